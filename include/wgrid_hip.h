@@ -1,0 +1,148 @@
+/*
+ * wgrid_hip.h -- C-ABI of the MI355X w-gridder / degridder.
+ *
+ * Each entry point replaces one member of WSClean's w-gridder boundary,
+ * class WGriddingGridderBase (wgridder/gridder_simple.h:20-42), whose
+ * concrete implementation WGriddingGridder_Simple<NumT> calls ducc0's
+ * ms2dirty / dirty2ms (wgridder/gridder_simple.h:209-227,
+ * wgridder/gridder_simple_implementation.h:170-196).  Plain pointers and
+ * sizes only; std::complex<float> buffers are passed as float[2 * n].
+ *
+ * Every function returns 0 on success and a non-zero status on failure;
+ * wg_last_error() then returns a thread-local message.  A failed call
+ * leaves the handle usable.  Calls on one handle must not overlap; distinct
+ * handles may be used from different threads concurrently (each handle owns
+ * its own HIP stream on the current device).
+ */
+#ifndef WGRID_HIP_H_
+#define WGRID_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wg_handle wg_handle;
+
+/* Constructor arguments of WGriddingGridder_Simple
+ * (wgridder/gridder_simple.h:84-88), in the same order. */
+typedef struct wg_params {
+  size_t width;          /* untrimmed image width (pixels) */
+  size_t height;         /* untrimmed image height */
+  size_t width_t;        /* trimmed width gridded (<= width, even) */
+  size_t height_t;       /* trimmed height (<= height, even) */
+  double pixel_size_x;   /* radians */
+  double pixel_size_y;   /* radians */
+  double l_shift;        /* phase-centre shift of the image centre */
+  double m_shift;
+  size_t nthreads;       /* accepted for interface parity; unused on GPU */
+  double epsilon;        /* requested accuracy (RMS vs. direct FT) */
+  size_t verbosity;      /* 0, 1 or 2; larger values are an error */
+  int tuning;            /* non-zero: tuned variant (ms2dirty_tuning) */
+} wg_params;
+
+/* new WGriddingGridder_Simple(...)   (gridder_simple_implementation.h:33-50)
+ * Fails with a status (the reference throws) if verbosity > 2. */
+int wg_create(const wg_params* params, wg_handle** out);
+/* ~WGriddingGridder_Simple */
+int wg_destroy(wg_handle* h);
+
+/* WGriddingGridderBase::ConstantMemoryUsage (gridder_simple.h:23,
+ * gridder_simple_implementation.h:52-60) */
+int wg_constant_memory_usage(const wg_handle* h, size_t* bytes);
+/* WGriddingGridderBase::PerVisibilityMemoryUsage (gridder_simple.h:24,
+ * gridder_simple_implementation.h:62-68) */
+int wg_per_visibility_memory_usage(const wg_handle* h, size_t* bytes);
+
+/* WGriddingGridderBase::InitializeInversion (gridder_simple.h:25,
+ * gridder_simple_implementation.h:70-73) */
+int wg_initialize_inversion(wg_handle* h);
+
+/* WGriddingGridderBase::AddInversionData (gridder_simple.h:26-28,
+ * gridder_simple_implementation.h:75-89).  uvw: n_rows*3 doubles (metres),
+ * freq: n_chan doubles (Hz, any order), vis: n_rows*n_chan complex<float>
+ * (row-major, weighted).  Zero visibilities are skipped; calls accumulate. */
+int wg_add_inversion_data(wg_handle* h, size_t n_rows, size_t n_chan,
+                          const double* uvw, const double* freq,
+                          const float* vis);
+
+/* WGriddingGridderBase::FinalizeImage (gridder_simple.h:36,
+ * gridder_simple_implementation.h:141-144) */
+int wg_finalize_image(wg_handle* h, double multiplication_factor);
+
+/* WGriddingGridderBase::RealImage (gridder_simple.h:37,
+ * gridder_simple_implementation.h:146-156).  out: width*height floats,
+ * index x + y*width; trimmed image centred, NaN border. */
+int wg_real_image(wg_handle* h, float* out);
+
+/* WGriddingGridderBase::InitializePrediction (gridder_simple.h:38,
+ * gridder_simple_implementation.h:158-167).  image: width*height floats,
+ * index x + y*width; the centred trimmed region is used. */
+int wg_initialize_prediction(wg_handle* h, const float* image);
+
+/* WGriddingGridderBase::PredictVisibilities (gridder_simple.h:39-41,
+ * gridder_simple_implementation.h:169-195).  vis_out: n_rows*n_chan
+ * complex<float>, overwritten. */
+int wg_predict_visibilities(wg_handle* h, size_t n_rows, size_t n_chan,
+                            const double* uvw, const double* freq,
+                            float* vis_out);
+
+/* Thread-local message of the last failed call on this thread. */
+const char* wg_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Device-resident variants.  Same semantics as the calls above, but uvw,
+ * vis and vis_out are HIP device pointers (freq stays on the host).  They
+ * correspond to the same reference members (AddInversionData,
+ * PredictVisibilities); the caller keeps the data in HBM across calls.
+ */
+int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
+                                 const double* uvw_dev, const double* freq,
+                                 const float* vis_dev);
+int wg_predict_visibilities_device(wg_handle* h, size_t n_rows, size_t n_chan,
+                                   const double* uvw_dev, const double* freq,
+                                   float* vis_out_dev);
+
+/* ------------------------------------------------------------------------
+ * Major-cycle variants.  A predict followed by an invert on the same uvw
+ * (WSClean's major cycle, main/wsclean.cpp:1519-1632, with
+ * WGriddingMSGridder::PredictMeasurementSet / GridMeasurementSet,
+ * wgridder/wgriddingmsgridder.cpp:117-241) shares one plan and one sort.
+ * wg_prepare_visibilities builds the device-resident sorted set once;
+ * invert/predict then reuse it.  Zero visibilities are gridded as zeros. */
+int wg_prepare_visibilities(wg_handle* h, size_t n_rows, size_t n_chan,
+                            const double* uvw_dev, const double* freq);
+int wg_add_inversion_prepared(wg_handle* h, const float* vis_dev);
+int wg_predict_prepared(wg_handle* h, float* vis_out_dev);
+
+/* Plan of the last call as JSON (W, sigma, nu, nv, P, box, tiles...). */
+int wg_plan_json(const wg_handle* h, char* buf, size_t len);
+/* Per-stage device times (ms) accumulated since the last reset when
+ * timing is enabled: plan, sort, grid, fft, degrid, total. */
+int wg_set_timing(wg_handle* h, int enabled);
+int wg_stage_times(wg_handle* h, double* out6, int reset);
+/* Tuning overrides (0 = automatic): kernel support, oversampling, tile
+ * edge, planes per gridding sub-slab, FFT-slab byte budget. */
+int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab,
+                  size_t slab_bytes);
+
+/* ------------------------------------------------------------------------
+ * Test entry points (no reference counterpart): expose internals so that
+ * the bucket sort and the LDS FFT can be checked bit-for-bit / against a
+ * library FFT. */
+/* Bucket keys (unsorted, one per visibility) and the stable sorted
+ * permutation of the valid ones.  vis may be NULL (no zero skipping). */
+int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan,
+                  const double* uvw, const double* freq, const float* vis,
+                  uint32_t* keys_out, uint32_t* perm_out, size_t* n_valid,
+                  char* plan_json, size_t plan_len);
+/* In-place batched complex FFT of nlines lines of length n (host data). */
+int wgx_fft(int n, int nlines, int inverse, float* data);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WGRID_HIP_H_ */

@@ -1,0 +1,177 @@
+/*
+ * oracle/dft.c -- FP64 direct Fourier transform oracle.  TEST
+ * INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never part of the product.
+ *
+ * Restates WSClean's DirectMSGridder (gridding/directmsgridder.cpp):
+ *   - per-sample contribution, gridSample():            :92-132
+ *       angle = -2 pi (u l + v m + w (sqrt(1 - l^2 - m^2) - 1))
+ *       I += Re(V) cos(angle) - Im(V) sin(angle)
+ *   - the n - 1 table, GetSqrtLMLookupTable():            :205-229
+ *       sqrt(1 - l^2 - m^2) - 1 inside the unit circle, 0 outside
+ *   - the final image wrap, FinishInversion():             :56-89
+ *       after the wrap, pixel (x, y) of the image (index x + y * width)
+ *       has l = (width/2 - x) dx + l_shift, m = (y - height/2) dy + m_shift
+ *       (even width / height).
+ * u, v, w are in wavelengths: uvw [m] * freq / c.  The division by
+ * ImageWeight() is left to the caller.
+ *
+ * The predict direction (DirectMSGridder has none, :184-202) is the adjoint:
+ *   V(u, v, w) = sum_{x,y} I(x, y) exp(+2 pi i (u l + v m + w (n - 1)))
+ * whose sign is fixed by WSClean's point-source generator, which writes
+ * V = exp(+2 pi i (u dl + v dm + w dn)) (gridding/msgridderdata.h:820-843,
+ * RotateVisibilities :929-941).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stddef.h>
+#include <stdlib.h>
+
+#define C_LIGHT 299792458.0
+
+static double pixel_l(int x, int nx, double dx, double ls) {
+  return (double)(nx / 2 - x) * dx + ls;
+}
+static double pixel_m(int y, int ny, double dy, double ms) {
+  return (double)(y - ny / 2) * dy + ms;
+}
+static double n_minus_1(double l, double m) {
+  double r2 = l * l + m * m;
+  return r2 < 1.0 ? sqrt(1.0 - r2) - 1.0 : 0.0;
+}
+
+typedef struct {
+  int nx, ny;
+  double dx, dy, ls, ms;
+  size_t n_rows, n_chan;
+  const double* uvw;
+  const double* freq;
+  const float* vis;      /* complex64 as float pairs */
+  const double* img;     /* predict input, x + y * nx */
+  double* out;
+  size_t npix;
+  const int* px;
+  const int* py;
+  size_t nsel;
+  const size_t* sel;     /* flat visibility indices */
+  int tid, nthreads;
+} job_t;
+
+static void* invert_worker(void* arg) {
+  job_t* j = (job_t*)arg;
+  for (size_t p = (size_t)j->tid; p < j->npix; p += (size_t)j->nthreads) {
+    int x, y;
+    if (j->px) {
+      x = j->px[p];
+      y = j->py[p];
+    } else {
+      x = (int)(p % (size_t)j->nx);
+      y = (int)(p / (size_t)j->nx);
+    }
+    const double l = pixel_l(x, j->nx, j->dx, j->ls);
+    const double m = pixel_m(y, j->ny, j->dy, j->ms);
+    const double nm1 = n_minus_1(l, m);
+    double acc = 0.0;
+    for (size_t r = 0; r < j->n_rows; ++r) {
+      const double* uvw = j->uvw + 3 * r;
+      for (size_t c = 0; c < j->n_chan; ++c) {
+        const float* v = j->vis + 2 * (r * j->n_chan + c);
+        if (v[0] == 0.0f && v[1] == 0.0f) continue;
+        const double s = j->freq[c] / C_LIGHT;
+        const double angle =
+            -2.0 * M_PI * (uvw[0] * s * l + uvw[1] * s * m + uvw[2] * s * nm1);
+        acc += (double)v[0] * cos(angle) - (double)v[1] * sin(angle);
+      }
+    }
+    j->out[p] = acc;
+  }
+  return NULL;
+}
+
+static void* predict_worker(void* arg) {
+  job_t* j = (job_t*)arg;
+  const size_t nvis = j->sel ? j->nsel : j->n_rows * j->n_chan;
+  /* collect non-zero pixels once per thread */
+  size_t np = (size_t)j->nx * (size_t)j->ny, nnz = 0;
+  for (size_t p = 0; p < np; ++p)
+    if (j->img[p] != 0.0) ++nnz;
+  double* pl = (double*)malloc(sizeof(double) * (nnz ? nnz : 1) * 4);
+  size_t k = 0;
+  for (size_t p = 0; p < np; ++p) {
+    if (j->img[p] == 0.0) continue;
+    int x = (int)(p % (size_t)j->nx), y = (int)(p / (size_t)j->nx);
+    double l = pixel_l(x, j->nx, j->dx, j->ls), m = pixel_m(y, j->ny, j->dy, j->ms);
+    pl[4 * k] = l;
+    pl[4 * k + 1] = m;
+    pl[4 * k + 2] = n_minus_1(l, m);
+    pl[4 * k + 3] = j->img[p];
+    ++k;
+  }
+  for (size_t q = (size_t)j->tid; q < nvis; q += (size_t)j->nthreads) {
+    const size_t idx = j->sel ? j->sel[q] : q;
+    const size_t r = idx / j->n_chan, c = idx % j->n_chan;
+    const double s = j->freq[c] / C_LIGHT;
+    const double u = j->uvw[3 * r] * s, v = j->uvw[3 * r + 1] * s, w = j->uvw[3 * r + 2] * s;
+    double re = 0.0, im = 0.0;
+    for (size_t t = 0; t < nnz; ++t) {
+      const double angle = 2.0 * M_PI * (u * pl[4 * t] + v * pl[4 * t + 1] + w * pl[4 * t + 2]);
+      re += pl[4 * t + 3] * cos(angle);
+      im += pl[4 * t + 3] * sin(angle);
+    }
+    j->out[2 * q] = re;
+    j->out[2 * q + 1] = im;
+  }
+  free(pl);
+  return NULL;
+}
+
+static void run(job_t* base, void* (*fn)(void*), int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  pthread_t th[256];
+  job_t jobs[256];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t] = *base;
+    jobs[t].tid = t;
+    jobs[t].nthreads = nthreads;
+    pthread_create(&th[t], NULL, fn, &jobs[t]);
+  }
+  for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+
+/* Full dirty image, out[x + y * nx] (not divided by ImageWeight). */
+void oracle_dft_invert(int nx, int ny, double dx, double dy, double l_shift, double m_shift,
+                       size_t n_rows, size_t n_chan, const double* uvw, const double* freq,
+                       const float* vis, double* out, int nthreads) {
+  job_t j = {0};
+  j.nx = nx; j.ny = ny; j.dx = dx; j.dy = dy; j.ls = l_shift; j.ms = m_shift;
+  j.n_rows = n_rows; j.n_chan = n_chan; j.uvw = uvw; j.freq = freq; j.vis = vis;
+  j.out = out; j.npix = (size_t)nx * (size_t)ny;
+  run(&j, invert_worker, nthreads);
+}
+
+/* Dirty image at npix selected pixels (x[i], y[i]). */
+void oracle_dft_invert_pixels(int nx, int ny, double dx, double dy, double l_shift,
+                              double m_shift, size_t n_rows, size_t n_chan, const double* uvw,
+                              const double* freq, const float* vis, size_t npix, const int* xs,
+                              const int* ys, double* out, int nthreads) {
+  job_t j = {0};
+  j.nx = nx; j.ny = ny; j.dx = dx; j.dy = dy; j.ls = l_shift; j.ms = m_shift;
+  j.n_rows = n_rows; j.n_chan = n_chan; j.uvw = uvw; j.freq = freq; j.vis = vis;
+  j.out = out; j.npix = npix; j.px = xs; j.py = ys;
+  run(&j, invert_worker, nthreads);
+}
+
+/* Predicted visibilities out[2 * (row * n_chan + chan) + {0, 1}] from the
+ * model image img[x + y * nx]; only non-zero pixels are summed.  With sel
+ * non-NULL only the nsel flat indices sel[] are predicted, into out[2 * q]. */
+void oracle_dft_predict(int nx, int ny, double dx, double dy, double l_shift, double m_shift,
+                        const double* img, size_t n_rows, size_t n_chan, const double* uvw,
+                        const double* freq, size_t nsel, const size_t* sel, double* out,
+                        int nthreads) {
+  job_t j = {0};
+  j.nx = nx; j.ny = ny; j.dx = dx; j.dy = dy; j.ls = l_shift; j.ms = m_shift;
+  j.n_rows = n_rows; j.n_chan = n_chan; j.uvw = uvw; j.freq = freq; j.img = img;
+  j.out = out; j.nsel = nsel; j.sel = sel;
+  run(&j, predict_worker, nthreads);
+}

@@ -1,0 +1,218 @@
+"""Oracle for the w-gridder: TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, as the checker.  Nothing in the product imports it.
+
+Contents
+--------
+* ``dft_invert`` / ``dft_invert_pixels`` / ``dft_predict``: the FP64 direct
+  Fourier transform of oracle/dft.c, a restatement of WSClean's
+  DirectMSGridder (gridding/directmsgridder.cpp:56-132, :205-229) plus its
+  adjoint (sign from gridding/msgridderdata.h:820-843, :929-941).
+* ``dft_invert_reference_wrap``: a literal numpy restatement of
+  DirectMSGridder::gridSample + FinishInversion (the scratch layout and the
+  wrap, directmsgridder.cpp:56-89, :112-131) for tiny inputs.  It pins the
+  closed-form pixel -> (l, m) mapping used by dft.c.
+* ``bucket_keys``: numpy restatement of the device's FP64 visibility ->
+  (tile, first plane) mapping (wsclean_amd/csrc/device.hpp LocateVis), the
+  bucketing step that ducc0's ms2dirty/dirty2ms perform internally
+  (wgridder/gridder_simple.h:217-225; ducc0 itself is absent, so its own
+  bucketing is parity unpinned -- the bucket sort is pinned against a stable
+  CPU sort of these keys).
+* ``es_phi_hat``: Fourier transform of the ES kernel by adaptive scipy
+  quadrature, to pin the planner's table and correction factors.
+* ``cpu_wgridder_*``: the multithreaded C++ CPU w-gridder restatement
+  (oracle/wgridder_cpu.cpp), used as the timed CPU baseline.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+C_LIGHT = 299792458.0
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _nthreads(n):
+    return int(n) if n else min(16, os.cpu_count() or 1)
+
+
+def dft_invert(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, nthreads=0):
+    """Dirty image [ny, nx] (index [y, x]), not divided by the image weight."""
+    lib = load()
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex64)
+    out = np.zeros(nx * ny, np.float64)
+    f = lib.oracle_dft_invert
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
+        [ctypes.c_size_t, ctypes.c_size_t] + [ctypes.c_void_p] * 4 + [ctypes.c_int]
+    f(nx, ny, dx, dy, l_shift, m_shift, uvw.size // 3, freq.size, _p(uvw), _p(freq), _p(vis),
+      _p(out), _nthreads(nthreads))
+    return out.reshape(ny, nx)
+
+
+def dft_invert_pixels(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, xs, ys, nthreads=0):
+    lib = load()
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex64)
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    out = np.zeros(xs.size, np.float64)
+    f = lib.oracle_dft_invert_pixels
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
+        [ctypes.c_size_t, ctypes.c_size_t] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t] + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int]
+    f(nx, ny, dx, dy, l_shift, m_shift, uvw.size // 3, freq.size, _p(uvw), _p(freq), _p(vis),
+      xs.size, _p(xs), _p(ys), _p(out), _nthreads(nthreads))
+    return out
+
+
+def dft_predict(nx, ny, dx, dy, l_shift, m_shift, image, uvw, freq, sel=None, nthreads=0):
+    """Visibilities from image [ny, nx]; complex128 [n_rows, n_chan] (or [len(sel)])."""
+    lib = load()
+    img = np.ascontiguousarray(image, np.float64).ravel()
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    n_rows, n_chan = uvw.size // 3, freq.size
+    if sel is not None:
+        sel = np.ascontiguousarray(sel, np.uint64)
+        nsel = sel.size
+        out = np.zeros(2 * nsel, np.float64)
+        selp = _p(sel)
+    else:
+        nsel = 0
+        out = np.zeros(2 * n_rows * n_chan, np.float64)
+        selp = None
+    f = lib.oracle_dft_predict
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_void_p] + \
+        [ctypes.c_size_t, ctypes.c_size_t] + [ctypes.c_void_p] * 2 + [ctypes.c_size_t] + \
+        [ctypes.c_void_p] * 2 + [ctypes.c_int]
+    f(nx, ny, dx, dy, l_shift, m_shift, _p(img), n_rows, n_chan, _p(uvw), _p(freq), nsel, selp,
+      _p(out), _nthreads(nthreads))
+    v = out[0::2] + 1j * out[1::2]
+    return v if sel is not None else v.reshape(n_rows, n_chan)
+
+
+def dft_invert_reference_wrap(width, height, dx, dy, l_shift, m_shift, uvw, freq, vis):
+    """Literal restatement of DirectMSGridder for tiny inputs (pure numpy).
+
+    gridSample (directmsgridder.cpp:112-131) accumulates into a scratch layer
+    whose pixel (x, y) uses xSrc = (x + W/2) mod W, ySrc = (H - y + H/2) mod
+    H, l = (W/2 - xSrc) dx + l_shift, m = (ySrc - H/2) dy + m_shift; then
+    FinishInversion (:73-86) writes image[x + y W] = scratch[xSrc + ySrc W]
+    with the same xSrc / ySrc.  Returns image [height, width].
+    """
+    uvw = np.asarray(uvw, np.float64).reshape(-1, 3)
+    freq = np.asarray(freq, np.float64)
+    vis = np.asarray(vis, np.complex64).reshape(uvw.shape[0], freq.size)
+    scratch = np.zeros(width * height)
+    for y in range(height):
+        ysrc = (height - y) + height // 2
+        if ysrc >= height:
+            ysrc -= height
+        m = (ysrc - height // 2) * dy + m_shift
+        for x in range(width):
+            xsrc = x + width // 2
+            if xsrc >= width:
+                xsrc -= width
+            l = (width // 2 - xsrc) * dx + l_shift
+            nm1 = np.sqrt(1.0 - l * l - m * m) - 1.0 if l * l + m * m < 1.0 else 0.0
+            acc = 0.0
+            for r in range(uvw.shape[0]):
+                for c in range(freq.size):
+                    s = freq[c] / C_LIGHT
+                    u, v, w = uvw[r] * s
+                    ang = -2.0 * np.pi * (u * l + v * m + w * nm1)
+                    val = vis[r, c]
+                    acc += float(val.real) * np.cos(ang) - float(val.imag) * np.sin(ang)
+            scratch[y * width + x] = acc
+    image = np.zeros(width * height)
+    for y in range(height):
+        ysrc = (height - y) + height // 2
+        if ysrc >= height:
+            ysrc -= height
+        for x in range(width):
+            xsrc = x + width // 2
+            if xsrc >= width:
+                xsrc -= width
+            image[x + y * width] = scratch[xsrc + ysrc * width]
+    return image.reshape(height, width)
+
+
+def bucket_keys(plan, uvw, freq, vis=None):
+    """(keys, valid) per flat visibility index, FP64 like LocateVis."""
+    uvw = np.asarray(uvw, np.float64).reshape(-1, 3)
+    freq = np.asarray(freq, np.float64)
+    W, T = plan["W"], plan["T"]
+    nu, nv = plan["nu"], plan["nv"]
+    dx, dy = plan["dx"], plan["dy"]
+    ufac = float(nu) * dx
+    vfac = float(nv) * dy
+    scale = freq / C_LIGHT
+    flip = uvw[:, 2] < 0.0
+    um = np.where(flip, -uvw[:, 0], uvw[:, 0])
+    vm = np.where(flip, -uvw[:, 1], uvw[:, 1])
+    wm = np.where(flip, -uvw[:, 2], uvw[:, 2])
+    u = um[:, None] * scale[None, :]
+    v = vm[:, None] * scale[None, :]
+    w = wm[:, None] * scale[None, :]
+    tu = u * ufac + 0.5 * nu
+    tv = v * vfac + 0.5 * nv
+    inv_dw = 1.0 / plan["dw"]
+    tw = (w - plan["w_lo"]) * inv_dw + float(W // 2 - 1)
+    hw = W // 2 - 1
+    a0u = np.floor(tu).astype(np.int64) - hw
+    a0v = np.floor(tv).astype(np.int64) - hw
+    p0 = np.floor(tw).astype(np.int64) - hw
+    if plan["wrap_u"]:
+        lu = np.mod(a0u, nu)
+    else:
+        lu = a0u - plan["box_u0"]
+    if plan["wrap_v"]:
+        lv = np.mod(a0v, nv)
+    else:
+        lv = a0v - plan["box_v0"]
+    tile = (lv // T) * plan["ntu"] + (lu // T)
+    keys = (tile * plan["P0"] + p0).astype(np.int64)
+    valid = np.ones(keys.shape, bool)
+    if vis is not None:
+        vis = np.asarray(vis).reshape(keys.shape)
+        valid = vis != 0
+    nkeys = plan["ntu"] * plan["ntv"] * plan["P0"]
+    keys = np.where(valid, keys, nkeys)
+    return keys.ravel().astype(np.uint32), valid.ravel()
+
+
+def es_phi_hat(W, beta, t):
+    """Phi(t) = int_{-W/2}^{W/2} exp(beta (sqrt(1 - (2s/W)^2) - 1)) cos(2 pi s t) ds."""
+    from scipy.integrate import quad
+
+    def f(s):
+        x = 2.0 * s / W
+        return np.exp(beta * (np.sqrt(max(0.0, 1.0 - x * x)) - 1.0)) * np.cos(2 * np.pi * s * t)
+
+    val, _ = quad(f, -W / 2, W / 2, limit=400, epsabs=1e-14, epsrel=1e-13)
+    return val

@@ -1,0 +1,259 @@
+"""GPU parity tests: the HIP w-gridder through the C-ABI against the FP64
+direct-FT oracle (DirectMSGridder restatement) and against exact
+properties.  Tolerances are the requested epsilon in relative L2 (the
+meaning of epsilon in doc/source/wgridding.rst:19-20) unless stated.
+"""
+import numpy as np
+import pytest
+
+from wsclean_amd import WGriddingGridder_Simple, WgError, fft_lines, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def make(n, dx, eps, ls=0.0, ms=0.0, width=None):
+    w = width or n
+    return WGriddingGridder_Simple(w, w, n, n, dx, dx, ls, ms, 1, eps)
+
+
+def invert(p, eps, ls=0.0, ms=0.0, n=None):
+    n = n or p["npix"]
+    g = make(n, p["dx"], eps, ls, ms)
+    g.InitializeInversion()
+    g.AddInversionData(p["uvw"].shape[0], len(p["freq"]), p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(1.0)
+    img = g.RealImage().reshape(n, n)
+    return img, g
+
+
+# ------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [16, 60, 96, 1000, 2048, 2560, 3000, 4096, 10240, 16384])
+def test_lds_fft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = (rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))).astype(np.complex64)
+    f = fft_lines(x)
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    assert rel_l2(f, ref) < 2e-6
+    b = fft_lines(x, inverse=True)
+    refb = np.fft.ifft(x.astype(np.complex128), axis=1) * n
+    assert rel_l2(b, refb) < 2e-6
+
+
+# ------------------------------------------------------------------ sort
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-5])
+def test_bucket_sort_bit_exact(oracle, eps):
+    p = synth.small_problem(seed=7, n_ant=16, n_times=30, n_chan=3, npix=128)
+    vis = p["vis"].copy()
+    vis[::7, 1] = 0  # zero visibilities are skipped
+    g = make(128, p["dx"], eps)
+    keys, perm, plan = g.sort_keys(p["uvw"], p["freq"], vis)
+    ref_keys, valid = oracle.bucket_keys(plan, p["uvw"], p["freq"], vis)
+    np.testing.assert_array_equal(keys, ref_keys)
+    ref_perm = np.argsort(ref_keys, kind="stable")[: int(valid.sum())]
+    np.testing.assert_array_equal(perm, ref_perm.astype(np.uint32))
+
+
+def test_bucket_sort_bit_exact_large(oracle):
+    # enough visibilities for several radix blocks and all three digit passes
+    p = synth.small_problem(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512,
+                            pixel_arcsec=8.0, radius_m=4000.0)
+    g = make(512, p["dx"], 1e-5)
+    keys, perm, plan = g.sort_keys(p["uvw"], p["freq"], None)
+    ref_keys, _ = oracle.bucket_keys(plan, p["uvw"], p["freq"], None)
+    assert plan["key_bits"] > 8
+    np.testing.assert_array_equal(keys, ref_keys)
+    np.testing.assert_array_equal(perm, np.argsort(ref_keys, kind="stable").astype(np.uint32))
+
+
+# ---------------------------------------------------------------- invert
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-4, 1e-5])
+def test_invert_matches_direct_ft(oracle, eps):
+    p = synth.small_problem(seed=1001, n_ant=16, n_times=24, n_chan=2, npix=96, noise=0.01)
+    img, g = invert(p, eps)
+    ref = oracle.dft_invert(96, 96, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"], p["vis"])
+    err = rel_l2(img, ref)
+    print("eps", eps, "rel_l2", err, g.plan())
+    assert err < eps
+
+
+def test_invert_with_phase_centre_shift(oracle):
+    p = synth.small_problem(seed=3, n_ant=14, n_times=20, n_chan=2, npix=80)
+    ls, ms = 0.02, -0.015
+    img, g = invert(p, 1e-5, ls, ms)
+    ref = oracle.dft_invert(80, 80, p["dx"], p["dx"], ls, ms, p["uvw"], p["freq"], p["vis"])
+    assert rel_l2(img, ref) < 1e-5
+
+
+def test_invert_wide_field_many_w_planes(oracle):
+    # 4 degree field with 10 km baselines: many w planes (P >> W)
+    p = synth.small_problem(seed=5, n_ant=20, n_times=40, n_chan=2, npix=512,
+                            pixel_arcsec=30.0, radius_m=6000.0, n_src=8)
+    img, g = invert(p, 1e-4)
+    plan = g.plan()
+    assert plan["P"] > plan["W"] + 4, plan
+    rng = np.random.default_rng(0)
+    xs = rng.integers(0, 512, 600)
+    ys = rng.integers(0, 512, 600)
+    ref = oracle.dft_invert_pixels(512, 512, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
+                                   p["vis"], xs, ys)
+    assert rel_l2(img[ys, xs], ref) < 1e-4
+
+
+def test_invert_wrapping_uv(oracle):
+    # pixels too coarse for the longest baselines: the uv box wraps around
+    p = synth.small_problem(seed=9, n_ant=12, n_times=10, n_chan=2, npix=64,
+                            pixel_arcsec=60.0, radius_m=3000.0)
+    img, g = invert(p, 1e-4)
+    assert g.plan()["wrap_u"] == 1 or g.plan()["wrap_v"] == 1
+    ref = oracle.dft_invert(64, 64, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"], p["vis"])
+    assert rel_l2(img, ref) < 1e-4
+
+
+def test_point_source_peak(oracle):
+    p = synth.small_problem(seed=12, n_src=1, on_pixel=True, npix=64, n_chan=2)
+    img, g = invert(p, 1e-5)
+    x0, y0 = int(p["xs"][0]), int(p["ys"][0])
+    peak = img[y0, x0] / p["vis"].size
+    assert abs(peak - p["flux"][0]) < 1e-5 * p["flux"][0] * 10
+
+
+def test_chunk_additivity_and_frequency_reversal():
+    p = synth.small_problem(seed=2, n_ant=12, n_times=20, n_chan=3, npix=64)
+    one, _ = invert(p, 1e-4)
+    g = make(64, p["dx"], 1e-4)
+    g.InitializeInversion()
+    h = p["uvw"].shape[0] // 2
+    g.AddInversionData(h, 3, p["uvw"][:h], p["freq"], p["vis"][:h])
+    g.AddInversionData(p["uvw"].shape[0] - h, 3, p["uvw"][h:], p["freq"], p["vis"][h:])
+    g.FinalizeImage(1.0)
+    two = g.RealImage().reshape(64, 64)
+    assert rel_l2(two, one) < 1e-4
+    q = dict(p)
+    q["freq"] = p["freq"][::-1].copy()
+    q["vis"] = np.ascontiguousarray(p["vis"][:, ::-1])
+    rev, _ = invert(q, 1e-4)
+    assert rel_l2(rev, one) < 1e-5
+
+
+def test_zero_visibilities_give_zero_image():
+    p = synth.small_problem(seed=2, npix=32)
+    p["vis"][:] = 0
+    img, _ = invert(p, 1e-4)
+    assert np.all(img == 0)
+
+
+def test_real_image_trim_and_nan_border():
+    p = synth.small_problem(seed=4, npix=48)
+    g = WGriddingGridder_Simple(64, 60, 48, 48, p["dx"], p["dx"], 0.0, 0.0, 1, 1e-4)
+    g.InitializeInversion()
+    g.AddInversionData(p["uvw"].shape[0], 2, p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(0.5)
+    img = g.RealImage().reshape(60, 64)
+    inner = img[6:54, 8:56]
+    assert np.all(np.isfinite(inner))
+    assert np.isnan(img[:6]).all() and np.isnan(img[54:]).all()
+    assert np.isnan(img[:, :8]).all() and np.isnan(img[:, 56:]).all()
+    ref, _ = invert(p, 1e-4, n=48)
+    assert rel_l2(inner, 0.5 * ref) < 1e-5
+
+
+# --------------------------------------------------------------- predict
+
+def predict(n, dx, eps, model, uvw, freq, ls=0.0, ms=0.0, width=None):
+    w = width or n
+    g = WGriddingGridder_Simple(w, w, n, n, dx, dx, ls, ms, 1, eps)
+    g.InitializePrediction(model.astype(np.float32).ravel())
+    out = np.zeros((uvw.shape[0], len(freq)), np.complex64)
+    g.PredictVisibilities(uvw.shape[0], len(freq), uvw, freq, out)
+    return out, g
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-4, 1e-5])
+def test_predict_matches_direct_ft(oracle, eps):
+    p = synth.small_problem(seed=31, n_ant=16, n_times=24, n_chan=2, npix=96)
+    rng = np.random.default_rng(1)
+    model = np.zeros((96, 96))
+    ys = rng.integers(5, 90, 12)
+    xs = rng.integers(5, 90, 12)
+    model[ys, xs] = rng.uniform(0.1, 1.0, 12)
+    out, g = predict(96, p["dx"], eps, model, p["uvw"], p["freq"])
+    ref = oracle.dft_predict(96, 96, p["dx"], p["dx"], 0.0, 0.0, model, p["uvw"], p["freq"])
+    err = rel_l2(out, ref)
+    print("eps", eps, "rel_l2", err, g.plan())
+    assert err < eps
+
+
+def test_predict_shift_and_wide_field(oracle):
+    p = synth.small_problem(seed=8, n_ant=20, n_times=30, n_chan=2, npix=384,
+                            pixel_arcsec=30.0, radius_m=5000.0)
+    rng = np.random.default_rng(2)
+    model = np.zeros((384, 384))
+    model[rng.integers(0, 384, 30), rng.integers(0, 384, 30)] = rng.uniform(-1, 1, 30)
+    ls, ms = -0.01, 0.02
+    out, g = predict(384, p["dx"], 1e-4, model, p["uvw"], p["freq"], ls, ms)
+    sel = rng.choice(out.size, 500, replace=False)
+    ref = oracle.dft_predict(384, 384, p["dx"], p["dx"], ls, ms, model, p["uvw"], p["freq"],
+                             sel=sel)
+    assert rel_l2(out.ravel()[sel], ref) < 1e-4
+
+
+def test_predict_zero_image_and_untrim():
+    p = synth.small_problem(seed=4, npix=48)
+    out, _ = predict(48, p["dx"], 1e-4, np.zeros((48, 48)), p["uvw"], p["freq"])
+    assert np.all(out == 0)
+
+
+def test_adjoint_invert_predict():
+    # Re<A x, y> = <x, A^H y> holds to epsilon for the gridder pair
+    p = synth.small_problem(seed=13, n_ant=14, n_times=20, n_chan=2, npix=64, noise=0.1)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(64, 64))
+    Ax, _ = predict(64, p["dx"], 1e-5, x, p["uvw"], p["freq"])
+    AHy, _ = invert(p, 1e-5)
+    lhs = float(np.sum(x * AHy))
+    rhs = float(np.real(np.sum(p["vis"].astype(np.complex128) * np.conj(Ax))))
+    assert abs(lhs - rhs) < 1e-5 * np.linalg.norm(x) * np.linalg.norm(AHy)
+
+
+def test_major_cycle_prepared_matches_stateless():
+    import torch
+    p = synth.small_problem(seed=21, n_ant=14, n_times=20, n_chan=2, npix=64)
+    ref_img, _ = invert(p, 1e-4)
+    g = make(64, p["dx"], 1e-4)
+    dev = torch.device("cuda:0")
+    uvw = torch.from_numpy(p["uvw"]).to(dev)
+    vis = torch.from_numpy(p["vis"].view(np.float32)).to(dev)
+    g.PrepareVisibilities(uvw, p["freq"], p["uvw"].shape[0], 2)
+    g.InitializeInversion()
+    g.AddInversionPrepared(vis)
+    g.FinalizeImage(1.0)
+    img = g.RealImage().reshape(64, 64)
+    assert rel_l2(img, ref_img) < 1e-5
+    out = torch.zeros_like(vis)
+    g.InitializePrediction(img.ravel())
+    g.PredictPrepared(out)
+    ref_out, _ = predict(64, p["dx"], 1e-4, img, p["uvw"], p["freq"])
+    got = out.cpu().numpy().view(np.complex64).reshape(ref_out.shape)
+    assert rel_l2(got, ref_out) < 1e-5
+
+
+# ---------------------------------------------------------------- errors
+
+def test_errors_leave_handle_usable():
+    with pytest.raises(WgError):
+        WGriddingGridder_Simple(64, 64, 64, 64, 1e-4, 1e-4, 0, 0, 1, 1e-4, verbosity=3)
+    p = synth.small_problem(seed=4, npix=32)
+    g = make(32, p["dx"], 1e-4)
+    g.InitializeInversion()
+    with pytest.raises(WgError):
+        g.AddInversionData(p["uvw"].shape[0], 2, p["uvw"], -p["freq"], p["vis"])
+    g.AddInversionData(p["uvw"].shape[0], 2, p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(1.0)
+    assert np.isfinite(g.RealImage()).all()

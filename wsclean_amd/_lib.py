@@ -1,0 +1,89 @@
+"""ctypes binding of libwgrid_hip.so (the C-ABI in include/wgrid_hip.h).
+
+The library is built in-tree by ``python -m wsclean_amd.build``.  Loading
+fails loudly if it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwgrid_hip.so")
+
+c_size = ctypes.c_size_t
+c_dbl_p = ctypes.POINTER(ctypes.c_double)
+c_flt_p = ctypes.POINTER(ctypes.c_float)
+c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class WgParams(ctypes.Structure):
+    _fields_ = [
+        ("width", c_size), ("height", c_size),
+        ("width_t", c_size), ("height_t", c_size),
+        ("pixel_size_x", ctypes.c_double), ("pixel_size_y", ctypes.c_double),
+        ("l_shift", ctypes.c_double), ("m_shift", ctypes.c_double),
+        ("nthreads", c_size), ("epsilon", ctypes.c_double),
+        ("verbosity", c_size), ("tuning", ctypes.c_int),
+    ]
+
+
+# (name, restype, argtypes) for every symbol declared in include/wgrid_hip.h
+SIGNATURES = [
+    ("wg_create", ctypes.c_int, [ctypes.POINTER(WgParams), ctypes.POINTER(ctypes.c_void_p)]),
+    ("wg_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_constant_memory_usage", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
+    ("wg_per_visibility_memory_usage", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
+    ("wg_initialize_inversion", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_add_inversion_data", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_finalize_image", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("wg_real_image", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_initialize_prediction", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_predict_visibilities", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_last_error", ctypes.c_char_p, []),
+    ("wg_add_inversion_data_device", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_predict_visibilities_device", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_prepare_visibilities", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_add_inversion_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_predict_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_plan_json", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_size]),
+    ("wg_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("wg_stage_times", ctypes.c_int, [ctypes.c_void_p, c_dbl_p, ctypes.c_int]),
+    ("wg_set_tuning", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, c_size]),
+    ("wgx_sort_keys", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),
+    ("wgx_fft", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+]
+
+_lib = None
+
+
+class WgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libwgrid_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WgError("libwgrid_hip.so is not built; run python -m wsclean_amd.build")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        msg = _lib.wg_last_error()
+        raise WgError(msg.decode() if msg else "wgrid error %d" % status)

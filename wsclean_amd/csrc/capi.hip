@@ -1,0 +1,404 @@
+// C-ABI of the MI355X w-gridder (declared in include/wgrid_hip.h).  The
+// handle mirrors WGriddingGridder_Simple<NumT>: it owns the trimmed image
+// (fp64, x-major index x * height_t + y as in
+// wgridder/gridder_simple_implementation.h:147-167) in HBM between calls.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/wgrid_hip.h"
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr double kSigmaMax = 2.0;  // WGriddingGridder_Simple::sigma_max
+
+__global__ void ScaleKernel(double* __restrict__ p, size_t n, double f) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] *= f;
+}
+
+// out[(i + dx) + (j + dy) * width] = img[i * height_t + j], NaN elsewhere.
+__global__ void RealImageKernel(const double* __restrict__ img, float* __restrict__ out,
+                                int width, int height, int width_t, int height_t) {
+  const size_t n = size_t(width) * height;
+  const int dx = (width - width_t) / 2, dy = (height - height_t) / 2;
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += size_t(gridDim.x) * blockDim.x) {
+    const int x = int(k % width), y = int(k / width);
+    const int i = x - dx, j = y - dy;
+    out[k] = (i >= 0 && i < width_t && j >= 0 && j < height_t)
+                 ? float(img[size_t(i) * height_t + j])
+                 : __builtin_nanf("");
+  }
+}
+
+// img[i * height_t + j] = image[(i + dx) + (j + dy) * width]
+__global__ void LoadModelKernel(const float* __restrict__ image, double* __restrict__ img,
+                                int width, int height, int width_t, int height_t) {
+  const size_t n = size_t(width_t) * height_t;
+  const int dx = (width - width_t) / 2, dy = (height - height_t) / 2;
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += size_t(gridDim.x) * blockDim.x) {
+    const int i = int(k / height_t), j = int(k % height_t);
+    img[k] = double(image[size_t(i + dx) + size_t(j + dy) * width]);
+  }
+}
+
+unsigned Blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  return unsigned(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+template <typename F>
+int Guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return 1;
+  }
+}
+
+}  // namespace
+
+struct wg_handle {
+  wg_params prm{};
+  std::unique_ptr<wg::Engine> eng;
+  double* img = nullptr;  // [width_t][height_t] fp64
+  wg::VisSet vs_call;
+  wg::VisSet vs_prep;
+  bool prepared = false;
+  std::string plan_json;
+  std::mutex mu;
+
+  size_t npix() const { return prm.width_t * prm.height_t; }
+  hipStream_t s() const { return eng->Stream(); }
+};
+
+extern "C" {
+
+const char* wg_last_error(void) { return g_last_error.c_str(); }
+
+int wg_create(const wg_params* params, wg_handle** out) {
+  return Guard([&] {
+    WG_CHECK(params && out, "null argument");
+    WG_CHECK(params->verbosity <= 2, "verbosity must be 0, 1, or 2");
+    WG_CHECK(params->width_t <= params->width && params->height_t <= params->height,
+             "trimmed size exceeds image size");
+    WG_CHECK(params->width_t > 0 && params->height_t > 0, "empty image");
+    auto h = std::make_unique<wg_handle>();
+    h->prm = *params;
+    wg::ImageGeometry g;
+    g.nx = int(params->width_t);
+    g.ny = int(params->height_t);
+    g.dx = params->pixel_size_x;
+    g.dy = params->pixel_size_y;
+    g.l_shift = params->l_shift;
+    g.m_shift = params->m_shift;
+    g.epsilon = params->epsilon;
+    wg::PlanOptions opt;
+    h->eng = std::make_unique<wg::Engine>(g, opt);
+    WG_HIP(hipMalloc(&h->img, h->npix() * sizeof(double)));
+    WG_HIP(hipMemsetAsync(h->img, 0, h->npix() * sizeof(double), h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
+    *out = h.release();
+  });
+}
+
+int wg_destroy(wg_handle* h) {
+  return Guard([&] {
+    if (!h) return;
+    if (h->img) (void)hipFree(h->img);
+    delete h;
+  });
+}
+
+int wg_constant_memory_usage(const wg_handle* h, size_t* bytes) {
+  return Guard([&] {
+    WG_CHECK(h && bytes, "null argument");
+    // Same accounting as gridder_simple_implementation.h:53-60.
+    const size_t num_size = h->prm.epsilon <= 1.01e-5 ? sizeof(double) : sizeof(float);
+    size_t c = size_t(kSigmaMax * kSigmaMax * double(h->prm.width_t) * double(h->prm.height_t) *
+                      8.0);
+    c += h->prm.width_t * h->prm.height_t * num_size;
+    *bytes = c;
+  });
+}
+
+int wg_per_visibility_memory_usage(const wg_handle* h, size_t* bytes) {
+  return Guard([&] {
+    WG_CHECK(h && bytes, "null argument");
+    *bytes = 8;  // gridder_simple_implementation.h:62-68
+  });
+}
+
+int wg_initialize_inversion(wg_handle* h) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    WG_HIP(hipMemsetAsync(h->img, 0, h->npix() * sizeof(double), h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+static void AddInversion(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                         const double* freq, const float* vis_dev) {
+  wg::Engine& e = *h->eng;
+  e.Prepare(h->vs_call, "call", n_rows, n_chan, uvw_dev, freq,
+            reinterpret_cast<const float2*>(vis_dev));
+  h->plan_json = h->vs_call.plan.describe();
+  e.Invert(h->vs_call, reinterpret_cast<const float2*>(vis_dev), h->img);
+  WG_HIP(hipStreamSynchronize(h->s()));
+}
+
+int wg_add_inversion_data(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
+                          const double* freq, const float* vis) {
+  return Guard([&] {
+    WG_CHECK(h && freq && (n_rows == 0 || (uvw && vis)), "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (n_rows == 0) return;
+    wg::DeviceArena& a = h->eng->Arena();
+    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
+    float* dvis = static_cast<float*>(a.Get("h2d.vis", n_rows * n_chan * 8));
+    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice,
+                          h->s()));
+    WG_HIP(hipMemcpyAsync(dvis, vis, n_rows * n_chan * 8, hipMemcpyHostToDevice, h->s()));
+    AddInversion(h, n_rows, n_chan, duvw, freq, dvis);
+  });
+}
+
+int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
+                                 const double* uvw_dev, const double* freq,
+                                 const float* vis_dev) {
+  return Guard([&] {
+    WG_CHECK(h && freq && (n_rows == 0 || (uvw_dev && vis_dev)), "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (n_rows == 0) return;
+    AddInversion(h, n_rows, n_chan, uvw_dev, freq, vis_dev);
+  });
+}
+
+int wg_finalize_image(wg_handle* h, double f) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    ScaleKernel<<<Blocks(h->npix()), 256, 0, h->s()>>>(h->img, h->npix(), f);
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_real_image(wg_handle* h, float* out) {
+  return Guard([&] {
+    WG_CHECK(h && out, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t n = h->prm.width * h->prm.height;
+    float* d = static_cast<float*>(h->eng->Arena().Get("d2h.image", n * 4));
+    RealImageKernel<<<Blocks(n), 256, 0, h->s()>>>(h->img, d, int(h->prm.width),
+                                                 int(h->prm.height), int(h->prm.width_t),
+                                                 int(h->prm.height_t));
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_initialize_prediction(wg_handle* h, const float* image) {
+  return Guard([&] {
+    WG_CHECK(h && image, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t n = h->prm.width * h->prm.height;
+    float* d = static_cast<float*>(h->eng->Arena().Get("h2d.image", n * 4));
+    WG_HIP(hipMemcpyAsync(d, image, n * 4, hipMemcpyHostToDevice, h->s()));
+    LoadModelKernel<<<Blocks(h->npix()), 256, 0, h->s()>>>(d, h->img, int(h->prm.width),
+                                                         int(h->prm.height),
+                                                         int(h->prm.width_t),
+                                                         int(h->prm.height_t));
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+static void Predict(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                    const double* freq, float* out_dev) {
+  wg::Engine& e = *h->eng;
+  e.Prepare(h->vs_call, "call", n_rows, n_chan, uvw_dev, freq, nullptr);
+  h->plan_json = h->vs_call.plan.describe();
+  e.Predict(h->vs_call, h->img, reinterpret_cast<float2*>(out_dev));
+  WG_HIP(hipStreamSynchronize(h->s()));
+}
+
+int wg_predict_visibilities(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
+                            const double* freq, float* vis_out) {
+  return Guard([&] {
+    WG_CHECK(h && freq && (n_rows == 0 || (uvw && vis_out)), "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (n_rows == 0) return;
+    wg::DeviceArena& a = h->eng->Arena();
+    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
+    float* dout = static_cast<float*>(a.Get("h2d.vis", n_rows * n_chan * 8));
+    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice,
+                          h->s()));
+    Predict(h, n_rows, n_chan, duvw, freq, dout);
+    WG_HIP(hipMemcpyAsync(vis_out, dout, n_rows * n_chan * 8, hipMemcpyDeviceToHost, h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_predict_visibilities_device(wg_handle* h, size_t n_rows, size_t n_chan,
+                                   const double* uvw_dev, const double* freq,
+                                   float* vis_out_dev) {
+  return Guard([&] {
+    WG_CHECK(h && freq && (n_rows == 0 || (uvw_dev && vis_out_dev)), "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (n_rows == 0) return;
+    Predict(h, n_rows, n_chan, uvw_dev, freq, vis_out_dev);
+  });
+}
+
+int wg_prepare_visibilities(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                            const double* freq) {
+  return Guard([&] {
+    WG_CHECK(h && freq && uvw_dev, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->prepared = false;
+    h->eng->Prepare(h->vs_prep, "prep", n_rows, n_chan, uvw_dev, freq, nullptr);
+    h->plan_json = h->vs_prep.plan.describe();
+    h->prepared = true;
+  });
+}
+
+int wg_add_inversion_prepared(wg_handle* h, const float* vis_dev) {
+  return Guard([&] {
+    WG_CHECK(h && vis_dev, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    WG_CHECK(h->prepared, "wg_prepare_visibilities has not been called");
+    h->eng->Invert(h->vs_prep, reinterpret_cast<const float2*>(vis_dev), h->img);
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_predict_prepared(wg_handle* h, float* vis_out_dev) {
+  return Guard([&] {
+    WG_CHECK(h && vis_out_dev, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    WG_CHECK(h->prepared, "wg_prepare_visibilities has not been called");
+    h->eng->Predict(h->vs_prep, h->img, reinterpret_cast<float2*>(vis_out_dev));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_plan_json(const wg_handle* h, char* buf, size_t len) {
+  return Guard([&] {
+    WG_CHECK(h && buf && len > 0, "null argument");
+    std::strncpy(buf, h->plan_json.c_str(), len - 1);
+    buf[len - 1] = 0;
+  });
+}
+
+int wg_set_timing(wg_handle* h, int enabled) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    h->eng->timing = enabled != 0;
+  });
+}
+
+int wg_stage_times(wg_handle* h, double* out6, int reset) {
+  return Guard([&] {
+    WG_CHECK(h && out6, "null argument");
+    wg::StageTimes& t = h->eng->Times();
+    out6[0] = t.plan;
+    out6[1] = t.sort;
+    out6[2] = t.grid;
+    out6[3] = t.fft;
+    out6[4] = t.degrid;
+    out6[5] = t.plan + t.sort + t.grid + t.fft + t.degrid;
+    if (reset) t = wg::StageTimes{};
+  });
+}
+
+int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab, size_t slab_bytes) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    wg::PlanOptions& o = h->eng->Options();
+    o.force_W = W;
+    o.force_sigma = sigma;
+    o.T = tile;
+    o.Sg = sub_slab;
+    if (slab_bytes) o.slab_bytes = slab_bytes;
+  });
+}
+
+int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
+                  const double* freq, const float* vis, uint32_t* keys_out, uint32_t* perm_out,
+                  size_t* n_valid, char* plan_json, size_t plan_len) {
+  return Guard([&] {
+    WG_CHECK(h && uvw && freq && keys_out && perm_out && n_valid, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    wg::Engine& e = *h->eng;
+    wg::DeviceArena& a = e.Arena();
+    const size_t nvis = n_rows * n_chan;
+    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice,
+                          e.Stream()));
+    float* dvis = nullptr;
+    if (vis) {
+      dvis = static_cast<float*>(a.Get("h2d.vis", nvis * 8));
+      WG_HIP(hipMemcpyAsync(dvis, vis, nvis * 8, hipMemcpyHostToDevice, e.Stream()));
+    }
+    e.Prepare(h->vs_call, "call", n_rows, n_chan, duvw, freq, reinterpret_cast<float2*>(dvis));
+    // unsorted keys: recompute into a scratch array
+    uint32_t* k = static_cast<uint32_t*>(a.Get("x.keys", nvis * 4));
+    uint32_t* v = static_cast<uint32_t*>(a.Get("x.vals", nvis * 4));
+    wg::LaunchKeys(h->vs_call.dp, duvw, h->vs_call.scale, n_rows, n_chan,
+                   reinterpret_cast<float2*>(dvis), k, v, e.Stream());
+    WG_HIP(hipMemcpyAsync(keys_out, k, nvis * 4, hipMemcpyDeviceToHost, e.Stream()));
+    WG_HIP(hipMemcpyAsync(perm_out, h->vs_call.perm, h->vs_call.n_valid * 4,
+                          hipMemcpyDeviceToHost, e.Stream()));
+    WG_HIP(hipStreamSynchronize(e.Stream()));
+    *n_valid = h->vs_call.n_valid;
+    h->plan_json = h->vs_call.plan.describe();
+    if (plan_json && plan_len) {
+      std::strncpy(plan_json, h->plan_json.c_str(), plan_len - 1);
+      plan_json[plan_len - 1] = 0;
+    }
+  });
+}
+
+int wgx_fft(int n, int nlines, int inverse, float* data) {
+  return Guard([&] {
+    WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");
+    std::vector<float2> tw(n);
+    for (int m = 0; m < n; ++m) {
+      const double ang = 2.0 * M_PI * double(m) / double(n);
+      tw[m] = make_float2(float(std::cos(ang)), float(-std::sin(ang)));
+    }
+    float2 *dtw = nullptr, *dd = nullptr;
+    const size_t bytes = size_t(n) * nlines * sizeof(float2);
+    WG_HIP(hipMalloc(&dtw, n * sizeof(float2)));
+    WG_HIP(hipMalloc(&dd, bytes));
+    WG_HIP(hipMemcpy(dtw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    WG_HIP(hipMemcpy(dd, data, bytes, hipMemcpyHostToDevice));
+    wg::LaunchLineFFT(dd, nlines, wg::MakeFFTDesc(n, dtw), inverse != 0, nullptr);
+    WG_HIP(hipDeviceSynchronize());
+    WG_HIP(hipMemcpy(data, dd, bytes, hipMemcpyDeviceToHost));
+    (void)hipFree(dd);
+    (void)hipFree(dtw);
+  });
+}
+
+}  // extern "C"

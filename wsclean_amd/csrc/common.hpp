@@ -1,0 +1,113 @@
+// Shared definitions for the MI355X w-gridder: plan description, error
+// handling and small device helpers.  Everything here is plain C++ / HIP for
+// gfx950; no CUDA shims and no alternative backends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace wg {
+
+constexpr double kSpeedOfLight = 299792458.0;
+
+// Thrown by host code; the C-ABI converts it into a status code + message.
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define WG_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      throw ::wg::Error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                        " at " + __FILE__ + ":" + std::to_string(__LINE__) + \
+                        " (" #call ")");                                    \
+  } while (0)
+
+#define WG_CHECK(cond, msg)                      \
+  do {                                           \
+    if (!(cond)) throw ::wg::Error(std::string(msg)); \
+  } while (0)
+
+// One row of the ES kernel parameter table (gen_es_table.py).
+struct KernelShape {
+  int W;          // support in cells (even)
+  double sigma;   // oversampling factor
+  double beta;    // ES shape parameter
+  double err1d;   // max 1-D RMS aliasing error over the image
+};
+
+const std::vector<KernelShape>& KernelTable();
+
+// Geometry of the image the gridder produces (the trimmed image of
+// WGriddingGridder_Simple) and the accuracy request.
+struct ImageGeometry {
+  int nx = 0, ny = 0;        // width_t, height_t
+  double dx = 0, dy = 0;     // pixel size (radians)
+  double l_shift = 0, m_shift = 0;
+  double epsilon = 1e-4;
+};
+
+// Coordinate extents of one visibility set after the Hermitian flip
+// (w >= 0), in wavelengths.
+struct VisExtent {
+  double umin, umax, vmin, vmax, wmin, wmax;
+};
+
+// Everything the kernels need to know about one gridding problem.
+struct Plan {
+  ImageGeometry img;
+  // kernel
+  int W = 0;
+  double sigma = 0, beta = 0, err1d = 0;
+  int nu = 0, nv = 0;          // padded grid (even, 2^a 3^b 5^c)
+  double ufac = 0, vfac = 0;   // grid cells per wavelength: nu*dx, nv*dy
+  // n - 1 over the image and the w axis
+  double zmin = 0, zmax = 0, zc = 0;
+  double w_lo = 0, dw = 1, inv_dw = 1;
+  int P0 = 1;                  // number of distinct first planes p0
+  int P = 1;                   // number of w planes (P0 + W - 1)
+  // occupied uv box (grid storage coordinates) and its tiling
+  int T = 32;                  // tile edge (cells)
+  int box_u0 = 0, box_v0 = 0;  // grid coordinate of box column/row 0
+  bool wrap_u = false, wrap_v = false;
+  int ntu = 0, ntv = 0;        // tiles along u / v
+  int BU = 0, BV = 0;          // box storage extent (ntu*T + W - 1, ...)
+  // sort
+  uint32_t nkeys = 0;          // valid keys are [0, nkeys); nkeys = invalid
+  int key_bits = 0;
+  // plane batching
+  int Sg = 4;                  // planes per gridding sub-slab (LDS bound)
+  int Sf = 0;                  // planes per FFT slab (HBM bound)
+
+  std::string describe() const;
+};
+
+// Tunables that select a plan (the autotuner fills these in).
+struct PlanOptions {
+  int force_W = 0;             // 0: choose from epsilon
+  double force_sigma = 0;      // 0: choose
+  int T = 0;                   // 0: default
+  int Sg = 0;                  // 0: default
+  size_t slab_bytes = size_t(8) << 30;  // HBM budget for one FFT slab
+  double sigma_min = 1.1, sigma_max = 2.0;
+  long long nvis_hint = 0;     // for the cost model
+};
+
+int GoodFFTSize(int n);        // smallest even 2^a 3^b 5^c >= n
+std::vector<int> FFTRadices(int n);  // Stockham stage radices for n
+
+// Compute the n-1 range of the image and the full plan.
+Plan MakePlan(const ImageGeometry& g, const VisExtent& e,
+              const PlanOptions& opt);
+
+// Fourier transform of the ES kernel with support W evaluated at t cycles
+// per grid cell (unnormalised: integral of phi over its support at t=0).
+double EsKernelFT(int W, double beta, double t);
+
+}  // namespace wg

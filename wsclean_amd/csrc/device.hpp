@@ -1,0 +1,117 @@
+// Device-side plan constants and the FP64 visibility -> (tile, cell, plane)
+// mapping shared by every kernel that needs it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace wg {
+
+// Plain-old-data copy of the plan that kernels take by value.
+struct DevPlan {
+  double ufac, vfac;      // grid cells per wavelength
+  double half_nu, half_nv;
+  double w_lo, inv_dw, wofs;  // wofs = W/2 - 1
+  double l_shift, m_shift, zc;
+  double dw;
+  double dx_rad, dy_rad;
+  float beta;
+  float two_over_w;
+  int W, T, ntu, ntv, P0, P, nu, nv;
+  int box_u0, box_v0, wrap_u, wrap_v;
+  int BU, BV;
+  int nx, ny;
+  uint32_t nkeys;
+};
+
+DevPlan ToDevPlan(const Plan& p);
+
+// Location of one visibility on the gridding lattice.
+struct VisLoc {
+  int tile;      // tv * ntu + tu
+  int du, dv;    // first kernel cell relative to the tile origin
+  int p0;        // first w plane
+  float fu, fv, fw;  // fractional part of the continuous grid coordinate
+  bool flip;     // (u, v, w, V) -> (-u, -v, -w, V*) was applied
+  double u, v, w;    // flipped coordinates in wavelengths
+};
+
+// FP64 mapping.  Contraction is off so that the result is bit-identical to
+// the same IEEE-754 double operations on the host (numpy restatement in the
+// tests).  The first kernel cell is floor(t) - W/2 + 1 and the fractional
+// part t - floor(t) is kept separately, so that fp32 holds it to 6e-8.
+__device__ __forceinline__ VisLoc LocateVis(double um, double vm, double wm,
+                                            double scale, const DevPlan& dp) {
+#pragma clang fp contract(off)
+  VisLoc r;
+  r.flip = wm < 0.0;
+  if (r.flip) {
+    um = -um;
+    vm = -vm;
+    wm = -wm;
+  }
+  const double u = um * scale;
+  const double v = vm * scale;
+  const double w = wm * scale;
+  r.u = u;
+  r.v = v;
+  r.w = w;
+  const double tu = u * dp.ufac + dp.half_nu;
+  const double tv = v * dp.vfac + dp.half_nv;
+  const double tw = (w - dp.w_lo) * dp.inv_dw + dp.wofs;
+  const double flu = floor(tu), flv = floor(tv), flw = floor(tw);
+  r.fu = float(tu - flu);
+  r.fv = float(tv - flv);
+  r.fw = float(tw - flw);
+  const int hw = dp.W / 2 - 1;
+  int a0u = int(flu) - hw;
+  int a0v = int(flv) - hw;
+  r.p0 = int(flw) - hw;
+  int lu, lv;
+  if (dp.wrap_u) {
+    lu = a0u % dp.nu;
+    if (lu < 0) lu += dp.nu;
+  } else {
+    lu = a0u - dp.box_u0;
+  }
+  if (dp.wrap_v) {
+    lv = a0v % dp.nv;
+    if (lv < 0) lv += dp.nv;
+  } else {
+    lv = a0v - dp.box_v0;
+  }
+  const int tu_i = lu / dp.T, tv_i = lv / dp.T;
+  r.du = lu - tu_i * dp.T;
+  r.dv = lv - tv_i * dp.T;
+  r.tile = tv_i * dp.ntu + tu_i;
+  return r;
+}
+
+// ES kernel phi(x) = exp(beta (sqrt(1 - x^2) - 1)) for |x| <= 1.
+__device__ __forceinline__ float EsKernel(float x, float beta) {
+  float a = fmaxf(1.0f - x * x, 0.0f);
+  return expf(beta * (sqrtf(a) - 1.0f));
+}
+
+// Weights of the W taps for fractional coordinate f in [0, 1): tap k sits at
+// x = (k - W/2 + 1 - f) * 2 / W.
+template <int W>
+__device__ __forceinline__ void EsTaps(float f, float beta, float* out) {
+  constexpr float s = 2.0f / W;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    float x = (float(k - W / 2 + 1) - f) * s;
+    out[k] = EsKernel(x, beta);
+  }
+}
+
+// Packed per-visibility word: du (8 bits) | dv (8 bits) | p0 (15 bits) | flip.
+__device__ __forceinline__ uint32_t PackLoc(const VisLoc& l) {
+  return uint32_t(l.du) | (uint32_t(l.dv) << 8) | (uint32_t(l.p0) << 16) |
+         (l.flip ? 0x80000000u : 0u);
+}
+
+}  // namespace wg

@@ -1,0 +1,354 @@
+// Engine: plan -> keys -> stable radix sort -> bucket offsets -> sorted
+// coordinates -> work items (Prepare), and the plane-slab loops of the
+// invert and predict pipelines.  All device work is issued on the engine's
+// own stream (default priority).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace wg {
+
+void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32_t* counts,
+                uint32_t* pos, uint32_t* scan_scratch, WorkItem* items, hipStream_t s,
+                bool emit);
+
+DevPlan ToDevPlan(const Plan& p) {
+  DevPlan d{};
+  d.ufac = p.ufac;
+  d.vfac = p.vfac;
+  d.half_nu = 0.5 * p.nu;
+  d.half_nv = 0.5 * p.nv;
+  d.w_lo = p.w_lo;
+  d.inv_dw = p.inv_dw;
+  d.wofs = double(p.W / 2 - 1);
+  d.l_shift = p.img.l_shift;
+  d.m_shift = p.img.m_shift;
+  d.zc = p.zc;
+  d.dw = p.dw;
+  d.dx_rad = p.img.dx;
+  d.dy_rad = p.img.dy;
+  d.beta = float(p.beta);
+  d.two_over_w = 2.0f / float(p.W);
+  d.W = p.W;
+  d.T = p.T;
+  d.ntu = p.ntu;
+  d.ntv = p.ntv;
+  d.P0 = p.P0;
+  d.P = p.P;
+  d.nu = p.nu;
+  d.nv = p.nv;
+  d.box_u0 = p.box_u0;
+  d.box_v0 = p.box_v0;
+  d.wrap_u = p.wrap_u ? 1 : 0;
+  d.wrap_v = p.wrap_v ? 1 : 0;
+  d.BU = p.BU;
+  d.BV = p.BV;
+  d.nx = p.img.nx;
+  d.ny = p.img.ny;
+  d.nkeys = p.nkeys;
+  return d;
+}
+
+// ------------------------------------------------------------ DeviceArena
+
+DeviceArena::~DeviceArena() { ReleaseAll(); }
+
+void* DeviceArena::Get(const std::string& name, size_t bytes) {
+  Slot& s = slots_[name];
+  if (bytes == 0) bytes = 256;
+  if (s.bytes < bytes) {
+    if (s.ptr) WG_HIP(hipFree(s.ptr));
+    s.ptr = nullptr;
+    s.bytes = 0;
+    // round up so that slowly growing requests do not reallocate every call
+    size_t b = bytes + bytes / 8 + 256;
+    WG_HIP(hipMalloc(&s.ptr, b));
+    s.bytes = b;
+  }
+  return s.ptr;
+}
+
+void DeviceArena::Release(const std::string& name) {
+  auto it = slots_.find(name);
+  if (it == slots_.end()) return;
+  if (it->second.ptr) (void)hipFree(it->second.ptr);
+  slots_.erase(it);
+}
+
+void DeviceArena::ReleaseAll() {
+  for (auto& kv : slots_)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  slots_.clear();
+}
+
+size_t DeviceArena::Allocated() const {
+  size_t t = 0;
+  for (const auto& kv : slots_) t += kv.second.bytes;
+  return t;
+}
+
+// ------------------------------------------------------------ Engine
+
+Engine::Engine(const ImageGeometry& g, const PlanOptions& opt) : geom_(g), opt_(opt) {
+  WG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Engine::~Engine() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+  for (auto& kv : twiddles_) (void)hipFree(kv.second);
+}
+
+const float2* Engine::Twiddles(int n) {
+  auto it = twiddles_.find(n);
+  if (it != twiddles_.end()) return it->second;
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    // exact reduction: angle = 2 pi m / n in double
+    const double a = 2.0 * M_PI * double(m) / double(n);
+    h[m] = make_float2(float(std::cos(a)), float(-std::sin(a)));
+  }
+  float2* d = nullptr;
+  WG_HIP(hipMalloc(&d, n * sizeof(float2)));
+  WG_HIP(hipMemcpy(d, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  twiddles_[n] = d;
+  return d;
+}
+
+namespace {
+struct EventTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  bool on;
+  double* dst;
+  EventTimer(bool enabled, hipStream_t st, double* out) : s(st), on(enabled), dst(out) {
+    if (!on) return;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+  }
+  ~EventTimer() {
+    if (!on) return;
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    *dst += ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+}  // namespace
+
+void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n_chan,
+                     const double* uvw_dev, const double* freq_host, const float2* vis_dev) {
+  EventTimer tm(timing, stream_, &times_.plan);
+  WG_CHECK(n_chan > 0, "n_chan must be positive");
+  const size_t n_vis = n_rows * n_chan;
+  WG_CHECK(n_vis < 0xFFFFFFF0ull, "at most 2^32 - 16 visibilities per call");
+  vs.tag = tag;
+  vs.n_rows = n_rows;
+  vs.n_chan = n_chan;
+  vs.n_vis = n_vis;
+  vs.skip_zero = vis_dev != nullptr;
+  vs.uvw = const_cast<double*>(uvw_dev);
+
+  std::vector<double> scale(n_chan);
+  double smin = 1e300, smax = -1e300;
+  for (size_t c = 0; c < n_chan; ++c) {
+    WG_CHECK(freq_host[c] > 0, "frequencies must be positive");
+    scale[c] = freq_host[c] / kSpeedOfLight;
+    smin = std::min(smin, scale[c]);
+    smax = std::max(smax, scale[c]);
+  }
+  vs.scale = static_cast<double*>(arena_.Get(tag + ".scale", n_chan * sizeof(double)));
+  WG_HIP(hipMemcpyAsync(vs.scale, scale.data(), n_chan * sizeof(double),
+                        hipMemcpyHostToDevice, stream_));
+
+  VisExtent ext{0, 0, 0, 0, 0, 0};
+  if (n_rows > 0) {
+    double* part = static_cast<double*>(arena_.Get("extents", (1024 * 6 + 6) * sizeof(double)));
+    LaunchExtents(uvw_dev, n_rows, smin, smax, part, part + 1024 * 6, stream_);
+    double e[6];
+    WG_HIP(hipMemcpyAsync(e, part + 1024 * 6, sizeof(e), hipMemcpyDeviceToHost, stream_));
+    WG_HIP(hipStreamSynchronize(stream_));
+    ext = VisExtent{e[0], e[1], e[2], e[3], e[4], e[5]};
+    for (double v : e) WG_CHECK(std::isfinite(v), "non-finite uvw coordinates");
+  }
+  PlanOptions o = opt_;
+  o.nvis_hint = (long long)n_vis;
+  vs.plan = MakePlan(geom_, ext, o);
+  WG_CHECK(vs.plan.P0 < 32768, "too many w planes");
+  vs.dp = ToDevPlan(vs.plan);
+  const DevPlan& dp = vs.dp;
+
+  // ---- keys + stable sort ----
+  uint32_t* keys = static_cast<uint32_t*>(arena_.Get(tag + ".keys", n_vis * 4));
+  uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
+  if (n_vis) {
+    LaunchKeys(dp, uvw_dev, vs.scale, n_rows, n_chan, vis_dev, keys, vals, stream_);
+    uint32_t* scratch =
+        static_cast<uint32_t*>(arena_.Get("sort.scratch", RadixSortScratchWords(n_vis) * 4));
+    DeviceRadixSort(keys, vals, n_vis, vs.plan.key_bits, scratch, stream_);
+  }
+  vs.perm = vals;
+  vs.off = static_cast<uint32_t*>(arena_.Get(tag + ".off", (size_t(dp.nkeys) + 1) * 4));
+  DeviceBucketOffsets(keys, n_vis, dp.nkeys, vs.off, stream_);
+  uint32_t nvalid = 0;
+  WG_HIP(hipMemcpyAsync(&nvalid, vs.off + dp.nkeys, 4, hipMemcpyDeviceToHost, stream_));
+  WG_HIP(hipStreamSynchronize(stream_));
+  vs.n_valid = nvalid;
+
+  vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", vs.n_valid * sizeof(float4)));
+  LaunchCoords(dp, uvw_dev, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, stream_);
+  BuildWorkItems(vs);
+  BuildCorrection(vs);
+}
+
+void Engine::BuildWorkItems(VisSet& vs) {
+  const DevPlan& dp = vs.dp;
+  const int Sg = vs.plan.Sg;
+  vs.nsub = (dp.P + Sg - 1) / Sg;
+  const size_t n = size_t(dp.ntu) * dp.ntv * vs.nsub;
+  uint32_t* counts = static_cast<uint32_t*>(arena_.Get("items.counts", (n + 1) * 4));
+  vs.item_pos = static_cast<uint32_t*>(arena_.Get(vs.tag + ".item_pos", (n + 1) * 4));
+  uint32_t* scr = static_cast<uint32_t*>(arena_.Get("items.scan", ScanScratchWords(n) * 4));
+  BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, nullptr, stream_, false);
+  uint32_t total = 0;
+  WG_HIP(hipMemcpyAsync(&total, vs.item_pos + n, 4, hipMemcpyDeviceToHost, stream_));
+  WG_HIP(hipStreamSynchronize(stream_));
+  vs.n_items = total;
+  vs.items = static_cast<WorkItem*>(arena_.Get(vs.tag + ".items", (total + 1) * sizeof(WorkItem)));
+  if (total) BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, vs.items, stream_, true);
+}
+
+void Engine::BuildCorrection(VisSet& vs) {
+  const Plan& p = vs.plan;
+  const int nx = p.img.nx, ny = p.img.ny;
+  std::vector<double> pu(nx), pv(ny);
+  for (int x = 0; x < nx; ++x) pu[x] = EsKernelFT(p.W, p.beta, double(nx / 2 - x) / p.nu);
+  for (int y = 0; y < ny; ++y) pv[y] = EsKernelFT(p.W, p.beta, double(y - ny / 2) / p.nv);
+  // Phi_w(t) = sum_q a_q cos(b_q t), Gauss-Legendre in theta (x = sin theta)
+  const int nq = 64;
+  std::vector<double> gx(nq), gw(nq);
+  {
+    // nodes by Newton iteration on the Legendre polynomial
+    for (int i = 0; i < (nq + 1) / 2; ++i) {
+      double z = std::cos(M_PI * (i + 0.75) / (nq + 0.5)), pp = 0;
+      for (int it = 0; it < 100; ++it) {
+        double p1 = 1, p2 = 0;
+        for (int j = 1; j <= nq; ++j) {
+          double p3 = p2;
+          p2 = p1;
+          p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = nq * (z * p1 - p2) / (z * z - 1.0);
+        double z1 = z;
+        z = z1 - p1 / pp;
+        if (std::fabs(z - z1) < 1e-15) break;
+      }
+      gx[i] = -z;
+      gx[nq - 1 - i] = z;
+      gw[i] = gw[nq - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+    }
+  }
+  std::vector<double> qa(nq), qb(nq);
+  for (int q = 0; q < nq; ++q) {
+    const double th = 0.5 * M_PI * gx[q];
+    const double c = std::cos(th);
+    qa[q] = 0.5 * p.W * 0.5 * M_PI * gw[q] * std::exp(p.beta * (c - 1.0)) * c;
+    qb[q] = M_PI * p.W * std::sin(th);
+  }
+  double* dpu = static_cast<double*>(arena_.Get(vs.tag + ".phiu", nx * 8));
+  double* dpv = static_cast<double*>(arena_.Get(vs.tag + ".phiv", ny * 8));
+  double* dq = static_cast<double*>(arena_.Get(vs.tag + ".quad", 2 * nq * 8));
+  WG_HIP(hipMemcpyAsync(dpu, pu.data(), nx * 8, hipMemcpyHostToDevice, stream_));
+  WG_HIP(hipMemcpyAsync(dpv, pv.data(), ny * 8, hipMemcpyHostToDevice, stream_));
+  WG_HIP(hipMemcpyAsync(dq, qa.data(), nq * 8, hipMemcpyHostToDevice, stream_));
+  WG_HIP(hipMemcpyAsync(dq + nq, qb.data(), nq * 8, hipMemcpyHostToDevice, stream_));
+  vs.corr = static_cast<float*>(arena_.Get(vs.tag + ".corr", size_t(nx) * ny * 4));
+  LaunchCorrection(vs.dp, dpu, dpv, dq, dq + nq, nq, vs.corr, stream_);
+  // the host vectors must outlive the async copies
+  WG_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
+  const DevPlan& dp = vs.dp;
+  const Plan& p = vs.plan;
+  if (vs.n_valid == 0) return;
+  float2* vsorted = nullptr;
+  {
+    EventTimer tm(timing, stream_, &times_.sort);
+    vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
+    LaunchGatherVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.n_valid, vis_dev, vsorted,
+                    stream_);
+  }
+  const size_t npix = size_t(dp.nx) * dp.ny;
+  float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
+  LaunchFillZero(dirty, npix * 4, stream_);
+  const FFTDesc fu = MakeFFTDesc(dp.nu, Twiddles(dp.nu));
+  const FFTDesc fv = MakeFFTDesc(dp.nv, Twiddles(dp.nv));
+  const size_t plane_box = size_t(dp.BU) * dp.BV;
+  const size_t plane_lines = size_t(dp.BV) * dp.nx;
+  float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
+  float2* rbuf = static_cast<float2*>(arena_.Get("rbuf", p.Sf * plane_lines * sizeof(float2)));
+  float2* cbuf = static_cast<float2*>(arena_.Get("cbuf", p.Sf * plane_lines * sizeof(float2)));
+  for (int f0 = 0; f0 < p.P; f0 += p.Sf) {
+    const int np = std::min(p.Sf, p.P - f0);
+    {
+      EventTimer tm(timing, stream_, &times_.grid);
+      LaunchFillZero(box, size_t(np) * plane_box * sizeof(float2), stream_);
+      for (int g = f0 / p.Sg; g * p.Sg < f0 + np; ++g) {
+        const int sub_lo = g * p.Sg;
+        const int sub_n = std::min(p.Sg, p.P - sub_lo);
+        LaunchGrid(dp, vs, vsorted, box, g, sub_lo, sub_n, f0, stream_);
+      }
+    }
+    EventTimer tm(timing, stream_, &times_.fft);
+    LaunchRowFFTInvert(dp, fu, box, rbuf, np, stream_);
+    LaunchTranspose(rbuf, cbuf, dp.BV, dp.nx, np, stream_);
+    LaunchColFFTInvert(dp, fv, cbuf, dirty, f0, np, stream_);
+  }
+  LaunchAccumulateImage(dirty, vs.corr, img_dev, npix, stream_);
+}
+
+void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
+  const DevPlan& dp = vs.dp;
+  const Plan& p = vs.plan;
+  if (vs.n_valid == 0) return;
+  float2* acc = static_cast<float2*>(arena_.Get("vis_acc", vs.n_valid * sizeof(float2)));
+  LaunchFillZero(acc, vs.n_valid * sizeof(float2), stream_);
+  const FFTDesc fu = MakeFFTDesc(dp.nu, Twiddles(dp.nu));
+  const FFTDesc fv = MakeFFTDesc(dp.nv, Twiddles(dp.nv));
+  const size_t plane_box = size_t(dp.BU) * dp.BV;
+  const size_t plane_lines = size_t(dp.BV) * dp.nx;
+  float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
+  float2* rbuf = static_cast<float2*>(arena_.Get("rbuf", p.Sf * plane_lines * sizeof(float2)));
+  float2* cbuf = static_cast<float2*>(arena_.Get("cbuf", p.Sf * plane_lines * sizeof(float2)));
+  for (int f0 = 0; f0 < p.P; f0 += p.Sf) {
+    const int np = std::min(p.Sf, p.P - f0);
+    {
+      EventTimer tm(timing, stream_, &times_.fft);
+      LaunchColFFTPredict(dp, fv, img_dev, vs.corr, cbuf, f0, np, stream_);
+      LaunchTranspose(cbuf, rbuf, dp.nx, dp.BV, np, stream_);
+      LaunchRowFFTPredict(dp, fu, rbuf, box, np, stream_);
+    }
+    EventTimer tm(timing, stream_, &times_.degrid);
+    for (int g = f0 / p.Sg; g * p.Sg < f0 + np; ++g) {
+      const int sub_lo = g * p.Sg;
+      const int sub_n = std::min(p.Sg, p.P - sub_lo);
+      LaunchDegrid(dp, vs, box, acc, g, sub_lo, sub_n, f0, stream_);
+    }
+  }
+  LaunchScatterVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.coord, vs.n_valid, acc,
+                   vis_out_dev, stream_);
+}
+
+}  // namespace wg

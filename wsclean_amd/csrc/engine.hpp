@@ -1,0 +1,173 @@
+// Device engine: owns a HIP stream and a cache of device buffers, and runs
+// the invert (grid) and predict (degrid) pipelines for one gridder handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace wg {
+
+// ---- primitives implemented in sort.hip ----
+void DeviceExclusiveScan(const uint32_t* in, uint32_t* out, size_t n,
+                         uint32_t* scratch, hipStream_t s, uint32_t* total);
+size_t ScanScratchWords(size_t n);
+size_t RadixSortScratchWords(size_t n);
+void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
+                     uint32_t* scratch, hipStream_t s);
+void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nkeys,
+                         uint32_t* off, hipStream_t s);
+
+// A work item: the visibilities [begin, end) (sorted order) of one uv tile
+// that touch the planes of one gridding sub-slab.
+struct WorkItem {
+  uint32_t tile;
+  uint32_t begin;
+  uint32_t end;
+};
+
+// Grow-only cache of named device allocations.
+class DeviceArena {
+ public:
+  ~DeviceArena();
+  void* Get(const std::string& name, size_t bytes);
+  void Release(const std::string& name);
+  void ReleaseAll();
+  size_t Allocated() const;
+
+ private:
+  struct Slot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Slot> slots_;
+};
+
+// Per-call timing of the pipeline stages (ms, measured with HIP events).
+struct StageTimes {
+  double plan = 0, sort = 0, grid = 0, fft = 0, degrid = 0, total = 0;
+};
+
+// Visibility set prepared on the device: plan, sort and sorted coordinates.
+// Invert and predict over the same uvw/freq reuse it (a major cycle).
+struct VisSet {
+  Plan plan;
+  DevPlan dp;
+  size_t n_rows = 0, n_chan = 0, n_vis = 0, n_valid = 0;
+  bool skip_zero = false;  // zero visibilities were left out (invert only)
+  // device arrays (owned by the arena under names prefixed with `tag`)
+  std::string tag;
+  double* uvw = nullptr;      // [n_rows][3]
+  double* scale = nullptr;    // [n_chan] freq / c
+  uint32_t* perm = nullptr;   // [n_valid] sorted -> flat index
+  uint32_t* off = nullptr;    // [nkeys + 1] bucket offsets
+  float4* coord = nullptr;    // [n_valid] fu, fv, fw, packed cell/plane/flip
+  WorkItem* items = nullptr;  // work items of all gridding sub-slabs
+  uint32_t* item_pos = nullptr;  // [nsub * ntiles + 1] scan of item counts
+  int nsub = 0;
+  size_t n_items = 0;
+  float* corr = nullptr;      // [nx][ny] 1 / (Phi_u Phi_v Phi_w) with sign
+};
+
+class Engine {
+ public:
+  Engine(const ImageGeometry& g, const PlanOptions& opt);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  const ImageGeometry& Geometry() const { return geom_; }
+  PlanOptions& Options() { return opt_; }
+  hipStream_t Stream() const { return stream_; }
+
+  // Build (or rebuild) the visibility set `vs` from device uvw / host freq.
+  // With `vis` non-null, zero visibilities are excluded from the sort.
+  void Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n_chan,
+               const double* uvw_dev, const double* freq_host,
+               const float2* vis_dev);
+
+  // dirty_acc[x][y] += corrected image of the visibilities (device, fp64).
+  void Invert(VisSet& vs, const float2* vis_dev, double* img_dev);
+  // vis_out[row][chan] = prediction from model img_dev ([x][y], fp64).
+  void Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev);
+
+  DeviceArena& Arena() { return arena_; }
+  StageTimes& Times() { return times_; }
+  bool timing = false;
+
+ private:
+  const float2* Twiddles(int n);
+  void BuildCorrection(VisSet& vs);
+  void BuildWorkItems(VisSet& vs);
+
+  ImageGeometry geom_;
+  PlanOptions opt_;
+  hipStream_t stream_ = nullptr;
+  DeviceArena arena_;
+  std::map<int, float2*> twiddles_;
+  StageTimes times_;
+};
+
+// ---- FFT / plane kernels (fft.hip) ----
+struct FFTDesc {
+  int n;
+  int nst;
+  int rad[16];
+  const float2* tw;
+};
+FFTDesc MakeFFTDesc(int n, const float2* tw);
+
+void LaunchRowFFTInvert(const DevPlan& dp, const FFTDesc& fu, const float2* box,
+                        float2* rbuf, int nplanes, hipStream_t s);
+void LaunchColFFTInvert(const DevPlan& dp, const FFTDesc& fv, const float2* cbuf,
+                        float* dirty, int plane_lo, int nplanes, hipStream_t s);
+void LaunchColFFTPredict(const DevPlan& dp, const FFTDesc& fv, const double* img,
+                         const float* corr, float2* cbuf, int plane_lo,
+                         int nplanes, hipStream_t s);
+void LaunchRowFFTPredict(const DevPlan& dp, const FFTDesc& fu, const float2* rbuf,
+                         float2* box, int nplanes, hipStream_t s);
+void LaunchTranspose(const float2* in, float2* out, int rows, int cols,
+                     int nplanes, hipStream_t s);
+void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
+                   hipStream_t s);
+
+// ---- gridding kernels (grid.hip) ----
+void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
+                float2* box, int sub, int sub_lo, int sub_n, int slab_lo,
+                hipStream_t s);
+void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
+                  float2* acc, int sub, int sub_lo, int sub_n, int slab_lo,
+                  hipStream_t s);
+size_t GridLdsBytes(const DevPlan& dp, int sub_n);
+
+// ---- per-visibility kernels (vis.hip) ----
+void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
+                   double* partial, double* out, hipStream_t s);
+void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
+                size_t n_rows, size_t n_chan, const float2* vis, uint32_t* keys,
+                uint32_t* vals, hipStream_t s);
+void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale,
+                  size_t n_chan, const uint32_t* perm, size_t n_valid,
+                  float4* coord, hipStream_t s);
+void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale,
+                     size_t n_chan, const uint32_t* perm, size_t n_valid,
+                     const float2* vis, float2* vis_sorted, hipStream_t s);
+void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale,
+                      size_t n_chan, const uint32_t* perm, const float4* coord,
+                      size_t n_valid, const float2* acc, float2* vis_out,
+                      hipStream_t s);
+void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_v,
+                      const double* quad_a, const double* quad_b, int nquad,
+                      float* corr, hipStream_t s);
+void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
+                           size_t n, hipStream_t s);
+void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace wg

@@ -1,0 +1,461 @@
+// Hand-written LDS Stockham FFT (radix 2/3/4/5, FP32) and the per-plane
+// line kernels of the w-gridder:
+//
+//   invert : box rows --FFT(u)--> crop columns --transpose--> image columns
+//            --FFT(v), w-screen exp(-2 pi i w_p (n-1-zc)), accumulate-->
+//            dirty[x][y]
+//   predict: model[x][y] * correction * exp(+2 pi i w_p (n-1-zc)) --FFT(v)-->
+//            box rows of the column spectrum --transpose--> --FFT(u)--> box
+//
+// Only the occupied uv box (rows/columns holding visibilities) and the
+// cropped image are moved through HBM; the zero-padded remainder of each
+// nu x nv plane only ever exists in LDS.  One line (<= 16384 complex = 128
+// KiB) is transformed per workgroup, entirely inside LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace wg {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) {
+  return make_float2(a.x * s, a.y * s);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void Dft(float2* v);
+
+template <>
+__device__ __forceinline__ void Dft<2, false>(float2* v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <>
+__device__ __forceinline__ void Dft<2, true>(float2* v) {
+  Dft<2, false>(v);
+}
+
+template <bool INV>
+__device__ __forceinline__ void Dft4(float2* v) {
+  float2 t0 = cadd(v[0], v[2]);
+  float2 t1 = csub(v[0], v[2]);
+  float2 t2 = cadd(v[1], v[3]);
+  float2 t3 = rot90<INV>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+template <>
+__device__ __forceinline__ void Dft<4, false>(float2* v) { Dft4<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<4, true>(float2* v) { Dft4<true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void Dft3(float2* v) {
+  constexpr float s3 = 0.86602540378443864676f;
+  float2 t1 = cadd(v[1], v[2]);
+  float2 t2 = csub(v[1], v[2]);
+  float2 y0 = cadd(v[0], t1);
+  float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+  float2 r = rot90<INV>(cscale(t2, s3));  // -i s3 t2 forward
+  v[0] = y0;
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+template <>
+__device__ __forceinline__ void Dft<3, false>(float2* v) { Dft3<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<3, true>(float2* v) { Dft3<true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void Dft5(float2* v) {
+  constexpr float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+  constexpr float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+  constexpr float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+  constexpr float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+  float2 t1 = cadd(v[1], v[4]);
+  float2 t2 = cadd(v[2], v[3]);
+  float2 t3 = csub(v[1], v[4]);
+  float2 t4 = csub(v[2], v[3]);
+  float2 y0 = cadd(v[0], cadd(t1, t2));
+  float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  float2 r1 = rot90<INV>(b1);  // -i b1 forward
+  float2 r2 = rot90<INV>(b2);
+  v[0] = y0;
+  v[1] = cadd(a1, r1);
+  v[4] = csub(a1, r1);
+  v[2] = cadd(a2, r2);
+  v[3] = csub(a2, r2);
+}
+template <>
+__device__ __forceinline__ void Dft<5, false>(float2* v) { Dft5<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<5, true>(float2* v) { Dft5<true>(v); }
+
+// One Stockham autosort stage of radix R on an LDS-resident line of length
+// n.  Ns is the product of the radices of the previous stages.  Every
+// thread first reads all its butterflies into registers, then (after a
+// barrier) writes them, so a single LDS buffer suffices.
+template <int R, bool INV, int BLOCK, int MAXN>
+__device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns,
+                                              const float2* __restrict__ tw) {
+  constexpr int NB = (MAXN / R + BLOCK - 1) / BLOCK;
+  const int nb = n / R;
+  const int tws = n / (Ns * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = threadIdx.x + b * BLOCK;
+    if (j < nb) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nb];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * tws];
+          if (INV) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      }
+      Dft<R, INV>(v[b]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = threadIdx.x + b * BLOCK;
+    if (j < nb) {
+      const int k = j % Ns;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-LDS FFT; unnormalised, forward = exp(-2 pi i jk/n).  Caller must
+// have synchronised after filling buf.
+template <bool INV, int BLOCK, int MAXN>
+__device__ void LdsFFT(float2* buf, const FFTDesc& d) {
+  int Ns = 1;
+  for (int s = 0; s < d.nst; ++s) {
+    const int R = d.rad[s];
+    switch (R) {
+      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
+      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
+      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
+      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
+    }
+    Ns *= R;
+  }
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void ZeroLds(float2* buf, int n) {
+  for (int i = threadIdx.x; i < n; i += BLOCK) buf[i] = make_float2(0.f, 0.f);
+}
+
+// buf[(origin + i) mod n] += src[i] for i in [0, len), folding periodically.
+template <int BLOCK>
+__device__ __forceinline__ void FoldLoad(float2* buf, int n, const float2* __restrict__ src,
+                                         int len, int origin) {
+  for (int k = 0; k * n < origin + len; ++k) {
+    const int i0 = max(0, k * n - origin);
+    const int i1 = min(len, (k + 1) * n - origin);
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+      float2& d = buf[origin + i - k * n];
+      const float2 a = src[i];
+      d.x += a.x;
+      d.y += a.y;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int Wrap(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// n - 1 - zc at image pixel (x, y), FP64 (stable form, 0 beyond horizon).
+__device__ __forceinline__ double PixelZ(const DevPlan& dp, int x, int y) {
+  const double l = double(dp.nx / 2 - x) * dp.dx_rad + dp.l_shift;
+  const double m = double(y - dp.ny / 2) * dp.dy_rad + dp.m_shift;
+  const double r2 = l * l + m * m;
+  const double z = r2 < 1.0 ? -r2 / (sqrt(1.0 - r2) + 1.0) : 0.0;
+  return z - dp.zc;
+}
+
+__device__ __forceinline__ double PlaneW(const DevPlan& dp, int p) {
+  return dp.w_lo + double(p - (dp.W / 2 - 1)) * dp.dw;
+}
+
+// exp(sign * 2 pi i * cycles) with the integer part removed in FP64.
+__device__ __forceinline__ float2 Phasor(double cycles, float sign) {
+  const double f = cycles - rint(cycles);
+  float s, c;
+  sincospif(2.0f * float(f), &s, &c);
+  return make_float2(c, sign * s);
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) RowFFTInvertKernel(
+    DevPlan dp, FFTDesc fu, const float2* __restrict__ box, float2* __restrict__ rbuf) {
+  extern __shared__ float2 buf[];
+  const int j = blockIdx.x, pl = blockIdx.y;
+  const float2* src = box + (size_t(pl) * dp.BV + j) * dp.BU;
+  ZeroLds<BLOCK>(buf, fu.n);
+  __syncthreads();
+  FoldLoad<BLOCK>(buf, fu.n, src, dp.BU, dp.box_u0);
+  LdsFFT<false, BLOCK, MAXN>(buf, fu);
+  float2* out = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
+  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) out[x] = buf[Wrap(dp.nx / 2 - x, fu.n)];
+}
+
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
+    DevPlan dp, FFTDesc fv, const float2* __restrict__ cbuf, float* __restrict__ dirty,
+    int plane_lo, int nplanes) {
+  extern __shared__ float2 buf[];
+  constexpr int NY = MAXN / BLOCK;
+  const int x = blockIdx.x;
+  float acc[NY];
+  double zz[NY];
+#pragma unroll
+  for (int k = 0; k < NY; ++k) {
+    const int y = threadIdx.x + k * BLOCK;
+    acc[k] = 0.f;
+    zz[k] = y < dp.ny ? PixelZ(dp, x, y) : 0.0;
+  }
+  for (int pl = 0; pl < nplanes; ++pl) {
+    ZeroLds<BLOCK>(buf, fv.n);
+    __syncthreads();
+    FoldLoad<BLOCK>(buf, fv.n, cbuf + (size_t(pl) * dp.nx + x) * dp.BV, dp.BV, dp.box_v0);
+    LdsFFT<false, BLOCK, MAXN>(buf, fv);
+    const double wp = PlaneW(dp, plane_lo + pl);
+#pragma unroll
+    for (int k = 0; k < NY; ++k) {
+      const int y = threadIdx.x + k * BLOCK;
+      if (y < dp.ny) {
+        const float2 val = buf[Wrap(y - dp.ny / 2, fv.n)];
+        const float2 ph = Phasor(wp * zz[k], -1.f);
+        acc[k] += val.x * ph.x - val.y * ph.y;
+      }
+    }
+    __syncthreads();
+  }
+  float* out = dirty + size_t(x) * dp.ny;
+#pragma unroll
+  for (int k = 0; k < NY; ++k) {
+    const int y = threadIdx.x + k * BLOCK;
+    if (y < dp.ny) out[y] += acc[k];
+  }
+}
+
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
+    DevPlan dp, FFTDesc fv, const double* __restrict__ img, const float* __restrict__ corr,
+    float2* __restrict__ cbuf, int plane_lo, int nplanes) {
+  extern __shared__ float2 buf[];
+  constexpr int NY = MAXN / BLOCK;
+  const int x = blockIdx.x;
+  float mv[NY];
+  double zz[NY];
+#pragma unroll
+  for (int k = 0; k < NY; ++k) {
+    const int y = threadIdx.x + k * BLOCK;
+    if (y < dp.ny) {
+      const size_t idx = size_t(x) * dp.ny + y;
+      mv[k] = float(img[idx]) * corr[idx];
+      zz[k] = PixelZ(dp, x, y);
+    } else {
+      mv[k] = 0.f;
+      zz[k] = 0.0;
+    }
+  }
+  for (int pl = 0; pl < nplanes; ++pl) {
+    ZeroLds<BLOCK>(buf, fv.n);
+    __syncthreads();
+    const double wp = PlaneW(dp, plane_lo + pl);
+#pragma unroll
+    for (int k = 0; k < NY; ++k) {
+      const int y = threadIdx.x + k * BLOCK;
+      if (y < dp.ny) buf[Wrap(y - dp.ny / 2, fv.n)] = cscale(Phasor(wp * zz[k], 1.f), mv[k]);
+    }
+    __syncthreads();
+    LdsFFT<true, BLOCK, MAXN>(buf, fv);
+    float2* out = cbuf + (size_t(pl) * dp.nx + x) * dp.BV;
+    for (int j = threadIdx.x; j < dp.BV; j += BLOCK) out[j] = buf[Wrap(dp.box_v0 + j, fv.n)];
+    __syncthreads();
+  }
+}
+
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) RowFFTPredictKernel(
+    DevPlan dp, FFTDesc fu, const float2* __restrict__ rbuf, float2* __restrict__ box) {
+  extern __shared__ float2 buf[];
+  const int j = blockIdx.x, pl = blockIdx.y;
+  ZeroLds<BLOCK>(buf, fu.n);
+  __syncthreads();
+  const float2* src = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
+  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) buf[Wrap(dp.nx / 2 - x, fu.n)] = src[x];
+  __syncthreads();
+  LdsFFT<true, BLOCK, MAXN>(buf, fu);
+  float2* out = box + (size_t(pl) * dp.BV + j) * dp.BU;
+  for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Wrap(dp.box_u0 + i, fu.n)];
+}
+
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) LineFFTKernel(float2* __restrict__ data, FFTDesc d,
+                                                       int inverse) {
+  extern __shared__ float2 buf[];
+  float2* line = data + size_t(blockIdx.x) * d.n;
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) buf[i] = line[i];
+  __syncthreads();
+  if (inverse)
+    LdsFFT<true, BLOCK, MAXN>(buf, d);
+  else
+    LdsFFT<false, BLOCK, MAXN>(buf, d);
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[i];
+}
+
+// out[p][c][r] = in[p][r][c]; 32x32 tiles through LDS.
+__global__ void __launch_bounds__(256) TransposeKernel(const float2* __restrict__ in,
+                                                       float2* __restrict__ out, int rows,
+                                                       int cols) {
+  __shared__ float2 tile[32][33];
+  const size_t plane = size_t(blockIdx.z) * rows * cols;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[k][tx] = in[plane + size_t(r) * cols + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;
+    if (r < rows && c < cols) out[plane + size_t(c) * rows + r] = tile[tx][k];
+  }
+}
+
+// Kernels that use more than the default dynamic LDS must opt in once.
+template <typename K>
+void AllowLds(K kernel, size_t bytes) {
+  if (bytes > 48 * 1024)
+    WG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+template <typename F>
+void Dispatch(int n, F&& f) {
+  if (n <= 2048)
+    f(std::integral_constant<int, 256>(), std::integral_constant<int, 2048>());
+  else if (n <= 8192)
+    f(std::integral_constant<int, 512>(), std::integral_constant<int, 8192>());
+  else
+    f(std::integral_constant<int, 1024>(), std::integral_constant<int, 16384>());
+}
+
+}  // namespace
+
+FFTDesc MakeFFTDesc(int n, const float2* tw) {
+  FFTDesc d{};
+  d.n = n;
+  std::vector<int> r = FFTRadices(n);
+  WG_CHECK(r.size() <= 16, "too many FFT stages");
+  d.nst = int(r.size());
+  for (size_t i = 0; i < r.size(); ++i) d.rad[i] = r[i];
+  d.tw = tw;
+  return d;
+}
+
+void LaunchRowFFTInvert(const DevPlan& dp, const FFTDesc& fu, const float2* box,
+                        float2* rbuf, int nplanes, hipStream_t s) {
+  Dispatch(fu.n, [&](auto B, auto M) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
+    AllowLds(RowFFTInvertKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
+    RowFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, fu.n * sizeof(float2), s>>>(
+        dp, fu, box, rbuf);
+  });
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchColFFTInvert(const DevPlan& dp, const FFTDesc& fv, const float2* cbuf,
+                        float* dirty, int plane_lo, int nplanes, hipStream_t s) {
+  Dispatch(fv.n, [&](auto B, auto M) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
+    AllowLds(ColFFTInvertKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
+    ColFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, fv.n * sizeof(float2), s>>>(
+        dp, fv, cbuf, dirty, plane_lo, nplanes);
+  });
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchColFFTPredict(const DevPlan& dp, const FFTDesc& fv, const double* img,
+                         const float* corr, float2* cbuf, int plane_lo, int nplanes,
+                         hipStream_t s) {
+  Dispatch(fv.n, [&](auto B, auto M) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
+    AllowLds(ColFFTPredictKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
+    ColFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, fv.n * sizeof(float2), s>>>(
+        dp, fv, img, corr, cbuf, plane_lo, nplanes);
+  });
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchRowFFTPredict(const DevPlan& dp, const FFTDesc& fu, const float2* rbuf,
+                         float2* box, int nplanes, hipStream_t s) {
+  Dispatch(fu.n, [&](auto B, auto M) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
+    AllowLds(RowFFTPredictKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
+    RowFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, fu.n * sizeof(float2), s>>>(
+        dp, fu, rbuf, box);
+  });
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchTranspose(const float2* in, float2* out, int rows, int cols, int nplanes,
+                     hipStream_t s) {
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32, nplanes);
+  TransposeKernel<<<grid, 256, 0, s>>>(in, out, rows, cols);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse, hipStream_t s) {
+  Dispatch(d.n, [&](auto B, auto M) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
+    AllowLds(LineFFTKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
+    LineFFTKernel<BLOCK, MAXN><<<dim3(nlines), BLOCK, d.n * sizeof(float2), s>>>(
+        data, d, inverse ? 1 : 0);
+  });
+  WG_HIP(hipGetLastError());
+}
+
+}  // namespace wg

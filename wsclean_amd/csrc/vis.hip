@@ -1,0 +1,265 @@
+// Per-visibility kernels: coordinate extents, bucket keys, sorted
+// coordinates, the visibility gather (invert) / scatter (predict) with the
+// phase-centre shift, and the per-pixel grid correction.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace wg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double WaveMin(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double WaveMax(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Per block: min/max of the flipped u, v, w (wavelengths) over its rows.
+__global__ void __launch_bounds__(kBlock) ExtentsKernel(const double* __restrict__ uvw,
+                                                        size_t n_rows, double smin, double smax,
+                                                        double* __restrict__ partial) {
+  double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_rows;
+       i += size_t(gridDim.x) * kBlock) {
+#pragma clang fp contract(off)
+    double u = uvw[3 * i], v = uvw[3 * i + 1], w = uvw[3 * i + 2];
+    if (w < 0.0) {
+      u = -u;
+      v = -v;
+      w = -w;
+    }
+    const double u1 = u * smin, u2 = u * smax;
+    const double v1 = v * smin, v2 = v * smax;
+    const double w1 = w * smin, w2 = w * smax;
+    r[0] = fmin(r[0], fmin(u1, u2));
+    r[1] = fmax(r[1], fmax(u1, u2));
+    r[2] = fmin(r[2], fmin(v1, v2));
+    r[3] = fmax(r[3], fmax(v1, v2));
+    r[4] = fmin(r[4], fmin(w1, w2));
+    r[5] = fmax(r[5], fmax(w1, w2));
+  }
+  __shared__ double sh[kBlock / 64][6];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < 6; ++k) {
+    const double v = (k & 1) ? WaveMax(r[k]) : WaveMin(r[k]);
+    if (lane == 0) sh[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double v = sh[0][k];
+    for (int w = 1; w < kBlock / 64; ++w) v = (k & 1) ? fmax(v, sh[w][k]) : fmin(v, sh[w][k]);
+    partial[size_t(blockIdx.x) * 6 + k] = v;
+  }
+}
+
+__global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nblocks,
+                                   double* __restrict__ out) {
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double v = (k & 1) ? -DBL_MAX : DBL_MAX;
+    for (int b = 0; b < nblocks; ++b)
+      v = (k & 1) ? fmax(v, partial[size_t(b) * 6 + k]) : fmin(v, partial[size_t(b) * 6 + k]);
+    out[k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* __restrict__ uvw,
+                                                     const double* __restrict__ scale,
+                                                     size_t n_rows, size_t n_chan,
+                                                     const float2* __restrict__ vis,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+  const size_t n = n_rows * n_chan;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t row = i / n_chan, ch = i - row * n_chan;
+    uint32_t key = dp.nkeys;
+    bool valid = true;
+    if (vis) {
+      const float2 V = vis[i];
+      valid = (V.x != 0.f) || (V.y != 0.f);
+    }
+    if (valid) {
+      const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+      key = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
+    }
+    keys[i] = key;
+    vals[i] = uint32_t(i);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw,
+                                                       const double* __restrict__ scale,
+                                                       size_t n_chan,
+                                                       const uint32_t* __restrict__ perm,
+                                                       size_t n_valid,
+                                                       float4* __restrict__ coord) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t idx = perm[i];
+    const size_t row = idx / n_chan, ch = idx - row * n_chan;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
+  }
+}
+
+// exp(sign 2 pi i (u l_shift + v m_shift + w zc)) for flipped coordinates.
+__device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L, float sign) {
+  const double cyc = L.u * dp.l_shift + L.v * dp.m_shift + L.w * dp.zc;
+  const double f = cyc - rint(cyc);
+  float s, c;
+  sincospif(2.0f * float(f), &s, &c);
+  return make_float2(c, sign * s);
+}
+
+__global__ void __launch_bounds__(kBlock) GatherVisKernel(
+    DevPlan dp, const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
+    const uint32_t* __restrict__ perm, size_t n_valid, const float2* __restrict__ vis,
+    float2* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t idx = perm[i];
+    const size_t row = idx / n_chan, ch = idx - row * n_chan;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    float2 V = vis[idx];
+    if (L.flip) V.y = -V.y;
+    const float2 ph = ShiftPhasor(dp, L, -1.f);
+    out[i] = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) ScatterVisKernel(
+    DevPlan dp, const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
+    const uint32_t* __restrict__ perm, size_t n_valid, const float2* __restrict__ acc,
+    float2* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t idx = perm[i];
+    const size_t row = idx / n_chan, ch = idx - row * n_chan;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    const float2 a = acc[i];
+    const float2 ph = ShiftPhasor(dp, L, 1.f);
+    float2 V = make_float2(a.x * ph.x - a.y * ph.y, a.x * ph.y + a.y * ph.x);
+    if (L.flip) V.y = -V.y;
+    out[idx] = V;
+  }
+}
+
+// corr[x][y] = (-1)^(q_l + q_m) / (Phi_u(q_l/nu) Phi_v(q_m/nv) Phi_w(dw (n-1-zc)))
+// Phi_w is integrated per pixel in FP64: sum_i a_i cos(b_i t).
+__global__ void __launch_bounds__(kBlock) CorrectionKernel(
+    DevPlan dp, const double* __restrict__ phi_u, const double* __restrict__ phi_v,
+    const double* __restrict__ qa, const double* __restrict__ qb, int nq,
+    float* __restrict__ corr) {
+  const size_t n = size_t(dp.nx) * dp.ny;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock) {
+    const int x = int(i / dp.ny), y = int(i % dp.ny);
+    const double l = double(dp.nx / 2 - x) * dp.dx_rad + dp.l_shift;
+    const double m = double(y - dp.ny / 2) * dp.dy_rad + dp.m_shift;
+    const double r2 = l * l + m * m;
+    const double z = (r2 < 1.0 ? -r2 / (sqrt(1.0 - r2) + 1.0) : 0.0) - dp.zc;
+    const double t = dp.dw * z;
+    double pw = 0.0;
+    for (int q = 0; q < nq; ++q) pw += qa[q] * cos(qb[q] * t);
+    const int ql = dp.nx / 2 - x, qm = y - dp.ny / 2;
+    const double sgn = ((ql + qm) & 1) ? -1.0 : 1.0;
+    corr[i] = float(sgn / (phi_u[x] * phi_v[y] * pw));
+  }
+}
+
+__global__ void AccumulateImageKernel(const float* __restrict__ dirty,
+                                      const float* __restrict__ corr, double* __restrict__ img,
+                                      size_t n) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock)
+    img[i] += double(dirty[i]) * double(corr[i]);
+}
+
+__global__ void FillZeroKernel(uint32_t* __restrict__ p, size_t nwords) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nwords;
+       i += size_t(gridDim.x) * kBlock)
+    p[i] = 0u;
+}
+
+unsigned Blocks(size_t n) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  return unsigned(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+}  // namespace
+
+void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
+                   double* partial, double* out, hipStream_t s) {
+  const unsigned nb = std::min<unsigned>(Blocks(n_rows), 1024);
+  ExtentsKernel<<<nb, kBlock, 0, s>>>(uvw, n_rows, smin, smax, partial);
+  ExtentsFinalKernel<<<1, 64, 0, s>>>(partial, int(nb), out);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale, size_t n_rows,
+                size_t n_chan, const float2* vis, uint32_t* keys, uint32_t* vals,
+                hipStream_t s) {
+  KeysKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw, scale, n_rows, n_chan, vis,
+                                                       keys, vals);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
+                  const uint32_t* perm, size_t n_valid, float4* coord, hipStream_t s) {
+  if (!n_valid) return;
+  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, coord);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
+                     const uint32_t* perm, size_t n_valid, const float2* vis,
+                     float2* vis_sorted, hipStream_t s) {
+  if (!n_valid) return;
+  GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, vis,
+                                                     vis_sorted);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
+                      const uint32_t* perm, const float4* coord, size_t n_valid,
+                      const float2* acc, float2* vis_out, hipStream_t s) {
+  (void)coord;
+  if (!n_valid) return;
+  ScatterVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, acc,
+                                                      vis_out);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_v,
+                      const double* quad_a, const double* quad_b, int nquad, float* corr,
+                      hipStream_t s) {
+  CorrectionKernel<<<Blocks(size_t(dp.nx) * dp.ny), kBlock, 0, s>>>(dp, phi_u, phi_v, quad_a,
+                                                                    quad_b, nquad, corr);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchAccumulateImage(const float* dirty, const float* corr, double* img, size_t n,
+                           hipStream_t s) {
+  AccumulateImageKernel<<<Blocks(n), kBlock, 0, s>>>(dirty, corr, img, n);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchFillZero(void* p, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  WG_CHECK(bytes % 4 == 0, "fill size must be a multiple of 4");
+  FillZeroKernel<<<Blocks(bytes / 4), kBlock, 0, s>>>(static_cast<uint32_t*>(p), bytes / 4);
+  WG_HIP(hipGetLastError());
+}
+
+}  // namespace wg
