@@ -1,0 +1,210 @@
+"""Benchmark: grid + degrid throughput of the MI355X w-gridder.
+
+Workload (default): 8192 x 8192 image, 2 arcsec pixels, 128 antennas in a
+seeded 10 km log-radial layout (8128 baselines), 2880 x 10 s at dec -30,
+64 channels 120-180 MHz (1.498e9 visibilities), 500 power-law point
+sources + 5 Gaussian blobs + noise 0.05, epsilon 1e-5.  Inputs are
+generated on the device and resident in HBM before the timed region.
+
+One step = one major cycle on the GPU: plan + bucket sort of the
+visibilities (wg_prepare_visibilities), invert (grid) and predict (degrid)
+of all visibilities.  value = N_vis / step time (vis/s, grid + degrid).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--channels C]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--channels", type=int, default=64)
+    ap.add_argument("--npix", type=int, default=8192)
+    ap.add_argument("--eps", type=float, default=1e-5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--sg", type=int, default=0)
+    ap.add_argument("--W", type=int, default=0)
+    ap.add_argument("--sigma", type=float, default=0.0)
+    return ap.parse_args()
+
+
+def make_inputs(n_chan, npix, seed=1002):
+    from wsclean_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    enu = synth.antennas_logradial(rng, 128, 10000.0)
+    uvw = synth.earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
+    freq = np.linspace(120e6, 180e6, 64)[:n_chan] if n_chan <= 64 else \
+        np.linspace(120e6, 180e6, n_chan)
+    dx = 2.0 * synth.ARCSEC
+    l, m, flux, _, _ = synth.point_sky(rng, 500, npix, dx)
+    gl, gm, gflux, _, _ = synth.point_sky(rng, 5, npix, dx)
+    gsig = rng.uniform(10.0, 60.0, 5) * synth.ARCSEC
+    src = np.zeros((505, 5))
+    src[:500, 0], src[:500, 1], src[:500, 3] = l, m, flux
+    src[500:, 0], src[500:, 1], src[500:, 3], src[500:, 4] = gl, gm, gflux, gsig
+    src[:, 2] = np.sqrt(1.0 - src[:, 0] ** 2 - src[:, 1] ** 2) - 1.0
+    return uvw, freq, dx, src
+
+
+def roofline_bytes(plan, n_vis, n_rows, nx, ny):
+    """Compulsory traffic of one grid + degrid (CTM): vis + index + uvw per
+    direction, and per w plane the flush, a 2-pass FFT and the image
+    accumulate / model read, evaluated on the plan used (nu, nv, P)."""
+    G = 8.0 * plan["nu"] * plan["nv"]
+    P = plan["P"]
+    b_vis = 12.0 * n_vis + 24.0 * n_rows
+    inv = 5 * G + 16.0 * nx * ny
+    pred = 6 * G + 4.0 * nx * ny
+    return 2 * b_vis + P * (inv + pred)
+
+
+def cpu_baseline(uvw_h, freq, dx, src, npix, plan, max_seconds):
+    """FP64 CPU w-gridder (oracle/wgridder_cpu.cpp, 'CPU restatement, not
+    ducc0') on a bounded sample of the same workload: the full image and the
+    same kernel (W, beta, nu, nv), one channel (the middle one) and the
+    central part of the track, sized for ~max_seconds on the host cores.
+    Returns invert + predict visibilities per second of the sample."""
+    from oracle import oracle
+    from wsclean_amd import synth
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
+    n_bl = 8128
+    n_t = uvw_h.shape[0] // n_bl
+    # central 1/4 of the track of one channel
+    t0, t1 = n_t * 3 // 8, n_t * 5 // 8
+    sub = uvw_h[t0 * n_bl:t1 * n_bl]
+    f = np.array([freq[len(freq) // 2]])
+    l, m, S = src[:64, 0], src[:64, 1], src[:64, 3]
+    vis = synth.predict_sources(sub[:200000], f, l, m, S).astype(np.complex64)
+    vis = np.resize(vis, (sub.shape[0], 1)).astype(np.complex64)
+    kw = dict(W=plan["W"], beta=plan["beta"], sigma=plan["sigma"], nu=plan["nu"],
+              nv=plan["nv"], nthreads=cores)
+    t = time.perf_counter()
+    img = oracle.cpu_wgrid_invert(npix, npix, dx, dx, 0.0, 0.0, sub, f, vis, **kw)
+    oracle.cpu_wgrid_predict(npix, npix, dx, dx, 0.0, 0.0, img, sub, f, **kw)
+    dt = time.perf_counter() - t
+    n = sub.shape[0]
+    return {"value": n / dt, "unit": "vis/s", "cores": cores, "kind": "port",
+            "sample": "CPU restatement (FP64, not ducc0): invert+predict of %d visibilities "
+                      "(1 channel at %.1f MHz, central quarter of the track), %d^2 image, "
+                      "same kernel W=%d sigma=%.2f; %.1f s" % (n, f[0] / 1e6, npix, plan["W"],
+                                                              plan["sigma"], dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this build "
+              "has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from wsclean_amd import WGriddingGridder_Simple, _lib
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    t_gen = time.time()
+    uvw_h, freq, dx, src = make_inputs(args.channels, args.npix)
+    n_rows, n_chan = uvw_h.shape[0], len(freq)
+    n_vis = n_rows * n_chan
+    uvw = torch.from_numpy(uvw_h).to(dev)
+    vis = torch.empty((n_rows, n_chan, 2), dtype=torch.float32, device=dev)
+    out = torch.empty_like(vis)
+    lib = _lib.load()
+    srcc = np.ascontiguousarray(src)
+    _lib.check(lib.wgx_synth_visibilities(
+        ctypes_ptr(uvw), n_rows, n_chan, freq.ctypes.data_as(_c_void_p()), srcc.shape[0],
+        srcc.ctypes.data_as(_c_void_p()), 0.05, 1002, ctypes_ptr(vis)))
+    torch.cuda.synchronize()
+    t_gen = time.time() - t_gen
+
+    n = args.npix
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, args.eps)
+    g.set_timing(True)
+    if args.tile or args.sg or args.W or args.sigma:
+        g.set_tuning(W=args.W, sigma=args.sigma, tile=args.tile, sub_slab=args.sg)
+
+    def step():
+        g.PrepareVisibilities(uvw, freq, n_rows, n_chan)
+        g.InitializeInversion()
+        g.AddInversionPrepared(vis)
+        g.FinalizeImage(1.0 / n_vis)
+        g.PredictPrepared(out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    g.stage_times(reset=True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = g.stage_times(reset=True)
+    plan = g.plan()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = n_vis / (ms / 1e3)
+    ctm = roofline_bytes(plan, n_vis, n_rows, n, n)
+    stages = {k: v / args.steps for k, v in st.items()}
+    dominant = max(("grid", "fft", "degrid", "sort"), key=lambda k: stages[k])
+    line = {
+        "metric": "vis/sec grid+degrid at eps=1e-5, 8192^2 w-gridding; p50/p99 request latency",
+        "value": value,
+        "unit": "vis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 coordinates and phases)",
+        "data": "synthetic (seeded earth-rotation uvw, 505-source sky, device-generated)",
+        "config": {"workload": "large w-gridding: %d^2 image, 2 arcsec, 128 antennas, 2880 x "
+                               "10 s, %d channels 120-180 MHz, eps %.0e" % (n, n_chan, args.eps),
+                   "n_vis": n_vis, "n_rows": n_rows, "n_chan": n_chan,
+                   "plan": {k: plan[k] for k in ("W", "sigma", "nu", "nv", "P", "T", "Sg", "Sf",
+                                                 "BU", "BV")}},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "stages_ms": stages,
+        "ctm_roofline": {"bytes": ctm, "frac": ctm / (ms / 1e3) / HBM_PEAK},
+        "roofline": None,
+        "cpu_baseline": None,
+        "gen_s": t_gen,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+def ctypes_ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _c_void_p():
+    import ctypes
+    return ctypes.c_void_p
+
+
+if __name__ == "__main__":
+    main()

@@ -140,6 +140,14 @@ int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan,
                   char* plan_json, size_t plan_len);
 /* In-place batched complex FFT of nlines lines of length n (host data). */
 int wgx_fft(int n, int nlines, int inverse, float* data);
+/* Synthetic visibilities on the device (benchmark inputs):
+ * vis(row, chan) = sum_s S_s exp(-2 pi^2 sigma_s^2 (u^2 + v^2))
+ *                  * exp(+2 pi i (u l_s + v m_s + w (n_s - 1))) + noise,
+ * src: n_src x {l, m, n - 1, S, sigma} (host, FP64), noise: Gaussian sigma
+ * per real/imaginary part from a counter-based generator seeded by seed. */
+int wgx_synth_visibilities(const double* uvw_dev, size_t n_rows, size_t n_chan,
+                           const double* freq, int n_src, const double* src,
+                           double noise_sigma, uint64_t seed, float* vis_dev);
 
 #ifdef __cplusplus
 }

@@ -196,14 +196,69 @@ def bucket_keys(plan, uvw, freq, vis=None):
     else:
         lv = a0v - plan["box_v0"]
     tile = (lv // T) * plan["ntu"] + (lu // T)
-    keys = (tile * plan["P0"] + p0).astype(np.int64)
+    bucket = tile * plan["P0"] + p0
+    keys = ((bucket * T + (lv % T)) * T + (lu % T)).astype(np.int64)
     valid = np.ones(keys.shape, bool)
     if vis is not None:
         vis = np.asarray(vis).reshape(keys.shape)
         valid = vis != 0
-    nkeys = plan["ntu"] * plan["ntv"] * plan["P0"]
+    nkeys = plan["ntu"] * plan["ntv"] * plan["P0"] * T * T
     keys = np.where(valid, keys, nkeys)
     return keys.ravel().astype(np.uint32), valid.ravel()
+
+
+def _good_size(n):
+    m = n + (n & 1)
+    while True:
+        r = m
+        for p in (2, 3, 5):
+            while r % p == 0:
+                r //= p
+        if r == 1:
+            return m
+        m += 2
+
+
+def cpu_wgrid_params(nx, ny, W, sigma, beta):
+    return dict(W=W, beta=beta, sigma=sigma, nu=_good_size(int(np.ceil(sigma * nx))),
+                nv=_good_size(int(np.ceil(sigma * ny))))
+
+
+def _wgrid_fn(name, out_type):
+    lib = load()
+    f = getattr(lib, name)
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int] + \
+        [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_size_t] * 2 + \
+        [ctypes.c_void_p] * 4 + [ctypes.c_int]
+    f.restype = ctypes.c_int
+    return f
+
+
+def cpu_wgrid_invert(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, W, beta, sigma, nu, nv,
+                     nthreads=0):
+    """FP64 CPU w-gridder dirty image [ny, nx] (not divided by the weight)."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex64)
+    out = np.zeros(nx * ny, np.float64)
+    f = _wgrid_fn("oracle_wgrid_invert", None)
+    f(nx, ny, dx, dy, l_shift, m_shift, W, beta, sigma, nu, nv, uvw.size // 3, freq.size,
+      _p(uvw), _p(freq), _p(vis), _p(out), _nthreads(nthreads))
+    return out.reshape(ny, nx)
+
+
+def cpu_wgrid_predict(nx, ny, dx, dy, l_shift, m_shift, image, uvw, freq, W, beta, sigma, nu,
+                      nv, nthreads=0):
+    """FP64 CPU w-gridder prediction, complex128 [n_rows, n_chan]."""
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    img = np.ascontiguousarray(image, np.float64)
+    n_rows, n_chan = uvw.size // 3, freq.size
+    out = np.zeros(2 * n_rows * n_chan, np.float64)
+    f = _wgrid_fn("oracle_wgrid_predict", None)
+    f(nx, ny, dx, dy, l_shift, m_shift, W, beta, sigma, nu, nv, n_rows, n_chan, _p(uvw),
+      _p(freq), _p(img), _p(out), _nthreads(nthreads))
+    return (out[0::2] + 1j * out[1::2]).reshape(n_rows, n_chan)
 
 
 def es_phi_hat(W, beta, t):

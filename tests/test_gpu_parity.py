@@ -62,7 +62,7 @@ def test_bucket_sort_bit_exact(oracle, eps):
 def test_bucket_sort_bit_exact_large(oracle):
     # enough visibilities for several radix blocks and all three digit passes
     p = synth.small_problem(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512,
-                            pixel_arcsec=8.0, radius_m=4000.0)
+                            pixel_arcsec=30.0, radius_m=6000.0, dec_deg=-70.0, dt_s=600.0)
     g = make(512, p["dx"], 1e-5)
     keys, perm, plan = g.sort_keys(p["uvw"], p["freq"], None)
     ref_keys, _ = oracle.bucket_keys(plan, p["uvw"], p["freq"], None)
@@ -84,17 +84,19 @@ def test_invert_matches_direct_ft(oracle, eps):
 
 
 def test_invert_with_phase_centre_shift(oracle):
-    p = synth.small_problem(seed=3, n_ant=14, n_times=20, n_chan=2, npix=80)
-    ls, ms = 0.02, -0.015
+    # a facet-like image far from the phase centre, with its sky inside it
+    ls, ms = 0.05, -0.03
+    p = synth.small_problem(seed=3, n_ant=14, n_times=20, n_chan=2, npix=80, sky_lm=(ls, ms))
     img, g = invert(p, 1e-5, ls, ms)
     ref = oracle.dft_invert(80, 80, p["dx"], p["dx"], ls, ms, p["uvw"], p["freq"], p["vis"])
     assert rel_l2(img, ref) < 1e-5
 
 
 def test_invert_wide_field_many_w_planes(oracle):
-    # 4 degree field with 10 km baselines: many w planes (P >> W)
+    # 4 degree field, 12 km baselines, 6.5 h track at dec -70: P >> W
     p = synth.small_problem(seed=5, n_ant=20, n_times=40, n_chan=2, npix=512,
-                            pixel_arcsec=30.0, radius_m=6000.0, n_src=8)
+                            pixel_arcsec=30.0, radius_m=6000.0, n_src=8, dec_deg=-70.0,
+                            dt_s=600.0)
     img, g = invert(p, 1e-4)
     plan = g.plan()
     assert plan["P"] > plan["W"] + 4, plan
@@ -196,7 +198,7 @@ def test_predict_shift_and_wide_field(oracle):
     rng = np.random.default_rng(2)
     model = np.zeros((384, 384))
     model[rng.integers(0, 384, 30), rng.integers(0, 384, 30)] = rng.uniform(-1, 1, 30)
-    ls, ms = -0.01, 0.02
+    ls, ms = -0.01, 0.02  # the model lives on the shifted image
     out, g = predict(384, p["dx"], 1e-4, model, p["uvw"], p["freq"], ls, ms)
     sel = rng.choice(out.size, 500, replace=False)
     ref = oracle.dft_predict(384, 384, p["dx"], p["dx"], ls, ms, model, p["uvw"], p["freq"],

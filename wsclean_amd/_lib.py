@@ -58,6 +58,9 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),
     ("wgx_fft", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    ("wgx_synth_visibilities", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]),
 ]
 
 _lib = None
@@ -74,6 +77,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise WgError("libwgrid_hip.so is not built; run python -m wsclean_amd.build")
+    # PyTorch ships its own HIP runtime (libamdhip64.so, SONAME
+    # libamdhip64.so.7).  Two HIP runtimes in one process do not share the
+    # device, so when torch is installed it is loaded first and this
+    # library's libamdhip64.so.7 dependency binds to torch's copy; device
+    # pointers from torch tensors are then valid here and vice versa.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

@@ -401,4 +401,25 @@ int wgx_fft(int n, int nlines, int inverse, float* data) {
   });
 }
 
+int wgx_synth_visibilities(const double* uvw_dev, size_t n_rows, size_t n_chan,
+                           const double* freq, int n_src, const double* src, double noise_sigma,
+                           uint64_t seed, float* vis_dev) {
+  return Guard([&] {
+    WG_CHECK(uvw_dev && freq && vis_dev && (n_src == 0 || src), "null argument");
+    std::vector<double> scale(n_chan);
+    for (size_t c = 0; c < n_chan; ++c) scale[c] = freq[c] / wg::kSpeedOfLight;
+    double *dscale = nullptr, *dsrc = nullptr;
+    WG_HIP(hipMalloc(&dscale, n_chan * sizeof(double)));
+    WG_HIP(hipMalloc(&dsrc, (size_t(n_src) * 5 + 1) * sizeof(double)));
+    WG_HIP(hipMemcpy(dscale, scale.data(), n_chan * sizeof(double), hipMemcpyHostToDevice));
+    if (n_src)
+      WG_HIP(hipMemcpy(dsrc, src, size_t(n_src) * 5 * sizeof(double), hipMemcpyHostToDevice));
+    wg::LaunchSynth(uvw_dev, n_rows, n_chan, dscale, dsrc, n_src, noise_sigma, seed,
+                    reinterpret_cast<float2*>(vis_dev), nullptr);
+    WG_HIP(hipDeviceSynchronize());
+    (void)hipFree(dscale);
+    (void)hipFree(dsrc);
+  });
+}
+
 }  // extern "C"

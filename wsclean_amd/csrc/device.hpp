@@ -24,7 +24,8 @@ struct DevPlan {
   int box_u0, box_v0, wrap_u, wrap_v;
   int BU, BV;
   int nx, ny;
-  uint32_t nkeys;
+  uint32_t nkeys;     // invalid key; valid keys are below
+  uint32_t nbuckets;  // (tile, p0) buckets; key = (bucket * T + dv) * T + du
 };
 
 DevPlan ToDevPlan(const Plan& p);

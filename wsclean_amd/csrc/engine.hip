@@ -51,6 +51,7 @@ DevPlan ToDevPlan(const Plan& p) {
   d.nx = p.img.nx;
   d.ny = p.img.ny;
   d.nkeys = p.nkeys;
+  d.nbuckets = p.nbuckets;
   return d;
 }
 
@@ -199,10 +200,11 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
     DeviceRadixSort(keys, vals, n_vis, vs.plan.key_bits, scratch, stream_);
   }
   vs.perm = vals;
-  vs.off = static_cast<uint32_t*>(arena_.Get(tag + ".off", (size_t(dp.nkeys) + 1) * 4));
-  DeviceBucketOffsets(keys, n_vis, dp.nkeys, vs.off, stream_);
+  vs.off = static_cast<uint32_t*>(arena_.Get(tag + ".off", (size_t(dp.nbuckets) + 1) * 4));
+  DeviceBucketOffsets(keys, n_vis, dp.nbuckets, uint32_t(dp.T) * uint32_t(dp.T), vs.off,
+                      stream_);
   uint32_t nvalid = 0;
-  WG_HIP(hipMemcpyAsync(&nvalid, vs.off + dp.nkeys, 4, hipMemcpyDeviceToHost, stream_));
+  WG_HIP(hipMemcpyAsync(&nvalid, vs.off + dp.nbuckets, 4, hipMemcpyDeviceToHost, stream_));
   WG_HIP(hipStreamSynchronize(stream_));
   vs.n_valid = nvalid;
 

@@ -22,8 +22,9 @@ size_t ScanScratchWords(size_t n);
 size_t RadixSortScratchWords(size_t n);
 void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
                      uint32_t* scratch, hipStream_t s);
-void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nkeys,
-                         uint32_t* off, hipStream_t s);
+// off[b] = first sorted position with key >= b * stride, b in [0, nbuckets].
+void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
+                         uint32_t stride, uint32_t* off, hipStream_t s);
 
 // A work item: the visibilities [begin, end) (sorted order) of one uv tile
 // that touch the planes of one gridding sub-slab.
@@ -169,5 +170,8 @@ void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+void LaunchSynth(const double* uvw, size_t n_rows, size_t n_chan, const double* scale,
+                 const double* src, int n_src, double noise, uint64_t seed, float2* vis,
+                 hipStream_t s);
 
 }  // namespace wg

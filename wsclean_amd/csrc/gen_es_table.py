@@ -69,7 +69,13 @@ def table():
             fine = np.linspace(b0 - 0.05, b0 + 0.05, 21)
             e1 = errors(fine)
             ib = int(np.argmin(e1))
-            rows.append((W, sigma, float(fine[ib] * W), float(e1[ib])))
+            beta = float(fine[ib] * W)
+            # Amplification of grid-domain rounding noise by the correction
+            # 1/Phi(t): RMS over |t| <= 1/(2 sigma) and its maximum (edge).
+            ph = es_phi_hat(W, np.array([beta]), t)[0]
+            amp = ph[0] / ph
+            rows.append((W, sigma, beta, float(e1[ib]), float(np.sqrt(np.mean(amp ** 2))),
+                         float(amp.max())))
     return rows
 
 
@@ -79,13 +85,13 @@ def main():
     out = os.path.join(here, "es_table.inc")
     with open(out, "w") as f:
         f.write("// Generated by gen_es_table.py: {W, sigma, beta, 1-D RMS "
-                "aliasing error}.\n")
-        for W, s, b, e in rows:
-            f.write("{%d, %.2f, %.6f, %.6e},\n" % (W, s, b, e))
+                "aliasing error, RMS and max correction amplification}.\n")
+        for W, s, b, e, ar, am in rows:
+            f.write("{%d, %.2f, %.6f, %.6e, %.6e, %.6e},\n" % (W, s, b, e, ar, am))
     print("wrote", out, len(rows), "rows")
-    for W, s, b, e in rows:
-        if s in (1.25, 1.5, 2.0):
-            print(W, s, round(b / W, 4), "%.2e" % e)
+    for W, s, b, e, ar, am in rows:
+        if s in (1.2, 1.25, 1.5, 1.75, 2.0):
+            print(W, s, round(b / W, 4), "%.2e" % e, "amp rms %.1f max %.1f" % (ar, am))
 
 
 if __name__ == "__main__":

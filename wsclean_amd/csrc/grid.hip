@@ -18,52 +18,133 @@ namespace wg {
 namespace {
 
 constexpr int kGridBlock = 256;
-constexpr uint32_t kChunk = 2048;  // visibilities per work item
+// Visibilities per work item.  Large, so that flushing the E x E x Sg
+// subgrid with global atomics is amortised over many visibilities.
+constexpr uint32_t kChunk = 65536;
 
+// Wave-cooperative gridding.  A wave walks its share of the work item one
+// visibility at a time; its lanes cover the W x W kernel footprint (lane ->
+// column k = lane % W, rows l = lane / W + r * (64 / W)) and accumulate the
+// footprint cells of every plane of the sub-slab in registers.  Visibilities
+// are sorted by exact first cell inside each (tile, first plane) bucket, so
+// long runs share one footprint; the registers go to the LDS subgrid with
+// (collision-free) LDS atomics only when the run's first cell changes.  With
+// the dense uv cores of real arrays, where thousands of visibilities share a
+// cell, this replaces ~2 W^3 contended atomics per visibility by ~3 kernel
+// evaluations and ~2 R Sg FMAs per lane.
 template <int W>
 __global__ void __launch_bounds__(kGridBlock) GridKernel(
     DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
     const float2* __restrict__ vis, float2* __restrict__ box, int sub_lo, int sub_n,
     int slab_lo) {
-  extern __shared__ float2 sub[];
+  constexpr int RPP = 64 / W;                 // footprint rows per pass
+  constexpr int R = (W + RPP - 1) / RPP;      // row passes per lane
+  constexpr int S = kMaxSubSlab;
+  extern __shared__ float sub_re[];           // [sub_n][E][E] real, then imaginary
   const int E = dp.T + W - 1;
   const int ncell = sub_n * E * E;
+  float* sub_im = sub_re + ncell;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kcol = lane % W, lrow = lane / W;
+  const float xk = float(kcol - W / 2 + 1);
   const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
   for (uint32_t it = it0 + blockIdx.x; it < it1; it += gridDim.x) {
     const WorkItem wi = items[it];
-    for (int i = threadIdx.x; i < ncell; i += kGridBlock) sub[i] = make_float2(0.f, 0.f);
+    for (int i = threadIdx.x; i < ncell; i += kGridBlock) {
+      sub_re[i] = 0.f;
+      sub_im[i] = 0.f;
+    }
     __syncthreads();
-    for (uint32_t i = wi.begin + threadIdx.x; i < wi.end; i += kGridBlock) {
-      const float4 c = coord[i];
-      const float2 V = vis[i];
-      const uint32_t pk = __float_as_uint(c.w);
-      const int du = pk & 255u, dv = (pk >> 8) & 255u, p0 = (pk >> 16) & 0x7fffu;
-      float ku[W], kv[W];
-      EsTaps<W>(c.x, dp.beta, ku);
-      EsTaps<W>(c.y, dp.beta, kv);
-      const int plo = max(p0, sub_lo), phi = min(p0 + W, sub_lo + sub_n);
-      for (int p = plo; p < phi; ++p) {
-        const float kw =
-            EsKernel((float(p - p0 - W / 2 + 1) - c.z) * dp.two_over_w, dp.beta);
-        const float2 vw = make_float2(V.x * kw, V.y * kw);
-        float2* base = sub + ((p - sub_lo) * E + dv) * E + du;
+    const uint32_t n = wi.end - wi.begin;
+    const uint32_t per = (n + kGridBlock / 64 - 1) / (kGridBlock / 64);
+    const uint32_t w0 = wi.begin + min(n, wave * per);
+    const uint32_t w1 = wi.begin + min(n, (wave + 1) * per);
+    float are[R][S], aim[R][S];
 #pragma unroll
-        for (int l = 0; l < W; ++l) {
-          const float2 vl = make_float2(vw.x * kv[l], vw.y * kv[l]);
-          float2* row = base + l * E;
+    for (int r = 0; r < R; ++r)
 #pragma unroll
-          for (int k = 0; k < W; ++k) {
-            atomicAdd(&row[k].x, vl.x * ku[k]);
-            atomicAdd(&row[k].y, vl.y * ku[k]);
+      for (int q = 0; q < S; ++q) are[r][q] = aim[r][q] = 0.f;
+    int cur = -1;            // packed (du, dv, p0) of the run, wave-uniform
+    int cur_lo = 0, cur_hi = 0;
+
+    auto flush = [&]() {
+      const int du = cur & 255, dv = (cur >> 8) & 255;
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        if (q >= cur_lo && q < cur_hi) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int l = lrow + r * RPP;
+            if (lrow < RPP && l < W) {
+              const int idx = (q * E + dv + l) * E + du + kcol;
+              atomicAdd(&sub_re[idx], are[r][q]);
+              atomicAdd(&sub_im[idx], aim[r][q]);
+            }
+            are[r][q] = 0.f;
+            aim[r][q] = 0.f;
+          }
+        }
+      }
+    };
+
+    for (uint32_t base = w0; base < w1; base += 64) {
+      const uint32_t idx = base + lane;
+      float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+      float2 V = make_float2(0.f, 0.f);
+      if (idx < w1) {
+        c = coord[idx];
+        V = vis[idx];
+      }
+      const int cnt = int(min(64u, w1 - base));
+      for (int j = 0; j < cnt; ++j) {
+        const float fu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), j));
+        const float fv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
+        const float fw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.z), j));
+        const int pk = __builtin_amdgcn_readlane(__float_as_int(c.w), j) & 0x7fffffff;
+        const float vr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(V.x), j));
+        const float vi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(V.y), j));
+        if (pk != cur) {
+          if (cur >= 0) flush();
+          cur = pk;
+          const int p0 = (pk >> 16) & 0x7fff;
+          cur_lo = max(p0, sub_lo) - sub_lo;
+          cur_hi = min(p0 + W, sub_lo + sub_n) - sub_lo;
+        }
+        const int p0 = (pk >> 16) & 0x7fff;
+        // lane q evaluates the w weight of plane sub_lo + q
+        float kwl = 0.f;
+        {
+          const int t = sub_lo + (lane & (S - 1)) - p0;
+          if (t >= 0 && t < W) kwl = EsKernel((float(t - W / 2 + 1) - fw) * dp.two_over_w, dp.beta);
+        }
+        const float ku = EsKernel((xk - fu) * dp.two_over_w, dp.beta);
+        float kv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int l = lrow + r * RPP;
+          kv[r] = (l < W) ? ku * EsKernel((float(l - W / 2 + 1) - fv) * dp.two_over_w, dp.beta)
+                          : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          if (q >= cur_lo && q < cur_hi) {
+            const float kw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kwl), q));
+            const float wr = vr * kw, wi_ = vi * kw;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              are[r][q] += wr * kv[r];
+              aim[r][q] += wi_ * kv[r];
+            }
           }
         }
       }
     }
+    if (cur >= 0) flush();
     __syncthreads();
     const int tu = int(wi.tile % uint32_t(dp.ntu)), tv = int(wi.tile / uint32_t(dp.ntu));
     for (int i = threadIdx.x; i < ncell; i += kGridBlock) {
-      const float2 val = sub[i];
+      const float2 val = make_float2(sub_re[i], sub_im[i]);
       if (val.x != 0.f || val.y != 0.f) {
         const int p = i / (E * E), r = (i / E) % E, cc = i % E;
         float2* dst = box + (size_t(sub_lo + p - slab_lo) * dp.BV + size_t(tv * dp.T + r)) *

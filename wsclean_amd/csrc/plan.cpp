@@ -1,6 +1,6 @@
 // Host-side planning: kernel shape, padded grid, w axis, occupied uv box and
 // its tiling.  The arithmetic that decides which grid cell or w plane a
-// visibility lands in is repeated bit-for-bit on the device (see keys.hip),
+// visibility lands in is repeated bit-for-bit on the device (LocateVis),
 // so everything here is plain IEEE double arithmetic without contraction.
 #include <algorithm>
 #include <cmath>
@@ -138,8 +138,7 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
     if (opt.force_W && k.W != opt.force_W) continue;
     if (opt.force_sigma > 0 && std::fabs(k.sigma - opt.force_sigma) > 1e-9) continue;
     if (k.sigma < opt.sigma_min - 1e-9 || k.sigma > opt.sigma_max + 1e-9) continue;
-    // 3 dimensions of aliasing plus FP32 rounding: keep a factor 2 margin.
-    if (!opt.force_W && 2.0 * k.err1d > g.epsilon) continue;
+    if (!opt.force_W && ShapeErrorEstimate(k) > g.epsilon) continue;
     int nu = GoodFFTSize(int(std::ceil(k.sigma * g.nx)));
     int nv = GoodFFTSize(int(std::ceil(k.sigma * g.ny)));
     double dw = dz > 0 ? 1.0 / (k.sigma * dz) : 1e300;
@@ -161,6 +160,7 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   p.sigma = chosen.sigma;
   p.beta = chosen.beta;
   p.err1d = chosen.err1d;
+  p.err_est = ShapeErrorEstimate(chosen);
   p.nu = GoodFFTSize(int(std::ceil(p.sigma * g.nx)));
   p.nv = GoodFFTSize(int(std::ceil(p.sigma * g.ny)));
   WG_CHECK(p.nu <= 16384 && p.nv <= 16384,
@@ -180,7 +180,7 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   }
 
   // ---- occupied uv box ----
-  p.T = opt.T > 0 ? opt.T : 32;
+  p.T = opt.T > 0 ? opt.T : 16;
   WG_CHECK(p.T >= 8 && p.T <= 128, "tile size must be in [8, 128]");
   auto box = [&](double cmin, double cmax, double fac, int n, int& origin,
                  bool& wrap, int& ntiles, int& ext) {
@@ -206,14 +206,27 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   box(e.vmin, e.vmax, p.vfac, p.nv, p.box_v0, p.wrap_v, p.ntv, p.BV);
   WG_CHECK(p.BU <= 2 * p.nu && p.BV <= 2 * p.nv, "uv box larger than twice the grid");
 
-  uint64_t nk = uint64_t(p.ntu) * uint64_t(p.ntv) * uint64_t(p.P0);
-  WG_CHECK(nk < 0xFFFFFFF0ull, "too many (tile, plane) buckets");
+  // sort key = ((tile * P0 + p0) * T + dv) * T + du: buckets of one tile and
+  // first plane, ordered by the exact first cell inside the bucket
+  const uint64_t nb = uint64_t(p.ntu) * uint64_t(p.ntv) * uint64_t(p.P0);
+  const uint64_t nk = nb * uint64_t(p.T) * uint64_t(p.T);
+  WG_CHECK(nk < 0xFFFFFFF0ull, "too many (tile, plane, cell) keys: use a smaller tile");
+  p.nbuckets = uint32_t(nb);
   p.nkeys = uint32_t(nk);
   p.key_bits = 1;
   while ((uint64_t(1) << p.key_bits) <= uint64_t(p.nkeys)) ++p.key_bits;
 
   // ---- plane batching ----
-  p.Sg = opt.Sg > 0 ? opt.Sg : 4;
+  p.Sg = opt.Sg > 0 ? opt.Sg : 8;
+  p.Sg = std::min(p.Sg, kMaxSubSlab);
+  {
+    // the gridding subgrid (Sg planes of (T + W - 1)^2 complex) must fit
+    // the 160 KiB of LDS of one CU
+    const size_t E = size_t(p.T + p.W - 1);
+    const int max_sg = int((160 * 1024) / (E * E * 8));
+    WG_CHECK(max_sg >= 1, "tile too large for LDS");
+    p.Sg = std::max(1, std::min(p.Sg, max_sg));
+  }
   const double per_plane = 8.0 * p.BV * (double(p.BU) + 2.0 * g.nx);
   int sf = int(std::max(1.0, std::floor(double(opt.slab_bytes) / per_plane)));
   sf = std::max(p.Sg, (sf / p.Sg) * p.Sg);
@@ -225,7 +238,8 @@ std::string Plan::describe() const {
   std::ostringstream s;
   s.precision(17);
   s << "{\"W\":" << W << ",\"sigma\":" << sigma << ",\"beta\":" << beta
-    << ",\"err1d\":" << err1d << ",\"nu\":" << nu << ",\"nv\":" << nv
+    << ",\"err1d\":" << err1d << ",\"err_est\":" << err_est << ",\"nu\":" << nu
+    << ",\"nv\":" << nv
     << ",\"P\":" << P << ",\"P0\":" << P0 << ",\"dw\":" << dw
     << ",\"w_lo\":" << w_lo << ",\"zmin\":" << zmin << ",\"zmax\":" << zmax
     << ",\"T\":" << T << ",\"ntu\":" << ntu << ",\"ntv\":" << ntv
@@ -233,7 +247,8 @@ std::string Plan::describe() const {
     << ",\"box_v0\":" << box_v0 << ",\"wrap_u\":" << (wrap_u ? 1 : 0)
     << ",\"wrap_v\":" << (wrap_v ? 1 : 0) << ",\"key_bits\":" << key_bits
     << ",\"Sg\":" << Sg << ",\"Sf\":" << Sf << ",\"dx\":" << img.dx
-    << ",\"dy\":" << img.dy << ",\"zc\":" << zc << ",\"nkeys\":" << nkeys << "}";
+    << ",\"dy\":" << img.dy << ",\"zc\":" << zc << ",\"nkeys\":" << nkeys
+    << ",\"nbuckets\":" << nbuckets << "}";
   return s.str();
 }
 

@@ -178,17 +178,18 @@ __global__ void RadixScatterKernel(const uint32_t* __restrict__ kin,
   }
 }
 
-// off[k] = first sorted position whose key >= k, for k in [0, nk].
-__global__ void BucketOffsetsKernel(const uint32_t* __restrict__ keys, size_t n,
-                                    uint32_t nk, uint32_t* __restrict__ off) {
-  const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k > nk) return;
+// off[b] = first sorted position whose key >= b * stride, for b in [0, nb].
+__global__ void BucketOffsetsKernel(const uint32_t* __restrict__ keys, size_t n, uint32_t nb,
+                                    uint32_t stride, uint32_t* __restrict__ off) {
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  const uint64_t k = uint64_t(b) * stride;
   size_t lo = 0, hi = n;
   while (lo < hi) {
     size_t mid = (lo + hi) >> 1;
-    if (keys[mid] < uint32_t(k)) lo = mid + 1; else hi = mid;
+    if (uint64_t(keys[mid]) < k) lo = mid + 1; else hi = mid;
   }
-  off[k] = uint32_t(lo);
+  off[b] = uint32_t(lo);
 }
 
 }  // namespace
@@ -249,11 +250,11 @@ void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
   }
 }
 
-void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nkeys,
-                         uint32_t* off, hipStream_t s) {
-  const size_t cnt = size_t(nkeys) + 1;
+void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
+                         uint32_t stride, uint32_t* off, hipStream_t s) {
+  const size_t cnt = size_t(nbuckets) + 1;
   BucketOffsetsKernel<<<dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, s>>>(
-      sorted_keys, n, nkeys, off);
+      sorted_keys, n, nbuckets, stride, off);
   WG_HIP(hipGetLastError());
 }
 

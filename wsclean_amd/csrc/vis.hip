@@ -91,7 +91,8 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
     }
     if (valid) {
       const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
-      key = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
+      const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
+      key = (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
     }
     keys[i] = key;
     vals[i] = uint32_t(i);
@@ -197,7 +198,68 @@ unsigned Blocks(size_t n) {
   return unsigned(b < 65536 ? (b ? b : 1) : 65536);
 }
 
+// splitmix64 finaliser: counter-based, so the noise of visibility i does not
+// depend on the launch shape.
+__device__ __forceinline__ uint64_t Mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// V(row, chan) = sum_s S_s g_s(u, v) exp(+2 pi i (u l_s + v m_s + w (n_s - 1)))
+// + complex Gaussian noise; g_s = exp(-2 pi^2 sigma_s^2 (u^2 + v^2)).
+// Sources are [l, m, n - 1, S, sigma] in FP64; phases are reduced in FP64.
+constexpr int kMaxSrcLds = 1024;
+__global__ void __launch_bounds__(kBlock) SynthKernel(
+    const double* __restrict__ uvw, size_t n_rows, size_t n_chan,
+    const double* __restrict__ scale, const double* __restrict__ src, int n_src,
+    double noise, uint64_t seed, float2* __restrict__ vis) {
+  __shared__ double s_src[kMaxSrcLds * 5];
+  for (int i = threadIdx.x; i < n_src * 5; i += kBlock) s_src[i] = src[i];
+  __syncthreads();
+  const size_t n = n_rows * n_chan;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t row = i / n_chan, ch = i - row * n_chan;
+    const double s = scale[ch];
+    const double u = uvw[3 * row] * s, v = uvw[3 * row + 1] * s, w = uvw[3 * row + 2] * s;
+    const double uv2 = u * u + v * v;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < n_src; ++k) {
+      const double* q = s_src + 5 * k;
+      const double cyc = u * q[0] + v * q[1] + w * q[2];
+      const double f = cyc - rint(cyc);
+      double amp = q[3];
+      if (q[4] > 0.0) amp *= exp(-2.0 * M_PI * M_PI * q[4] * q[4] * uv2);
+      float sn, cs;
+      sincospif(2.0f * float(f), &sn, &cs);
+      re += amp * cs;
+      im += amp * sn;
+    }
+    if (noise > 0.0) {
+      const uint64_t a = Mix64(seed ^ (uint64_t(i) * 2ull));
+      const uint64_t b = Mix64(seed ^ (uint64_t(i) * 2ull + 1ull));
+      const double u1 = (double(a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+      const double u2 = double(b >> 11) * (1.0 / 9007199254740992.0);
+      const double r = sqrt(-2.0 * log(u1)) * noise;
+      re += r * cos(2.0 * M_PI * u2);
+      im += r * sin(2.0 * M_PI * u2);
+    }
+    vis[i] = make_float2(float(re), float(im));
+  }
+}
+
 }  // namespace
+
+void LaunchSynth(const double* uvw, size_t n_rows, size_t n_chan, const double* scale,
+                 const double* src, int n_src, double noise, uint64_t seed, float2* vis,
+                 hipStream_t s) {
+  WG_CHECK(n_src >= 0 && n_src <= kMaxSrcLds, "at most 1024 synthetic sources");
+  SynthKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(uvw, n_rows, n_chan, scale, src, n_src,
+                                                         noise, seed, vis);
+  WG_HIP(hipGetLastError());
+}
 
 void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
                    double* partial, double* out, hipStream_t s) {

@@ -99,14 +99,17 @@ def add_noise(rng, vis, sigma):
 
 def small_problem(seed=1001, n_ant=12, n_times=20, n_chan=2, npix=64, pixel_arcsec=20.0,
                   f0=150e6, df=5e6, dec_deg=-30.0, radius_m=1500.0, n_src=5, noise=0.0,
-                  on_pixel=False):
-    """A small seeded problem for parity tests: returns a dict."""
+                  on_pixel=False, dt_s=60.0, sky_lm=(0.0, 0.0)):
+    """A small seeded problem for parity tests: returns a dict.  The sky is
+    centred on sky_lm (use the image's l/m shift to keep it in the field)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     enu = antennas_disc(rng, n_ant, radius_m)
-    uvw = earth_rotation_uvw(enu, n_times, 60.0, np.deg2rad(dec_deg))
+    uvw = earth_rotation_uvw(enu, n_times, dt_s, np.deg2rad(dec_deg))
     freq = f0 + df * np.arange(n_chan)
     dx = pixel_arcsec * ARCSEC
     l, m, flux, xs, ys = point_sky(rng, n_src, npix, dx, on_pixel=on_pixel)
+    l = l + sky_lm[0]
+    m = m + sky_lm[1]
     vis = predict_sources(uvw, freq, l, m, flux)
     if noise > 0:
         vis = add_noise(rng, vis, noise)
