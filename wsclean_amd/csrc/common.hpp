@@ -111,7 +111,7 @@ struct PlanOptions {
 };
 
 // Largest number of w planes one gridding sub-slab accumulates in registers.
-constexpr int kMaxSubSlab = 8;
+constexpr int kMaxSubSlab = 16;
 
 int GoodFFTSize(int n);        // smallest even 2^a 3^b 5^c >= n
 std::vector<int> FFTRadices(int n);  // Stockham stage radices for n

@@ -18,7 +18,7 @@ struct DevPlan {
   double l_shift, m_shift, zc;
   double dw;
   double dx_rad, dy_rad;
-  float beta;
+  float beta2;   // beta * log2(e) for the exp2-based ES kernel
   float two_over_w;
   int W, T, ntu, ntv, P0, P, nu, nv;
   int box_u0, box_v0, wrap_u, wrap_v;
@@ -91,21 +91,24 @@ __device__ __forceinline__ VisLoc LocateVis(double um, double vm, double wm,
   return r;
 }
 
-// ES kernel phi(x) = exp(beta (sqrt(1 - x^2) - 1)) for |x| <= 1.
-__device__ __forceinline__ float EsKernel(float x, float beta) {
-  float a = fmaxf(1.0f - x * x, 0.0f);
-  return expf(beta * (sqrtf(a) - 1.0f));
+// ES kernel phi(x) = exp(beta (sqrt(1 - x^2) - 1)) = 2^(b2 (sqrt(1 - x^2) - 1))
+// with b2 = beta log2(e), for |x| <= 1.  The exponent lies in [-b2, 0]
+// (b2 < 60), so the hardware exp2 and sqrt (about 1 ulp each) need no range
+// reduction.
+__device__ __forceinline__ float EsKernel(float x, float b2) {
+  const float a = fmaxf(1.0f - x * x, 0.0f);
+  return __builtin_amdgcn_exp2f(b2 * (__builtin_amdgcn_sqrtf(a) - 1.0f));
 }
 
 // Weights of the W taps for fractional coordinate f in [0, 1): tap k sits at
 // x = (k - W/2 + 1 - f) * 2 / W.
 template <int W>
-__device__ __forceinline__ void EsTaps(float f, float beta, float* out) {
+__device__ __forceinline__ void EsTaps(float f, float b2, float* out) {
   constexpr float s = 2.0f / W;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
     float x = (float(k - W / 2 + 1) - f) * s;
-    out[k] = EsKernel(x, beta);
+    out[k] = EsKernel(x, b2);
   }
 }
 

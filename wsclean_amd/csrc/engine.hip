@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "device.hpp"
@@ -32,7 +33,7 @@ DevPlan ToDevPlan(const Plan& p) {
   d.dw = p.dw;
   d.dx_rad = p.img.dx;
   d.dy_rad = p.img.dy;
-  d.beta = float(p.beta);
+  d.beta2 = float(p.beta * 1.4426950408889634);
   d.two_over_w = 2.0f / float(p.W);
   d.W = p.W;
   d.T = p.T;
@@ -234,51 +235,52 @@ void Engine::BuildWorkItems(VisSet& vs) {
 void Engine::BuildCorrection(VisSet& vs) {
   const Plan& p = vs.plan;
   const int nx = p.img.nx, ny = p.img.ny;
-  std::vector<double> pu(nx), pv(ny);
-  for (int x = 0; x < nx; ++x) pu[x] = EsKernelFT(p.W, p.beta, double(nx / 2 - x) / p.nu);
-  for (int y = 0; y < ny; ++y) pv[y] = EsKernelFT(p.W, p.beta, double(y - ny / 2) / p.nv);
-  // Phi_w(t) = sum_q a_q cos(b_q t), Gauss-Legendre in theta (x = sin theta)
+  // The correction depends only on the image geometry, the kernel shape, the
+  // padded grid and the w axis scale/centre: reuse it while those match.
+  char key[512];
+  std::snprintf(key, sizeof(key), "%d %d %.17g %.17g %.17g %.17g %d %.17g %d %d %.17g %.17g",
+                nx, ny, p.img.dx, p.img.dy, p.img.l_shift, p.img.m_shift, p.W, p.beta, p.nu,
+                p.nv, p.dw, p.zc);
+  vs.corr = static_cast<float*>(arena_.Get(vs.tag + ".corr", size_t(nx) * ny * 4));
+  if (vs.corr_key == key) return;
+  // Phi(t) = sum_q a_q cos(b_q t): Gauss-Legendre in theta with x = sin(theta)
   const int nq = 64;
   std::vector<double> gx(nq), gw(nq);
-  {
+  for (int i = 0; i < (nq + 1) / 2; ++i) {
     // nodes by Newton iteration on the Legendre polynomial
-    for (int i = 0; i < (nq + 1) / 2; ++i) {
-      double z = std::cos(M_PI * (i + 0.75) / (nq + 0.5)), pp = 0;
-      for (int it = 0; it < 100; ++it) {
-        double p1 = 1, p2 = 0;
-        for (int j = 1; j <= nq; ++j) {
-          double p3 = p2;
-          p2 = p1;
-          p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
-        }
-        pp = nq * (z * p1 - p2) / (z * z - 1.0);
-        double z1 = z;
-        z = z1 - p1 / pp;
-        if (std::fabs(z - z1) < 1e-15) break;
+    double z = std::cos(M_PI * (i + 0.75) / (nq + 0.5)), pp = 0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1, p2 = 0;
+      for (int j = 1; j <= nq; ++j) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
       }
-      gx[i] = -z;
-      gx[nq - 1 - i] = z;
-      gw[i] = gw[nq - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+      pp = nq * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-15) break;
     }
+    gx[i] = -z;
+    gx[nq - 1 - i] = z;
+    gw[i] = gw[nq - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
   }
-  std::vector<double> qa(nq), qb(nq);
-  for (int q = 0; q < nq; ++q) {
-    const double th = 0.5 * M_PI * gx[q];
+  std::vector<double> q(2 * nq);
+  for (int k = 0; k < nq; ++k) {
+    const double th = 0.5 * M_PI * gx[k];
     const double c = std::cos(th);
-    qa[q] = 0.5 * p.W * 0.5 * M_PI * gw[q] * std::exp(p.beta * (c - 1.0)) * c;
-    qb[q] = M_PI * p.W * std::sin(th);
+    q[k] = 0.5 * p.W * 0.5 * M_PI * gw[k] * std::exp(p.beta * (c - 1.0)) * c;
+    q[nq + k] = M_PI * p.W * std::sin(th);
   }
+  double* dq = static_cast<double*>(arena_.Get(vs.tag + ".quad", 2 * nq * 8));
   double* dpu = static_cast<double*>(arena_.Get(vs.tag + ".phiu", nx * 8));
   double* dpv = static_cast<double*>(arena_.Get(vs.tag + ".phiv", ny * 8));
-  double* dq = static_cast<double*>(arena_.Get(vs.tag + ".quad", 2 * nq * 8));
-  WG_HIP(hipMemcpyAsync(dpu, pu.data(), nx * 8, hipMemcpyHostToDevice, stream_));
-  WG_HIP(hipMemcpyAsync(dpv, pv.data(), ny * 8, hipMemcpyHostToDevice, stream_));
-  WG_HIP(hipMemcpyAsync(dq, qa.data(), nq * 8, hipMemcpyHostToDevice, stream_));
-  WG_HIP(hipMemcpyAsync(dq + nq, qb.data(), nq * 8, hipMemcpyHostToDevice, stream_));
-  vs.corr = static_cast<float*>(arena_.Get(vs.tag + ".corr", size_t(nx) * ny * 4));
+  WG_HIP(hipMemcpyAsync(dq, q.data(), 2 * nq * 8, hipMemcpyHostToDevice, stream_));
+  LaunchPhiTables(vs.dp, dq, dq + nq, nq, dpu, dpv, stream_);
   LaunchCorrection(vs.dp, dpu, dpv, dq, dq + nq, nq, vs.corr, stream_);
-  // the host vectors must outlive the async copies
+  // the host vector must outlive the async copy
   WG_HIP(hipStreamSynchronize(stream_));
+  vs.corr_key = key;
 }
 
 void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {

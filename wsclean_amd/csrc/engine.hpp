@@ -75,6 +75,7 @@ struct VisSet {
   int nsub = 0;
   size_t n_items = 0;
   float* corr = nullptr;      // [nx][ny] 1 / (Phi_u Phi_v Phi_w) with sign
+  std::string corr_key;       // parameters the correction was built for
 };
 
 class Engine {
@@ -167,6 +168,8 @@ void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale,
 void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_v,
                       const double* quad_a, const double* quad_b, int nquad,
                       float* corr, hipStream_t s);
+void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad_b, int nquad,
+                     double* phi_u, double* phi_v, hipStream_t s);
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);

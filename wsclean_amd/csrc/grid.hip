@@ -32,7 +32,7 @@ constexpr uint32_t kChunk = 65536;
 // the dense uv cores of real arrays, where thousands of visibilities share a
 // cell, this replaces ~2 W^3 contended atomics per visibility by ~3 kernel
 // evaluations and ~2 R Sg FMAs per lane.
-template <int W>
+template <int W, int S>  // S: register depth in planes (>= sub_n, power of 2)
 __global__ void __launch_bounds__(kGridBlock) GridKernel(
     DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
@@ -40,7 +40,6 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
     int slab_lo) {
   constexpr int RPP = 64 / W;                 // footprint rows per pass
   constexpr int R = (W + RPP - 1) / RPP;      // row passes per lane
-  constexpr int S = kMaxSubSlab;
   extern __shared__ float sub_re[];           // [sub_n][E][E] real, then imaginary
   const int E = dp.T + W - 1;
   const int ncell = sub_n * E * E;
@@ -116,14 +115,14 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
         float kwl = 0.f;
         {
           const int t = sub_lo + (lane & (S - 1)) - p0;
-          if (t >= 0 && t < W) kwl = EsKernel((float(t - W / 2 + 1) - fw) * dp.two_over_w, dp.beta);
+          if (t >= 0 && t < W) kwl = EsKernel((float(t - W / 2 + 1) - fw) * dp.two_over_w, dp.beta2);
         }
-        const float ku = EsKernel((xk - fu) * dp.two_over_w, dp.beta);
+        const float ku = EsKernel((xk - fu) * dp.two_over_w, dp.beta2);
         float kv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int l = lrow + r * RPP;
-          kv[r] = (l < W) ? ku * EsKernel((float(l - W / 2 + 1) - fv) * dp.two_over_w, dp.beta)
+          kv[r] = (l < W) ? ku * EsKernel((float(l - W / 2 + 1) - fv) * dp.two_over_w, dp.beta2)
                           : 0.f;
         }
 #pragma unroll
@@ -181,13 +180,13 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
       const uint32_t pk = __float_as_uint(c.w);
       const int du = pk & 255u, dv = (pk >> 8) & 255u, p0 = (pk >> 16) & 0x7fffu;
       float ku[W], kv[W];
-      EsTaps<W>(c.x, dp.beta, ku);
-      EsTaps<W>(c.y, dp.beta, kv);
+      EsTaps<W>(c.x, dp.beta2, ku);
+      EsTaps<W>(c.y, dp.beta2, kv);
       const int plo = max(p0, sub_lo), phi = min(p0 + W, sub_lo + sub_n);
       float sx = 0.f, sy = 0.f;
       for (int p = plo; p < phi; ++p) {
         const float kw =
-            EsKernel((float(p - p0 - W / 2 + 1) - c.z) * dp.two_over_w, dp.beta);
+            EsKernel((float(p - p0 - W / 2 + 1) - c.z) * dp.two_over_w, dp.beta2);
         const float2* base = sub + ((p - sub_lo) * E + dv) * E + du;
         float px = 0.f, py = 0.f;
 #pragma unroll
@@ -309,12 +308,20 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
                 int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
+  WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
   DispatchW(dp.W, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    AllowLds(GridKernel<W>, lds);
-    GridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(
-        dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord,
-        vis_sorted, box, sub_lo, sub_n, slab_lo);
+    if (sub_n <= 8) {
+      AllowLds(GridKernel<W, 8>, lds);
+      GridKernel<W, 8><<<GridBlocks(), kGridBlock, lds, s>>>(
+          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord,
+          vis_sorted, box, sub_lo, sub_n, slab_lo);
+    } else {
+      AllowLds(GridKernel<W, 16>, lds);
+      GridKernel<W, 16><<<GridBlocks(), kGridBlock, lds, s>>>(
+          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord,
+          vis_sorted, box, sub_lo, sub_n, slab_lo);
+    }
   });
   WG_HIP(hipGetLastError());
 }

@@ -179,6 +179,27 @@ __global__ void __launch_bounds__(kBlock) CorrectionKernel(
   }
 }
 
+// phi_u[x] = Phi((nx/2 - x) / nu), phi_v[y] = Phi((y - ny/2) / nv), FP64.
+__global__ void PhiTablesKernel(DevPlan dp, const double* __restrict__ qa,
+                                const double* __restrict__ qb, int nq, double* __restrict__ pu,
+                                double* __restrict__ pv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double t;
+  double* out;
+  if (i < dp.nx) {
+    t = double(dp.nx / 2 - i) / double(dp.nu);
+    out = pu + i;
+  } else if (i < dp.nx + dp.ny) {
+    t = double(i - dp.nx - dp.ny / 2) / double(dp.nv);
+    out = pv + (i - dp.nx);
+  } else {
+    return;
+  }
+  double s = 0.0;
+  for (int q = 0; q < nq; ++q) s += qa[q] * cos(qb[q] * t);
+  *out = s;
+}
+
 __global__ void AccumulateImageKernel(const float* __restrict__ dirty,
                                       const float* __restrict__ corr, double* __restrict__ img,
                                       size_t n) {
@@ -308,6 +329,13 @@ void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_
                       hipStream_t s) {
   CorrectionKernel<<<Blocks(size_t(dp.nx) * dp.ny), kBlock, 0, s>>>(dp, phi_u, phi_v, quad_a,
                                                                     quad_b, nquad, corr);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad_b, int nquad,
+                     double* phi_u, double* phi_v, hipStream_t s) {
+  const int n = dp.nx + dp.ny;
+  PhiTablesKernel<<<(n + 255) / 256, 256, 0, s>>>(dp, quad_a, quad_b, nquad, phi_u, phi_v);
   WG_HIP(hipGetLastError());
 }
 
