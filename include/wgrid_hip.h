@@ -140,6 +140,13 @@ int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan,
                   char* plan_json, size_t plan_len);
 /* In-place batched complex FFT of nlines lines of length n (host data). */
 int wgx_fft(int n, int nlines, int inverse, float* data);
+/* Host-only: the plan (JSON) the library makes for an image geometry and
+ * flipped-coordinate extents {umin, umax, vmin, vmax, wmin, wmax} (in
+ * wavelengths), with the tuning overrides of wg_set_tuning.  Needs no GPU;
+ * used to check kernel index ranges on the CPU. */
+int wgx_make_plan(const wg_params* params, const double* extents6, long long nvis,
+                  int W, double sigma, int tile, int sub_slab, char* plan_json,
+                  size_t plan_len);
 /* Synthetic visibilities on the device (benchmark inputs):
  * vis(row, chan) = sum_s S_s exp(-2 pi^2 sigma_s^2 (u^2 + v^2))
  *                  * exp(+2 pi i (u l_s + v m_s + w (n_s - 1))) + noise,

@@ -1,0 +1,185 @@
+"""CPU check of the index ranges of every HIP kernel of the pipeline.
+
+The plan comes from the library itself (wgx_make_plan, host-only); the
+bucket keys, the stable sort, the bucket offsets and the work items are
+rebuilt in numpy exactly as the device builds them (the key restatement is
+pinned bit-for-bit against the device by the GPU sort tests).  Every array
+index the kernels form is then checked against the size of the buffer it
+addresses, for the geometries the GPU tests and the benchmark use.  No GPU.
+"""
+import ctypes
+import json
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from wsclean_amd import _lib, synth
+
+CHUNK = 65536      # wsclean_amd/csrc/grid.hip kChunk
+MAX_FFT = 16384    # largest LDS FFT line
+C = synth.C_LIGHT
+
+
+def extents(uvw, freq):
+    s = freq / C
+    smin, smax = s.min(), s.max()
+    u, v, w = uvw[:, 0].copy(), uvw[:, 1].copy(), uvw[:, 2].copy()
+    f = w < 0
+    u[f], v[f], w[f] = -u[f], -v[f], -w[f]
+    out = []
+    for a in (u, v, w):
+        x1, x2 = a * smin, a * smax
+        out += [float(np.minimum(x1, x2).min()), float(np.maximum(x1, x2).max())]
+    return np.array(out)
+
+
+def make_plan(n, dx, eps, uvw, freq, ls=0.0, ms=0.0, **tune):
+    lib = _lib.load()
+    prm = _lib.WgParams(n, n, n, n, dx, dx, ls, ms, 1, eps, 0, 0)
+    e = extents(uvw, freq)
+    buf = ctypes.create_string_buffer(4096)
+    _lib.check(lib.wgx_make_plan(ctypes.byref(prm), e.ctypes.data_as(ctypes.c_void_p),
+                                 uvw.shape[0] * len(freq), tune.get("W", 0),
+                                 tune.get("sigma", 0.0), tune.get("tile", 0),
+                                 tune.get("sub_slab", 0), buf, 4096))
+    p = json.loads(buf.value.decode())
+    p["nx"] = p["ny"] = n
+    return p
+
+
+def check_plan_bounds(plan, uvw, freq, vis=None):
+    W, T, P0, P = plan["W"], plan["T"], plan["P0"], plan["P"]
+    ntu, ntv, BU, BV = plan["ntu"], plan["ntv"], plan["BU"], plan["BV"]
+    Sg, Sf, nx, ny = plan["Sg"], plan["Sf"], plan["nx"], plan["ny"]
+    E = T + W - 1
+    ntiles = ntu * ntv
+    nbuckets = ntiles * P0
+    assert plan["nbuckets"] == nbuckets and plan["nkeys"] == nbuckets * T * T
+    assert plan["nu"] <= MAX_FFT and plan["nv"] <= MAX_FFT
+    assert Sf % Sg == 0 and 1 <= Sg <= 32
+    assert Sg * E * E * 8 <= 160 * 1024                      # LDS subgrid
+    assert BU == ntu * T + W - 1 and BV == ntv * T + W - 1
+
+    keys, valid = oracle.bucket_keys(plan, uvw, freq, vis)
+    kv = keys[valid].astype(np.int64)
+    assert kv.size == 0 or (kv.min() >= 0 and kv.max() < plan["nkeys"])
+    sk = np.sort(keys, kind="stable").astype(np.int64)
+    n_valid = int(valid.sum())
+    off = np.searchsorted(sk, np.arange(nbuckets + 1, dtype=np.int64) * T * T, side="left")
+    assert off[-1] == n_valid
+
+    # decode the sorted keys exactly as the kernels unpack them
+    s = sk[:n_valid]
+    du, dv = s % T, (s // T) % T
+    bucket = s // (T * T)
+    tile, p0 = bucket // P0, bucket % P0
+    assert (p0 < P0).all() and (tile < ntiles).all() and (p0 < 32768).all()
+
+    nsub = (P + Sg - 1) // Sg
+    counts = np.zeros((nsub, ntiles), np.int64)
+    ranges = {}
+    for g in range(nsub):
+        lo, hi = max(0, g * Sg - W + 1), min(P0 - 1, g * Sg + Sg - 1)
+        if lo > hi:
+            continue
+        t = np.arange(ntiles)
+        b = off[t * P0 + lo]
+        e = off[t * P0 + hi + 1]
+        counts[g] = (e - b + CHUNK - 1) // CHUNK
+        ranges[g] = (b, e)
+    item_pos = np.concatenate([[0], np.cumsum(counts.ravel())])
+    per_sub = np.array([item_pos[(g + 1) * ntiles] - item_pos[g * ntiles] for g in range(nsub)])
+    max_items = int(per_sub.max()) if nsub else 0
+    scratch_cells = max_items * Sg * E * E
+
+    for g, (b, e) in ranges.items():
+        sub_lo, sub_n = g * Sg, min(Sg, P - g * Sg)
+        assert 1 <= sub_n <= Sg
+        slab_lo = (sub_lo // Sf) * Sf
+        assert sub_lo + sub_n <= slab_lo + Sf                 # sub-slab inside the slab
+        nz = np.nonzero(e > b)[0]
+        for t in nz:
+            first = item_pos[g * ntiles + t] - item_pos[g * ntiles]
+            nitem = item_pos[g * ntiles + t + 1] - item_pos[g * ntiles + t]
+            assert first + nitem <= max_items
+            seg = slice(b[t], e[t])
+            # GridKernel: vis of the item belong to its tile and sub-slab range
+            assert (tile[seg] == t).all()
+            assert (p0[seg] <= sub_lo + sub_n - 1).all() and (p0[seg] + W > sub_lo).all()
+            q_hi = np.minimum(p0[seg] + W, sub_lo + sub_n) - sub_lo
+            lds_max = ((q_hi - 1) * E + dv[seg] + W - 1) * E + du[seg] + W - 1
+            assert lds_max.max() < sub_n * E * E
+            assert (first + nitem) * sub_n * E * E <= scratch_cells
+            # DegridKernel / GatherBoxKernel: the tile footprint inside the box
+            tu, tv = t % ntu, t // ntu
+            assert tv * T + E <= BV and tu * T + E <= BU
+            box_max = ((sub_lo + sub_n - 1 - slab_lo) * BV + tv * T + E - 1) * BU + tu * T + E - 1
+            assert box_max < Sf * BU * BV
+
+    # GatherBoxKernel: every cell's covering tiles and slot offsets
+    r = np.arange(BV)
+    c = np.arange(BU)
+    tv0 = np.maximum(0, np.trunc((r - E + T) / T).astype(int))
+    tv1 = np.minimum(ntv - 1, r // T)
+    tu0 = np.maximum(0, np.trunc((c - E + T) / T).astype(int))
+    tu1 = np.minimum(ntu - 1, c // T)
+    assert (tv1 >= tv0).all() and (tu1 >= tu0).all()
+    for lo_, hi_, x in ((tv0, tv1, r), (tu0, tu1, c)):
+        for k in range(3):
+            t = lo_ + k
+            m = t <= hi_
+            d = x[m] - t[m] * T
+            assert (d >= 0).all() and (d < E).all()
+
+    # FFT line kernels: folded loads and cropped stores stay inside the line
+    for n, origin, L, crop, half in ((plan["nu"], plan["box_u0"], BU, nx, nx // 2),
+                                     (plan["nv"], plan["box_v0"], BV, ny, ny // 2)):
+        i = np.arange(L)
+        pos = (origin + i) % n
+        assert pos.min() >= 0 and pos.max() < n
+        assert origin + L <= 3 * n
+        assert crop <= n
+    return dict(n_valid=n_valid, max_items=max_items, nsub=nsub)
+
+
+PROBLEMS = [
+    # (problem kwargs, npix, eps, shift)
+    (dict(seed=1001, n_ant=16, n_times=24, n_chan=2, npix=96, noise=0.01), 96, 1e-5, (0, 0)),
+    (dict(seed=3, n_ant=14, n_times=20, n_chan=2, npix=80, sky_lm=(0.05, -0.03)), 80, 1e-5,
+     (0.05, -0.03)),
+    (dict(seed=5, n_ant=20, n_times=40, n_chan=2, npix=512, pixel_arcsec=30.0, radius_m=6000.0,
+          n_src=8, dec_deg=-70.0, dt_s=600.0), 512, 1e-4, (0, 0)),
+    (dict(seed=9, n_ant=12, n_times=10, n_chan=2, npix=64, pixel_arcsec=60.0,
+          radius_m=3000.0), 64, 1e-4, (0, 0)),
+    (dict(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512, pixel_arcsec=30.0,
+          radius_m=6000.0, dec_deg=-70.0, dt_s=600.0), 512, 1e-5, (0, 0)),
+    (dict(seed=8, n_ant=20, n_times=30, n_chan=2, npix=384, pixel_arcsec=30.0,
+          radius_m=5000.0), 384, 1e-4, (-0.01, 0.02)),
+]
+
+
+@pytest.mark.parametrize("kw,n,eps,shift", PROBLEMS)
+@pytest.mark.parametrize("tune", [{}, dict(tile=32, sub_slab=8), dict(sub_slab=8),
+                                  dict(tile=8, sub_slab=32), dict(tile=12, sub_slab=16)])
+def test_kernel_index_ranges(kw, n, eps, shift, tune):
+    p = synth.small_problem(**kw)
+    plan = make_plan(n, p["dx"], eps, p["uvw"], p["freq"], shift[0], shift[1], **tune)
+    check_plan_bounds(plan, p["uvw"], p["freq"], p["vis"])
+    check_plan_bounds(plan, p["uvw"], p["freq"], None)
+
+
+@pytest.mark.parametrize("tune", [{}, dict(tile=32, sub_slab=8), dict(tile=8, sub_slab=32),
+                                  dict(tile=12, sub_slab=16)])
+def test_kernel_index_ranges_benchmark_geometry(tune):
+    # the 8192^2, 2 arcsec, 128-antenna track of the benchmark; one channel
+    # of every 8th time step (the index ranges depend on the coordinate
+    # distribution, not on the visibility count)
+    import bench
+    uvw, freq, dx, _ = bench.make_inputs(64, 8192)
+    nbl = 8128
+    uvw = uvw.reshape(-1, nbl, 3)[::8].reshape(-1, 3)
+    plan = make_plan(8192, dx, 1e-5, uvw, freq, **tune)
+    f1 = freq[[0, 31, 63]]
+    # the plan above covers the full band; check the extreme channels
+    check_plan_bounds(plan, uvw, f1, None)

@@ -58,6 +58,9 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),
     ("wgx_fft", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    ("wgx_make_plan", ctypes.c_int,
+     [ctypes.POINTER(WgParams), ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
+      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_size]),
     ("wgx_synth_visibilities", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]),

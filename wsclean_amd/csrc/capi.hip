@@ -379,6 +379,30 @@ int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
   });
 }
 
+int wgx_make_plan(const wg_params* params, const double* e, long long nvis, int W, double sigma,
+                  int tile, int sub_slab, char* plan_json, size_t plan_len) {
+  return Guard([&] {
+    WG_CHECK(params && e && plan_json && plan_len, "null argument");
+    wg::ImageGeometry g;
+    g.nx = int(params->width_t);
+    g.ny = int(params->height_t);
+    g.dx = params->pixel_size_x;
+    g.dy = params->pixel_size_y;
+    g.l_shift = params->l_shift;
+    g.m_shift = params->m_shift;
+    g.epsilon = params->epsilon;
+    wg::PlanOptions o;
+    o.force_W = W;
+    o.force_sigma = sigma;
+    o.T = tile;
+    o.Sg = sub_slab;
+    o.nvis_hint = nvis;
+    const wg::Plan p = wg::MakePlan(g, wg::VisExtent{e[0], e[1], e[2], e[3], e[4], e[5]}, o);
+    std::strncpy(plan_json, p.describe().c_str(), plan_len - 1);
+    plan_json[plan_len - 1] = 0;
+  });
+}
+
 int wgx_fft(int n, int nlines, int inverse, float* data) {
   return Guard([&] {
     WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");

@@ -110,8 +110,8 @@ struct PlanOptions {
   long long nvis_hint = 0;     // for the cost model
 };
 
-// Largest number of w planes one gridding sub-slab accumulates in registers.
-constexpr int kMaxSubSlab = 16;
+// Largest number of w planes of one gridding sub-slab (its LDS subgrid).
+constexpr int kMaxSubSlab = 32;
 
 int GoodFFTSize(int n);        // smallest even 2^a 3^b 5^c >= n
 std::vector<int> FFTRadices(int n);  // Stockham stage radices for n

@@ -224,10 +224,20 @@ void Engine::BuildWorkItems(VisSet& vs) {
   vs.item_pos = static_cast<uint32_t*>(arena_.Get(vs.tag + ".item_pos", (n + 1) * 4));
   uint32_t* scr = static_cast<uint32_t*>(arena_.Get("items.scan", ScanScratchWords(n) * 4));
   BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, nullptr, stream_, false);
-  uint32_t total = 0;
-  WG_HIP(hipMemcpyAsync(&total, vs.item_pos + n, 4, hipMemcpyDeviceToHost, stream_));
+  // item counts at the sub-slab boundaries (every ntiles-th scan entry),
+  // gathered on the device and copied with a plain 1-D copy
+  const size_t ntiles = size_t(dp.ntu) * dp.ntv;
+  std::vector<uint32_t> bounds(vs.nsub + 1);
+  uint32_t* dbounds = static_cast<uint32_t*>(arena_.Get("items.bounds", (vs.nsub + 1) * 4));
+  LaunchStridedGather(vs.item_pos, ntiles, vs.nsub + 1, dbounds, stream_);
+  WG_HIP(hipMemcpyAsync(bounds.data(), dbounds, (vs.nsub + 1) * 4, hipMemcpyDeviceToHost,
+                        stream_));
   WG_HIP(hipStreamSynchronize(stream_));
+  const uint32_t total = bounds[vs.nsub];
   vs.n_items = total;
+  vs.max_items_per_sub = 0;
+  for (int g = 0; g < vs.nsub; ++g)
+    vs.max_items_per_sub = std::max<size_t>(vs.max_items_per_sub, bounds[g + 1] - bounds[g]);
   vs.items = static_cast<WorkItem*>(arena_.Get(vs.tag + ".items", (total + 1) * sizeof(WorkItem)));
   if (total) BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, vs.items, stream_, true);
 }
@@ -304,15 +314,17 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
   float2* rbuf = static_cast<float2*>(arena_.Get("rbuf", p.Sf * plane_lines * sizeof(float2)));
   float2* cbuf = static_cast<float2*>(arena_.Get("cbuf", p.Sf * plane_lines * sizeof(float2)));
+  float2* scratch = static_cast<float2*>(
+      arena_.Get("grid.scratch", GridScratchBytes(dp, p.Sg, vs.max_items_per_sub)));
   for (int f0 = 0; f0 < p.P; f0 += p.Sf) {
     const int np = std::min(p.Sf, p.P - f0);
     {
+      // every box cell of the slab's planes is written by the gather pass
       EventTimer tm(timing, stream_, &times_.grid);
-      LaunchFillZero(box, size_t(np) * plane_box * sizeof(float2), stream_);
       for (int g = f0 / p.Sg; g * p.Sg < f0 + np; ++g) {
         const int sub_lo = g * p.Sg;
         const int sub_n = std::min(p.Sg, p.P - sub_lo);
-        LaunchGrid(dp, vs, vsorted, box, g, sub_lo, sub_n, f0, stream_);
+        LaunchGrid(dp, vs, vsorted, box, scratch, g, sub_lo, sub_n, f0, stream_);
       }
     }
     EventTimer tm(timing, stream_, &times_.fft);

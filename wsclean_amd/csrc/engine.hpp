@@ -74,6 +74,7 @@ struct VisSet {
   uint32_t* item_pos = nullptr;  // [nsub * ntiles + 1] scan of item counts
   int nsub = 0;
   size_t n_items = 0;
+  size_t max_items_per_sub = 0;  // largest work-item count of one sub-slab
   float* corr = nullptr;      // [nx][ny] 1 / (Phi_u Phi_v Phi_w) with sign
   std::string corr_key;       // parameters the correction was built for
 };
@@ -141,9 +142,13 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
                    hipStream_t s);
 
 // ---- gridding kernels (grid.hip) ----
+// Grids one sub-slab into the box planes [sub_lo, sub_lo + sub_n) (every
+// cell of those planes is written); scratch holds one subgrid per work item
+// of the sub-slab (GridScratchBytes).
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                float2* box, int sub, int sub_lo, int sub_n, int slab_lo,
-                hipStream_t s);
+                float2* box, float2* scratch, int sub, int sub_lo, int sub_n,
+                int slab_lo, hipStream_t s);
+size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub);
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
                   float2* acc, int sub, int sub_lo, int sub_n, int slab_lo,
                   hipStream_t s);
@@ -173,6 +178,9 @@ void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+// out[k] = in[k * stride] for k in [0, count)
+void LaunchStridedGather(const uint32_t* in, size_t stride, size_t count, uint32_t* out,
+                         hipStream_t s);
 void LaunchSynth(const double* uvw, size_t n_rows, size_t n_chan, const double* scale,
                  const double* src, int n_src, double noise, uint64_t seed, float2* vis,
                  hipStream_t s);

@@ -3,12 +3,16 @@
 // A work item is one uv tile (T x T first-cells) and a range of its sorted
 // visibilities that touch the planes [sub_lo, sub_lo + sub_n) of one
 // gridding sub-slab.  Its footprint in the grid is E x E cells per plane
-// (E = T + W - 1).  Gridding accumulates every visibility's W x W x W
-// kernel footprint into the LDS subgrid with LDS float atomics and flushes
-// the non-zero cells to the HBM plane box with global float atomics
-// (neighbouring tiles overlap by W - 1 cells).  Degridding stages the same
-// subgrid from HBM into LDS and gathers per visibility in registers.
+// (E = T + W - 1).  Gridding accumulates the visibilities' W x W x W kernel
+// footprints into the LDS subgrid and stores it, with plain coalesced
+// stores, into the item's own scratch slot; a gather pass then forms every
+// box cell as the compensated sum of the slots of the (at most 4) tiles
+// covering it (neighbouring tiles overlap by W - 1 cells), in a fixed order.
+// No global atomics, and the box is deterministic.  Degridding stages the
+// box subgrid from HBM into LDS and gathers per visibility in registers.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device.hpp"
 #include "engine.hpp"
@@ -18,34 +22,41 @@ namespace wg {
 namespace {
 
 constexpr int kGridBlock = 256;
-// Visibilities per work item.  Large, so that flushing the E x E x Sg
-// subgrid with global atomics is amortised over many visibilities.
+// Visibilities per work item: large, so that writing (and gathering) the
+// E x E x Sg subgrid of an item is amortised over many visibilities, and
+// small enough that the dense uv core spreads over many workgroups.
 constexpr uint32_t kChunk = 65536;
 
 // Wave-cooperative gridding.  A wave walks its share of the work item one
 // visibility at a time; its lanes cover the W x W kernel footprint (lane ->
-// column k = lane % W, rows l = lane / W + r * (64 / W)) and accumulate the
-// footprint cells of every plane of the sub-slab in registers.  Visibilities
-// are sorted by exact first cell inside each (tile, first plane) bucket, so
-// long runs share one footprint; the registers go to the LDS subgrid with
-// (collision-free) LDS atomics only when the run's first cell changes.  With
-// the dense uv cores of real arrays, where thousands of visibilities share a
-// cell, this replaces ~2 W^3 contended atomics per visibility by ~3 kernel
-// evaluations and ~2 R Sg FMAs per lane.
-template <int W, int S>  // S: register depth in planes (>= sub_n, power of 2)
+// column k = lane % W, rows l = lane / W + r * (64 / W)) and accumulate, in
+// registers, the footprint cells of the W planes p0 .. p0 + W - 1 the
+// visibility touches (run-relative taps, so the work per visibility does
+// not grow with the sub-slab depth).  Visibilities are sorted by exact
+// first cell inside each (tile, first plane) bucket, so long runs share one
+// footprint and one p0; the registers go to the LDS subgrid with
+// collision-free LDS atomics only when the run changes.  With the dense uv
+// cores of real arrays, where thousands of visibilities share a cell, this
+// replaces ~2 W^3 contended atomics per visibility by 3 kernel evaluations
+// and 2 R W FMAs per lane.
+template <int W>
 __global__ void __launch_bounds__(kGridBlock) GridKernel(
     DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
-    const float2* __restrict__ vis, float2* __restrict__ box, int sub_lo, int sub_n,
-    int slab_lo) {
+    const float2* __restrict__ vis, float2* __restrict__ scratch, int sub_lo, int sub_n) {
   constexpr int RPP = 64 / W;                 // footprint rows per pass
   constexpr int R = (W + RPP - 1) / RPP;      // row passes per lane
   extern __shared__ float sub_re[];           // [sub_n][E][E] real, then imaginary
   const int E = dp.T + W - 1;
   const int ncell = sub_n * E * E;
   float* sub_im = sub_re + ncell;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index is uniform within a wave; readfirstlane tells the
+  // compiler so, which keeps the whole visibility loop under scalar control
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int kcol = lane % W, lrow = lane / W;
+  // tap abscissa of the lane's column; lane t (< W) also evaluates the w
+  // weight of tap t (plane p0 + t)
   const float xk = float(kcol - W / 2 + 1);
   const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
   for (uint32_t it = it0 + blockIdx.x; it < it1; it += gridDim.x) {
@@ -59,33 +70,36 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
     const uint32_t per = (n + kGridBlock / 64 - 1) / (kGridBlock / 64);
     const uint32_t w0 = wi.begin + min(n, wave * per);
     const uint32_t w1 = wi.begin + min(n, (wave + 1) * per);
-    float are[R][S], aim[R][S];
+    float are[R][W], aim[R][W];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int q = 0; q < S; ++q) are[r][q] = aim[r][q] = 0.f;
-    int cur = -1;            // packed (du, dv, p0) of the run, wave-uniform
-    int cur_lo = 0, cur_hi = 0;
+      for (int t = 0; t < W; ++t) are[r][t] = aim[r][t] = 0.f;
+    // packed (du, dv, p0) of the current run, wave-uniform (SGPR), so run
+    // changes are scalar branches
+    int cur = -1;
+    const bool lane_ok = lrow < RPP;
 
-    auto flush = [&]() {
-      const int du = cur & 255, dv = (cur >> 8) & 255;
-#pragma unroll
-      for (int q = 0; q < S; ++q) {
-        if (q >= cur_lo && q < cur_hi) {
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int l = lrow + r * RPP;
-            if (lrow < RPP && l < W) {
-              const int idx = (q * E + dv + l) * E + du + kcol;
-              atomicAdd(&sub_re[idx], are[r][q]);
-              atomicAdd(&sub_im[idx], aim[r][q]);
-            }
-            are[r][q] = 0.f;
-            aim[r][q] = 0.f;
-          }
-        }
-      }
-    };
+#define WG_FLUSH_RUN()                                                  \
+  do {                                                                  \
+    const int du_ = cur & 255, dv_ = (cur >> 8) & 255;                  \
+    const int q0_ = ((cur >> 16) & 0x7fff) - sub_lo;                    \
+    _Pragma("unroll") for (int t = 0; t < W; ++t) {                     \
+      const int q = q0_ + t;                                            \
+      if (q >= 0 && q < sub_n) {                                        \
+        _Pragma("unroll") for (int r = 0; r < R; ++r) {                 \
+          const int l = lrow + r * RPP;                                 \
+          if (lane_ok && l < W) {                                       \
+            const int idx = (q * E + dv_ + l) * E + du_ + kcol;         \
+            atomicAdd(&sub_re[idx], are[r][t]);                         \
+            atomicAdd(&sub_im[idx], aim[r][t]);                         \
+          }                                                             \
+        }                                                               \
+      }                                                                 \
+    }                                                                   \
+    _Pragma("unroll") for (int r = 0; r < R; ++r)                       \
+      _Pragma("unroll") for (int t = 0; t < W; ++t) are[r][t] = aim[r][t] = 0.f; \
+  } while (0)
 
     for (uint32_t base = w0; base < w1; base += 64) {
       const uint32_t idx = base + lane;
@@ -100,23 +114,17 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
         const float fu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.x), j));
         const float fv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.y), j));
         const float fw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.z), j));
-        const int pk = __builtin_amdgcn_readlane(__float_as_int(c.w), j) & 0x7fffffff;
+        const int pk = __builtin_amdgcn_readfirstlane(
+            __builtin_amdgcn_readlane(__float_as_int(c.w), j) & 0x7fffffff);
         const float vr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(V.x), j));
         const float vi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(V.y), j));
         if (pk != cur) {
-          if (cur >= 0) flush();
+          if (cur >= 0) WG_FLUSH_RUN();
           cur = pk;
-          const int p0 = (pk >> 16) & 0x7fff;
-          cur_lo = max(p0, sub_lo) - sub_lo;
-          cur_hi = min(p0 + W, sub_lo + sub_n) - sub_lo;
         }
-        const int p0 = (pk >> 16) & 0x7fff;
-        // lane q evaluates the w weight of plane sub_lo + q
-        float kwl = 0.f;
-        {
-          const int t = sub_lo + (lane & (S - 1)) - p0;
-          if (t >= 0 && t < W) kwl = EsKernel((float(t - W / 2 + 1) - fw) * dp.two_over_w, dp.beta2);
-        }
+        // per-tap w weights times the visibility, one tap per lane
+        const float kwl = EsKernel((xk - fw) * dp.two_over_w, dp.beta2);
+        const float wrl = vr * kwl, wil = vi * kwl;
         const float ku = EsKernel((xk - fu) * dp.two_over_w, dp.beta2);
         float kv[R];
 #pragma unroll
@@ -126,33 +134,67 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
                           : 0.f;
         }
 #pragma unroll
-        for (int q = 0; q < S; ++q) {
-          if (q >= cur_lo && q < cur_hi) {
-            const float kw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kwl), q));
-            const float wr = vr * kw, wi_ = vi * kw;
+        for (int t = 0; t < W; ++t) {
+          const float wr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wrl), t));
+          const float wi_ = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wil), t));
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              are[r][q] += wr * kv[r];
-              aim[r][q] += wi_ * kv[r];
-            }
+          for (int r = 0; r < R; ++r) {
+            are[r][t] += wr * kv[r];
+            aim[r][t] += wi_ * kv[r];
           }
         }
       }
     }
-    if (cur >= 0) flush();
+    if (cur >= 0) WG_FLUSH_RUN();
+#undef WG_FLUSH_RUN
     __syncthreads();
-    const int tu = int(wi.tile % uint32_t(dp.ntu)), tv = int(wi.tile / uint32_t(dp.ntu));
-    for (int i = threadIdx.x; i < ncell; i += kGridBlock) {
-      const float2 val = make_float2(sub_re[i], sub_im[i]);
-      if (val.x != 0.f || val.y != 0.f) {
-        const int p = i / (E * E), r = (i / E) % E, cc = i % E;
-        float2* dst = box + (size_t(sub_lo + p - slab_lo) * dp.BV + size_t(tv * dp.T + r)) *
-                                dp.BU + size_t(tu * dp.T + cc);
-        atomicAdd(&dst->x, val.x);
-        atomicAdd(&dst->y, val.y);
+    // the item's partial subgrid goes to its own scratch slot with plain
+    // coalesced stores; GatherBoxKernel sums the slots into the box
+    float2* slot = scratch + size_t(it - it0) * size_t(ncell);
+    for (int i = threadIdx.x; i < ncell; i += kGridBlock)
+      slot[i] = make_float2(sub_re[i], sub_im[i]);
+    __syncthreads();
+  }
+}
+
+// box[p][r][c] for the planes of one sub-slab: the compensated (Kahan) sum
+// of the scratch slots of every work item whose tile footprint covers
+// (r, c), in a fixed order (tiles by index, items by index), so the box is
+// deterministic.  Every box cell of the sub-slab's planes is written.
+__global__ void __launch_bounds__(256) GatherBoxKernel(
+    DevPlan dp, int W, const uint32_t* __restrict__ item_pos, int pos_lo,
+    const float2* __restrict__ scratch, float2* __restrict__ box, int sub_lo, int sub_n,
+    int slab_lo) {
+  const int E = dp.T + W - 1;
+  const size_t plane_cells = size_t(dp.BU) * dp.BV;
+  const size_t n = plane_cells * sub_n;
+  const uint32_t base = item_pos[pos_lo];  // first work item of the sub-slab
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const int q = int(i / plane_cells);
+    const size_t rc = i - size_t(q) * plane_cells;
+    const int r = int(rc / dp.BU), c = int(rc - size_t(r) * dp.BU);
+    // tiles whose footprint [t T, t T + E) contains the cell
+    const int tv0 = max(0, (r - E + dp.T) / dp.T), tv1 = min(dp.ntv - 1, r / dp.T);
+    const int tu0 = max(0, (c - E + dp.T) / dp.T), tu1 = min(dp.ntu - 1, c / dp.T);
+    float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;
+    for (int tv = tv0; tv <= tv1; ++tv) {
+      for (int tu = tu0; tu <= tu1; ++tu) {
+        const int t = tv * dp.ntu + tu;
+        const uint32_t a = item_pos[pos_lo + t], b = item_pos[pos_lo + t + 1];
+        const size_t off = (size_t(q) * E + (r - tv * dp.T)) * E + (c - tu * dp.T);
+        for (uint32_t it = a; it < b; ++it) {
+          const float2 v = scratch[size_t(it - base) * size_t(sub_n) * E * E + off];
+          const float yx = v.x - cx, yy = v.y - cy;
+          const float tx = sx + yx, ty = sy + yy;
+          cx = (tx - sx) - yx;
+          cy = (ty - sy) - yy;
+          sx = tx;
+          sy = ty;
+        }
       }
     }
-    __syncthreads();
+    box[(size_t(sub_lo + q - slab_lo) * dp.BV + r) * dp.BU + c] = make_float2(sx, sy);
   }
 }
 
@@ -305,25 +347,28 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
 }
 
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* box,
-                int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
+                float2* scratch, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
+  const int pos_lo = int(sub * ntiles), pos_hi = int((sub + 1) * ntiles);
   DispatchW(dp.W, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    if (sub_n <= 8) {
-      AllowLds(GridKernel<W, 8>, lds);
-      GridKernel<W, 8><<<GridBlocks(), kGridBlock, lds, s>>>(
-          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord,
-          vis_sorted, box, sub_lo, sub_n, slab_lo);
-    } else {
-      AllowLds(GridKernel<W, 16>, lds);
-      GridKernel<W, 16><<<GridBlocks(), kGridBlock, lds, s>>>(
-          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord,
-          vis_sorted, box, sub_lo, sub_n, slab_lo);
-    }
+    AllowLds(GridKernel<W>, lds);
+    GridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(dp, vs.items, vs.item_pos, pos_lo, pos_hi,
+                                                         vs.coord, vis_sorted, scratch, sub_lo,
+                                                         sub_n);
   });
+  const size_t n = size_t(dp.BU) * dp.BV * sub_n;
+  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 65536));
+  GatherBoxKernel<<<nb, 256, 0, s>>>(dp, dp.W, vs.item_pos, pos_lo, scratch, box, sub_lo, sub_n,
+                                     slab_lo);
   WG_HIP(hipGetLastError());
+}
+
+size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub) {
+  const size_t E = size_t(dp.T + dp.W - 1);
+  return max_items_per_sub * size_t(Sg) * E * E * sizeof(float2);
 }
 
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc, int sub,

@@ -180,7 +180,7 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   }
 
   // ---- occupied uv box ----
-  p.T = opt.T > 0 ? opt.T : 16;
+  p.T = opt.T > 0 ? opt.T : 8;
   WG_CHECK(p.T >= 8 && p.T <= 128, "tile size must be in [8, 128]");
   auto box = [&](double cmin, double cmax, double fac, int n, int& origin,
                  bool& wrap, int& ntiles, int& ext) {
@@ -217,7 +217,13 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   while ((uint64_t(1) << p.key_bits) <= uint64_t(p.nkeys)) ++p.key_bits;
 
   // ---- plane batching ----
-  p.Sg = opt.Sg > 0 ? opt.Sg : 8;
+  // default: as many planes as fit ~64 KiB of LDS subgrid (two workgroups
+  // per CU); deeper sub-slabs re-read each visibility fewer times
+  {
+    const size_t E = size_t(p.T + p.W - 1);
+    const int fit = int((64 * 1024) / (E * E * 8));
+    p.Sg = opt.Sg > 0 ? opt.Sg : std::max(1, fit);
+  }
   p.Sg = std::min(p.Sg, kMaxSubSlab);
   {
     // the gridding subgrid (Sg planes of (T + W - 1)^2 complex) must fit

@@ -208,6 +208,12 @@ __global__ void AccumulateImageKernel(const float* __restrict__ dirty,
     img[i] += double(dirty[i]) * double(corr[i]);
 }
 
+__global__ void StridedGatherKernel(const uint32_t* __restrict__ in, size_t stride,
+                                    size_t count, uint32_t* __restrict__ out) {
+  const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < count) out[k] = in[k * stride];
+}
+
 __global__ void FillZeroKernel(uint32_t* __restrict__ p, size_t nwords) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nwords;
        i += size_t(gridDim.x) * kBlock)
@@ -342,6 +348,13 @@ void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img, size_t n,
                            hipStream_t s) {
   AccumulateImageKernel<<<Blocks(n), kBlock, 0, s>>>(dirty, corr, img, n);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchStridedGather(const uint32_t* in, size_t stride, size_t count, uint32_t* out,
+                         hipStream_t s) {
+  if (!count) return;
+  StridedGatherKernel<<<unsigned((count + 255) / 256), 256, 0, s>>>(in, stride, count, out);
   WG_HIP(hipGetLastError());
 }
 
