@@ -40,7 +40,21 @@ def parse():
     ap.add_argument("--sg", type=int, default=0)
     ap.add_argument("--W", type=int, default=0)
     ap.add_argument("--sigma", type=float, default=0.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the calibration "
+                         "kernel for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}))
 
 
 def make_inputs(n_chan, npix, seed=1002):
@@ -71,6 +85,40 @@ def roofline_bytes(plan, n_vis, n_rows, nx, ny):
     inv = 5 * G + 16.0 * nx * ny
     pred = 6 * G + 4.0 * nx * ny
     return 2 * b_vis + P * (inv + pred)
+
+
+def kernel_roofline(st, plan):
+    """Roofline of the dominant kernel, the ES-kernel gridding kernel
+    (GridKernel<W>), from its own HIP events on the engine stream.
+
+    Algorithmic bytes per launch = 24 B per visibility pass (float4
+    coordinates + complex64 visibility read once per sub-slab it touches)
+    + 8 B x sub_n x E^2 per work item (its subgrid written once); summed by
+    the engine over the timed launches.  traffic = measured HBM bytes per
+    launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration) when a counter summary for this kernel and plan is
+    committed, else None."""
+    n = st["grid_launches"]
+    if not n or st["grid_kernel_ms"] <= 0:
+        return None
+    per_launch = st["grid_bytes"] / n
+    t = st["grid_kernel_ms"] / n / 1e3
+    achieved = per_launch / t / 1e9
+    out = {"bound": "hbm", "kernel": "GridKernel<%d>" % plan["W"],
+           "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+           "bytes_per_launch": per_launch, "launch_ms": t * 1e3,
+           "launches": int(n),
+           "degrid": {"kernel": "DegridKernel<%d>" % plan["W"],
+                      "achieved": st["degrid_bytes"] / max(st["degrid_kernel_ms"], 1e-9) / 1e6,
+                      "launch_ms": st["degrid_kernel_ms"] / max(st["degrid_launches"], 1)}}
+    path = os.path.join(ROOT, "profiles", "gridkernel_8192_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            tr = json.load(f)
+        if tr.get("plan") == {k: plan[k] for k in tr.get("plan", {})}:
+            out["traffic"] = tr["bytes_per_dispatch"]
+    return out
 
 
 def cpu_baseline(uvw_h, freq, dx, src, npix, plan, max_seconds):
@@ -111,6 +159,9 @@ def main():
         print("bench.py measures one GPU: every box here has one MI355X and this build "
               "has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import torch
 
     from wsclean_amd import WGriddingGridder_Simple, _lib
@@ -164,7 +215,6 @@ def main():
     value = n_vis / (ms / 1e3)
     ctm = roofline_bytes(plan, n_vis, n_rows, n, n)
     stages = {k: v / args.steps for k, v in st.items()}
-    dominant = max(("grid", "fft", "degrid", "sort"), key=lambda k: stages[k])
     line = {
         "metric": "vis/sec grid+degrid at eps=1e-5, 8192^2 w-gridding; p50/p99 request latency",
         "value": value,
@@ -187,7 +237,7 @@ def main():
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "stages_ms": stages,
         "ctm_roofline": {"bytes": ctm, "frac": ctm / (ms / 1e3) / HBM_PEAK},
-        "roofline": None,
+        "roofline": kernel_roofline(st, plan),
         "cpu_baseline": None,
         "gen_s": t_gen,
     }

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <algorithm>
 #include <mutex>
 #include <string>
 
@@ -317,16 +318,19 @@ int wg_set_timing(wg_handle* h, int enabled) {
   });
 }
 
-int wg_stage_times(wg_handle* h, double* out6, int reset) {
+int wg_stage_times(wg_handle* h, double* out12, int reset) {
   return Guard([&] {
-    WG_CHECK(h && out6, "null argument");
+    WG_CHECK(h && out12, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    // every entry point synchronises its stream before returning, so all
+    // recorded events have completed
+    h->eng->CollectTimes();
     wg::StageTimes& t = h->eng->Times();
-    out6[0] = t.plan;
-    out6[1] = t.sort;
-    out6[2] = t.grid;
-    out6[3] = t.fft;
-    out6[4] = t.degrid;
-    out6[5] = t.plan + t.sort + t.grid + t.fft + t.degrid;
+    const double v[12] = {t.plan, t.sort, t.grid, t.fft, t.degrid,
+                          t.plan + t.sort + t.grid + t.fft + t.degrid,
+                          t.grid_kernel_ms, t.grid_launches, t.grid_bytes,
+                          t.degrid_kernel_ms, t.degrid_launches, t.degrid_bytes};
+    std::copy(v, v + 12, out12);
     if (reset) t = wg::StageTimes{};
   });
 }

@@ -15,8 +15,8 @@
 namespace wg {
 
 void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32_t* counts,
-                uint32_t* pos, uint32_t* scan_scratch, WorkItem* items, hipStream_t s,
-                bool emit);
+                uint32_t* pos, uint32_t* scan_scratch, WorkItem* items,
+                unsigned long long* sub_vis, hipStream_t s, bool emit);
 
 DevPlan ToDevPlan(const Plan& p) {
   DevPlan d{};
@@ -125,33 +125,41 @@ const float2* Engine::Twiddles(int n) {
 }
 
 namespace {
+// Records an event pair around a scope on the engine stream; the elapsed
+// time is accumulated into *dst by Engine::CollectTimes().
 struct EventTimer {
-  hipEvent_t a = nullptr, b = nullptr;
-  hipStream_t s;
-  bool on;
+  Engine* e;
   double* dst;
-  EventTimer(bool enabled, hipStream_t st, double* out) : s(st), on(enabled), dst(out) {
-    if (!on) return;
+  hipEvent_t a = nullptr;
+  EventTimer(Engine& eng, double* out) : e(&eng), dst(out) {
+    if (!e->timing) return;
     (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, s);
+    (void)hipEventRecord(a, e->Stream());
   }
   ~EventTimer() {
-    if (!on) return;
-    (void)hipEventRecord(b, s);
-    (void)hipEventSynchronize(b);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    *dst += ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (!e->timing || !a) return;
+    hipEvent_t b = nullptr;
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(b, e->Stream());
+    e->pending_.push_back(PendingTime{dst, a, b});
   }
 };
 }  // namespace
 
+void Engine::CollectTimes() {
+  for (PendingTime& t : pending_) {
+    float ms = 0;
+    if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess)
+      *t.dst += ms;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  pending_.clear();
+}
+
 void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n_chan,
                      const double* uvw_dev, const double* freq_host, const float2* vis_dev) {
-  EventTimer tm(timing, stream_, &times_.plan);
+  EventTimer tm(*this, &times_.plan);
   WG_CHECK(n_chan > 0, "n_chan must be positive");
   const size_t n_vis = n_rows * n_chan;
   WG_CHECK(n_vis < 0xFFFFFFF0ull, "at most 2^32 - 16 visibilities per call");
@@ -223,23 +231,35 @@ void Engine::BuildWorkItems(VisSet& vs) {
   uint32_t* counts = static_cast<uint32_t*>(arena_.Get("items.counts", (n + 1) * 4));
   vs.item_pos = static_cast<uint32_t*>(arena_.Get(vs.tag + ".item_pos", (n + 1) * 4));
   uint32_t* scr = static_cast<uint32_t*>(arena_.Get("items.scan", ScanScratchWords(n) * 4));
-  BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, nullptr, stream_, false);
+  // visibility passes per sub-slab (for the kernels' byte accounting)
+  unsigned long long* dvisp =
+      static_cast<unsigned long long*>(arena_.Get("items.visp", vs.nsub * 8));
+  LaunchFillZero(dvisp, vs.nsub * 8, stream_);
+  BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, nullptr, dvisp, stream_, false);
   // item counts at the sub-slab boundaries (every ntiles-th scan entry),
   // gathered on the device and copied with a plain 1-D copy
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   std::vector<uint32_t> bounds(vs.nsub + 1);
+  std::vector<unsigned long long> visp(vs.nsub);
   uint32_t* dbounds = static_cast<uint32_t*>(arena_.Get("items.bounds", (vs.nsub + 1) * 4));
   LaunchStridedGather(vs.item_pos, ntiles, vs.nsub + 1, dbounds, stream_);
   WG_HIP(hipMemcpyAsync(bounds.data(), dbounds, (vs.nsub + 1) * 4, hipMemcpyDeviceToHost,
                         stream_));
+  WG_HIP(hipMemcpyAsync(visp.data(), dvisp, vs.nsub * 8, hipMemcpyDeviceToHost, stream_));
   WG_HIP(hipStreamSynchronize(stream_));
+  vs.sub_vis.assign(visp.begin(), visp.end());
   const uint32_t total = bounds[vs.nsub];
   vs.n_items = total;
   vs.max_items_per_sub = 0;
-  for (int g = 0; g < vs.nsub; ++g)
-    vs.max_items_per_sub = std::max<size_t>(vs.max_items_per_sub, bounds[g + 1] - bounds[g]);
+  vs.sub_items.resize(vs.nsub);
+  for (int g = 0; g < vs.nsub; ++g) {
+    vs.sub_items[g] = bounds[g + 1] - bounds[g];
+    vs.max_items_per_sub = std::max<size_t>(vs.max_items_per_sub, vs.sub_items[g]);
+  }
   vs.items = static_cast<WorkItem*>(arena_.Get(vs.tag + ".items", (total + 1) * sizeof(WorkItem)));
-  if (total) BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, vs.items, stream_, true);
+  if (total)
+    BuildItems(dp, vs.off, Sg, vs.nsub, counts, vs.item_pos, scr, vs.items, nullptr, stream_,
+               true);
 }
 
 void Engine::BuildCorrection(VisSet& vs) {
@@ -299,7 +319,7 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   if (vs.n_valid == 0) return;
   float2* vsorted = nullptr;
   {
-    EventTimer tm(timing, stream_, &times_.sort);
+    EventTimer tm(*this, &times_.sort);
     vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
     LaunchGatherVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.n_valid, vis_dev, vsorted,
                     stream_);
@@ -320,14 +340,26 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
     const int np = std::min(p.Sf, p.P - f0);
     {
       // every box cell of the slab's planes is written by the gather pass
-      EventTimer tm(timing, stream_, &times_.grid);
+      EventTimer tm(*this, &times_.grid);
+      const size_t E = size_t(dp.T + dp.W - 1);
       for (int g = f0 / p.Sg; g * p.Sg < f0 + np; ++g) {
         const int sub_lo = g * p.Sg;
         const int sub_n = std::min(p.Sg, p.P - sub_lo);
-        LaunchGrid(dp, vs, vsorted, box, scratch, g, sub_lo, sub_n, f0, stream_);
+        {
+          EventTimer tk(*this, &times_.grid_kernel_ms);
+          LaunchGrid(dp, vs, vsorted, scratch, g, sub_lo, sub_n, stream_);
+        }
+        if (timing) {
+          // coordinates (16 B) and visibility (8 B) per pass, one subgrid
+          // store per work item
+          times_.grid_launches += 1;
+          times_.grid_bytes +=
+              24.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
+        }
+        LaunchGatherBox(dp, vs, box, scratch, g, sub_lo, sub_n, f0, stream_);
       }
     }
-    EventTimer tm(timing, stream_, &times_.fft);
+    EventTimer tm(*this, &times_.fft);
     LaunchRowFFTInvert(dp, fu, box, rbuf, np, stream_);
     LaunchTranspose(rbuf, cbuf, dp.BV, dp.nx, np, stream_);
     LaunchColFFTInvert(dp, fv, cbuf, dirty, f0, np, stream_);
@@ -351,16 +383,27 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
   for (int f0 = 0; f0 < p.P; f0 += p.Sf) {
     const int np = std::min(p.Sf, p.P - f0);
     {
-      EventTimer tm(timing, stream_, &times_.fft);
+      EventTimer tm(*this, &times_.fft);
       LaunchColFFTPredict(dp, fv, img_dev, vs.corr, cbuf, f0, np, stream_);
       LaunchTranspose(cbuf, rbuf, dp.nx, dp.BV, np, stream_);
       LaunchRowFFTPredict(dp, fu, rbuf, box, np, stream_);
     }
-    EventTimer tm(timing, stream_, &times_.degrid);
+    EventTimer tm(*this, &times_.degrid);
+    const size_t E = size_t(dp.T + dp.W - 1);
     for (int g = f0 / p.Sg; g * p.Sg < f0 + np; ++g) {
       const int sub_lo = g * p.Sg;
       const int sub_n = std::min(p.Sg, p.P - sub_lo);
-      LaunchDegrid(dp, vs, box, acc, g, sub_lo, sub_n, f0, stream_);
+      {
+        EventTimer tk(*this, &times_.degrid_kernel_ms);
+        LaunchDegrid(dp, vs, box, acc, g, sub_lo, sub_n, f0, stream_);
+      }
+      if (timing) {
+        // coordinates (16 B) read and accumulator (8 B) read + written per
+        // pass, one subgrid load per work item
+        times_.degrid_launches += 1;
+        times_.degrid_bytes +=
+            32.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
+      }
     }
   }
   LaunchScatterVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.coord, vs.n_valid, acc,

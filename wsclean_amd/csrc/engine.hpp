@@ -54,6 +54,18 @@ class DeviceArena {
 // Per-call timing of the pipeline stages (ms, measured with HIP events).
 struct StageTimes {
   double plan = 0, sort = 0, grid = 0, fft = 0, degrid = 0, total = 0;
+  // the gridding and degridding kernels alone: time, launches and the
+  // algorithmic HBM bytes they move (coordinates + visibilities read,
+  // subgrids written / read)
+  double grid_kernel_ms = 0, grid_launches = 0, grid_bytes = 0;
+  double degrid_kernel_ms = 0, degrid_launches = 0, degrid_bytes = 0;
+};
+
+// HIP events recorded on the engine stream; read back only after the
+// stream has been synchronised, so timing adds no synchronisation.
+struct PendingTime {
+  double* dst;
+  hipEvent_t a, b;
 };
 
 // Visibility set prepared on the device: plan, sort and sorted coordinates.
@@ -73,6 +85,8 @@ struct VisSet {
   WorkItem* items = nullptr;  // work items of all gridding sub-slabs
   uint32_t* item_pos = nullptr;  // [nsub * ntiles + 1] scan of item counts
   int nsub = 0;
+  std::vector<double> sub_vis;     // visibility passes of each sub-slab
+  std::vector<uint32_t> sub_items;  // work items of each sub-slab
   size_t n_items = 0;
   size_t max_items_per_sub = 0;  // largest work-item count of one sub-slab
   float* corr = nullptr;      // [nx][ny] 1 / (Phi_u Phi_v Phi_w) with sign
@@ -103,6 +117,9 @@ class Engine {
 
   DeviceArena& Arena() { return arena_; }
   StageTimes& Times() { return times_; }
+  // Accumulates the recorded event pairs into Times(); the caller must have
+  // synchronised the stream (every public entry point does).
+  void CollectTimes();
   bool timing = false;
 
  private:
@@ -116,6 +133,9 @@ class Engine {
   DeviceArena arena_;
   std::map<int, float2*> twiddles_;
   StageTimes times_;
+
+ public:
+  std::vector<PendingTime> pending_;
 };
 
 // ---- FFT / plane kernels (fft.hip) ----
@@ -142,12 +162,14 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
                    hipStream_t s);
 
 // ---- gridding kernels (grid.hip) ----
-// Grids one sub-slab into the box planes [sub_lo, sub_lo + sub_n) (every
-// cell of those planes is written); scratch holds one subgrid per work item
-// of the sub-slab (GridScratchBytes).
+// Grids one sub-slab: every work item writes its subgrid of the planes
+// [sub_lo, sub_lo + sub_n) into its scratch slot (GridScratchBytes); then
+// LaunchGatherBox writes every box cell of those planes.
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                float2* box, float2* scratch, int sub, int sub_lo, int sub_n,
-                int slab_lo, hipStream_t s);
+                float2* scratch, int sub, int sub_lo, int sub_n, hipStream_t s);
+void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
+                     const float2* scratch, int sub, int sub_lo, int sub_n,
+                     int slab_lo, hipStream_t s);
 size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub);
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
                   float2* acc, int sub, int sub_lo, int sub_n, int slab_lo,

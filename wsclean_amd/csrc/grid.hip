@@ -256,20 +256,34 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
   }
 }
 
-// Work-item counts for every (sub-slab, tile).
+// Work-item counts for every (sub-slab, tile); with `sub_vis` non-null also
+// the number of visibility passes of every sub-slab (sum over its tiles).
 __global__ void ItemCountKernel(DevPlan dp, const uint32_t* __restrict__ off, int Sg, int nsub,
-                                uint32_t* __restrict__ counts) {
+                                uint32_t* __restrict__ counts,
+                                unsigned long long* __restrict__ sub_vis) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= ntiles * nsub) return;
-  const int g = int(idx / ntiles);
-  const size_t t = idx % ntiles;
-  const int sub_lo = g * Sg;
-  const int lo = max(0, sub_lo - dp.W + 1);
-  const int hi = min(dp.P0 - 1, sub_lo + Sg - 1);
+  const bool in = idx < ntiles * nsub;
+  const int g = in ? int(idx / ntiles) : nsub;
   uint32_t n = 0;
-  if (lo <= hi) n = off[t * dp.P0 + hi + 1] - off[t * dp.P0 + lo];
-  counts[idx] = (n + kChunk - 1) / kChunk;
+  if (in) {
+    const size_t t = idx % ntiles;
+    const int sub_lo = g * Sg;
+    const int lo = max(0, sub_lo - dp.W + 1);
+    const int hi = min(dp.P0 - 1, sub_lo + Sg - 1);
+    if (lo <= hi) n = off[t * dp.P0 + hi + 1] - off[t * dp.P0 + lo];
+    counts[idx] = (n + kChunk - 1) / kChunk;
+  }
+  if (!sub_vis) return;
+  // one atomic per wave when the whole wave lies in one sub-slab
+  const int g0 = __builtin_amdgcn_readfirstlane(g);
+  if (__all(g == g0)) {
+    unsigned long long s = n;
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    if ((threadIdx.x & 63) == 0 && g0 < nsub) atomicAdd(&sub_vis[g0], s);
+  } else if (in) {
+    atomicAdd(&sub_vis[g], (unsigned long long)n);
+  }
 }
 
 __global__ void ItemEmitKernel(DevPlan dp, const uint32_t* __restrict__ off, int Sg, int nsub,
@@ -333,12 +347,12 @@ size_t GridLdsBytes(const DevPlan& dp, int sub_n) {
 }
 
 void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32_t* counts,
-                uint32_t* pos, uint32_t* scan_scratch, WorkItem* items, hipStream_t s,
-                bool emit) {
+                uint32_t* pos, uint32_t* scan_scratch, WorkItem* items,
+                unsigned long long* sub_vis, hipStream_t s, bool emit) {
   const size_t n = size_t(dp.ntu) * dp.ntv * nsub;
   const unsigned nb = unsigned((n + 255) / 256);
   if (!emit) {
-    ItemCountKernel<<<nb, 256, 0, s>>>(dp, off, Sg, nsub, counts);
+    ItemCountKernel<<<nb, 256, 0, s>>>(dp, off, Sg, nsub, counts, sub_vis);
     DeviceExclusiveScan(counts, pos, n, scan_scratch, s, pos + n);
   } else {
     ItemEmitKernel<<<nb, 256, 0, s>>>(dp, off, Sg, nsub, pos, items);
@@ -346,8 +360,8 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
   WG_HIP(hipGetLastError());
 }
 
-void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* box,
-                float2* scratch, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
+void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* scratch,
+                int sub, int sub_lo, int sub_n, hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
@@ -359,6 +373,13 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
                                                          vs.coord, vis_sorted, scratch, sub_lo,
                                                          sub_n);
   });
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box, const float2* scratch,
+                     int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
+  const size_t ntiles = size_t(dp.ntu) * dp.ntv;
+  const int pos_lo = int(sub * ntiles);
   const size_t n = size_t(dp.BU) * dp.BV * sub_n;
   const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 65536));
   GatherBoxKernel<<<nb, 256, 0, s>>>(dp, dp.W, vs.item_pos, pos_lo, scratch, box, sub_lo, sub_n,
