@@ -33,3 +33,28 @@ def test_params_struct_layout():
 def test_library_is_gfx950():
     so = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in so
+
+
+def test_cpp_wrapper_compiles_and_links(tmp_path):
+    # include/wgrid_hip.hpp (the WGriddingGridderBase-shaped C++ wrapper)
+    # compiles against the C header and links against the library
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "wgrid_hip.hpp"\n'
+        "#include <type_traits>\n"
+        "static_assert(!std::is_copy_constructible<wgrid_hip::Gridder>::value, \"\");\n"
+        "int run(size_t n) {\n"
+        "  wgrid_hip::Gridder g(n, n, n, n, 1e-5, 1e-5, 0.0, 0.0, 1, 1e-5);\n"
+        "  g.InitializeInversion();\n"
+        "  std::vector<float> img = g.RealImage();\n"
+        "  return int(img.size() + g.ConstantMemoryUsage());\n"
+        "}\n"
+        "int main(int argc, char**) { return argc > 5 ? run(64) : 0; }\n")
+    exe = tmp_path / "use"
+    lib_dir = os.path.dirname(_lib.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        str(src), "-o", str(exe), "-L", lib_dir, "-lwgrid_hip",
+                        "-Wl,--allow-shlib-undefined"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
