@@ -108,6 +108,36 @@ def test_invert_wide_field_many_w_planes(oracle):
     assert rel_l2(img[ys, xs], ref) < 1e-4
 
 
+def dense_core_problem():
+    # compact 300 m array, 40 antennas, 240 x 60 s, 16 channels: 3.0e6
+    # visibilities in a few hundred uv cells, so work items are full (65536
+    # visibilities) and runs of one cell are thousands long
+    return synth.small_problem(seed=23, n_ant=40, n_times=240, n_chan=16, npix=256,
+                               pixel_arcsec=20.0, radius_m=300.0, n_src=12, dt_s=60.0)
+
+
+def test_invert_dense_core(oracle):
+    # the fixed-point subgrid of the matrix-core gridder at its largest
+    # per-item range; tolerance epsilon = 1e-5 on 400 sampled pixels
+    p = dense_core_problem()
+    img, g = invert(p, 1e-5)
+    assert g.plan()["W"] <= 8
+    rng = np.random.default_rng(1)
+    xs = rng.integers(0, 256, 400)
+    ys = rng.integers(0, 256, 400)
+    ref = oracle.dft_invert_pixels(256, 256, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
+                                   p["vis"], xs, ys)
+    assert rel_l2(img[ys, xs], ref) < 1e-5
+
+
+def test_invert_is_deterministic():
+    # integer subgrid accumulation + fixed-order gather: bitwise repeatable
+    p = dense_core_problem()
+    a, _ = invert(p, 1e-5)
+    b, _ = invert(p, 1e-5)
+    np.testing.assert_array_equal(a, b)
+
+
 def test_invert_wrapping_uv(oracle):
     # pixels too coarse for the longest baselines: the uv box wraps around
     p = synth.small_problem(seed=9, n_ant=12, n_times=10, n_chan=2, npix=64,

@@ -59,6 +59,10 @@ def check_plan_bounds(plan, uvw, freq, vis=None):
     assert plan["nu"] <= MAX_FFT and plan["nv"] <= MAX_FFT
     assert Sf % Sg == 0 and 1 <= Sg <= 32
     assert Sg * E * E * 8 <= 160 * 1024                      # LDS subgrid
+    if W <= 8:
+        # matrix-core gridding: 16-byte aligned subgrid + 8 waves x 2 views
+        # x 2 x 64 staged float4 entries
+        assert (Sg * E * E * 8 + 15) // 16 * 16 + 8 * 2 * 128 * 16 <= 160 * 1024
     assert BU == ntu * T + W - 1 and BV == ntv * T + W - 1
 
     keys, valid = oracle.bucket_keys(plan, uvw, freq, vis)
@@ -156,6 +160,9 @@ PROBLEMS = [
           radius_m=6000.0, dec_deg=-70.0, dt_s=600.0), 512, 1e-5, (0, 0)),
     (dict(seed=8, n_ant=20, n_times=30, n_chan=2, npix=384, pixel_arcsec=30.0,
           radius_m=5000.0), 384, 1e-4, (-0.01, 0.02)),
+    # dense core: full work items, runs thousands long
+    (dict(seed=23, n_ant=40, n_times=240, n_chan=16, npix=256, pixel_arcsec=20.0,
+          radius_m=300.0, n_src=12, dt_s=60.0), 256, 1e-5, (0, 0)),
 ]
 
 

@@ -26,6 +26,9 @@ COMMON = [
     "-munsafe-fp-atomics",
     # keep the compiler from forming packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: the gridding loop carries them
+    # across branches, and the AGPR form costs 32 copies per group
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-result",
 ]
 

@@ -112,6 +112,11 @@ struct PlanOptions {
 
 // Largest number of w planes of one gridding sub-slab (its LDS subgrid).
 constexpr int kMaxSubSlab = 32;
+// LDS of a gridding workgroup beyond its subgrid (per-wave coordinate
+// staging of the matrix-core kernel); the subgrid gets the rest of 160 KiB.
+constexpr int kGridLdsReserve = 33 * 1024;
+// Default LDS per gridding workgroup, so that two are resident per CU.
+constexpr int kGridLdsTarget = 80 * 1024;
 
 int GoodFFTSize(int n);        // smallest even 2^a 3^b 5^c >= n
 std::vector<int> FFTRadices(int n);  // Stockham stage radices for n

@@ -318,11 +318,14 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   const Plan& p = vs.plan;
   if (vs.n_valid == 0) return;
   float2* vsorted = nullptr;
+  // [0] work-item queue of the gridding kernel, [1] max |component| of the
+  // sorted visibilities (sets the gridding's fixed-point scale)
+  uint32_t* words = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t)));
   {
     EventTimer tm(*this, &times_.sort);
     vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
     LaunchGatherVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.n_valid, vis_dev, vsorted,
-                    stream_);
+                    words + 1, stream_);
   }
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
@@ -347,7 +350,8 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
         const int sub_n = std::min(p.Sg, p.P - sub_lo);
         {
           EventTimer tk(*this, &times_.grid_kernel_ms);
-          LaunchGrid(dp, vs, vsorted, scratch, g, sub_lo, sub_n, stream_);
+          LaunchGrid(dp, vs, vsorted, scratch, words, reinterpret_cast<const float*>(words + 1),
+                     g, sub_lo, sub_n, stream_);
         }
         if (timing) {
           // coordinates (16 B) and visibility (8 B) per pass, one subgrid

@@ -165,8 +165,12 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
 // Grids one sub-slab: every work item writes its subgrid of the planes
 // [sub_lo, sub_lo + sub_n) into its scratch slot (GridScratchBytes); then
 // LaunchGatherBox writes every box cell of those planes.
+// `queue` is one device word, the work-item counter of the persistent
+// matrix-core kernel (cleared by the launch); `vmax` points to the largest
+// |component| of vis_sorted (from LaunchGatherVis).
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                float2* scratch, int sub, int sub_lo, int sub_n, hipStream_t s);
+                float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
+                int sub_n, hipStream_t s);
 void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
                      const float2* scratch, int sub, int sub_lo, int sub_n,
                      int slab_lo, hipStream_t s);
@@ -185,9 +189,12 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
 void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale,
                   size_t n_chan, const uint32_t* perm, size_t n_valid,
                   float4* coord, hipStream_t s);
+// Also writes the bits of max |Re|, |Im| of the gathered visibilities to
+// *vmax_bits (a non-negative float; the launch clears the word first).
 void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale,
                      size_t n_chan, const uint32_t* perm, size_t n_valid,
-                     const float2* vis, float2* vis_sorted, hipStream_t s);
+                     const float2* vis, float2* vis_sorted, uint32_t* vmax_bits,
+                     hipStream_t s);
 void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale,
                       size_t n_chan, const uint32_t* perm, const float4* coord,
                       size_t n_valid, const float2* acc, float2* vis_out,

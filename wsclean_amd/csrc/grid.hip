@@ -157,6 +157,295 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core gridding for W <= 8.
+//
+// Inside a run (visibilities with the same first cell and first plane) the
+// contribution to the W x W x W footprint is a sum of rank-1 terms,
+//   G[t][l][k] = sum_j V_j kw_j[t] kv_j[l] ku_j[k],
+// i.e. for a group of 4 visibilities and each plane pair m (t = 2m, 2m+1)
+// one FP32 MFMA 16x16x4:  D_m[(l,c)][(k,t')] += A[(l,c)][j] B_m[j][(k,t')]
+// with A = kv_j[l] (Re V_j | Im V_j) and B_m = ku_j[k] kw_j[2m + t'].
+// Operand layout of v_mfma_f32_16x16x4f32: lane L holds A[L % 16][L / 16],
+// B[L / 16][L % 16] and D[4 (L / 16) + r][L % 16], r = 0..3.  So in a group
+// lane L works on visibility j = L / 16 and on tap L % 8 of u and v; each
+// lane evaluates two kernel values per group (ku or kv, then one kw) and
+// exchanges the rest with DPP: row_ror:8 swaps the u/v halves of a row and
+// a quad broadcast delivers kw[2m + t'] (lane q of each quad evaluates
+// tap 2q + t').  Taps >= W are zero, so W < 8 uses the same layout.
+// Runs end at group granularity: a group only takes the leading visibilities
+// that share the current run; the run's 4 D blocks go to the LDS subgrid
+// with LDS atomics when the run changes.  Coordinates are staged per wave
+// through a 2 x 64-visibility LDS ring, loaded one batch ahead, and each
+// staged batch carries a 64-bit mask of the visibilities that start a run.
+constexpr int kMfmaWaves = 8;
+constexpr int kStage = 64;
+
+// One staged batch: visibilities [first, first + 64) of the wave's range
+// [.., end), one per lane, zero past the end.
+__device__ __forceinline__ void LoadStage(const float4* __restrict__ coord,
+                                          const float2* __restrict__ vis, uint32_t first,
+                                          uint32_t end, int lane, float4& c, float2& v) {
+  c = make_float4(0.f, 0.f, 0.f, 0.f);
+  v = make_float2(0.f, 0.f);
+  if (first + uint32_t(lane) < end) {
+    c = coord[first + lane];
+    v = vis[first + lane];
+  }
+}
+
+// Run key (packed cell and first plane) of the lane's staged visibility;
+// -2 past the end of the range, so the end starts a "run".
+__device__ __forceinline__ int StageKey(uint32_t first, uint32_t end, int lane, const float4& c) {
+  return first + uint32_t(lane) < end ? (__float_as_int(c.w) & 0x7fffffff) : -2;
+}
+
+// Bit i set: staged visibility i starts a run (its key differs from that of
+// the visibility before it; prev_last is the key before lane 0).
+__device__ __forceinline__ uint64_t RunStarts(int key, int prev_last, int lane) {
+  int prev = __shfl_up(key, 1);
+  if (lane == 0) prev = prev_last;
+  return __ballot(key != prev);
+}
+
+// Stores the lane's staged visibility at ring position `slot` in both views.
+__device__ __forceinline__ void StoreStage(float4* ring, int slot, const float4& c,
+                                           const float2& v) {
+  ring[slot] = make_float4(c.x, c.z, v.x, 0.f);
+  ring[2 * kStage + slot] = make_float4(c.y, c.z, v.y, 0.f);
+}
+
+// The three staged values one lane uses (16-byte aligned, one 96-bit read).
+struct alignas(16) Stage3 {
+  float x, y, z;
+};
+__device__ __forceinline__ Stage3 ReadStage(const float4* view, int slot) {
+  return *reinterpret_cast<const Stage3*>(view + slot);
+}
+
+// Orders one wave's LDS stores before its lanes read each other's values.
+__device__ __forceinline__ void WaveLdsSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float DppQuadBcast(float v, int m) {
+  // quad_perm [m, m, m, m]
+  switch (m) {
+    case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xf, 0xf, false));
+    case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xf, 0xf, false));
+    case 2: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xaa, 0xf, 0xf, false));
+    default: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xff, 0xf, 0xf, false));
+  }
+}
+
+__device__ __forceinline__ float DppRowRor8(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, false));
+}
+
+// ES kernel at x = xc - f * c2 (xc = tap abscissa * c2, nc2 = -c2), with the
+// scale folded into FMAs: exp2(b2 (sqrt(1 - x^2) - 1)).
+__device__ __forceinline__ float EsKernelAt(float f, float nc2, float xc, float b2) {
+  const float x = __builtin_fmaf(f, nc2, xc);
+  const float a = fmaxf(__builtin_fmaf(-x, x, 1.0f), 0.0f);
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(__builtin_amdgcn_sqrtf(a), b2, -b2));
+}
+
+template <int W>
+// LDS (subgrid + rings) allows two workgroups, i.e. four waves per SIMD: let
+// the register allocator use the registers that occupancy leaves free
+// instead of serialising the staged loads.
+__global__ void __launch_bounds__(64 * kMfmaWaves)
+    __attribute__((amdgpu_waves_per_eu(4, 4))) GridKernelMfma(
+    DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
+    int pos_lo, int pos_hi, const float4* __restrict__ coord,
+    const float2* __restrict__ vis, float2* __restrict__ scratch, int sub_lo, int sub_n,
+    uint32_t* __restrict__ queue, const float* __restrict__ vmax_ptr) {
+  static_assert(W <= 8 && W % 2 == 0, "matrix-core gridding covers W <= 8");
+  __shared__ uint32_t next_item;
+  // [sub_n][E][E] fixed-point re, im; then the staging rings.  The dynamic
+  // area follows next_item, so its alignment is stated explicitly.
+  extern __shared__ __attribute__((aligned(16))) int grid_fx_lds[];
+  int* sub_re = grid_fx_lds;
+  const int E = dp.T + W - 1;
+  const int ncell = sub_n * E * E;
+  int* sub_im = sub_re + ncell;
+  const float vmax = *vmax_ptr;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // staging rings after the subgrid, 16-byte aligned
+  // per wave two views of 2 x 64 staged visibilities: view 0 = (fu, fw, Re V),
+  // view 1 = (fv, fw, Im V); lane L reads view (L % 16 >= 8) only
+  float4* ring = reinterpret_cast<float4*>(sub_re + ((2 * ncell + 3) & ~3)) +
+                 wave * 2 * (2 * kStage);
+  const int j = lane >> 4;          // visibility of the group
+  const int r16 = lane & 15;
+  const int tap = r16 & 7;          // u tap (B columns) and v tap (A rows)
+  const bool hi = r16 >= 8;         // A: imaginary part; first evaluation: v
+  const int tsig = 2 * (r16 & 3) + (r16 >> 3);  // w tap this lane evaluates
+  const bool tap_ok = tap < W, w_ok = tsig < W;
+  const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
+  const float xc1 = float(tap - W / 2 + 1) * c2;
+  const float xcw = float(tsig - W / 2 + 1) * c2;
+  // flush coordinates of the lane's D entries: D_m[4 (L/16) + r][r16]
+  const int f_c = (lane >> 4) >> 1;                 // row i = 4 (L/16) + r -> c = i / 8
+  const int f_k = r16 & 7, f_tp = r16 >> 3;          // column -> k, t'
+  const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
+  // persistent workgroups take work items from a queue, in item order
+  for (;;) {
+    if (threadIdx.x == 0) next_item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t it = it0 + next_item;
+    if (it >= it1) break;
+    const WorkItem wi = items[it];
+    for (int i = threadIdx.x; i < ncell; i += 64 * kMfmaWaves) {
+      sub_re[i] = 0;
+      sub_im[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = wi.end - wi.begin;
+    // Fixed-point subgrid: every cell component is a sum of |V component| x
+    // kernel weights (<= 1) over at most n visibilities, so |cell| <=
+    // n * vmax <= 2^e and values scaled by 2^(30 - e) fit an int32 with
+    // headroom for the rounding of each flushed term.  Integer LDS atomics
+    // are fast and exact, so the subgrid is also independent of the order
+    // of the flushes.
+    int e = 0;
+    (void)frexpf(fmaxf(float(n) * vmax, 1e-30f), &e);
+    e = max(e, -90);  // all-zero items: keep the scale finite
+    const float fx_scale = (float(n) * vmax < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
+    const float fx_inv = ldexpf(1.0f, e - 30);
+    const uint32_t per = (n + kMfmaWaves - 1) / kMfmaWaves;
+    const uint32_t w0 = wi.begin + min(n, wave * per);
+    const uint32_t w1 = wi.begin + min(n, (wave + 1) * per);
+    mfma_f4 acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    int cur = -1;
+
+#define WG_FLUSH_MFMA()                                                          \
+  do {                                                                           \
+    const int du_ = cur & 255, dv_ = (cur >> 8) & 255;                           \
+    const int q0_ = ((cur >> 16) & 0x7fff) - sub_lo;                             \
+    int* dst_ = f_c ? sub_im : sub_re;                                           \
+    _Pragma("unroll") for (int m = 0; m < 4; ++m) {                              \
+      const int t_ = 2 * m + f_tp;                                               \
+      const int q_ = q0_ + t_;                                                   \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
+        const int l_ = (4 * (lane >> 4) + r) & 7;                                \
+        if (t_ < W && l_ < W && f_k < W && q_ >= 0 && q_ < sub_n)                \
+          atomicAdd(&dst_[(q_ * E + dv_ + l_) * E + du_ + f_k],                  \
+                    __float2int_rn(acc[m][r] * fx_scale));                       \
+        acc[m][r] = 0.f;                                                         \
+      }                                                                          \
+    }                                                                            \
+  } while (0)
+
+    if (w0 < w1) {
+      // Batch b = [w0 + 64 b, w0 + 64 b + 64) lives in ring slot b & 1.  The
+      // group at pos starts in batch b; the ring holds batches b and b + 1,
+      // the registers pc/pv hold batch b + 2 (in flight), and kb0/kb1 and
+      // sm0/sm1 are the keys and run-start masks of batches b and b + 1.
+      // Run lengths then come from scalar bit scans, so the next group's
+      // staged data can be read before this group's matrix ops issue.
+      float4 pc;
+      float2 pv;
+      LoadStage(coord, vis, w0, w1, lane, pc, pv);
+      StoreStage(ring, lane, pc, pv);
+      int kb0 = StageKey(w0, w1, lane, pc);
+      uint64_t sm0 = RunStarts(kb0, -1, lane);
+      LoadStage(coord, vis, w0 + kStage, w1, lane, pc, pv);
+      StoreStage(ring, kStage + lane, pc, pv);
+      int kb1 = StageKey(w0 + kStage, w1, lane, pc);
+      uint64_t sm1 = RunStarts(kb1, __builtin_amdgcn_readlane(kb0, 63), lane);
+      uint32_t bbase = w0, bslot = 0;
+      LoadStage(coord, vis, w0 + 2 * kStage, w1, lane, pc, pv);
+      WaveLdsSync();
+      // Each step takes two groups (visibilities pos .. pos + 7) of one run.
+      uint32_t pos = w0;
+      const float4* view = ring + (hi ? 2 * kStage : 0);
+      // the staged values of the current step live in (e0, e1) or (f0, f1)
+      // alternately, so the next step's reads stay in flight across the
+      // loop edge
+      Stage3 e0 = ReadStage(view, j), e1 = ReadStage(view, 4 + j), f0, f1;
+      auto step = [&](const Stage3& c0, const Stage3& c1, Stage3& n0, Stage3& n1) {
+        const uint32_t o = pos - bbase;  // < 64
+        if ((sm0 >> o) & 1ull) {
+          if (cur >= 0) WG_FLUSH_MFMA();
+          cur = __builtin_amdgcn_readlane(kb0, int(o));
+        }
+        // bit d - 1 of win: visibility pos + d starts a run
+        const uint64_t win = (o < 63 ? (sm0 >> (o + 1)) : 0ull) | (sm1 << (63 - o));
+        const int lead = __builtin_ctzll(win | 0x80ull) + 1;
+        // The next step's visibilities [npos, npos + 8) lie in batches b and
+        // b + 1 (npos - bbase <= 67), which the ring holds and the batch
+        // advance below does not overwrite: read them now, a step ahead.
+        const uint32_t npos = pos + uint32_t(lead);
+        const uint32_t ns = npos - w0 + uint32_t(j);
+        n0 = ReadStage(view, int(ns & uint32_t(2 * kStage - 1)));
+        n1 = ReadStage(view, int((ns + 4) & uint32_t(2 * kStage - 1)));
+        // kernel values: lanes r16 < 8 evaluate ku[tap], r16 >= 8 kv[tap]
+        const float k10 = EsKernelAt(c0.x, nc2, xc1, b2);
+        const float k11 = EsKernelAt(c1.x, nc2, xc1, b2);
+        const float kw0 = w_ok ? EsKernelAt(c0.y, nc2, xcw, b2) : 0.f;
+        const float kw1 = w_ok ? EsKernelAt(c1.y, nc2, xcw, b2) : 0.f;
+        const float k10o = DppRowRor8(k10), k11o = DppRowRor8(k11);
+        // visibilities past the run contribute zero rows and columns
+        const bool take0 = tap_ok && j < lead, take1 = tap_ok && j + 4 < lead;
+        const float a0 = take0 ? (hi ? k10 : k10o) * c0.z : 0.f;
+        const float a1 = take1 ? (hi ? k11 : k11o) * c1.z : 0.f;
+        const float u0 = take0 ? (hi ? k10o : k10) : 0.f;
+        const float u1 = take1 ? (hi ? k11o : k11) : 0.f;
+        float b0[4], b1[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          b0[m] = u0 * DppQuadBcast(kw0, m);
+          b1[m] = u1 * DppQuadBcast(kw1, m);
+        }
+        if (npos - bbase >= uint32_t(kStage) && npos < w1) {
+          // the next group starts in batch b + 1: batch b + 2 replaces b
+          StoreStage(ring, bslot * kStage + lane, pc, pv);
+          const int kb2 = StageKey(bbase + 2 * kStage, w1, lane, pc);
+          const uint64_t sm2 = RunStarts(kb2, __builtin_amdgcn_readlane(kb1, 63), lane);
+          kb0 = kb1;
+          sm0 = sm1;
+          kb1 = kb2;
+          sm1 = sm2;
+          bbase += kStage;
+          bslot ^= 1u;
+          LoadStage(coord, vis, bbase + 2 * kStage, w1, lane, pc, pv);
+          WaveLdsSync();
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0[m], acc[m], 0, 0, 0);
+        if (lead > 4) {
+#pragma unroll
+          for (int m = 0; m < 4; ++m)
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[m], acc[m], 0, 0, 0);
+        }
+        pos = npos;
+      };
+      for (;;) {
+        if (pos >= w1) break;
+        step(e0, e1, f0, f1);
+        if (pos >= w1) break;
+        step(f0, f1, e0, e1);
+      }
+      if (cur >= 0) WG_FLUSH_MFMA();
+    }
+#undef WG_FLUSH_MFMA
+    __syncthreads();
+    float2* slot = scratch + size_t(it - it0) * size_t(ncell);
+    for (int i = threadIdx.x; i < ncell; i += 64 * kMfmaWaves)
+      slot[i] = make_float2(float(sub_re[i]) * fx_inv, float(sub_im[i]) * fx_inv);
+    __syncthreads();
+  }
+}
+
 // box[p][r][c] for the planes of one sub-slab: the compensated (Kahan) sum
 // of the scratch slots of every work item whose tile footprint covers
 // (r, c), in a fixed order (tiles by index, items by index), so the box is
@@ -361,17 +650,38 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
 }
 
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* scratch,
-                int sub, int sub_lo, int sub_n, hipStream_t s) {
+                uint32_t* queue, const float* vmax, int sub, int sub_lo, int sub_n,
+                hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
   const int pos_lo = int(sub * ntiles), pos_hi = int((sub + 1) * ntiles);
   DispatchW(dp.W, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    AllowLds(GridKernel<W>, lds);
-    GridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(dp, vs.items, vs.item_pos, pos_lo, pos_hi,
-                                                         vs.coord, vis_sorted, scratch, sub_lo,
-                                                         sub_n);
+    if constexpr (W <= 8) {
+      // subgrid (16-byte aligned) + per-wave staging rings
+      constexpr size_t ring = size_t(kMfmaWaves) * 2 * (2 * kStage) * sizeof(float4);
+      static_assert(ring + 16 <= size_t(kGridLdsReserve), "staging rings exceed the LDS reserve");
+      const size_t total = ((lds + 15) & ~size_t(15)) + ring;
+      WG_CHECK(total <= 160 * 1024, "gridding LDS exceeds 160 KiB");
+      AllowLds(GridKernelMfma<W>, total);
+      // one persistent workgroup per resident slot; they drain the queue
+      int per_cu = 0, dev = 0, cus = 256;
+      WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(GridKernelMfma<W>), 64 * kMfmaWaves, total));
+      WG_HIP(hipGetDevice(&dev));
+      WG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const int blocks = std::max(1, per_cu) * cus;
+      LaunchFillZero(queue, sizeof(uint32_t), s);
+      GridKernelMfma<W><<<blocks, 64 * kMfmaWaves, total, s>>>(
+          dp, vs.items, vs.item_pos, pos_lo, pos_hi, vs.coord, vis_sorted, scratch, sub_lo, sub_n,
+          queue, vmax);
+    } else {
+      AllowLds(GridKernel<W>, lds);
+      GridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(dp, vs.items, vs.item_pos, pos_lo,
+                                                           pos_hi, vs.coord, vis_sorted, scratch,
+                                                           sub_lo, sub_n);
+    }
   });
   WG_HIP(hipGetLastError());
 }

@@ -217,19 +217,20 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   while ((uint64_t(1) << p.key_bits) <= uint64_t(p.nkeys)) ++p.key_bits;
 
   // ---- plane batching ----
-  // default: as many planes as fit ~64 KiB of LDS subgrid (two workgroups
-  // per CU); deeper sub-slabs re-read each visibility fewer times
+  // default: as many planes as keep a gridding workgroup (subgrid + staging)
+  // within half the LDS of a CU, so two are resident; deeper sub-slabs
+  // re-read each visibility fewer times
   {
     const size_t E = size_t(p.T + p.W - 1);
-    const int fit = int((64 * 1024) / (E * E * 8));
+    const int fit = int((kGridLdsTarget - kGridLdsReserve) / (E * E * 8));
     p.Sg = opt.Sg > 0 ? opt.Sg : std::max(1, fit);
   }
   p.Sg = std::min(p.Sg, kMaxSubSlab);
   {
-    // the gridding subgrid (Sg planes of (T + W - 1)^2 complex) must fit
-    // the 160 KiB of LDS of one CU
+    // the gridding subgrid (Sg planes of (T + W - 1)^2 complex) and the
+    // staging reserve must fit the 160 KiB of LDS of one CU
     const size_t E = size_t(p.T + p.W - 1);
-    const int max_sg = int((160 * 1024) / (E * E * 8));
+    const int max_sg = int((160 * 1024 - kGridLdsReserve) / (E * E * 8));
     WG_CHECK(max_sg >= 1, "tile too large for LDS");
     p.Sg = std::max(1, std::min(p.Sg, max_sg));
   }

@@ -126,7 +126,8 @@ __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L
 __global__ void __launch_bounds__(kBlock) GatherVisKernel(
     DevPlan dp, const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
     const uint32_t* __restrict__ perm, size_t n_valid, const float2* __restrict__ vis,
-    float2* __restrict__ out) {
+    float2* __restrict__ out, uint32_t* __restrict__ vmax_bits) {
+  float vmax = 0.f;  // largest |component| of the gathered visibilities
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
     const size_t idx = perm[i];
@@ -135,8 +136,13 @@ __global__ void __launch_bounds__(kBlock) GatherVisKernel(
     float2 V = vis[idx];
     if (L.flip) V.y = -V.y;
     const float2 ph = ShiftPhasor(dp, L, -1.f);
-    out[i] = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
+    const float2 o = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
+    out[i] = o;
+    vmax = fmaxf(vmax, fmaxf(fabsf(o.x), fabsf(o.y)));
   }
+  // non-negative floats order like their bit patterns
+  for (int m = 32; m >= 1; m >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(vmax));
 }
 
 __global__ void __launch_bounds__(kBlock) ScatterVisKernel(
@@ -313,10 +319,11 @@ void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale, siz
 
 void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
                      const uint32_t* perm, size_t n_valid, const float2* vis,
-                     float2* vis_sorted, hipStream_t s) {
+                     float2* vis_sorted, uint32_t* vmax_bits, hipStream_t s) {
+  LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
   if (!n_valid) return;
   GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, vis,
-                                                     vis_sorted);
+                                                     vis_sorted, vmax_bits);
   WG_HIP(hipGetLastError());
 }
 
