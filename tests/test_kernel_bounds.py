@@ -63,6 +63,8 @@ def check_plan_bounds(plan, uvw, freq, vis=None):
         # matrix-core gridding: 16-byte aligned subgrid + 8 waves x 2 views
         # x 2 x 64 staged float4 entries
         assert (Sg * E * E * 8 + 15) // 16 * 16 + 8 * 2 * 128 * 16 <= 160 * 1024
+        # matrix-core degridding: box block + 8 waves x 128 x (float4 + float2)
+        assert (Sg * E * E * 8 + 15) // 16 * 16 + 8 * 128 * 24 <= 160 * 1024
     assert BU == ntu * T + W - 1 and BV == ntv * T + W - 1
 
     keys, valid = oracle.bucket_keys(plan, uvw, freq, vis)

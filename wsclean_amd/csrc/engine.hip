@@ -384,6 +384,7 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
   float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
   float2* rbuf = static_cast<float2*>(arena_.Get("rbuf", p.Sf * plane_lines * sizeof(float2)));
   float2* cbuf = static_cast<float2*>(arena_.Get("cbuf", p.Sf * plane_lines * sizeof(float2)));
+  uint32_t* queue = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t)));
   for (int f0 = 0; f0 < p.P; f0 += p.Sf) {
     const int np = std::min(p.Sf, p.P - f0);
     {
@@ -399,7 +400,7 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
       const int sub_n = std::min(p.Sg, p.P - sub_lo);
       {
         EventTimer tk(*this, &times_.degrid_kernel_ms);
-        LaunchDegrid(dp, vs, box, acc, g, sub_lo, sub_n, f0, stream_);
+        LaunchDegrid(dp, vs, box, acc, queue, g, sub_lo, sub_n, f0, stream_);
       }
       if (timing) {
         // coordinates (16 B) read and accumulator (8 B) read + written per

@@ -175,9 +175,11 @@ void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
                      const float2* scratch, int sub, int sub_lo, int sub_n,
                      int slab_lo, hipStream_t s);
 size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub);
+// `queue`: one device word, the work-item counter of the persistent
+// matrix-core degridder (cleared by the launch).
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
-                  float2* acc, int sub, int sub_lo, int sub_n, int slab_lo,
-                  hipStream_t s);
+                  float2* acc, uint32_t* queue, int sub, int sub_lo, int sub_n,
+                  int slab_lo, hipStream_t s);
 size_t GridLdsBytes(const DevPlan& dp, int sub_n);
 
 // ---- per-visibility kernels (vis.hip) ----

@@ -545,6 +545,193 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core degridding for W <= 8.
+//
+// Inside a run the visibilities read the same W x W x W block G of the box:
+//   V_j = sum_t kw_j[t] sum_l kv_j[l] sum_k G[t][l][k] ku_j[k].
+// The u contraction is a GEMM with the block as the A operand: for a group
+// of 16 visibilities and each plane t, two FP32 MFMA 16x16x4 steps (k = 0..3,
+// 4..7) give  D_t[(l,c)][j] = sum_k G[t][l][k].c ku_j[k].  Lane L holds
+// D_t[4 (L/16) + r][L % 16], i.e. visibility j = L % 16, component
+// c = L / 32 and taps l = 4 (L/16 % 2) + r; it finishes the v and w
+// contractions in registers and the four lanes of a visibility combine
+// their parts with lane exchanges.  The 16 A registers are reloaded from the
+// LDS box block only when the run changes.  Kernel values: each lane
+// evaluates two u taps (its B operand), two v taps and two w taps and gets
+// the other taps its sums need from the lanes of the same visibility.
+// Coordinates and the accumulated visibilities are staged per wave through
+// an LDS ring with run-start masks, as in the gridder.
+constexpr int kDegridGroup = 16;
+
+template <int W>
+__global__ void __launch_bounds__(64 * kMfmaWaves)
+    __attribute__((amdgpu_waves_per_eu(4, 4))) DegridKernelMfma(
+    DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
+    int pos_lo, int pos_hi, const float4* __restrict__ coord, const float2* __restrict__ box,
+    float2* __restrict__ acc, int sub_lo, int sub_n, int slab_lo, uint32_t* __restrict__ queue) {
+  static_assert(W <= 8 && W % 2 == 0, "matrix-core degridding covers W <= 8");
+  __shared__ uint32_t next_item;
+  extern __shared__ __attribute__((aligned(16))) float2 degrid_lds[];
+  float2* sub = degrid_lds;  // [sub_n][E][E] box block of the item's tile
+  const int E = dp.T + W - 1;
+  const int ncell = sub_n * E * E;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // per wave 2 x 64 staged (coordinate, accumulated visibility) entries
+  float4* ring_c = reinterpret_cast<float4*>(degrid_lds + ((ncell + 1) & ~1)) +
+                   wave * (2 * kStage);
+  float2* ring_a = reinterpret_cast<float2*>(reinterpret_cast<float4*>(
+                       degrid_lds + ((ncell + 1) & ~1)) + kMfmaWaves * (2 * kStage)) +
+                   wave * (2 * kStage);
+  const int j = lane & 15;        // visibility of the group (B column, D column)
+  const int g = lane >> 4;        // B row k = 4 s + g; D rows 4 g + r
+  const int cc = g >> 1;          // D component of the lane: 0 re, 1 im
+  const int lh = g & 1;           // D v taps l = 4 lh + r
+  const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
+  // abscissae (times 2/W) of the taps this lane evaluates
+  const float xu0 = float(g - W / 2 + 1) * c2, xu1 = float(4 + g - W / 2 + 1) * c2;
+  const int lv0 = 4 * lh + 2 * (g >> 1);       // v taps lv0, lv0 + 1
+  const float xv0 = float(lv0 - W / 2 + 1) * c2, xv1 = float(lv0 + 1 - W / 2 + 1) * c2;
+  const float xw0 = float(2 * g - W / 2 + 1) * c2, xw1 = float(2 * g + 1 - W / 2 + 1) * c2;
+  const bool u0_ok = g < W, u1_ok = 4 + g < W, v0_ok = lv0 < W, v1_ok = lv0 + 1 < W;
+  const bool w0_ok = 2 * g < W, w1_ok = 2 * g + 1 < W;
+  // A operand lane mapping: A[i = L % 16][kk = L / 16] = G[t][l = i % 8][4 s + kk].(i / 8)
+  const int a_l = lane & 7, a_c = (lane >> 3) & 1, a_k = lane >> 4;
+  const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
+  for (;;) {
+    if (threadIdx.x == 0) next_item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t it = it0 + next_item;
+    if (it >= it1) break;
+    const WorkItem wi = items[it];
+    const int tu = int(wi.tile % uint32_t(dp.ntu)), tv = int(wi.tile / uint32_t(dp.ntu));
+    for (int i = threadIdx.x; i < ncell; i += 64 * kMfmaWaves) {
+      const int p = i / (E * E), r = (i / E) % E, c = i % E;
+      sub[i] = box[(size_t(sub_lo + p - slab_lo) * dp.BV + size_t(tv * dp.T + r)) * dp.BU +
+                   size_t(tu * dp.T + c)];
+    }
+    __syncthreads();
+    const uint32_t n = wi.end - wi.begin;
+    const uint32_t per = (n + kMfmaWaves - 1) / kMfmaWaves;
+    const uint32_t w0 = wi.begin + min(n, wave * per);
+    const uint32_t w1 = wi.begin + min(n, (wave + 1) * per);
+    if (w0 < w1) {
+      auto stage = [&](uint32_t first, float4& c, float2& a) {
+        c = make_float4(0.f, 0.f, 0.f, 0.f);
+        a = make_float2(0.f, 0.f);
+        if (first + uint32_t(lane) < w1) {
+          c = coord[first + lane];
+          a = acc[first + lane];
+        }
+      };
+      float4 pc;
+      float2 pa;
+      stage(w0, pc, pa);
+      ring_c[lane] = pc;
+      ring_a[lane] = pa;
+      int kb0 = StageKey(w0, w1, lane, pc);
+      uint64_t sm0 = RunStarts(kb0, -1, lane);
+      stage(w0 + kStage, pc, pa);
+      ring_c[kStage + lane] = pc;
+      ring_a[kStage + lane] = pa;
+      int kb1 = StageKey(w0 + kStage, w1, lane, pc);
+      uint64_t sm1 = RunStarts(kb1, __builtin_amdgcn_readlane(kb0, 63), lane);
+      uint32_t bbase = w0, bslot = 0;
+      stage(w0 + 2 * kStage, pc, pa);
+      WaveLdsSync();
+      float areg[8][2];  // A operands of the current run: [t][s]
+      int cur = -1;
+      uint32_t pos = w0;
+      float4 cv = ring_c[j];
+      float2 av = ring_a[j];
+      while (pos < w1) {
+        const uint32_t o = pos - bbase;  // < 64
+        if ((sm0 >> o) & 1ull) {
+          cur = __builtin_amdgcn_readlane(kb0, int(o));
+          const int du = cur & 255, dv = (cur >> 8) & 255;
+          const int q0 = ((cur >> 16) & 0x7fff) - sub_lo;
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+              const int q = q0 + t, k = 4 * s + a_k;
+              float v = 0.f;
+              if (t < W && a_l < W && k < W && q >= 0 && q < sub_n) {
+                const float2 gv = sub[(q * E + dv + a_l) * E + du + k];
+                v = a_c ? gv.y : gv.x;
+              }
+              areg[t][s] = v;
+            }
+          }
+        }
+        // bit d - 1 of win: visibility pos + d starts a run
+        const uint64_t win = (o < 63 ? (sm0 >> (o + 1)) : 0ull) | (sm1 << (63 - o));
+        const int lead = __builtin_ctzll(win | (1ull << (kDegridGroup - 1))) + 1;
+        const uint32_t npos = pos + uint32_t(lead);
+        // next group's staged data (batches b, b + 1; see the gridder)
+        const uint32_t ns = (npos - w0 + uint32_t(j)) & uint32_t(2 * kStage - 1);
+        const float4 cn = ring_c[ns];
+        const float2 an = ring_a[ns];
+        const bool take = j < lead;
+        // B operands: u taps 4 s + g of visibility j
+        const float bu0 = (take && u0_ok) ? EsKernelAt(cv.x, nc2, xu0, b2) : 0.f;
+        const float bu1 = (take && u1_ok) ? EsKernelAt(cv.x, nc2, xu1, b2) : 0.f;
+        // v taps lv0, lv0 + 1 here; the other pair of this lane's half from
+        // lane L ^ 32
+        const float v0 = v0_ok ? EsKernelAt(cv.y, nc2, xv0, b2) : 0.f;
+        const float v1 = v1_ok ? EsKernelAt(cv.y, nc2, xv1, b2) : 0.f;
+        const float v2 = __shfl_xor(v0, 32), v3 = __shfl_xor(v1, 32);
+        // kv for l = 4 lh + r, r = 0..3 (lanes with g < 2 hold the first pair)
+        const float kv0 = g < 2 ? v0 : v2, kv1 = g < 2 ? v1 : v3;
+        const float kv2 = g < 2 ? v2 : v0, kv3 = g < 2 ? v3 : v1;
+        // w taps 2 g, 2 g + 1 here; all eight from the four lanes of j
+        // (lane 16 t + j holds taps 2 t, 2 t + 1 of visibility j)
+        float kw[8];
+        {
+          const float w0v = w0_ok ? EsKernelAt(cv.z, nc2, xw0, b2) : 0.f;
+          const float w1v = w1_ok ? EsKernelAt(cv.z, nc2, xw1, b2) : 0.f;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            kw[2 * t] = __shfl(w0v, 16 * t + j);
+            kw[2 * t + 1] = __shfl(w1v, 16 * t + j);
+          }
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          mfma_f4 d = mfma_f4{0.f, 0.f, 0.f, 0.f};
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[t][0], bu0, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[t][1], bu1, d, 0, 0, 0);
+          part += kw[t] * (d[0] * kv0 + d[1] * kv1 + d[2] * kv2 + d[3] * kv3);
+        }
+        // l halves (lanes L, L ^ 16), then the imaginary part to g = 0
+        part += __shfl_xor(part, 16);
+        const float im = __shfl_xor(part, 32);
+        if (g == 0 && take) acc[pos + j] = make_float2(av.x + part, av.y + im);
+        if (npos - bbase >= uint32_t(kStage) && npos < w1) {
+          ring_c[bslot * kStage + lane] = pc;
+          ring_a[bslot * kStage + lane] = pa;
+          const int kb2 = StageKey(bbase + 2 * kStage, w1, lane, pc);
+          const uint64_t sm2 = RunStarts(kb2, __builtin_amdgcn_readlane(kb1, 63), lane);
+          kb0 = kb1;
+          sm0 = sm1;
+          kb1 = kb2;
+          sm1 = sm2;
+          bbase += kStage;
+          bslot ^= 1u;
+          stage(bbase + 2 * kStage, pc, pa);
+          WaveLdsSync();
+        }
+        cv = cn;
+        av = an;
+        pos = npos;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Work-item counts for every (sub-slab, tile); with `sub_vis` non-null also
 // the number of visibility passes of every sub-slab (sum over its tiles).
 __global__ void ItemCountKernel(DevPlan dp, const uint32_t* __restrict__ off, int Sg, int nsub,
@@ -702,16 +889,35 @@ size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub) {
   return max_items_per_sub * size_t(Sg) * E * E * sizeof(float2);
 }
 
-void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc, int sub,
-                  int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
+void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc,
+                  uint32_t* queue, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
+  WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
   DispatchW(dp.W, [&](auto Wc) {
     constexpr int W = decltype(Wc)::value;
-    AllowLds(DegridKernel<W>, lds);
-    DegridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(
-        dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
-        acc, sub_lo, sub_n, slab_lo);
+    if constexpr (W <= 8) {
+      // box block (16-byte aligned) + per-wave rings of float4 + float2
+      constexpr size_t ring = size_t(kMfmaWaves) * (2 * kStage) * 24;
+      static_assert(ring + 16 <= size_t(kGridLdsReserve), "staging rings exceed the reserve");
+      const size_t total = ((lds + 15) & ~size_t(15)) + ring;
+      WG_CHECK(total <= 160 * 1024, "degridding LDS exceeds 160 KiB");
+      AllowLds(DegridKernelMfma<W>, total);
+      int per_cu = 0, dev = 0, cus = 256;
+      WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(DegridKernelMfma<W>), 64 * kMfmaWaves, total));
+      WG_HIP(hipGetDevice(&dev));
+      WG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      LaunchFillZero(queue, sizeof(uint32_t), s);
+      DegridKernelMfma<W><<<std::max(1, per_cu) * cus, 64 * kMfmaWaves, total, s>>>(
+          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
+          acc, sub_lo, sub_n, slab_lo, queue);
+    } else {
+      AllowLds(DegridKernel<W>, lds);
+      DegridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(
+          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
+          acc, sub_lo, sub_n, slab_lo);
+    }
   });
   WG_HIP(hipGetLastError());
 }
