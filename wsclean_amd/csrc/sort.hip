@@ -2,8 +2,9 @@
 //
 // keys are uint32 bucket ids (tile * P0 + p0; nkeys marks skipped zero
 // visibilities), values are the flat visibility index row * n_chan + chan.
-// The sort is a stable LSD radix sort with 8-bit digits, so the permutation
-// is identical to a stable CPU sort (numpy argsort(kind="stable")).
+// The sort is a stable LSD radix sort (digits of up to 9 bits, tiles of
+// 8192 elements reordered in LDS before the global writes), so the
+// permutation is identical to a stable CPU sort (numpy argsort(kind="stable")).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,9 +16,15 @@ namespace wg {
 
 namespace {
 
-constexpr int kRsBlock = 256;
-constexpr int kRsRounds = 32;
-constexpr int kRsTile = kRsBlock * kRsRounds;
+// Radix sort tiles: 512 threads x 16 elements; digits of at most 9 bits
+// (27-bit keys sort in three passes).
+constexpr int kRsThreads = 512;
+constexpr int kRsWaves = kRsThreads / 64;
+constexpr int kRsPerThread = 16;
+constexpr int kRsTile = kRsThreads * kRsPerThread;
+constexpr int kRsMaxBits = 9;
+constexpr int kRsMaxBuckets = 1 << kRsMaxBits;
+static_assert(kRsMaxBuckets <= kRsThreads, "one thread per bucket in the tile scan");
 
 constexpr int kScanBlock = 1024;
 constexpr int kScanPer = 8;
@@ -114,68 +121,123 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
   }
 }
 
-__global__ void RadixHistKernel(const uint32_t* __restrict__ keys, size_t n,
-                                int shift, uint32_t* __restrict__ counts,
-                                uint32_t nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+// Digit histogram of one tile: counts[d * nblocks + block].  Wave w of the
+// block reads elements [tile + w * 1024, tile + (w + 1) * 1024).
+__global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
+    const uint32_t* __restrict__ keys, size_t n, int shift, uint32_t mask,
+    uint32_t* __restrict__ counts, uint32_t nblocks) {
+  __shared__ uint32_t h[kRsWaves][kRsMaxBuckets];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int R = int(mask) + 1;
+  for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
-  const size_t base = size_t(blockIdx.x) * kRsTile;
-  for (int r = 0; r < kRsRounds; ++r) {
-    size_t i = base + size_t(r) * kRsBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  const size_t base = size_t(blockIdx.x) * kRsTile + size_t(wave) * (kRsTile / kRsWaves);
+#pragma unroll 4
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const size_t i = base + size_t(it) * 64 + lane;
+    if (i < n) atomicAdd(&h[wave][(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
-  counts[size_t(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < R; d += kRsThreads) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kRsWaves; ++w) s += h[w][d];
+    counts[size_t(d) * nblocks + blockIdx.x] = s;
+  }
 }
 
-__global__ void RadixScatterKernel(const uint32_t* __restrict__ kin,
-                                   const uint32_t* __restrict__ vin,
-                                   uint32_t* __restrict__ kout,
-                                   uint32_t* __restrict__ vout, size_t n,
-                                   int shift, const uint32_t* __restrict__ offs,
-                                   uint32_t nblocks) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[kRsBlock / 64][256];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  run[t] = offs[size_t(t) * nblocks + blockIdx.x];
-  const size_t base = size_t(blockIdx.x) * kRsTile;
+// Stable scatter of one tile.  Each wave ranks its 1024 elements by digit
+// (lanes with equal digits found by ballots over the digit bits; wave-
+// private running counts in LDS, so the order is element order), the block
+// turns the per-wave counts into tile positions, reorders the keys and then
+// the values through LDS, and writes each digit's run of the tile with
+// consecutive threads to consecutive addresses.
+__global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
+    int bits, const uint32_t* __restrict__ offs, uint32_t nblocks) {
+  __shared__ uint32_t cnt[kRsWaves][kRsMaxBuckets];  // per-wave counts, then offsets
+  __shared__ uint32_t start[kRsMaxBuckets];          // tile-local start of each digit
+  __shared__ uint32_t gbase[kRsMaxBuckets];          // global position - tile position
+  __shared__ uint32_t buf[kRsTile];                  // keys, then values, in tile order
+  __shared__ uint16_t dig[kRsTile];                  // digit of each tile position
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t mask = (1u << bits) - 1u;
+  const int R = 1 << bits;
+  for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&cnt[0][0])[i] = 0;
+  __syncthreads();
+  const size_t tile = size_t(blockIdx.x) * kRsTile;
+  const size_t base = tile + size_t(wave) * (kRsTile / kRsWaves);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int r = 0; r < kRsRounds; ++r) {
+  uint32_t key[kRsPerThread], pos[kRsPerThread];
 #pragma unroll
-    for (int w = 0; w < kRsBlock / 64; ++w) wc[w][t] = 0;
-    __syncthreads();
-    const size_t i = base + size_t(r) * kRsBlock + t;
-    const bool act = i < n;
-    const uint32_t k = act ? kin[i] : 0u;
-    const uint32_t v = act ? vin[i] : 0u;
-    const uint32_t d = (k >> shift) & 255u;
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const size_t i = base + size_t(it) * 64 + lane;
+    key[it] = i < n ? kin[i] : 0u;
+  }
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const bool act = base + size_t(it) * 64 + lane < n;
+    const uint32_t d = (key[it] >> shift) & mask;
     uint64_t peers = __ballot(act);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < bits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
     }
-    const uint32_t rank = __popcll(peers & lt);
-    if (act && rank == 0) wc[wave][d] = __popcll(peers);
-    __syncthreads();
-    uint32_t pos = 0;
-    if (act) {
-      pos = run[d] + rank;
-      for (int w = 0; w < wave; ++w) pos += wc[w][d];
-    }
-    __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int w = 0; w < kRsBlock / 64; ++w) add += wc[w][t];
-    run[t] += add;
-    if (act) {
-      kout[pos] = k;
-      vout[pos] = v;
-    }
-    __syncthreads();
+    const uint32_t r = uint32_t(__popcll(peers & lt));
+    const uint32_t c = act ? cnt[wave][d] : 0u;
+    pos[it] = c + r;
+    if (act && r == 0) cnt[wave][d] = c + uint32_t(__popcll(peers));
   }
+  __syncthreads();
+  // wave offsets inside each digit, digit totals, then tile starts
+  uint32_t tot = 0;
+  for (int d = threadIdx.x; d < R; d += kRsThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kRsWaves; ++w) {
+      const uint32_t t = cnt[w][d];
+      cnt[w][d] = run;
+      run += t;
+    }
+    start[d] = run;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;
+    const uint32_t v = d < R ? start[d] : 0u;
+    uint32_t ex = 0;
+    (void)BlockExclusiveScan(v, ex);
+    tot = ex;
+    __syncthreads();
+    if (d < R) {
+      start[d] = tot;
+      gbase[d] = offs[size_t(d) * nblocks + blockIdx.x] - tot;
+    }
+  }
+  __syncthreads();
+  const int count = int(min(size_t(kRsTile), n - tile));
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    if (base + size_t(it) * 64 + lane < n) {
+      const uint32_t d = (key[it] >> shift) & mask;
+      const uint32_t p = start[d] + cnt[wave][d] + pos[it];
+      pos[it] = p;
+      buf[p] = key[it];
+      dig[p] = uint16_t(d);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < count; i += kRsThreads) kout[gbase[dig[i]] + i] = buf[i];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const size_t i = base + size_t(it) * 64 + lane;
+    if (i < n) buf[pos[it]] = vin[i];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < count; i += kRsThreads) vout[gbase[dig[i]] + i] = buf[i];
 }
 
 // off[b] = first sorted position whose key >= b * stride, for b in [0, nb].
@@ -212,33 +274,38 @@ size_t ScanScratchWords(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; 
 
 size_t RadixSortScratchWords(size_t n) {
   const size_t nblocks = (n + kRsTile - 1) / kRsTile;
-  const size_t counts = 256 * std::max<size_t>(nblocks, 1);
+  const size_t counts = size_t(kRsMaxBuckets) * std::max<size_t>(nblocks, 1);
   return 2 * counts + ScanScratchWords(counts) + 2 * n;
 }
 
-// Sorts (keys, vals) in place by the low key_bits of keys; stable.
+// Sorts (keys, vals) in place by the low key_bits of keys; stable.  The
+// digit width is ceil(key_bits / passes) with passes = ceil(key_bits / 9).
 void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
                      uint32_t* scratch, hipStream_t s) {
   if (n <= 1) return;
+  WG_CHECK(key_bits >= 1 && key_bits <= 32, "key width out of range");
   const size_t nblocks = (n + kRsTile - 1) / kRsTile;
   WG_CHECK(nblocks < (1u << 31), "radix sort too large");
-  const size_t ncounts = 256 * nblocks;
+  const int npass = (key_bits + kRsMaxBits - 1) / kRsMaxBits;
+  const int digit = (key_bits + npass - 1) / npass;
   uint32_t* counts = scratch;
-  uint32_t* offs = counts + ncounts;
-  uint32_t* scan_scratch = offs + ncounts;
-  uint32_t* kalt = scan_scratch + ScanScratchWords(ncounts);
+  uint32_t* offs = counts + size_t(kRsMaxBuckets) * nblocks;
+  uint32_t* scan_scratch = offs + size_t(kRsMaxBuckets) * nblocks;
+  uint32_t* kalt = scan_scratch + ScanScratchWords(size_t(kRsMaxBuckets) * nblocks);
   uint32_t* valt = kalt + n;
   uint32_t* kin = keys;
   uint32_t* vin = vals;
   uint32_t* ko = kalt;
   uint32_t* vo = valt;
   int passes = 0;
-  for (int shift = 0; shift < key_bits; shift += 8) {
-    RadixHistKernel<<<dim3(unsigned(nblocks)), dim3(kRsBlock), 0, s>>>(
-        kin, n, shift, counts, uint32_t(nblocks));
+  for (int shift = 0; shift < key_bits; shift += digit) {
+    const int bits = std::min(digit, key_bits - shift);
+    const size_t ncounts = (size_t(1) << bits) * nblocks;
+    RadixHistKernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
+        kin, n, shift, (1u << bits) - 1u, counts, uint32_t(nblocks));
     DeviceExclusiveScan(counts, offs, ncounts, scan_scratch, s, nullptr);
-    RadixScatterKernel<<<dim3(unsigned(nblocks)), dim3(kRsBlock), 0, s>>>(
-        kin, vin, ko, vo, n, shift, offs, uint32_t(nblocks));
+    RadixScatterKernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
+        kin, vin, ko, vo, n, shift, bits, offs, uint32_t(nblocks));
     std::swap(kin, ko);
     std::swap(vin, vo);
     ++passes;
