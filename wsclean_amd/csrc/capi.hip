@@ -371,8 +371,11 @@ int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
     wg::LaunchKeys(h->vs_call.dp, duvw, h->vs_call.scale, n_rows, n_chan,
                    reinterpret_cast<float2*>(dvis), k, v, e.Stream());
     WG_HIP(hipMemcpyAsync(keys_out, k, nvis * 4, hipMemcpyDeviceToHost, e.Stream()));
-    WG_HIP(hipMemcpyAsync(perm_out, h->vs_call.perm, h->vs_call.n_valid * 4,
-                          hipMemcpyDeviceToHost, e.Stream()));
+    // the sorted permutation without the flip flags
+    uint32_t* pm = static_cast<uint32_t*>(a.Get("x.perm", h->vs_call.n_valid * 4 + 4));
+    wg::LaunchMaskBits(h->vs_call.perm, h->vs_call.n_valid, ~wg::kPermFlip, pm, e.Stream());
+    WG_HIP(hipMemcpyAsync(perm_out, pm, h->vs_call.n_valid * 4, hipMemcpyDeviceToHost,
+                          e.Stream()));
     WG_HIP(hipStreamSynchronize(e.Stream()));
     *n_valid = h->vs_call.n_valid;
     h->plan_json = h->vs_call.plan.describe();

@@ -112,6 +112,10 @@ __device__ __forceinline__ void EsTaps(float f, float b2, float* out) {
   }
 }
 
+// Top bit of a sorted permutation entry: the visibility was flipped to w >= 0
+// (the flat index itself is below 2^31).
+constexpr uint32_t kPermFlip = 0x80000000u;
+
 // Packed per-visibility word: du (8 bits) | dv (8 bits) | p0 (15 bits) | flip.
 __device__ __forceinline__ uint32_t PackLoc(const VisLoc& l) {
   return uint32_t(l.du) | (uint32_t(l.dv) << 8) | (uint32_t(l.p0) << 16) |

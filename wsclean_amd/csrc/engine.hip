@@ -162,7 +162,8 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   EventTimer tm(*this, &times_.plan);
   WG_CHECK(n_chan > 0, "n_chan must be positive");
   const size_t n_vis = n_rows * n_chan;
-  WG_CHECK(n_vis < 0xFFFFFFF0ull, "at most 2^32 - 16 visibilities per call");
+  // flat indices live in the low 31 bits of the sorted permutation
+  WG_CHECK(n_vis < 0x80000000ull, "at most 2^31 - 1 visibilities per call");
   vs.tag = tag;
   vs.n_rows = n_rows;
   vs.n_chan = n_chan;
@@ -206,7 +207,12 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
     LaunchKeys(dp, uvw_dev, vs.scale, n_rows, n_chan, vis_dev, keys, vals, stream_);
     uint32_t* scratch =
         static_cast<uint32_t*>(arena_.Get("sort.scratch", RadixSortScratchWords(n_vis) * 4));
-    DeviceRadixSort(keys, vals, n_vis, vs.plan.key_bits, scratch, stream_);
+    uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
+    uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
+    if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_)) {
+      keys = keys2;
+      vals = vals2;
+    }
   }
   vs.perm = vals;
   vs.off = static_cast<uint32_t*>(arena_.Get(tag + ".off", (size_t(dp.nbuckets) + 1) * 4));
@@ -218,7 +224,9 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   vs.n_valid = nvalid;
 
   vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", vs.n_valid * sizeof(float4)));
-  LaunchCoords(dp, uvw_dev, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, stream_);
+  vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", vs.n_valid * sizeof(float2)));
+  LaunchCoords(dp, uvw_dev, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, vs.phasor,
+               stream_);
   BuildWorkItems(vs);
   BuildCorrection(vs);
 }
@@ -324,8 +332,7 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   {
     EventTimer tm(*this, &times_.sort);
     vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
-    LaunchGatherVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.n_valid, vis_dev, vsorted,
-                    words + 1, stream_);
+    LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, words + 1, stream_);
   }
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
@@ -411,8 +418,10 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
       }
     }
   }
-  LaunchScatterVis(dp, vs.uvw, vs.scale, vs.n_chan, vs.perm, vs.coord, vs.n_valid, acc,
-                   vis_out_dev, stream_);
+  {
+    EventTimer tm(*this, &times_.sort);
+    LaunchScatterVis(vs.perm, vs.phasor, vs.n_valid, acc, vis_out_dev, stream_);
+  }
 }
 
 }  // namespace wg

@@ -20,8 +20,10 @@ void DeviceExclusiveScan(const uint32_t* in, uint32_t* out, size_t n,
                          uint32_t* scratch, hipStream_t s, uint32_t* total);
 size_t ScanScratchWords(size_t n);
 size_t RadixSortScratchWords(size_t n);
-void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
-                     uint32_t* scratch, hipStream_t s);
+// Stable sort of (keys, vals) by the low key_bits; keys_alt / vals_alt are
+// the ping-pong buffers.  Returns true when the result is in the alt pair.
+bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s);
 // off[b] = first sorted position with key >= b * stride, b in [0, nbuckets].
 void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
                          uint32_t stride, uint32_t* off, hipStream_t s);
@@ -79,7 +81,8 @@ struct VisSet {
   std::string tag;
   double* uvw = nullptr;      // [n_rows][3]
   double* scale = nullptr;    // [n_chan] freq / c
-  uint32_t* perm = nullptr;   // [n_valid] sorted -> flat index
+  uint32_t* perm = nullptr;   // [n_valid] sorted -> flat index | kPermFlip
+  float2* phasor = nullptr;   // [n_valid] exp(-2 pi i (u l_s + v m_s + w zc))
   uint32_t* off = nullptr;    // [nkeys + 1] bucket offsets
   float4* coord = nullptr;    // [n_valid] fu, fv, fw, packed cell/plane/flip
   WorkItem* items = nullptr;  // work items of all gridding sub-slabs
@@ -188,19 +191,22 @@ void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
 void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
                 size_t n_rows, size_t n_chan, const float2* vis, uint32_t* keys,
                 uint32_t* vals, hipStream_t s);
+// Sorted coordinates and phasors; sets the flip bit (kPermFlip) in perm.
 void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale,
-                  size_t n_chan, const uint32_t* perm, size_t n_valid,
-                  float4* coord, hipStream_t s);
-// Also writes the bits of max |Re|, |Im| of the gathered visibilities to
-// *vmax_bits (a non-negative float; the launch clears the word first).
-void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale,
-                     size_t n_chan, const uint32_t* perm, size_t n_valid,
+                  size_t n_chan, uint32_t* perm, size_t n_valid, float4* coord,
+                  float2* phasor, hipStream_t s);
+// out[i] = in[i] & mask
+void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,
+                    hipStream_t s);
+// vis_sorted[i] = flip(vis[perm[i]]) * phasor[i].  Also writes the bits of
+// max |Re|, |Im| of the result to *vmax_bits (a non-negative float; the
+// launch clears the word first).
+void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
                      const float2* vis, float2* vis_sorted, uint32_t* vmax_bits,
                      hipStream_t s);
-void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale,
-                      size_t n_chan, const uint32_t* perm, const float4* coord,
-                      size_t n_valid, const float2* acc, float2* vis_out,
-                      hipStream_t s);
+// vis_out[perm[i]] = flip(acc[i] * conj(phasor[i])).
+void LaunchScatterVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
+                      const float2* acc, float2* vis_out, hipStream_t s);
 void LaunchCorrection(const DevPlan& dp, const double* phi_u, const double* phi_v,
                       const double* quad_a, const double* quad_b, int nquad,
                       float* corr, hipStream_t s);

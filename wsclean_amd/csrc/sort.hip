@@ -275,14 +275,16 @@ size_t ScanScratchWords(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; 
 size_t RadixSortScratchWords(size_t n) {
   const size_t nblocks = (n + kRsTile - 1) / kRsTile;
   const size_t counts = size_t(kRsMaxBuckets) * std::max<size_t>(nblocks, 1);
-  return 2 * counts + ScanScratchWords(counts) + 2 * n;
+  return 2 * counts + ScanScratchWords(counts);
 }
 
-// Sorts (keys, vals) in place by the low key_bits of keys; stable.  The
-// digit width is ceil(key_bits / passes) with passes = ceil(key_bits / 9).
-void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
-                     uint32_t* scratch, hipStream_t s) {
-  if (n <= 1) return;
+// Sorts (keys, vals) by the low key_bits of keys; stable.  The digit width
+// is ceil(key_bits / passes) with passes = ceil(key_bits / 9).  The passes
+// alternate between (keys, vals) and (keys_alt, vals_alt); the return value
+// says which pair holds the result (true: the alt pair).
+bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s) {
+  if (n <= 1) return false;
   WG_CHECK(key_bits >= 1 && key_bits <= 32, "key width out of range");
   const size_t nblocks = (n + kRsTile - 1) / kRsTile;
   WG_CHECK(nblocks < (1u << 31), "radix sort too large");
@@ -291,13 +293,11 @@ void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
   uint32_t* counts = scratch;
   uint32_t* offs = counts + size_t(kRsMaxBuckets) * nblocks;
   uint32_t* scan_scratch = offs + size_t(kRsMaxBuckets) * nblocks;
-  uint32_t* kalt = scan_scratch + ScanScratchWords(size_t(kRsMaxBuckets) * nblocks);
-  uint32_t* valt = kalt + n;
   uint32_t* kin = keys;
   uint32_t* vin = vals;
-  uint32_t* ko = kalt;
-  uint32_t* vo = valt;
-  int passes = 0;
+  uint32_t* ko = keys_alt;
+  uint32_t* vo = vals_alt;
+  bool in_alt = false;
   for (int shift = 0; shift < key_bits; shift += digit) {
     const int bits = std::min(digit, key_bits - shift);
     const size_t ncounts = (size_t(1) << bits) * nblocks;
@@ -308,13 +308,10 @@ void DeviceRadixSort(uint32_t* keys, uint32_t* vals, size_t n, int key_bits,
         kin, vin, ko, vo, n, shift, bits, offs, uint32_t(nblocks));
     std::swap(kin, ko);
     std::swap(vin, vo);
-    ++passes;
+    in_alt = !in_alt;
   }
   WG_HIP(hipGetLastError());
-  if (passes & 1) {
-    WG_HIP(hipMemcpyAsync(keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    WG_HIP(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-  }
+  return in_alt;
 }
 
 void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,

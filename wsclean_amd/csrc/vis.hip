@@ -99,21 +99,6 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
   }
 }
 
-__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw,
-                                                       const double* __restrict__ scale,
-                                                       size_t n_chan,
-                                                       const uint32_t* __restrict__ perm,
-                                                       size_t n_valid,
-                                                       float4* __restrict__ coord) {
-  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
-       i += size_t(gridDim.x) * kBlock) {
-    const size_t idx = perm[i];
-    const size_t row = idx / n_chan, ch = idx - row * n_chan;
-    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
-    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
-  }
-}
-
 // exp(sign 2 pi i (u l_shift + v m_shift + w zc)) for flipped coordinates.
 __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L, float sign) {
   const double cyc = L.u * dp.l_shift + L.v * dp.m_shift + L.w * dp.zc;
@@ -123,19 +108,43 @@ __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L
   return make_float2(c, sign * s);
 }
 
+// Per sorted visibility: the gridding coordinates, the phase-centre /
+// w-centre phasor exp(-2 pi i (u l_shift + v m_shift + w zc)) and, in the top
+// bit of perm[i], whether (u, v, w, V) was flipped to w >= 0.  The gather
+// and scatter of visibilities then need no uvw reads and no FP64 work.
+__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw,
+                                                       const double* __restrict__ scale,
+                                                       size_t n_chan, uint32_t* __restrict__ perm,
+                                                       size_t n_valid, float4* __restrict__ coord,
+                                                       float2* __restrict__ phasor) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
+       i += size_t(gridDim.x) * kBlock) {
+    const uint32_t idx = perm[i];
+    const size_t row = idx / n_chan, ch = idx - row * n_chan;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
+    phasor[i] = ShiftPhasor(dp, L, -1.f);
+    perm[i] = idx | (L.flip ? kPermFlip : 0u);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) MaskKernel(const uint32_t* __restrict__ in, size_t n,
+                                                     uint32_t mask, uint32_t* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock)
+    out[i] = in[i] & mask;
+}
+
 __global__ void __launch_bounds__(kBlock) GatherVisKernel(
-    DevPlan dp, const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
-    const uint32_t* __restrict__ perm, size_t n_valid, const float2* __restrict__ vis,
-    float2* __restrict__ out, uint32_t* __restrict__ vmax_bits) {
+    const uint32_t* __restrict__ perm, const float2* __restrict__ phasor, size_t n_valid,
+    const float2* __restrict__ vis, float2* __restrict__ out, uint32_t* __restrict__ vmax_bits) {
   float vmax = 0.f;  // largest |component| of the gathered visibilities
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
-    const size_t idx = perm[i];
-    const size_t row = idx / n_chan, ch = idx - row * n_chan;
-    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
-    float2 V = vis[idx];
-    if (L.flip) V.y = -V.y;
-    const float2 ph = ShiftPhasor(dp, L, -1.f);
+    const uint32_t pw = perm[i];
+    float2 V = vis[pw & ~kPermFlip];
+    if (pw & kPermFlip) V.y = -V.y;
+    const float2 ph = phasor[i];
     const float2 o = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
     out[i] = o;
     vmax = fmaxf(vmax, fmaxf(fabsf(o.x), fabsf(o.y)));
@@ -146,19 +155,16 @@ __global__ void __launch_bounds__(kBlock) GatherVisKernel(
 }
 
 __global__ void __launch_bounds__(kBlock) ScatterVisKernel(
-    DevPlan dp, const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
-    const uint32_t* __restrict__ perm, size_t n_valid, const float2* __restrict__ acc,
-    float2* __restrict__ out) {
+    const uint32_t* __restrict__ perm, const float2* __restrict__ phasor, size_t n_valid,
+    const float2* __restrict__ acc, float2* __restrict__ out) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
-    const size_t idx = perm[i];
-    const size_t row = idx / n_chan, ch = idx - row * n_chan;
-    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    const uint32_t pw = perm[i];
     const float2 a = acc[i];
-    const float2 ph = ShiftPhasor(dp, L, 1.f);
-    float2 V = make_float2(a.x * ph.x - a.y * ph.y, a.x * ph.y + a.y * ph.x);
-    if (L.flip) V.y = -V.y;
-    out[idx] = V;
+    const float2 ph = phasor[i];  // conjugated below: the predict direction
+    float2 V = make_float2(a.x * ph.x + a.y * ph.y, a.y * ph.x - a.x * ph.y);
+    if (pw & kPermFlip) V.y = -V.y;
+    out[pw & ~kPermFlip] = V;
   }
 }
 
@@ -311,29 +317,33 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale, size_
 }
 
 void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
-                  const uint32_t* perm, size_t n_valid, float4* coord, hipStream_t s) {
+                  uint32_t* perm, size_t n_valid, float4* coord, float2* phasor,
+                  hipStream_t s) {
   if (!n_valid) return;
-  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, coord);
+  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, coord,
+                                                  phasor);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchGatherVis(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
-                     const uint32_t* perm, size_t n_valid, const float2* vis,
-                     float2* vis_sorted, uint32_t* vmax_bits, hipStream_t s) {
+void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out, hipStream_t s) {
+  if (!n) return;
+  MaskKernel<<<Blocks(n), kBlock, 0, s>>>(in, n, mask, out);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
+                     const float2* vis, float2* vis_sorted, uint32_t* vmax_bits, hipStream_t s) {
   LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
   if (!n_valid) return;
-  GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, vis,
-                                                     vis_sorted, vmax_bits);
+  GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(perm, phasor, n_valid, vis, vis_sorted,
+                                                     vmax_bits);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchScatterVis(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
-                      const uint32_t* perm, const float4* coord, size_t n_valid,
+void LaunchScatterVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
                       const float2* acc, float2* vis_out, hipStream_t s) {
-  (void)coord;
   if (!n_valid) return;
-  ScatterVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, acc,
-                                                      vis_out);
+  ScatterVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(perm, phasor, n_valid, acc, vis_out);
   WG_HIP(hipGetLastError());
 }
 
