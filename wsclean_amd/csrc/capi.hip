@@ -413,16 +413,12 @@ int wgx_make_plan(const wg_params* params, const double* e, long long nvis, int 
 int wgx_fft(int n, int nlines, int inverse, float* data) {
   return Guard([&] {
     WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");
-    std::vector<float2> tw(n);
-    for (int m = 0; m < n; ++m) {
-      const double ang = 2.0 * M_PI * double(m) / double(n);
-      tw[m] = make_float2(float(std::cos(ang)), float(-std::sin(ang)));
-    }
+    const std::vector<float2> tw = wg::FFTTwiddleTable(n);
     float2 *dtw = nullptr, *dd = nullptr;
     const size_t bytes = size_t(n) * nlines * sizeof(float2);
-    WG_HIP(hipMalloc(&dtw, n * sizeof(float2)));
+    WG_HIP(hipMalloc(&dtw, tw.size() * sizeof(float2)));
     WG_HIP(hipMalloc(&dd, bytes));
-    WG_HIP(hipMemcpy(dtw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    WG_HIP(hipMemcpy(dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
     WG_HIP(hipMemcpy(dd, data, bytes, hipMemcpyHostToDevice));
     wg::LaunchLineFFT(dd, nlines, wg::MakeFFTDesc(n, dtw), inverse != 0, nullptr);
     WG_HIP(hipDeviceSynchronize());

@@ -111,15 +111,10 @@ Engine::~Engine() {
 const float2* Engine::Twiddles(int n) {
   auto it = twiddles_.find(n);
   if (it != twiddles_.end()) return it->second;
-  std::vector<float2> h(n);
-  for (int m = 0; m < n; ++m) {
-    // exact reduction: angle = 2 pi m / n in double
-    const double a = 2.0 * M_PI * double(m) / double(n);
-    h[m] = make_float2(float(std::cos(a)), float(-std::sin(a)));
-  }
+  const std::vector<float2> h = FFTTwiddleTable(n);
   float2* d = nullptr;
-  WG_HIP(hipMalloc(&d, n * sizeof(float2)));
-  WG_HIP(hipMemcpy(d, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  WG_HIP(hipMalloc(&d, h.size() * sizeof(float2)));
+  WG_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
   twiddles_[n] = d;
   return d;
 }

@@ -146,9 +146,12 @@ struct FFTDesc {
   int n;
   int nst;
   int rad[16];
-  const float2* tw;
+  uint32_t magic[16];  // floor(2^32 / Ns) + 1 per stage (Ns: product of earlier radices)
+  const float2* tw;    // device copy of FFTTwiddleTable(n)
 };
 FFTDesc MakeFFTDesc(int n, const float2* tw);
+// Two-level twiddle table of the LDS FFT (host values, FP64-rounded to FP32).
+std::vector<float2> FFTTwiddleTable(int n);
 
 void LaunchRowFFTInvert(const DevPlan& dp, const FFTDesc& fu, const float2* box,
                         float2* rbuf, int nplanes, hipStream_t s);

@@ -22,6 +22,12 @@ namespace wg {
 
 namespace {
 
+// Two-level twiddle table: kTwLo entries exp(-2 pi i r / n), r < kTwLo,
+// then ceil(n / kTwLo) entries exp(-2 pi i kTwLo m / n); kTwSlots bounds its
+// LDS copy for n <= 16384.
+constexpr int kTwLo = 128;
+constexpr int kTwSlots = kTwLo + 16384 / kTwLo;
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -115,13 +121,58 @@ __device__ __forceinline__ void Dft<5, false>(float2* v) { Dft5<false>(v); }
 template <>
 __device__ __forceinline__ void Dft<5, true>(float2* v) { Dft5<true>(v); }
 
+// Radix 8 as two radix-2 halves and two radix-4 DFTs:
+// X[2m] = DFT4(v_n + v_{n+4})[m],  X[2m+1] = DFT4((v_n - v_{n+4}) w8^n)[m].
+template <bool INV>
+__device__ __forceinline__ void Dft8(float2* v) {
+  constexpr float r = 0.70710678118654752440f;
+  float2 a[4], b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    a[i] = cadd(v[i], v[i + 4]);
+    b[i] = csub(v[i], v[i + 4]);
+  }
+  // b1 *= w8, b2 *= w8^2, b3 *= w8^3 (w8 = exp(-+ 2 pi i / 8))
+  b[1] = INV ? make_float2(r * (b[1].x - b[1].y), r * (b[1].x + b[1].y))
+             : make_float2(r * (b[1].x + b[1].y), r * (b[1].y - b[1].x));
+  b[2] = rot90<INV>(b[2]);
+  b[3] = INV ? make_float2(-r * (b[3].x + b[3].y), r * (b[3].x - b[3].y))
+             : make_float2(r * (b[3].y - b[3].x), -r * (b[3].x + b[3].y));
+  Dft4<INV>(a);
+  Dft4<INV>(b);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    v[2 * m] = a[m];
+    v[2 * m + 1] = b[m];
+  }
+}
+template <>
+__device__ __forceinline__ void Dft<8, false>(float2* v) { Dft8<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<8, true>(float2* v) { Dft8<true>(v); }
+
+// LDS line layout: one pad slot after every 16 complex values, so the
+// strided writes of the early Stockham stages spread over the banks.
+__device__ __forceinline__ int Pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int PaddedLen(int n) { return n + (n >> 4) + 1; }
+
+// Twiddle exp(-2 pi i k / n) from the two-level LDS table (lo[k % 128] *
+// hi[k / 128]; FP32 product of correctly rounded factors).
+__device__ __forceinline__ float2 Twiddle(const float2* twl, int k) {
+  const float2 lo = twl[k & (kTwLo - 1)];
+  const float2 hi = twl[kTwLo + (k >> 7)];
+  return cmul(lo, hi);
+}
+
 // One Stockham autosort stage of radix R on an LDS-resident line of length
-// n.  Ns is the product of the radices of the previous stages.  Every
-// thread first reads all its butterflies into registers, then (after a
-// barrier) writes them, so a single LDS buffer suffices.
+// n.  Ns is the product of the radices of the previous stages and `magic`
+// = floor(2^32 / Ns) + 1, so j / Ns = umulhi(j, magic) exactly for the
+// j < n / R <= 8192 used here.  Every thread first reads all its
+// butterflies into registers, then (after a barrier) writes them, so a
+// single LDS buffer suffices.
 template <int R, bool INV, int BLOCK, int MAXN>
-__device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns,
-                                              const float2* __restrict__ tw) {
+__device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32_t magic,
+                                              const float2* twl) {
   constexpr int NB = (MAXN / R + BLOCK - 1) / BLOCK;
   const int nb = n / R;
   const int tws = n / (Ns * R);
@@ -130,13 +181,13 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns,
   for (int b = 0; b < NB; ++b) {
     const int j = threadIdx.x + b * BLOCK;
     if (j < nb) {
-      const int k = j % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nb];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[Pad(j + r * nb)];
       if (Ns > 1) {
+        const int k = j - int(__umulhi(uint32_t(j), magic)) * Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[k * r * tws];
+          float2 w = Twiddle(twl, k * r * tws);
           if (INV) w.y = -w.y;
           v[b][r] = cmul(v[b][r], w);
         }
@@ -149,35 +200,44 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns,
   for (int b = 0; b < NB; ++b) {
     const int j = threadIdx.x + b * BLOCK;
     if (j < nb) {
-      const int k = j % Ns;
+      const int k = Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[Pad(base + r * Ns)] = v[b][r];
     }
   }
   __syncthreads();
 }
 
 // Full in-LDS FFT; unnormalised, forward = exp(-2 pi i jk/n).  Caller must
-// have synchronised after filling buf.
+// have synchronised after filling buf and the twiddle table twl.
 template <bool INV, int BLOCK, int MAXN>
-__device__ void LdsFFT(float2* buf, const FFTDesc& d) {
+__device__ void LdsFFT(float2* buf, const FFTDesc& d, const float2* twl) {
   int Ns = 1;
   for (int s = 0; s < d.nst; ++s) {
     const int R = d.rad[s];
+    const uint32_t mg = d.magic[s];
     switch (R) {
-      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
-      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
-      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
-      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, d.n, Ns, d.tw); break;
+      case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
     }
     Ns *= R;
   }
 }
 
+// Copies the line's twiddle table to LDS (kTwLo + ceil(n / 128) entries).
+template <int BLOCK>
+__device__ __forceinline__ void LoadTwiddles(float2* twl, const FFTDesc& d) {
+  const int cnt = kTwLo + (d.n + kTwLo - 1) / kTwLo;
+  for (int i = threadIdx.x; i < cnt; i += BLOCK) twl[i] = d.tw[i];
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void ZeroLds(float2* buf, int n) {
-  for (int i = threadIdx.x; i < n; i += BLOCK) buf[i] = make_float2(0.f, 0.f);
+  for (int i = threadIdx.x; i < PaddedLen(n); i += BLOCK) buf[i] = make_float2(0.f, 0.f);
 }
 
 // buf[(origin + i) mod n] += src[i] for i in [0, len), folding periodically.
@@ -188,11 +248,13 @@ __device__ __forceinline__ void FoldLoad(float2* buf, int n, const float2* __res
     const int i0 = max(0, k * n - origin);
     const int i1 = min(len, (k + 1) * n - origin);
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-      float2& d = buf[origin + i - k * n];
+      float2& d = buf[Pad(origin + i - k * n)];
       const float2 a = src[i];
       d.x += a.x;
       d.y += a.y;
     }
+    // successive folds add to the same slots
+    __syncthreads();
     __syncthreads();
   }
 }
@@ -228,22 +290,28 @@ __device__ __forceinline__ float2 Phasor(double cycles, float sign) {
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) RowFFTInvertKernel(
     DevPlan dp, FFTDesc fu, const float2* __restrict__ box, float2* __restrict__ rbuf) {
-  extern __shared__ float2 buf[];
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
   const int j = blockIdx.x, pl = blockIdx.y;
   const float2* src = box + (size_t(pl) * dp.BV + j) * dp.BU;
+  LoadTwiddles<BLOCK>(twl, fu);
   ZeroLds<BLOCK>(buf, fu.n);
   __syncthreads();
   FoldLoad<BLOCK>(buf, fu.n, src, dp.BU, dp.box_u0);
-  LdsFFT<false, BLOCK, MAXN>(buf, fu);
+  LdsFFT<false, BLOCK, MAXN>(buf, fu, twl);
   float2* out = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
-  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) out[x] = buf[Wrap(dp.nx / 2 - x, fu.n)];
+  for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
+    out[x] = buf[Pad(Wrap(dp.nx / 2 - x, fu.n))];
 }
 
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
     DevPlan dp, FFTDesc fv, const float2* __restrict__ cbuf, float* __restrict__ dirty,
     int plane_lo, int nplanes) {
-  extern __shared__ float2 buf[];
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
   constexpr int NY = MAXN / BLOCK;
   const int x = blockIdx.x;
   float acc[NY];
@@ -254,17 +322,18 @@ __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
     acc[k] = 0.f;
     zz[k] = y < dp.ny ? PixelZ(dp, x, y) : 0.0;
   }
+  LoadTwiddles<BLOCK>(twl, fv);
   for (int pl = 0; pl < nplanes; ++pl) {
     ZeroLds<BLOCK>(buf, fv.n);
     __syncthreads();
     FoldLoad<BLOCK>(buf, fv.n, cbuf + (size_t(pl) * dp.nx + x) * dp.BV, dp.BV, dp.box_v0);
-    LdsFFT<false, BLOCK, MAXN>(buf, fv);
+    LdsFFT<false, BLOCK, MAXN>(buf, fv, twl);
     const double wp = PlaneW(dp, plane_lo + pl);
 #pragma unroll
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
       if (y < dp.ny) {
-        const float2 val = buf[Wrap(y - dp.ny / 2, fv.n)];
+        const float2 val = buf[Pad(Wrap(y - dp.ny / 2, fv.n))];
         const float2 ph = Phasor(wp * zz[k], -1.f);
         acc[k] += val.x * ph.x - val.y * ph.y;
       }
@@ -283,7 +352,9 @@ template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
     DevPlan dp, FFTDesc fv, const double* __restrict__ img, const float* __restrict__ corr,
     float2* __restrict__ cbuf, int plane_lo, int nplanes) {
-  extern __shared__ float2 buf[];
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
   constexpr int NY = MAXN / BLOCK;
   const int x = blockIdx.x;
   float mv[NY];
@@ -300,6 +371,7 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
       zz[k] = 0.0;
     }
   }
+  LoadTwiddles<BLOCK>(twl, fv);
   for (int pl = 0; pl < nplanes; ++pl) {
     ZeroLds<BLOCK>(buf, fv.n);
     __syncthreads();
@@ -307,12 +379,14 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
 #pragma unroll
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
-      if (y < dp.ny) buf[Wrap(y - dp.ny / 2, fv.n)] = cscale(Phasor(wp * zz[k], 1.f), mv[k]);
+      if (y < dp.ny)
+        buf[Pad(Wrap(y - dp.ny / 2, fv.n))] = cscale(Phasor(wp * zz[k], 1.f), mv[k]);
     }
     __syncthreads();
-    LdsFFT<true, BLOCK, MAXN>(buf, fv);
+    LdsFFT<true, BLOCK, MAXN>(buf, fv, twl);
     float2* out = cbuf + (size_t(pl) * dp.nx + x) * dp.BV;
-    for (int j = threadIdx.x; j < dp.BV; j += BLOCK) out[j] = buf[Wrap(dp.box_v0 + j, fv.n)];
+    for (int j = threadIdx.x; j < dp.BV; j += BLOCK)
+      out[j] = buf[Pad(Wrap(dp.box_v0 + j, fv.n))];
     __syncthreads();
   }
 }
@@ -320,30 +394,36 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) RowFFTPredictKernel(
     DevPlan dp, FFTDesc fu, const float2* __restrict__ rbuf, float2* __restrict__ box) {
-  extern __shared__ float2 buf[];
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
   const int j = blockIdx.x, pl = blockIdx.y;
+  LoadTwiddles<BLOCK>(twl, fu);
   ZeroLds<BLOCK>(buf, fu.n);
   __syncthreads();
   const float2* src = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
-  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) buf[Wrap(dp.nx / 2 - x, fu.n)] = src[x];
+  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) buf[Pad(Wrap(dp.nx / 2 - x, fu.n))] = src[x];
   __syncthreads();
-  LdsFFT<true, BLOCK, MAXN>(buf, fu);
+  LdsFFT<true, BLOCK, MAXN>(buf, fu, twl);
   float2* out = box + (size_t(pl) * dp.BV + j) * dp.BU;
-  for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Wrap(dp.box_u0 + i, fu.n)];
+  for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Pad(Wrap(dp.box_u0 + i, fu.n))];
 }
 
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) LineFFTKernel(float2* __restrict__ data, FFTDesc d,
                                                        int inverse) {
-  extern __shared__ float2 buf[];
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
   float2* line = data + size_t(blockIdx.x) * d.n;
-  for (int i = threadIdx.x; i < d.n; i += BLOCK) buf[i] = line[i];
+  LoadTwiddles<BLOCK>(twl, d);
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) buf[Pad(i)] = line[i];
   __syncthreads();
   if (inverse)
-    LdsFFT<true, BLOCK, MAXN>(buf, d);
+    LdsFFT<true, BLOCK, MAXN>(buf, d, twl);
   else
-    LdsFFT<false, BLOCK, MAXN>(buf, d);
-  for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[i];
+    LdsFFT<false, BLOCK, MAXN>(buf, d, twl);
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[Pad(i)];
 }
 
 // out[p][c][r] = in[p][r][c]; 32x32 tiles through LDS.
@@ -364,6 +444,9 @@ __global__ void __launch_bounds__(256) TransposeKernel(const float2* __restrict_
     if (r < rows && c < cols) out[plane + size_t(c) * rows + r] = tile[tx][k];
   }
 }
+
+// LDS of one line kernel: twiddle table + padded line of length n.
+size_t LineLds(int n) { return size_t(kTwSlots + PaddedLen(n)) * sizeof(float2); }
 
 // Kernels that use more than the default dynamic LDS must opt in once.
 template <typename K>
@@ -386,22 +469,41 @@ void Dispatch(int n, F&& f) {
 }  // namespace
 
 FFTDesc MakeFFTDesc(int n, const float2* tw) {
+  WG_CHECK(n >= 1 && n <= 16384, "FFT length out of range");
   FFTDesc d{};
   d.n = n;
   std::vector<int> r = FFTRadices(n);
   WG_CHECK(r.size() <= 16, "too many FFT stages");
   d.nst = int(r.size());
-  for (size_t i = 0; i < r.size(); ++i) d.rad[i] = r[i];
+  uint64_t ns = 1;
+  for (size_t i = 0; i < r.size(); ++i) {
+    d.rad[i] = r[i];
+    // j / Ns = umulhi(j, magic) for j < 2^18 (see StockhamStage)
+    d.magic[i] = uint32_t((uint64_t(1) << 32) / ns + 1);
+    ns *= uint64_t(r[i]);
+  }
   d.tw = tw;
   return d;
+}
+
+std::vector<float2> FFTTwiddleTable(int n) {
+  const int nhi = (n + kTwLo - 1) / kTwLo;
+  std::vector<float2> t(kTwLo + nhi);
+  for (int i = 0; i < kTwLo + nhi; ++i) {
+    const long m = i < kTwLo ? long(i) : long(i - kTwLo) * kTwLo;
+    // exact reduction: angle = 2 pi (m mod n) / n in double
+    const double a = 2.0 * M_PI * double(m % n) / double(n);
+    t[i] = make_float2(float(std::cos(a)), float(-std::sin(a)));
+  }
+  return t;
 }
 
 void LaunchRowFFTInvert(const DevPlan& dp, const FFTDesc& fu, const float2* box,
                         float2* rbuf, int nplanes, hipStream_t s) {
   Dispatch(fu.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(RowFFTInvertKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
-    RowFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, fu.n * sizeof(float2), s>>>(
+    AllowLds(RowFFTInvertKernel<BLOCK, MAXN>, LineLds(MAXN));
+    RowFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, LineLds(fu.n), s>>>(
         dp, fu, box, rbuf);
   });
   WG_HIP(hipGetLastError());
@@ -411,8 +513,8 @@ void LaunchColFFTInvert(const DevPlan& dp, const FFTDesc& fv, const float2* cbuf
                         float* dirty, int plane_lo, int nplanes, hipStream_t s) {
   Dispatch(fv.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(ColFFTInvertKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
-    ColFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, fv.n * sizeof(float2), s>>>(
+    AllowLds(ColFFTInvertKernel<BLOCK, MAXN>, LineLds(MAXN));
+    ColFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
         dp, fv, cbuf, dirty, plane_lo, nplanes);
   });
   WG_HIP(hipGetLastError());
@@ -423,8 +525,8 @@ void LaunchColFFTPredict(const DevPlan& dp, const FFTDesc& fv, const double* img
                          hipStream_t s) {
   Dispatch(fv.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(ColFFTPredictKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
-    ColFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, fv.n * sizeof(float2), s>>>(
+    AllowLds(ColFFTPredictKernel<BLOCK, MAXN>, LineLds(MAXN));
+    ColFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
         dp, fv, img, corr, cbuf, plane_lo, nplanes);
   });
   WG_HIP(hipGetLastError());
@@ -434,8 +536,8 @@ void LaunchRowFFTPredict(const DevPlan& dp, const FFTDesc& fu, const float2* rbu
                          float2* box, int nplanes, hipStream_t s) {
   Dispatch(fu.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(RowFFTPredictKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
-    RowFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, fu.n * sizeof(float2), s>>>(
+    AllowLds(RowFFTPredictKernel<BLOCK, MAXN>, LineLds(MAXN));
+    RowFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, LineLds(fu.n), s>>>(
         dp, fu, rbuf, box);
   });
   WG_HIP(hipGetLastError());
@@ -451,8 +553,8 @@ void LaunchTranspose(const float2* in, float2* out, int rows, int cols, int npla
 void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse, hipStream_t s) {
   Dispatch(d.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(LineFFTKernel<BLOCK, MAXN>, MAXN * sizeof(float2));
-    LineFFTKernel<BLOCK, MAXN><<<dim3(nlines), BLOCK, d.n * sizeof(float2), s>>>(
+    AllowLds(LineFFTKernel<BLOCK, MAXN>, LineLds(MAXN));
+    LineFFTKernel<BLOCK, MAXN><<<dim3(nlines), BLOCK, LineLds(d.n), s>>>(
         data, d, inverse ? 1 : 0);
   });
   WG_HIP(hipGetLastError());

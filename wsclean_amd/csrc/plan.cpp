@@ -79,6 +79,7 @@ int GoodFFTSize(int n) {
 std::vector<int> FFTRadices(int n) {
   std::vector<int> r;
   int m = n;
+  while (m % 8 == 0) { r.push_back(8); m /= 8; }
   while (m % 4 == 0) { r.push_back(4); m /= 4; }
   while (m % 2 == 0) { r.push_back(2); m /= 2; }
   while (m % 3 == 0) { r.push_back(3); m /= 3; }
