@@ -87,36 +87,58 @@ def roofline_bytes(plan, n_vis, n_rows, nx, ny):
     return 2 * b_vis + P * (inv + pred)
 
 
-def kernel_roofline(st, plan):
-    """Roofline of the dominant kernel, the ES-kernel gridding kernel
-    (GridKernel<W>), from its own HIP events on the engine stream.
+MFMA_F32_PEAK = 157.3e12   # MI355X dense FP32 matrix FLOP/s
 
-    Algorithmic bytes per launch = 24 B per visibility pass (float4
-    coordinates + complex64 visibility read once per sub-slab it touches)
-    + 8 B x sub_n x E^2 per work item (its subgrid written once); summed by
-    the engine over the timed launches.  traffic = measured HBM bytes per
-    launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration) when a counter summary for this kernel and plan is
-    committed, else None."""
+
+def kernel_roofline(st, plan):
+    """Roofline of the dominant kernel, the ES-kernel gridding kernel, from
+    its own HIP events on the engine stream.
+
+    For W <= 8 the gridding runs on the FP32 matrix cores
+    (GridKernelMfma<W>): bound "mfma", algorithmic FLOPs = 4 W^3 per
+    visibility pass (each of the W^3 footprint cells takes a complex x real
+    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  The HBM
+    view is reported beside it: algorithmic bytes per launch = 24 B per
+    visibility pass (float4 coordinates + complex64 visibility, read once
+    per sub-slab the visibility touches) + 8 B x sub_n x E^2 per work item
+    (its subgrid written once), against 8 TB/s.  traffic = measured HBM
+    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the
+    copy-kernel calibration) from the committed counter summary of this
+    kernel and plan, else None."""
     n = st["grid_launches"]
     if not n or st["grid_kernel_ms"] <= 0:
         return None
-    per_launch = st["grid_bytes"] / n
+    W = plan["W"]
     t = st["grid_kernel_ms"] / n / 1e3
-    achieved = per_launch / t / 1e9
-    out = {"bound": "hbm", "kernel": "GridKernel<%d>" % plan["W"],
-           "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-           "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
-           "bytes_per_launch": per_launch, "launch_ms": t * 1e3,
-           "launches": int(n),
-           "degrid": {"kernel": "DegridKernel<%d>" % plan["W"],
-                      "achieved": st["degrid_bytes"] / max(st["degrid_kernel_ms"], 1e-9) / 1e6,
-                      "launch_ms": st["degrid_kernel_ms"] / max(st["degrid_launches"], 1)}}
+    per_launch_bytes = st["grid_bytes"] / n
+    per_launch_flops = 4.0 * W ** 3 * st["grid_passes"] / n
+    gbs = per_launch_bytes / t / 1e9
+    mfma = W <= 8
+    kernel = ("GridKernelMfma<%d>" if mfma else "GridKernel<%d>") % W
+    if mfma:
+        achieved = per_launch_flops / t / 1e12
+        out = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
+               "peak": MFMA_F32_PEAK / 1e12, "unit": "TFLOP/s",
+               "frac": achieved / (MFMA_F32_PEAK / 1e12)}
+    else:
+        out = {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK / 1e9,
+               "unit": "GB/s", "frac": gbs / (HBM_PEAK / 1e9)}
+    dt = st["degrid_kernel_ms"] / max(st["degrid_launches"], 1) / 1e3
+    out.update({
+        "traffic": None, "launch_ms": t * 1e3, "launches": int(n),
+        "flops_per_launch": per_launch_flops, "bytes_per_launch": per_launch_bytes,
+        "hbm_view": {"achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": gbs / (HBM_PEAK / 1e9)},
+        "degrid": {"kernel": ("DegridKernelMfma<%d>" if mfma else "DegridKernel<%d>") % W,
+                   "launch_ms": dt * 1e3,
+                   "achieved_tflops": 4.0 * W ** 3 * st["degrid_passes"]
+                   / max(st["degrid_launches"], 1) / max(dt, 1e-12) / 1e12}})
     path = os.path.join(ROOT, "profiles", "gridkernel_8192_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
             tr = json.load(f)
-        if tr.get("plan") == {k: plan[k] for k in tr.get("plan", {})}:
+        if tr.get("kernel") == kernel and \
+                tr.get("plan") == {k: plan[k] for k in tr.get("plan", {})}:
             out["traffic"] = tr["bytes_per_dispatch"]
     return out
 

@@ -120,12 +120,14 @@ int wg_predict_prepared(wg_handle* h, float* vis_out_dev);
 /* Plan of the last call as JSON (W, sigma, nu, nv, P, box, tiles...). */
 int wg_plan_json(const wg_handle* h, char* buf, size_t len);
 /* Per-stage device times (ms, HIP events on the handle's stream) accumulated
- * since the last reset while timing is enabled.  out12 = plan, sort, grid,
+ * since the last reset while timing is enabled.  out14 = plan, sort, grid,
  * fft, degrid, total; then for the gridding kernel alone: time (ms),
- * launches, algorithmic bytes; and the same three for the degridding
- * kernel.  No reference counterpart (measurement hook). */
+ * launches, algorithmic bytes; the same three for the degridding kernel;
+ * then the visibility passes of the gridding and of the degridding kernel
+ * (a visibility counts once per plane sub-slab it touches).  No reference
+ * counterpart (measurement hook). */
 int wg_set_timing(wg_handle* h, int enabled);
-int wg_stage_times(wg_handle* h, double* out12, int reset);
+int wg_stage_times(wg_handle* h, double* out14, int reset);
 /* Tuning overrides (0 = automatic): kernel support, oversampling, tile
  * edge, planes per gridding sub-slab, FFT-slab byte budget. */
 int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab,

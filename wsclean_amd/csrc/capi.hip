@@ -318,19 +318,20 @@ int wg_set_timing(wg_handle* h, int enabled) {
   });
 }
 
-int wg_stage_times(wg_handle* h, double* out12, int reset) {
+int wg_stage_times(wg_handle* h, double* out14, int reset) {
   return Guard([&] {
-    WG_CHECK(h && out12, "null argument");
+    WG_CHECK(h && out14, "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
     // every entry point synchronises its stream before returning, so all
     // recorded events have completed
     h->eng->CollectTimes();
     wg::StageTimes& t = h->eng->Times();
-    const double v[12] = {t.plan, t.sort, t.grid, t.fft, t.degrid,
+    const double v[14] = {t.plan, t.sort, t.grid, t.fft, t.degrid,
                           t.plan + t.sort + t.grid + t.fft + t.degrid,
                           t.grid_kernel_ms, t.grid_launches, t.grid_bytes,
-                          t.degrid_kernel_ms, t.degrid_launches, t.degrid_bytes};
-    std::copy(v, v + 12, out12);
+                          t.degrid_kernel_ms, t.degrid_launches, t.degrid_bytes,
+                          t.grid_passes, t.degrid_passes};
+    std::copy(v, v + 14, out14);
     if (reset) t = wg::StageTimes{};
   });
 }

@@ -359,6 +359,7 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
           // coordinates (16 B) and visibility (8 B) per pass, one subgrid
           // store per work item
           times_.grid_launches += 1;
+          times_.grid_passes += vs.sub_vis[g];
           times_.grid_bytes +=
               24.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
         }
@@ -408,6 +409,7 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
         // coordinates (16 B) read and accumulator (8 B) read + written per
         // pass, one subgrid load per work item
         times_.degrid_launches += 1;
+        times_.degrid_passes += vs.sub_vis[g];
         times_.degrid_bytes +=
             32.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
       }

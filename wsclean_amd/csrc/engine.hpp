@@ -61,6 +61,8 @@ struct StageTimes {
   // subgrids written / read)
   double grid_kernel_ms = 0, grid_launches = 0, grid_bytes = 0;
   double degrid_kernel_ms = 0, degrid_launches = 0, degrid_bytes = 0;
+  // visibility passes (a visibility counts once per sub-slab it touches)
+  double grid_passes = 0, degrid_passes = 0;
 };
 
 // HIP events recorded on the engine stream; read back only after the

@@ -173,11 +173,12 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_set_timing(self._h, 1 if on else 0))
 
     def stage_times(self, reset=True):
-        out = (ctypes.c_double * 12)()
+        out = (ctypes.c_double * 14)()
         _lib.check(self._lib.wg_stage_times(self._h, out, 1 if reset else 0))
         return dict(zip(["plan", "sort", "grid", "fft", "degrid", "total",
                          "grid_kernel_ms", "grid_launches", "grid_bytes",
-                         "degrid_kernel_ms", "degrid_launches", "degrid_bytes"], list(out)))
+                         "degrid_kernel_ms", "degrid_launches", "degrid_bytes",
+                         "grid_passes", "degrid_passes"], list(out)))
 
     def set_tuning(self, W=0, sigma=0.0, tile=0, sub_slab=0, slab_bytes=0):
         _lib.check(self._lib.wg_set_tuning(self._h, int(W), float(sigma), int(tile),
