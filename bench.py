@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--npix", type=int, default=8192)
     ap.add_argument("--eps", type=float, default=1e-5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-per-channel", action="store_true",
+                    help="skip the one-call-per-channel GPU comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--sg", type=int, default=0)
@@ -263,9 +265,46 @@ def main():
         "cpu_baseline": None,
         "gen_s": t_gen,
     }
+    if not args.no_per_channel:
+        line["per_channel_gpu"] = per_channel_leg(uvw, vis, freq, n, dx, args.eps, n_rows, value)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
     print(json.dumps(line))
+
+
+def per_channel_leg(uvw, vis, freq, n, dx, eps, n_rows, batched_value):
+    """The same workload as a plain one-call-per-channel GPU gridder: for
+    every channel a separate plan + sort, invert and predict (the channel's
+    visibilities copied to contiguous device buffers before timing).  One
+    pass, timed with the device synchronised on both sides."""
+    import torch
+
+    from wsclean_amd import WGriddingGridder_Simple
+    n_chan = len(freq)
+    vis_t = vis.permute(1, 0, 2).contiguous()     # [chan][rows][2]
+    out_t = torch.empty_like(vis_t)
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, eps)
+
+    def one(c):
+        f = freq[c:c + 1]
+        g.PrepareVisibilities(uvw, f, n_rows, 1)
+        g.AddInversionPrepared(vis_t[c])
+        g.PredictPrepared(out_t[c])
+
+    g.InitializeInversion()
+    one(0)   # warm the buffers
+    torch.cuda.synchronize()
+    g.InitializeInversion()
+    t0 = time.perf_counter()
+    for c in range(n_chan):
+        one(c)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    v = n_rows * n_chan / dt
+    del vis_t, out_t
+    torch.cuda.empty_cache()
+    return {"value": v, "unit": "vis/s", "ms": dt * 1e3, "calls": n_chan,
+            "batched_over_per_channel": batched_value / v}
 
 
 def ctypes_ptr(t):
