@@ -92,6 +92,27 @@ int wg_predict_visibilities(wg_handle* h, size_t n_rows, size_t n_chan,
 /* Thread-local message of the last failed call on this thread. */
 const char* wg_last_error(void);
 
+/* WGriddingGridderBase::AddInversionDataWithCorrectionCallback
+ * (wgridder/gridder_simple.h:29-35; VisibilityCallbackBuffer,
+ * wgridder/callback_implementation.h:34-136).  The reference reaches the
+ * facet solutions through an MsGridder callback; across the C-ABI they are
+ * passed explicitly: `gains` is the cached solution table
+ * [n_times][n_chan][n_antennas][nparms] complex<float> (nparms 2: diagonal
+ * X, Y; 4: full Jones), indexed per row by time_offsets[row] and the antenna
+ * pair antennas[2 row], antennas[2 row + 1] (gridding/visibilitymodifier.h:
+ * 539-594).  vis is [n_rows][n_chan][n_pol] complex<float>.  gain_mode in
+ * GainMode order: 0 xx, 1 yy, 2 trace (n_pol 1), 3 2-vis diagonal (n_pol 2),
+ * 4 full (n_pol 4).  Each visibility gets the forward gain (apply_forward,
+ * direction-dependent PSF mode) and then the conjugate gain, the
+ * polarizations are collapsed (2: sum; 4: Stokes I = (xx + yy) / 2) and the
+ * result is gridded as by wg_add_inversion_data. */
+int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, size_t n_rows,
+                                    const double* uvw, const double* freq, size_t n_chan,
+                                    const uint32_t* antennas, const float* vis,
+                                    const uint64_t* time_offsets, const float* gains,
+                                    size_t n_times, size_t n_antennas, size_t nparms,
+                                    int apply_forward);
+
 /* ------------------------------------------------------------------------
  * Device-resident variants.  Same semantics as the calls above, but uvw,
  * vis and vis_out are HIP device pointers (freq stays on the host).  They

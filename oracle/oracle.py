@@ -271,3 +271,91 @@ def es_phi_hat(W, beta, t):
 
     val, _ = quad(f, -W / 2, W / 2, limit=400, epsabs=1e-14, epsrel=1e-13)
     return val
+
+
+# ---------------------------------------------------------------------------
+# Per-antenna gain corrections and polarization collapse (the correction
+# callback of WGriddingGridder_Simple, wgridder/callback_implementation.h:
+# 78-97).  Gain modes in the order of GainMode (gridding/gainmode.h:16-31).
+GAIN_XX, GAIN_YY, GAIN_TRACE, GAIN_2VIS_DIAGONAL, GAIN_FULL = range(5)
+GAIN_NPOL = {GAIN_XX: 1, GAIN_YY: 1, GAIN_TRACE: 1, GAIN_2VIS_DIAGONAL: 2, GAIN_FULL: 4}
+
+
+def _jones(resp, nparms):
+    """2x2 gain matrices [..., 2, 2] from a cached solution row: nparms 2 is
+    diag(p0, p1), nparms 4 is [[p0, p1], [p2, p3]]
+    (gridding/visibilitymodifier.h:554-557, :578-579)."""
+    resp = np.asarray(resp)
+    g = np.zeros(resp.shape[:-1] + (2, 2), np.complex128)
+    if nparms == 2:
+        g[..., 0, 0] = resp[..., 0]
+        g[..., 1, 1] = resp[..., 1]
+    else:
+        g[..., 0, 0], g[..., 0, 1] = resp[..., 0], resp[..., 1]
+        g[..., 1, 0], g[..., 1, 1] = resp[..., 2], resp[..., 3]
+    return g
+
+
+def _apply_gain(mode, v, g1, g2, conjugated):
+    """One visibility (n_pol values) corrected by the gains of its two
+    antennas: ApplyGain (forward) and ApplyConjugatedGain
+    (gridding/visibilitymodifier.h:346-401).  Element order of a 2x2 matrix:
+    [0] xx, [1] xy, [2] yx, [3] yy."""
+    a = g1.reshape(4)
+    b = g2.reshape(4)
+    v = np.array(v, np.complex128)
+    if mode == GAIN_XX:
+        return np.array([np.conj(a[0]) * v[0] * b[0] if conjugated else a[0] * v[0] * np.conj(b[0])])
+    if mode == GAIN_YY:
+        return np.array([np.conj(a[3]) * v[0] * b[3] if conjugated else a[3] * v[0] * np.conj(b[3])])
+    if mode == GAIN_TRACE:
+        dd = np.sum(b * np.conj(a)) if conjugated else np.sum(a * np.conj(b))
+        return np.array([v[0] * 0.5 * dd])
+    if mode == GAIN_2VIS_DIAGONAL:
+        # the second line uses the updated first value, as the reference does
+        if conjugated:
+            v0 = np.conj(a[0]) * v[0] * b[0] + np.conj(a[2]) * v[1] * b[2]
+            v1 = np.conj(a[3]) * v[1] * b[3] + np.conj(a[1]) * v0 * b[1]
+        else:
+            v0 = a[0] * v[0] * np.conj(b[0]) + a[1] * v[1] * np.conj(b[1])
+            v1 = a[3] * v[1] * np.conj(b[3]) + a[2] * v0 * np.conj(b[2])
+        return np.array([v0, v1])
+    V = v.reshape(2, 2)
+    if conjugated:
+        R = g1.conj().T @ V @ g2
+    else:
+        R = g1 @ V @ g2.conj().T
+    return R.reshape(4)
+
+
+def correct_and_collapse(vis, antennas, time_offsets, gains, n_ant, nparms, mode, forward):
+    """vis [rows, chan, n_pol] complex -> corrected Stokes-I [rows, chan].
+
+    gains: cached solutions [n_times, n_chan, n_ant, nparms] complex, indexed
+    (time_offset * n_chan + ch) * n_ant * nparms
+    (gridding/visibilitymodifier.h:550-553).  forward applies the forward
+    gain first (direction-dependent PSF mode, :559-562).  Collapse:
+    2 polarizations -> v0 + v1 (gridding/msgridderdata.h:40-44), 4 -> Stokes
+    I = (xx + yy) / 2 (aocommon Polarization::ConvertFromLinear; aocommon is
+    absent from the reference mount, parity unpinned)."""
+    vis = np.asarray(vis)
+    n_rows, n_chan, npol = vis.shape
+    assert npol == GAIN_NPOL[mode]
+    out = np.zeros((n_rows, n_chan), np.complex128)
+    for r in range(n_rows):
+        a1, a2 = antennas[r]
+        t = time_offsets[r]
+        for c in range(n_chan):
+            g1 = _jones(gains[t, c, a1], nparms)
+            g2 = _jones(gains[t, c, a2], nparms)
+            v = vis[r, c]
+            if forward:
+                v = _apply_gain(mode, v, g1, g2, False)
+            v = _apply_gain(mode, v, g1, g2, True)
+            if npol == 1:
+                out[r, c] = v[0]
+            elif npol == 2:
+                out[r, c] = v[0] + v[1]
+            else:
+                out[r, c] = 0.5 * (v[0] + v[3])
+    return out

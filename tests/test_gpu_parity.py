@@ -289,3 +289,40 @@ def test_errors_leave_handle_usable():
     g.AddInversionData(p["uvw"].shape[0], 2, p["uvw"], p["freq"], p["vis"])
     g.FinalizeImage(1.0)
     assert np.isfinite(g.RealImage()).all()
+
+
+# ------------------------------------------------ correction callback (f3)
+
+@pytest.mark.parametrize("mode,nparms,forward", [(2, 2, False), (3, 2, False), (4, 4, False),
+                                                 (4, 4, True), (0, 4, False)])
+def test_corrected_inversion_matches_host_correction(oracle, mode, nparms, forward):
+    # GPU gain correction + collapse + gridding against the oracle's
+    # correction on the host followed by the plain inversion; the two paths
+    # differ only by FP32 rounding of the correction (tolerance 1e-5)
+    p = synth.small_problem(seed=31, n_ant=12, n_times=16, n_chan=3, npix=96)
+    rng = np.random.default_rng(mode * 10 + nparms)
+    n_rows, n_chan = p["uvw"].shape[0], len(p["freq"])
+    npol = oracle.GAIN_NPOL[mode]
+    n_ant, n_times = 12, 4
+    vis = np.repeat(p["vis"][:, :, None], npol, axis=2)
+    vis = vis + 0.1 * (rng.normal(size=vis.shape) + 1j * rng.normal(size=vis.shape))
+    vis = vis.astype(np.complex64)
+    g = (1.0 + 0.2 * (rng.normal(size=(n_times, n_chan, n_ant, nparms))
+                      + 1j * rng.normal(size=(n_times, n_chan, n_ant, nparms))))
+    if nparms == 4:
+        g[..., 1:3] -= 1.0
+    g = g.astype(np.complex64)
+    ant = rng.integers(0, n_ant, (n_rows, 2)).astype(np.uint32)
+    toff = rng.integers(0, n_times, n_rows).astype(np.uint64)
+    host = oracle.correct_and_collapse(vis, ant, toff, g, n_ant, nparms, mode, forward)
+    ref_g = make(96, p["dx"], 1e-5)
+    ref_g.InitializeInversion()
+    ref_g.AddInversionData(n_rows, n_chan, p["uvw"], p["freq"], host.astype(np.complex64))
+    ref_g.FinalizeImage(1.0)
+    ref = ref_g.RealImage()
+    gg = make(96, p["dx"], 1e-5)
+    gg.InitializeInversion()
+    gg.AddInversionDataWithCorrectionCallback(mode, npol, n_rows, p["uvw"], p["freq"], n_chan,
+                                              ant, vis, toff, g, apply_forward=forward)
+    gg.FinalizeImage(1.0)
+    assert rel_l2(gg.RealImage(), ref) < 1e-5

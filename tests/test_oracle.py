@@ -203,3 +203,65 @@ def test_cpu_wgridder_predict_matches_dft(oracle):
                                  **prm)
     ref = oracle.dft_predict(n, n, p["dx"], p["dx"], 0.0, 0.0, model, p["uvw"], p["freq"])
     assert np.linalg.norm(V - ref) / np.linalg.norm(ref) < 1e-5
+
+
+# ------------------------------------------------- gain corrections (f3)
+
+def _rand_jones(rng, n, diag=False):
+    g = (rng.normal(size=(n, 2, 2)) + 1j * rng.normal(size=(n, 2, 2))) * 0.3
+    g += np.eye(2)
+    if diag:
+        g[:, 0, 1] = g[:, 1, 0] = 0
+    return g
+
+
+def test_gain_identity_leaves_visibilities(oracle):
+    rng = np.random.default_rng(3)
+    for mode, npol in oracle.GAIN_NPOL.items():
+        v = rng.normal(size=npol) + 1j * rng.normal(size=npol)
+        for conj in (False, True):
+            np.testing.assert_allclose(oracle._apply_gain(mode, v, np.eye(2), np.eye(2), conj), v)
+
+
+def test_gain_modes_against_matrix_algebra(oracle):
+    # xx / yy / trace and the 2-vis diagonal mode are the entries of the full
+    # 2x2 form G1^H V G2 (conjugate) and G1 V G2^H (forward) that they select
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        a, b = _rand_jones(rng, 2)
+        ad, bd = _rand_jones(rng, 2, diag=True)
+        V = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        full_c = a.conj().T @ V @ b
+        full_f = a @ V @ b.conj().T
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_FULL, V.ravel(), a, b, True),
+                                   full_c.ravel())
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_FULL, V.ravel(), a, b, False),
+                                   full_f.ravel())
+        s = V[0, 0]
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_XX, [s], a, b, True)[0],
+                                   np.conj(a[0, 0]) * s * b[0, 0])
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_YY, [s], a, b, False)[0],
+                                   a[1, 1] * s * np.conj(b[1, 1]))
+        # trace: 0.5 tr(G1^H G2) (conjugate), 0.5 tr(G1 G2^H) (forward)
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_TRACE, [s], a, b, True)[0],
+                                   s * 0.5 * np.trace(a.conj().T @ b))
+        np.testing.assert_allclose(oracle._apply_gain(oracle.GAIN_TRACE, [s], a, b, False)[0],
+                                   s * 0.5 * np.trace(a @ b.conj().T))
+        # diagonal gains: the 2-vis mode is the diagonal of the full form
+        d = oracle._apply_gain(oracle.GAIN_2VIS_DIAGONAL, [V[0, 0], V[1, 1]], ad, bd, True)
+        full_d = ad.conj().T @ V @ bd
+        np.testing.assert_allclose(d, [full_d[0, 0], full_d[1, 1]])
+
+
+def test_correct_and_collapse_unitary_round_trip(oracle):
+    # forward then conjugate gain with unitary Jones matrices is the identity
+    rng = np.random.default_rng(5)
+    n_rows, n_chan, n_ant, n_times = 6, 3, 4, 2
+    q, _ = np.linalg.qr(rng.normal(size=(n_times * n_chan * n_ant, 2, 2))
+                        + 1j * rng.normal(size=(n_times * n_chan * n_ant, 2, 2)))
+    gains = q.reshape(n_times, n_chan, n_ant, 4)
+    vis = rng.normal(size=(n_rows, n_chan, 4)) + 1j * rng.normal(size=(n_rows, n_chan, 4))
+    ant = rng.integers(0, n_ant, (n_rows, 2))
+    toff = rng.integers(0, n_times, n_rows)
+    out = oracle.correct_and_collapse(vis, ant, toff, gains, n_ant, 4, oracle.GAIN_FULL, True)
+    np.testing.assert_allclose(out, 0.5 * (vis[..., 0] + vis[..., 3]), atol=1e-12)

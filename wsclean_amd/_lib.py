@@ -41,6 +41,10 @@ SIGNATURES = [
     ("wg_predict_visibilities", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_last_error", ctypes.c_char_p, []),
+    ("wg_add_inversion_data_corrected", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, c_size,
+      ctypes.c_int]),
     ("wg_add_inversion_data_device", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_predict_visibilities_device", ctypes.c_int,

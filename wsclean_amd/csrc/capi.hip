@@ -182,6 +182,50 @@ int wg_add_inversion_data(wg_handle* h, size_t n_rows, size_t n_chan, const doub
   });
 }
 
+int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, size_t n_rows,
+                                    const double* uvw, const double* freq, size_t n_chan,
+                                    const uint32_t* antennas, const float* vis,
+                                    const uint64_t* time_offsets, const float* gains,
+                                    size_t n_times, size_t n_antennas, size_t nparms,
+                                    int apply_forward) {
+  return Guard([&] {
+    WG_CHECK(h && freq && (n_rows == 0 || (uvw && antennas && vis && time_offsets && gains)),
+             "null argument");
+    WG_CHECK(gain_mode >= 0 && gain_mode <= 4, "unknown gain mode");
+    static const size_t kNPol[5] = {1, 1, 1, 2, 4};
+    WG_CHECK(n_pol == kNPol[gain_mode], "polarization count does not match the gain mode");
+    WG_CHECK(nparms == 2 || nparms == 4, "solutions have 2 or 4 values per antenna");
+    WG_CHECK(n_antennas > 0 && n_times > 0, "empty solution table");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (n_rows == 0) return;
+    for (size_t r = 0; r < n_rows; ++r) {
+      WG_CHECK(antennas[2 * r] < n_antennas && antennas[2 * r + 1] < n_antennas,
+               "antenna index out of range");
+      WG_CHECK(time_offsets[r] < n_times, "time offset out of range");
+    }
+    wg::DeviceArena& a = h->eng->Arena();
+    const size_t nvis = n_rows * n_chan;
+    const size_t ngain = n_times * n_chan * n_antennas * nparms;
+    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
+    float2* dpol = static_cast<float2*>(a.Get("h2d.polvis", nvis * n_pol * sizeof(float2)));
+    uint32_t* dant = static_cast<uint32_t*>(a.Get("h2d.ant", n_rows * 2 * sizeof(uint32_t)));
+    uint64_t* dtoff = static_cast<uint64_t*>(a.Get("h2d.toff", n_rows * sizeof(uint64_t)));
+    float2* dgain = static_cast<float2*>(a.Get("h2d.gains", ngain * sizeof(float2)));
+    float2* dvis = static_cast<float2*>(a.Get("h2d.vis", nvis * sizeof(float2)));
+    hipStream_t s = h->s();
+    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    WG_HIP(hipMemcpyAsync(dpol, vis, nvis * n_pol * sizeof(float2), hipMemcpyHostToDevice, s));
+    WG_HIP(hipMemcpyAsync(dant, antennas, n_rows * 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          s));
+    WG_HIP(hipMemcpyAsync(dtoff, time_offsets, n_rows * sizeof(uint64_t), hipMemcpyHostToDevice,
+                          s));
+    WG_HIP(hipMemcpyAsync(dgain, gains, ngain * sizeof(float2), hipMemcpyHostToDevice, s));
+    wg::LaunchCorrectCollapse(dpol, dant, dtoff, dgain, n_rows, n_chan, n_antennas, int(nparms),
+                              gain_mode, int(n_pol), apply_forward != 0, dvis, s);
+    AddInversion(h, n_rows, n_chan, duvw, freq, reinterpret_cast<const float*>(dvis));
+  });
+}
+
 int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
                                  const double* uvw_dev, const double* freq,
                                  const float* vis_dev) {

@@ -220,6 +220,14 @@ void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+// out[row][ch] = Stokes-I collapse of the conjugate-gain (after the forward
+// gain when `forward`) corrected vis[row][ch][0..npol).  gains: cached
+// solutions [n_times][n_chan][n_ant][nparms]; mode in GainMode order
+// (xx, yy, trace, 2-vis diagonal, full).
+void LaunchCorrectCollapse(const float2* vis, const uint32_t* antennas,
+                           const uint64_t* time_offsets, const float2* gains, size_t n_rows,
+                           size_t n_chan, size_t n_ant, int nparms, int mode, int npol,
+                           bool forward, float2* out, hipStream_t s);
 // out[k] = in[k * stride] for k in [0, count)
 void LaunchStridedGather(const uint32_t* in, size_t stride, size_t count, uint32_t* out,
                          hipStream_t s);

@@ -226,6 +226,109 @@ __global__ void StridedGatherKernel(const uint32_t* __restrict__ in, size_t stri
   if (k < count) out[k] = in[k * stride];
 }
 
+// ---- per-antenna gain corrections (correction callback) ----
+// 2x2 complex matrices with elements [0] xx, [1] xy, [2] yx, [3] yy.
+struct C2x2 {
+  float2 e[4];
+};
+__device__ __forceinline__ float2 cm(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 ca(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+// A B (AH: A^H B, BH: A B^H)
+template <bool AH, bool BH>
+__device__ __forceinline__ C2x2 MatMul(const C2x2& A, const C2x2& B) {
+  C2x2 R;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const float2 a = AH ? cj(A.e[k * 2 + i]) : A.e[i * 2 + k];
+        const float2 b = BH ? cj(B.e[j * 2 + k]) : B.e[k * 2 + j];
+        s = ca(s, cm(a, b));
+      }
+      R.e[i * 2 + j] = s;
+    }
+  return R;
+}
+
+__device__ __forceinline__ C2x2 LoadJones(const float2* __restrict__ p, int nparms) {
+  C2x2 g;
+  if (nparms == 2) {
+    g.e[0] = p[0];
+    g.e[1] = g.e[2] = make_float2(0.f, 0.f);
+    g.e[3] = p[1];
+  } else {
+    for (int i = 0; i < 4; ++i) g.e[i] = p[i];
+  }
+  return g;
+}
+
+// ApplyGain (conj = false) / ApplyConjugatedGain (conj = true) of
+// gridding/visibilitymodifier.h:346-401 on the n_pol values of v.
+__device__ __forceinline__ void ApplyJones(int mode, bool conj, float2* v, const C2x2& a,
+                                           const C2x2& b) {
+  switch (mode) {
+    case 0:  // xx
+      v[0] = conj ? cm(cm(cj(a.e[0]), v[0]), b.e[0]) : cm(cm(a.e[0], v[0]), cj(b.e[0]));
+      break;
+    case 1:  // yy
+      v[0] = conj ? cm(cm(cj(a.e[3]), v[0]), b.e[3]) : cm(cm(a.e[3], v[0]), cj(b.e[3]));
+      break;
+    case 2: {  // trace: 0.5 * sum_i x_i conj(y_i)
+      float2 dd = make_float2(0.f, 0.f);
+      for (int i = 0; i < 4; ++i)
+        dd = ca(dd, conj ? cm(b.e[i], cj(a.e[i])) : cm(a.e[i], cj(b.e[i])));
+      v[0] = cm(v[0], make_float2(0.5f * dd.x, 0.5f * dd.y));
+      break;
+    }
+    case 3:  // two diagonal visibilities; the second uses the updated first
+      if (conj) {
+        v[0] = ca(cm(cm(cj(a.e[0]), v[0]), b.e[0]), cm(cm(cj(a.e[2]), v[1]), b.e[2]));
+        v[1] = ca(cm(cm(cj(a.e[3]), v[1]), b.e[3]), cm(cm(cj(a.e[1]), v[0]), b.e[1]));
+      } else {
+        v[0] = ca(cm(cm(a.e[0], v[0]), cj(b.e[0])), cm(cm(a.e[1], v[1]), cj(b.e[1])));
+        v[1] = ca(cm(cm(a.e[3], v[1]), cj(b.e[3])), cm(cm(a.e[2], v[0]), cj(b.e[2])));
+      }
+      break;
+    default: {  // full 2x2: a^H V b, or a V b^H
+      C2x2 V;
+      for (int i = 0; i < 4; ++i) V.e[i] = v[i];
+      const C2x2 R = conj ? MatMul<false, false>(MatMul<true, false>(a, V), b)
+                          : MatMul<false, true>(MatMul<false, false>(a, V), b);
+      for (int i = 0; i < 4; ++i) v[i] = R.e[i];
+    }
+  }
+}
+
+// out[row][ch] = collapse(conj-gain(forward-gain(vis[row][ch][0..npol))))
+__global__ void __launch_bounds__(kBlock) CorrectCollapseKernel(
+    const float2* __restrict__ vis, const uint32_t* __restrict__ antennas,
+    const uint64_t* __restrict__ time_offsets, const float2* __restrict__ gains,
+    size_t n_rows, size_t n_chan, size_t n_ant, int nparms, int mode, int npol, int forward,
+    float2* __restrict__ out) {
+  const size_t n = n_rows * n_chan;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t row = i / n_chan, ch = i - row * n_chan;
+    const size_t base = (size_t(time_offsets[row]) * n_chan + ch) * n_ant;
+    const C2x2 a = LoadJones(gains + (base + antennas[2 * row]) * nparms, nparms);
+    const C2x2 b = LoadJones(gains + (base + antennas[2 * row + 1]) * nparms, nparms);
+    float2 v[4];
+    for (int p = 0; p < npol; ++p) v[p] = vis[i * npol + p];
+    if (forward) ApplyJones(mode, false, v, a, b);
+    ApplyJones(mode, true, v, a, b);
+    float2 r = v[0];
+    if (npol == 2) r = ca(v[0], v[1]);
+    if (npol == 4) r = make_float2(0.5f * (v[0].x + v[3].x), 0.5f * (v[0].y + v[3].y));
+    out[i] = r;
+  }
+}
+
 __global__ void FillZeroKernel(uint32_t* __restrict__ p, size_t nwords) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nwords;
        i += size_t(gridDim.x) * kBlock)
@@ -379,6 +482,18 @@ void LaunchFillZero(void* p, size_t bytes, hipStream_t s) {
   if (!bytes) return;
   WG_CHECK(bytes % 4 == 0, "fill size must be a multiple of 4");
   FillZeroKernel<<<Blocks(bytes / 4), kBlock, 0, s>>>(static_cast<uint32_t*>(p), bytes / 4);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchCorrectCollapse(const float2* vis, const uint32_t* antennas,
+                           const uint64_t* time_offsets, const float2* gains, size_t n_rows,
+                           size_t n_chan, size_t n_ant, int nparms, int mode, int npol,
+                           bool forward, float2* out, hipStream_t s) {
+  const size_t n = n_rows * n_chan;
+  if (!n) return;
+  CorrectCollapseKernel<<<Blocks(n), kBlock, 0, s>>>(vis, antennas, time_offsets, gains, n_rows,
+                                                     n_chan, n_ant, nparms, mode, npol,
+                                                     forward ? 1 : 0, out);
   WG_HIP(hipGetLastError());
 }
 

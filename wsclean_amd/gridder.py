@@ -100,13 +100,29 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_add_inversion_data(self._h, n_rows, n_chan, _ptr(uvw),
                                                    _ptr(freq), _ptr(vis)))
 
-    def AddInversionDataWithCorrectionCallback(self, *args, **kwargs):
-        # The reference applies per-antenna facet solutions through a C++
-        # callback (wgridder/callback_implementation.h:34-136); callbacks do
-        # not cross a C-ABI.  Callers apply the corrections on the host and
-        # call AddInversionData.
-        raise NotImplementedError(
-            "apply facet corrections before the call and use AddInversionData")
+    def AddInversionDataWithCorrectionCallback(self, gain_mode, n_polarizations, n_rows, uvws,
+                                               frequencies, n_channels, antennas, visibilities,
+                                               time_offsets, gains, apply_forward=False):
+        """Mirror of AddInversionDataWithCorrectionCallback
+        (wgridder/gridder_simple.h:29-35).  The MsGridder callback state is
+        passed explicitly: antennas [n_rows, 2] antenna pairs, time_offsets
+        [n_rows] rows of `gains`, gains [n_times, n_chan, n_antennas, nparms]
+        complex64 (the cached solution table), visibilities [n_rows, n_chan,
+        n_pol] complex64.  gain_mode: 0 xx, 1 yy, 2 trace, 3 2-vis diagonal,
+        4 full (GainMode order)."""
+        uvw = np.ascontiguousarray(uvws, np.float64).reshape(-1, 3)
+        freq = np.ascontiguousarray(frequencies, np.float64)
+        ant = np.ascontiguousarray(antennas, np.uint32).reshape(-1, 2)
+        vis = np.ascontiguousarray(visibilities, np.complex64)
+        toff = np.ascontiguousarray(time_offsets, np.uint64)
+        g = np.ascontiguousarray(gains, np.complex64)
+        n_times, n_chan_g, n_ant, nparms = g.shape
+        if n_chan_g != n_channels or vis.size != n_rows * n_channels * n_polarizations:
+            raise ValueError("shape mismatch")
+        _lib.check(self._lib.wg_add_inversion_data_corrected(
+            self._h, int(gain_mode), int(n_polarizations), int(n_rows), _ptr(uvw), _ptr(freq),
+            int(n_channels), _ptr(ant), _ptr(vis), _ptr(toff), _ptr(g), int(n_times), int(n_ant),
+            int(nparms), 1 if apply_forward else 0))
 
     def FinalizeImage(self, multiplication_factor):
         _lib.check(self._lib.wg_finalize_image(self._h, float(multiplication_factor)))
