@@ -38,6 +38,12 @@ def parse():
     ap.add_argument("--no-per-channel", action="store_true",
                     help="skip the one-call-per-channel GPU comparison leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests in the served-stream leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served-stream leg and print its JSON")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--sg", type=int, default=0)
     ap.add_argument("--W", type=int, default=0)
@@ -191,6 +197,9 @@ def main():
     from wsclean_amd import WGriddingGridder_Simple, _lib
 
     torch.cuda.set_device(0)
+    if args.serve_only:
+        print(json.dumps(serving_leg(args)))
+        return
     dev = torch.device("cuda:0")
     t_gen = time.time()
     uvw_h, freq, dx, src = make_inputs(args.channels, args.npix)
@@ -267,6 +276,11 @@ def main():
     }
     if not args.no_per_channel:
         line["per_channel_gpu"] = per_channel_leg(uvw, vis, freq, n, dx, args.eps, n_rows, value)
+    g.close()
+    del uvw, vis, out
+    torch.cuda.empty_cache()
+    if args.serve_requests > 0:
+        line["serving"] = serving_leg(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
     print(json.dumps(line))
@@ -305,6 +319,43 @@ def per_channel_leg(uvw, vis, freq, n, dx, eps, n_rows, batched_value):
     torch.cuda.empty_cache()
     return {"value": v, "unit": "vis/s", "ms": dt * 1e3, "calls": n_chan,
             "batched_over_per_channel": batched_value / v}
+
+
+def serving_leg(args):
+    """Served request stream: seeded Poisson arrivals of requests with
+    image sizes 512^2 ... 8192^2 and 1e5 ... 5e7 visibilities from random
+    32-256 antenna arrays over 1-64 channels, each 5 major cycles (invert,
+    image to host, model, predict, residual) on device-resident sorted
+    visibilities, admitted by free slot and HBM.  Latency is arrival to
+    completion; throughput counts one grid + degrid per visibility and
+    cycle over the stream's span."""
+    import torch
+
+    from wsclean_amd.serve import Server, latency_stats, make_requests, run_request
+    # one untimed request loads the kernels' code objects
+    warm = make_requests(1, 1.0, seed=1, npix_choices=[512], vis_range=(1e5, 1.1e5), cycles=1,
+                         eps=args.eps)
+    run_request(warm[0])
+    del warm
+    t = time.time()
+    reqs = make_requests(args.serve_requests, args.serve_rate, seed=1005, cycles=5, eps=args.eps)
+    gen_s = time.time() - t
+    srv = Server(slots=args.serve_slots)
+    out = srv.serve(reqs)
+    s = latency_stats(out)
+    s.update({"workload": "served stream: %d requests, Poisson %.1f/s, 512^2-8192^2, "
+                          "1e5-5e7 vis, 5 major cycles, eps %.0e" % (len(reqs), args.serve_rate,
+                                                                     args.eps),
+              "slots": srv.slots, "peak_running": srv.peak_running,
+              "peak_admitted_gb": srv.peak_bytes / 1e9, "gen_s": gen_s,
+              "per_request": [{"npix": r.npix, "n_vis": r.n_vis, "n_chan": r.n_chan,
+                               "arrival_s": round(r.arrival, 4),
+                               "wait_ms": round((r.start - r.arrival) * 1e3, 2),
+                               "latency_ms": round((r.done - r.arrival) * 1e3, 2)}
+                              for r in out]})
+    del reqs, out
+    torch.cuda.empty_cache()
+    return s
 
 
 def ctypes_ptr(t):

@@ -125,6 +125,11 @@ int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
 int wg_predict_visibilities_device(wg_handle* h, size_t n_rows, size_t n_chan,
                                    const double* uvw_dev, const double* freq,
                                    float* vis_out_dev);
+/* RealImage / InitializePrediction (gridder_simple.h:37-38) with the
+ * width*height float image in device memory: a major cycle's dirty image
+ * and model stay in HBM. */
+int wg_real_image_device(wg_handle* h, float* image_dev);
+int wg_initialize_prediction_device(wg_handle* h, const float* image_dev);
 
 /* ------------------------------------------------------------------------
  * Major-cycle variants.  A predict followed by an invert on the same uvw

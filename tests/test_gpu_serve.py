@@ -1,0 +1,74 @@
+"""Served request stream on the GPU: each request's result is the same
+alone and in the mix (bitwise: every kernel of the path is deterministic),
+and the requests of the mix do overlap on the device."""
+import itertools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone_results():
+    import torch
+    from wsclean_amd.serve import Server, latency_stats, make_requests, run_request
+    reqs = make_requests(6, rate_hz=200.0, seed=17, npix_choices=[256, 512],
+                         vis_range=(3e4, 4e5), cycles=2)
+    print("served mix:", [(r.rid, r.npix, r.n_rows, r.n_chan) for r in reqs], flush=True)
+    alone = []
+    for r in reqs:
+        img = run_request(r)
+        alone.append((img.copy(), r.out.cpu().numpy().copy()))
+        r.out.zero_()
+    torch.cuda.synchronize()
+    srv = Server(slots=3)
+    out = srv.serve(reqs)
+    for r, (img, vis) in zip(out, alone):
+        assert np.isfinite(img).all()
+        assert np.abs(img).max() > 0
+        assert np.array_equal(r.image, img), "request %d image differs in the mix" % r.rid
+        assert np.array_equal(r.out.cpu().numpy(), vis), "request %d prediction differs" % r.rid
+    assert srv.peak_running >= 2
+    assert any(a.start < b.done and b.start < a.done for a, b in itertools.combinations(out, 2))
+    s = latency_stats(out)
+    print("served stats:", s, flush=True)
+    assert s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_device_image_calls_match_host_calls():
+    """RealImageDevice / InitializePredictionDevice give what RealImage /
+    InitializePrediction give (trimmed image centred, NaN border)."""
+    import torch
+    from wsclean_amd import WGriddingGridder_Simple, synth
+    rng = np.random.Generator(np.random.PCG64(5))
+    enu = synth.antennas_disc(rng, 24, 1500.0)
+    uvw_h = synth.earth_rotation_uvw(enu, 30, 10.0, np.deg2rad(-30.0))
+    freq = np.array([140e6, 160e6])
+    n_rows = uvw_h.shape[0]
+    vis_h = (rng.standard_normal((n_rows, 2)) + 1j * rng.standard_normal((n_rows, 2))).astype(
+        np.complex64)
+    dev = torch.device("cuda:0")
+    uvw = torch.from_numpy(uvw_h).to(dev)
+    vis = torch.from_numpy(vis_h.view(np.float32).reshape(n_rows, 2, 2)).to(dev)
+    w, h, wt, ht = 192, 160, 128, 128
+    g = WGriddingGridder_Simple(w, h, wt, ht, 20 * synth.ARCSEC, 20 * synth.ARCSEC, 0.0, 0.0, 1,
+                                1e-5)
+    g.PrepareVisibilities(uvw, freq, n_rows, 2)
+    g.InitializeInversion()
+    g.AddInversionPrepared(vis)
+    g.FinalizeImage(1.0 / (2 * n_rows))
+    host = g.RealImage()
+    img = torch.full((w * h,), 7.0, dtype=torch.float32, device=dev)
+    g.RealImageDevice(img)
+    assert np.array_equal(img.cpu().numpy(), host, equal_nan=True)
+    model = rng.standard_normal(w * h).astype(np.float32)
+    out_h = torch.empty_like(vis)
+    out_d = torch.empty_like(vis)
+    g.InitializePrediction(model)
+    g.PredictPrepared(out_h)
+    g.InitializePredictionDevice(torch.from_numpy(model).to(dev))
+    g.PredictPrepared(out_d)
+    assert torch.equal(out_h, out_d)
+    with pytest.raises(Exception):
+        g.RealImageDevice(torch.empty(w * h - 1, dtype=torch.float32, device=dev))
+    g.close()

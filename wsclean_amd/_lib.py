@@ -38,6 +38,8 @@ SIGNATURES = [
     ("wg_finalize_image", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     ("wg_real_image", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_initialize_prediction", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_real_image_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_initialize_prediction_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_predict_visibilities", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_last_error", ctypes.c_char_p, []),

@@ -113,7 +113,7 @@ int wg_create(const wg_params* params, wg_handle** out) {
     g.epsilon = params->epsilon;
     wg::PlanOptions opt;
     h->eng = std::make_unique<wg::Engine>(g, opt);
-    WG_HIP(hipMalloc(&h->img, h->npix() * sizeof(double)));
+    h->img = static_cast<double*>(h->eng->Arena().Get("image.acc", h->npix() * sizeof(double)));
     WG_HIP(hipMemsetAsync(h->img, 0, h->npix() * sizeof(double), h->s()));
     WG_HIP(hipStreamSynchronize(h->s()));
     *out = h.release();
@@ -123,8 +123,7 @@ int wg_create(const wg_params* params, wg_handle** out) {
 int wg_destroy(wg_handle* h) {
   return Guard([&] {
     if (!h) return;
-    if (h->img) (void)hipFree(h->img);
-    delete h;
+    delete h;  // the engine releases the image with its arena
   });
 }
 
@@ -247,19 +246,44 @@ int wg_finalize_image(wg_handle* h, double f) {
   });
 }
 
+// Writes the width*height float image (trimmed image centred, NaN border)
+// to device memory `d` on the handle's stream.
+static void RealImageTo(wg_handle* h, float* d) {
+  const size_t n = h->prm.width * h->prm.height;
+  RealImageKernel<<<Blocks(n), 256, 0, h->s()>>>(h->img, d, int(h->prm.width),
+                                               int(h->prm.height), int(h->prm.width_t),
+                                               int(h->prm.height_t));
+  WG_HIP(hipGetLastError());
+}
+
 int wg_real_image(wg_handle* h, float* out) {
   return Guard([&] {
     WG_CHECK(h && out, "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
     const size_t n = h->prm.width * h->prm.height;
     float* d = static_cast<float*>(h->eng->Arena().Get("d2h.image", n * 4));
-    RealImageKernel<<<Blocks(n), 256, 0, h->s()>>>(h->img, d, int(h->prm.width),
-                                                 int(h->prm.height), int(h->prm.width_t),
-                                                 int(h->prm.height_t));
-    WG_HIP(hipGetLastError());
+    RealImageTo(h, d);
     WG_HIP(hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, h->s()));
     WG_HIP(hipStreamSynchronize(h->s()));
   });
+}
+
+int wg_real_image_device(wg_handle* h, float* image_dev) {
+  return Guard([&] {
+    WG_CHECK(h && image_dev, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    RealImageTo(h, image_dev);
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+// Loads the centred trimmed region of the width*height float model `d`
+// (device memory) into the fp64 image on the handle's stream.
+static void LoadModelFrom(wg_handle* h, const float* d) {
+  LoadModelKernel<<<Blocks(h->npix()), 256, 0, h->s()>>>(d, h->img, int(h->prm.width),
+                                                       int(h->prm.height), int(h->prm.width_t),
+                                                       int(h->prm.height_t));
+  WG_HIP(hipGetLastError());
 }
 
 int wg_initialize_prediction(wg_handle* h, const float* image) {
@@ -269,11 +293,16 @@ int wg_initialize_prediction(wg_handle* h, const float* image) {
     const size_t n = h->prm.width * h->prm.height;
     float* d = static_cast<float*>(h->eng->Arena().Get("h2d.image", n * 4));
     WG_HIP(hipMemcpyAsync(d, image, n * 4, hipMemcpyHostToDevice, h->s()));
-    LoadModelKernel<<<Blocks(h->npix()), 256, 0, h->s()>>>(d, h->img, int(h->prm.width),
-                                                         int(h->prm.height),
-                                                         int(h->prm.width_t),
-                                                         int(h->prm.height_t));
-    WG_HIP(hipGetLastError());
+    LoadModelFrom(h, d);
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_initialize_prediction_device(wg_handle* h, const float* image_dev) {
+  return Guard([&] {
+    WG_CHECK(h && image_dev, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    LoadModelFrom(h, image_dev);
     WG_HIP(hipStreamSynchronize(h->s()));
   });
 }

@@ -60,16 +60,36 @@ DevPlan ToDevPlan(const Plan& p) {
 
 DeviceArena::~DeviceArena() { ReleaseAll(); }
 
+void DeviceArena::SetStream(hipStream_t s) {
+  stream_ = s;
+  int dev = 0, pools = 0;
+  async_ = hipGetDevice(&dev) == hipSuccess &&
+           hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, dev) ==
+               hipSuccess &&
+           pools != 0;
+}
+
+void DeviceArena::Free(void* p) {
+  if (!p) return;
+  if (async_)
+    (void)hipFreeAsync(p, stream_);
+  else
+    (void)hipFree(p);
+}
+
 void* DeviceArena::Get(const std::string& name, size_t bytes) {
   Slot& s = slots_[name];
   if (bytes == 0) bytes = 256;
   if (s.bytes < bytes) {
-    if (s.ptr) WG_HIP(hipFree(s.ptr));
+    Free(s.ptr);
     s.ptr = nullptr;
     s.bytes = 0;
     // round up so that slowly growing requests do not reallocate every call
     size_t b = bytes + bytes / 8 + 256;
-    WG_HIP(hipMalloc(&s.ptr, b));
+    if (async_)
+      WG_HIP(hipMallocAsync(&s.ptr, b, stream_));
+    else
+      WG_HIP(hipMalloc(&s.ptr, b));
     s.bytes = b;
   }
   return s.ptr;
@@ -78,13 +98,12 @@ void* DeviceArena::Get(const std::string& name, size_t bytes) {
 void DeviceArena::Release(const std::string& name) {
   auto it = slots_.find(name);
   if (it == slots_.end()) return;
-  if (it->second.ptr) (void)hipFree(it->second.ptr);
+  Free(it->second.ptr);
   slots_.erase(it);
 }
 
 void DeviceArena::ReleaseAll() {
-  for (auto& kv : slots_)
-    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& kv : slots_) Free(kv.second.ptr);
   slots_.clear();
 }
 
@@ -98,23 +117,25 @@ size_t DeviceArena::Allocated() const {
 
 Engine::Engine(const ImageGeometry& g, const PlanOptions& opt) : geom_(g), opt_(opt) {
   WG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  arena_.SetStream(stream_);
 }
 
 Engine::~Engine() {
   if (stream_) {
+    arena_.ReleaseAll();  // stream-ordered frees, before the stream goes
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
   }
-  for (auto& kv : twiddles_) (void)hipFree(kv.second);
 }
 
 const float2* Engine::Twiddles(int n) {
   auto it = twiddles_.find(n);
   if (it != twiddles_.end()) return it->second;
   const std::vector<float2> h = FFTTwiddleTable(n);
-  float2* d = nullptr;
-  WG_HIP(hipMalloc(&d, h.size() * sizeof(float2)));
-  WG_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+  float2* d = static_cast<float2*>(
+      arena_.Get("twiddles." + std::to_string(n), h.size() * sizeof(float2)));
+  WG_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
+  WG_HIP(hipStreamSynchronize(stream_));
   twiddles_[n] = d;
   return d;
 }

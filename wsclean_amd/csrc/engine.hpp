@@ -36,10 +36,15 @@ struct WorkItem {
   uint32_t end;
 };
 
-// Grow-only cache of named device allocations.
+// Grow-only cache of named device allocations, stream-ordered on the
+// owning engine's stream (hipMallocAsync / hipFreeAsync): growing or
+// releasing a slot never synchronizes the device, so handles that serve
+// requests side by side do not stall each other.  Every use of an arena
+// buffer is on that same stream.
 class DeviceArena {
  public:
   ~DeviceArena();
+  void SetStream(hipStream_t s);
   void* Get(const std::string& name, size_t bytes);
   void Release(const std::string& name);
   void ReleaseAll();
@@ -50,7 +55,10 @@ class DeviceArena {
     void* ptr = nullptr;
     size_t bytes = 0;
   };
+  void Free(void* p);
   std::map<std::string, Slot> slots_;
+  hipStream_t stream_ = nullptr;
+  bool async_ = false;  // device supports stream-ordered allocation
 };
 
 // Per-call timing of the pipeline stages (ms, measured with HIP events).

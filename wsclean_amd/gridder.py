@@ -164,6 +164,23 @@ class WGriddingGridder_Simple:
             self._h, n_rows, n_chan, ctypes.c_void_p(uvw_dev.data_ptr()), _ptr(freq),
             ctypes.c_void_p(vis_out_dev.data_ptr())))
 
+    def RealImageDevice(self, image_dev):
+        """RealImage into a device float32 tensor of width*height."""
+        self._check_image(image_dev)
+        _lib.check(self._lib.wg_real_image_device(self._h, ctypes.c_void_p(image_dev.data_ptr())))
+
+    def InitializePredictionDevice(self, image_dev):
+        """InitializePrediction from a device float32 tensor of width*height."""
+        self._check_image(image_dev)
+        _lib.check(self._lib.wg_initialize_prediction_device(
+            self._h, ctypes.c_void_p(image_dev.data_ptr())))
+
+    def _check_image(self, t):
+        import torch
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and t.numel() == self.width * self.height):
+            raise WgError("image must be a contiguous float32 device tensor of width*height")
+
     def PrepareVisibilities(self, uvw_dev, freq, n_rows, n_chan):
         freq = _f64(freq, (n_chan,))
         self._prep_freq = freq

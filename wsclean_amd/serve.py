@@ -1,0 +1,272 @@
+"""Served request stream on one GPU: continuous batching of w-gridding
+requests with HBM admission.
+
+A request is one imaging job: an image geometry, uvw / frequencies and
+visibilities resident in HBM, and a number of major cycles.  Each cycle
+inverts the residual visibilities into a dirty image, takes a model from
+it (a stand-in for the deconvolution: a tenth of the pixels above half the
+peak), predicts the model's visibilities and forms
+the next residual (WSClean's major cycle, main/wsclean.cpp:1519-1632).  The
+request keeps its sorted visibility set (wg_prepare_visibilities), its
+grids and its images on the device across the cycles; only the last dirty
+image goes to the host.
+
+Requests arrive at their own times (a seeded Poisson stream in
+``make_requests``).  A request is admitted, in arrival order, as soon as a
+worker slot is free and its device memory fits under the HBM budget beside
+the requests already running.  Each admitted request drives a gridder handle
+of its own, and each handle owns a HIP stream and a stream-ordered arena,
+so the work of admitted requests overlaps on the device and no handle's
+allocation or release waits on another's work.  The library calls release
+the interpreter lock while the GPU works, so one Python thread per running
+request keeps the slots busy.
+
+A request's result does not depend on what runs beside it: handles share
+nothing but the device, and every kernel of the path is deterministic
+(integer subgrid accumulation, fixed-order gathers and reductions).
+"""
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, synth
+from .gridder import WGriddingGridder_Simple
+
+
+@dataclass
+class Request:
+    rid: int
+    npix: int
+    pixel_rad: float
+    eps: float
+    uvw: object            # device tensor [n_rows, 3] float64, metres
+    freq: np.ndarray       # host [n_chan] float64, Hz
+    vis: object            # device tensor [n_rows, n_chan, 2] float32
+    out: object            # device tensor, same shape: the last predicted visibilities
+    cycles: int = 5
+    arrival: float = 0.0   # seconds after the stream starts
+    # filled in by the server
+    start: float = 0.0
+    done: float = 0.0
+    image: np.ndarray = field(default=None, repr=False)
+
+    @property
+    def n_rows(self):
+        return int(self.uvw.shape[0])
+
+    @property
+    def n_chan(self):
+        return len(self.freq)
+
+    @property
+    def n_vis(self):
+        return self.n_rows * self.n_chan
+
+    def hbm_bytes(self, slab_bytes=8 << 30):
+        """Upper estimate of the device memory the request allocates beyond
+        its inputs: per visibility the sorted set (keys and permutation with
+        their ping-pong copies, coordinates, phasors, sorted and accumulated
+        visibilities, grown by the arena's 1/8 rounding) and the residual;
+        per pixel the fp64 image, the dirty / correction / host-transfer
+        images; the FFT slab (w-planes of at most (2 npix)^2 uv cells plus
+        two line buffers, capped by the planner's slab budget); and 64 MiB
+        for plan tables."""
+        n2 = self.npix * self.npix
+        slab = min(slab_bytes, 64 * n2 * 32)
+        return int(72 * self.n_vis + 24 * self.n_rows + 32 * n2 + slab + (64 << 20))
+
+
+def major_cycle_model(img, out=None):
+    """The model a cycle predicts: 0.1 x the pixels at or above half the
+    image's peak, zero elsewhere (NaN pixels are ignored).  ``img`` is a
+    float32 torch tensor on any device; the model goes to ``out`` when
+    given."""
+    import torch
+    peak = torch.nan_to_num(img, nan=float("-inf")).amax()
+    if out is None:
+        out = torch.empty_like(img)
+    torch.mul(img, 0.1, out=out)
+    return out.masked_fill_(~(img >= 0.5 * peak), 0.0)
+
+
+def run_request(req):
+    """Runs one request to completion on a gridder handle of its own and
+    returns the last cycle's dirty image (float32 [npix, npix], host);
+    ``req.out`` holds the last cycle's predicted visibilities."""
+    import torch
+    n = req.npix
+    dev = req.vis.device
+    g = WGriddingGridder_Simple(n, n, n, n, req.pixel_rad, req.pixel_rad, 0.0, 0.0, 1, req.eps)
+    try:
+        g.PrepareVisibilities(req.uvw, req.freq, req.n_rows, req.n_chan)
+        residual = req.vis.clone()
+        img = torch.empty(n * n, dtype=torch.float32, device=dev)
+        model = torch.empty_like(img)
+        # torch work of this thread runs on its current stream; the library
+        # calls synchronise their own stream before returning, so a host
+        # synchronisation of this stream orders the two
+        stream = torch.cuda.current_stream(dev)
+        stream.synchronize()
+        for c in range(req.cycles):
+            g.InitializeInversion()
+            g.AddInversionPrepared(residual)
+            g.FinalizeImage(1.0 / req.n_vis)
+            g.RealImageDevice(img)
+            major_cycle_model(img, out=model)
+            stream.synchronize()
+            g.InitializePredictionDevice(model)
+            g.PredictPrepared(req.out)
+            if c + 1 < req.cycles:
+                torch.sub(req.vis, req.out, out=residual)
+                stream.synchronize()
+        return img.view(n, n).cpu().numpy()
+    finally:
+        g.close()
+
+
+class Server:
+    """Continuous batching over ``slots`` concurrent requests.
+
+    Admission is in arrival order: the head of the queue starts as soon as
+    a slot is free and ``hbm_bytes()`` of it fits under ``hbm_budget`` next
+    to the running requests (a request larger than the whole budget runs
+    alone).  Later arrivals wait behind the head, so large requests are not
+    starved.  ``hbm_budget=None`` takes 80% of the device memory free when
+    ``serve`` starts.  ``run`` is the per-request function (``run_request``).
+    """
+
+    def __init__(self, slots=4, hbm_budget=None, run=run_request):
+        if slots < 1:
+            raise ValueError("slots must be >= 1")
+        self.slots = int(slots)
+        self.hbm_budget = hbm_budget
+        self.run = run
+        self._cv = threading.Condition()
+        self._in_use = 0
+        self._running = 0
+        self.peak_running = 0
+        self.peak_bytes = 0
+
+    def _admit(self, req, budget):
+        need = req.hbm_bytes()
+        with self._cv:
+            while True:
+                fits = budget is None or self._in_use + need <= budget or self._running == 0
+                if self._running < self.slots and fits:
+                    self._in_use += need
+                    self._running += 1
+                    self.peak_running = max(self.peak_running, self._running)
+                    self.peak_bytes = max(self.peak_bytes, self._in_use)
+                    return need
+                self._cv.wait()
+
+    def _release(self, need):
+        with self._cv:
+            self._in_use -= need
+            self._running -= 1
+            self._cv.notify_all()
+
+    def serve(self, requests):
+        """Replays ``requests`` at their arrival times and returns them in
+        arrival order with start / done times (seconds from the stream
+        start) and results.  The first error of any request is raised once
+        every admitted request has ended."""
+        budget = self.hbm_budget
+        if budget is None:
+            import torch
+            budget = int(0.8 * torch.cuda.mem_get_info()[0])
+        order = sorted(requests, key=lambda r: r.arrival)
+        errors = []
+        threads = []
+        t0 = time.perf_counter()
+
+        def work(req, need):
+            try:
+                req.start = time.perf_counter() - t0
+                req.image = self.run(req)
+                req.done = time.perf_counter() - t0
+            except BaseException as e:
+                errors.append(e)
+            finally:
+                self._release(need)
+
+        for req in order:
+            delay = req.arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            if errors:
+                break
+            need = self._admit(req, budget)
+            th = threading.Thread(target=work, args=(req, need), daemon=True)
+            th.start()
+            threads.append(th)
+        for th in threads:
+            th.join()
+        if errors:
+            raise errors[0]
+        return order
+
+
+def latency_stats(requests):
+    """p50 / p99 / mean latency (arrival to completion) and the stream's
+    throughput in visibilities per second (every cycle's invert and
+    predict counted once, as one grid + degrid pass)."""
+    lat = np.array([r.done - r.arrival for r in requests])
+    span = max(r.done for r in requests) - min(r.arrival for r in requests)
+    nvis = sum(r.n_vis * r.cycles for r in requests)
+    return {"requests": len(requests),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3),
+            "max_ms": float(lat.max() * 1e3),
+            "throughput_vis_s": float(nvis / span) if span > 0 else None,
+            "span_s": float(span),
+            "vis_total": int(sum(r.n_vis for r in requests))}
+
+
+def make_requests(n_requests, rate_hz, seed=1005, device="cuda:0", npix_choices=None,
+                  vis_range=(1e5, 5e7), cycles=5, eps=1e-5):
+    """Seeded served-stream workload: Poisson arrivals at ``rate_hz``; image
+    size drawn from ``npix_choices`` (512^2 ... 8192^2), 1 arcsec pixels; a
+    random array of 32-256 antennas uniform in a 3 km disc, 1-64 channels
+    over 120-180 MHz, 10 s time steps at dec -30 deg until a visibility
+    count drawn log-uniformly from ``vis_range`` is reached; 20 power-law
+    point sources plus Gaussian noise (sigma 0.01), generated on the
+    device."""
+    import torch
+    rng = np.random.Generator(np.random.PCG64(seed))
+    npix_choices = npix_choices or [512, 1024, 2048, 4096, 8192]
+    lib = _lib.load()
+    dev = torch.device(device)
+    c_void_p = _lib.ctypes.c_void_p
+    reqs = []
+    t = 0.0
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        npix = int(rng.choice(npix_choices))
+        n_ant = int(rng.integers(32, 257))
+        n_chan = int(rng.integers(1, 65))
+        target = float(np.exp(rng.uniform(np.log(vis_range[0]), np.log(vis_range[1]))))
+        n_bl = n_ant * (n_ant - 1) // 2
+        n_times = max(1, int(round(target / (n_bl * n_chan))))
+        enu = synth.antennas_disc(rng, n_ant, 1500.0)
+        uvw_h = synth.earth_rotation_uvw(enu, n_times, 10.0, np.deg2rad(-30.0))
+        freq = np.linspace(120e6, 180e6, n_chan) if n_chan > 1 else np.array([150e6])
+        dx = synth.ARCSEC
+        l, m, flux, _, _ = synth.point_sky(rng, 20, npix, dx)
+        src = np.zeros((20, 5))
+        src[:, 0], src[:, 1], src[:, 3] = l, m, flux
+        src[:, 2] = np.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
+        uvw = torch.from_numpy(uvw_h).to(dev)
+        n_rows = uvw_h.shape[0]
+        vis = torch.empty((n_rows, n_chan, 2), dtype=torch.float32, device=dev)
+        _lib.check(lib.wgx_synth_visibilities(
+            c_void_p(uvw.data_ptr()), n_rows, n_chan, freq.ctypes.data_as(c_void_p), 20,
+            src.ctypes.data_as(c_void_p), 0.01, int(seed) * 1000 + rid,
+            c_void_p(vis.data_ptr())))
+        out = torch.zeros_like(vis)
+        reqs.append(Request(rid, npix, dx, eps, uvw, freq, vis, out, cycles, t))
+    torch.cuda.synchronize()
+    return reqs
