@@ -160,6 +160,45 @@ int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab,
                   size_t slab_bytes);
 
 /* ------------------------------------------------------------------------
+ * Facet batches.  WSClean images a faceted field with one gridder per facet
+ * (gridding/msgriddermanager.cpp:41-77) and, in BatchInvert
+ * (gridding/msgriddermanager.cpp:227-396 with
+ * WGriddingMSGridder::GridSharedMeasurementSetChunk,
+ * wgridder/wgriddingmsgridder.cpp:97-115), feeds one chunk read to all of
+ * them.  A facet batch replaces that set of WGriddingGridder_Simple objects:
+ * facet f has the constructor arguments facets[f] (its own image size,
+ * pixel size and l/m shift; the smallest epsilon of the batch applies to
+ * all), and its visibilities are the rows [row_offset[f], row_offset[f] +
+ * n_rows[f]) of the shared device buffers uvw_dev ([rows][3] f64) and
+ * vis_dev ([rows][n_chan] complex64); facets may share rows.  All facets
+ * share one sort and one gridding / degridding launch per plane sub-slab.
+ * Per-facet images follow RealImage / InitializePrediction
+ * (gridder_simple.h:37-38): width*height floats, trimmed image centred,
+ * NaN border.  wg_facet_batch_predict overwrites vis_out_dev in the facet
+ * layout: facet f's [n_rows[f]][n_chan] block follows the blocks of the
+ * facets before it (wg_facet_batch_virtual_count gives the total). */
+typedef struct wg_facet_batch wg_facet_batch;
+int wg_facet_batch_create(const wg_params* facets, size_t n_facets, wg_facet_batch** out);
+int wg_facet_batch_destroy(wg_facet_batch* b);
+int wg_facet_batch_prepare(wg_facet_batch* b, size_t n_chan, const double* freq,
+                           const double* uvw_dev, const size_t* row_offset,
+                           const size_t* n_rows);
+/* InitializeInversion / AddInversionData / FinalizeImage for every facet */
+int wg_facet_batch_initialize_inversion(wg_facet_batch* b);
+int wg_facet_batch_add_inversion(wg_facet_batch* b, const float* vis_dev);
+int wg_facet_batch_finalize_image(wg_facet_batch* b, size_t facet, double factor);
+int wg_facet_batch_real_image(wg_facet_batch* b, size_t facet, float* out);
+int wg_facet_batch_real_image_device(wg_facet_batch* b, size_t facet, float* image_dev);
+int wg_facet_batch_initialize_prediction(wg_facet_batch* b, size_t facet, const float* image);
+int wg_facet_batch_initialize_prediction_device(wg_facet_batch* b, size_t facet,
+                                                const float* image_dev);
+/* PredictVisibilities of every facet from its model */
+int wg_facet_batch_predict(wg_facet_batch* b, float* vis_out_dev);
+int wg_facet_batch_virtual_count(wg_facet_batch* b, size_t* n);
+/* Atlas and per-facet plans of the last prepare (JSON). */
+int wg_facet_batch_plan_json(wg_facet_batch* b, char* buf, size_t len);
+
+/* ------------------------------------------------------------------------
  * Test entry points (no reference counterpart): expose internals so that
  * the bucket sort and the LDS FFT can be checked bit-for-bit / against a
  * library FFT. */
@@ -178,6 +217,11 @@ int wgx_fft(int n, int nlines, int inverse, float* data);
 int wgx_make_plan(const wg_params* params, const double* extents6, long long nvis,
                   int W, double sigma, int tile, int sub_slab, char* plan_json,
                   size_t plan_len);
+/* Host-only: the facet-batch layout (JSON: atlas plan, per-facet tile
+ * origin and plan) for facets with flipped-coordinate extents extents6
+ * (6 per facet) and visibility counts nvis.  Needs no GPU. */
+int wgx_facet_atlas(const wg_params* facets, size_t n_facets, const double* extents6,
+                    const long long* nvis, char* json, size_t len);
 /* Synthetic visibilities on the device (benchmark inputs):
  * vis(row, chan) = sum_s S_s exp(-2 pi^2 sigma_s^2 (u^2 + v^2))
  *                  * exp(+2 pi i (u l_s + v m_s + w (n_s - 1))) + noise,

@@ -67,6 +67,28 @@ SIGNATURES = [
     ("wgx_make_plan", ctypes.c_int,
      [ctypes.POINTER(WgParams), ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_size]),
+    ("wg_facet_batch_create", ctypes.c_int,
+     [ctypes.POINTER(WgParams), c_size, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wg_facet_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_facet_batch_prepare", ctypes.c_int,
+     [ctypes.c_void_p, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
+    ("wg_facet_batch_initialize_inversion", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_facet_batch_add_inversion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_facet_batch_finalize_image", ctypes.c_int, [ctypes.c_void_p, c_size, ctypes.c_double]),
+    ("wg_facet_batch_real_image", ctypes.c_int, [ctypes.c_void_p, c_size, ctypes.c_void_p]),
+    ("wg_facet_batch_real_image_device", ctypes.c_int,
+     [ctypes.c_void_p, c_size, ctypes.c_void_p]),
+    ("wg_facet_batch_initialize_prediction", ctypes.c_int,
+     [ctypes.c_void_p, c_size, ctypes.c_void_p]),
+    ("wg_facet_batch_initialize_prediction_device", ctypes.c_int,
+     [ctypes.c_void_p, c_size, ctypes.c_void_p]),
+    ("wg_facet_batch_predict", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_facet_batch_virtual_count", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
+    ("wg_facet_batch_plan_json", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_size]),
+    ("wgx_facet_atlas", ctypes.c_int,
+     [ctypes.POINTER(WgParams), c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
+      c_size]),
     ("wgx_synth_visibilities", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]),

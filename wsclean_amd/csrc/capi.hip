@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/wgrid_hip.h"
 #include "device.hpp"
@@ -88,6 +89,17 @@ struct wg_handle {
 
   size_t npix() const { return prm.width_t * prm.height_t; }
   hipStream_t s() const { return eng->Stream(); }
+};
+
+struct wg_facet_batch {
+  std::vector<wg_params> prm;
+  std::unique_ptr<wg::FacetBatch> fb;
+  std::vector<double*> img;  // per facet [width_t][height_t] fp64
+  std::string plan_json;
+  std::mutex mu;
+
+  hipStream_t s() { return fb->Eng().Stream(); }
+  size_t npix(size_t f) const { return prm[f].width_t * prm[f].height_t; }
 };
 
 extern "C" {
@@ -484,6 +496,40 @@ int wgx_make_plan(const wg_params* params, const double* e, long long nvis, int 
   });
 }
 
+int wgx_facet_atlas(const wg_params* facets, size_t n_facets, const double* extents6,
+                    const long long* nvis, char* json, size_t len) {
+  return Guard([&] {
+    WG_CHECK(facets && extents6 && nvis && json && len && n_facets, "null argument");
+    std::vector<wg::ImageGeometry> geoms(n_facets);
+    std::vector<wg::VisExtent> exts(n_facets);
+    std::vector<long long> nv(nvis, nvis + n_facets);
+    double eps = facets[0].epsilon;
+    for (size_t f = 0; f < n_facets; ++f) eps = std::min(eps, facets[f].epsilon);
+    for (size_t f = 0; f < n_facets; ++f) {
+      geoms[f].nx = int(facets[f].width_t);
+      geoms[f].ny = int(facets[f].height_t);
+      geoms[f].dx = facets[f].pixel_size_x;
+      geoms[f].dy = facets[f].pixel_size_y;
+      geoms[f].l_shift = facets[f].l_shift;
+      geoms[f].m_shift = facets[f].m_shift;
+      geoms[f].epsilon = eps;
+      const double* e = extents6 + 6 * f;
+      exts[f] = wg::VisExtent{e[0], e[1], e[2], e[3], e[4], e[5]};
+    }
+    const wg::FacetLayout lay = wg::PlanFacets(geoms, exts, nv, wg::PlanOptions{});
+    std::string out = "{\"atlas\":" + lay.atlas.describe() + ",\"facets\":[";
+    for (size_t f = 0; f < n_facets; ++f) {
+      if (f) out += ",";
+      out += "{\"tu0\":" + std::to_string(lay.tu0[f]) + ",\"tv0\":" +
+             std::to_string(lay.tv0[f]) + ",\"plan\":" + lay.plans[f].describe() + "}";
+    }
+    out += "]}";
+    WG_CHECK(out.size() < len, "JSON buffer too small");
+    std::strncpy(json, out.c_str(), len - 1);
+    json[len - 1] = 0;
+  });
+}
+
 int wgx_fft(int n, int nlines, int inverse, float* data) {
   return Guard([&] {
     WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");
@@ -520,6 +566,172 @@ int wgx_synth_visibilities(const double* uvw_dev, size_t n_rows, size_t n_chan,
     WG_HIP(hipDeviceSynchronize());
     (void)hipFree(dscale);
     (void)hipFree(dsrc);
+  });
+}
+
+// ------------------------------------------------------------ facet batches
+
+int wg_facet_batch_create(const wg_params* facets, size_t n_facets, wg_facet_batch** out) {
+  return Guard([&] {
+    WG_CHECK(facets && out && n_facets > 0, "null argument or no facets");
+    auto b = std::make_unique<wg_facet_batch>();
+    std::vector<wg::ImageGeometry> geoms(n_facets);
+    double eps = facets[0].epsilon;
+    for (size_t f = 0; f < n_facets; ++f) {
+      const wg_params& p = facets[f];
+      WG_CHECK(p.width_t <= p.width && p.height_t <= p.height,
+               "trimmed size exceeds image size");
+      WG_CHECK(p.width_t > 0 && p.height_t > 0, "empty image");
+      eps = std::min(eps, p.epsilon);
+      geoms[f].nx = int(p.width_t);
+      geoms[f].ny = int(p.height_t);
+      geoms[f].dx = p.pixel_size_x;
+      geoms[f].dy = p.pixel_size_y;
+      geoms[f].l_shift = p.l_shift;
+      geoms[f].m_shift = p.m_shift;
+    }
+    for (auto& g : geoms) g.epsilon = eps;
+    b->prm.assign(facets, facets + n_facets);
+    b->fb = std::make_unique<wg::FacetBatch>(geoms, wg::PlanOptions{});
+    b->img.resize(n_facets);
+    for (size_t f = 0; f < n_facets; ++f) {
+      b->img[f] = static_cast<double*>(b->fb->Eng().Arena().Get(
+          "fb.image" + std::to_string(f), b->npix(f) * sizeof(double)));
+      WG_HIP(hipMemsetAsync(b->img[f], 0, b->npix(f) * sizeof(double), b->s()));
+    }
+    WG_HIP(hipStreamSynchronize(b->s()));
+    *out = b.release();
+  });
+}
+
+int wg_facet_batch_destroy(wg_facet_batch* b) {
+  return Guard([&] { delete b; });
+}
+
+int wg_facet_batch_prepare(wg_facet_batch* b, size_t n_chan, const double* freq,
+                           const double* uvw_dev, const size_t* row_offset,
+                           const size_t* n_rows) {
+  return Guard([&] {
+    WG_CHECK(b && freq && uvw_dev && row_offset && n_rows, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->fb->Prepare(n_chan, freq, uvw_dev, row_offset, n_rows);
+    b->plan_json = b->fb->Describe();
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_initialize_inversion(wg_facet_batch* b) {
+  return Guard([&] {
+    WG_CHECK(b, "null handle");
+    std::lock_guard<std::mutex> lk(b->mu);
+    for (size_t f = 0; f < b->img.size(); ++f)
+      WG_HIP(hipMemsetAsync(b->img[f], 0, b->npix(f) * sizeof(double), b->s()));
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_add_inversion(wg_facet_batch* b, const float* vis_dev) {
+  return Guard([&] {
+    WG_CHECK(b && vis_dev, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    WG_CHECK(b->fb->Prepared(), "wg_facet_batch_prepare has not been called");
+    b->fb->Invert(reinterpret_cast<const float2*>(vis_dev), b->img.data());
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_finalize_image(wg_facet_batch* b, size_t facet, double factor) {
+  return Guard([&] {
+    WG_CHECK(b && facet < b->img.size(), "null handle or facet out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    ScaleKernel<<<Blocks(b->npix(facet)), 256, 0, b->s()>>>(b->img[facet], b->npix(facet),
+                                                          factor);
+    WG_HIP(hipGetLastError());
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+static void FacetRealImage(wg_facet_batch* b, size_t f, float* d) {
+  const wg_params& p = b->prm[f];
+  RealImageKernel<<<Blocks(p.width * p.height), 256, 0, b->s()>>>(
+      b->img[f], d, int(p.width), int(p.height), int(p.width_t), int(p.height_t));
+  WG_HIP(hipGetLastError());
+}
+
+int wg_facet_batch_real_image(wg_facet_batch* b, size_t facet, float* out) {
+  return Guard([&] {
+    WG_CHECK(b && out && facet < b->img.size(), "null argument or facet out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    const size_t n = b->prm[facet].width * b->prm[facet].height;
+    float* d = static_cast<float*>(b->fb->Eng().Arena().Get("d2h.image", n * 4));
+    FacetRealImage(b, facet, d);
+    WG_HIP(hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, b->s()));
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_real_image_device(wg_facet_batch* b, size_t facet, float* image_dev) {
+  return Guard([&] {
+    WG_CHECK(b && image_dev && facet < b->img.size(), "null argument or facet out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    FacetRealImage(b, facet, image_dev);
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+static void FacetLoadModel(wg_facet_batch* b, size_t f, const float* d) {
+  const wg_params& p = b->prm[f];
+  LoadModelKernel<<<Blocks(b->npix(f)), 256, 0, b->s()>>>(
+      d, b->img[f], int(p.width), int(p.height), int(p.width_t), int(p.height_t));
+  WG_HIP(hipGetLastError());
+}
+
+int wg_facet_batch_initialize_prediction(wg_facet_batch* b, size_t facet, const float* image) {
+  return Guard([&] {
+    WG_CHECK(b && image && facet < b->img.size(), "null argument or facet out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    const size_t n = b->prm[facet].width * b->prm[facet].height;
+    float* d = static_cast<float*>(b->fb->Eng().Arena().Get("h2d.image", n * 4));
+    WG_HIP(hipMemcpyAsync(d, image, n * 4, hipMemcpyHostToDevice, b->s()));
+    FacetLoadModel(b, facet, d);
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_initialize_prediction_device(wg_facet_batch* b, size_t facet,
+                                                const float* image_dev) {
+  return Guard([&] {
+    WG_CHECK(b && image_dev && facet < b->img.size(), "null argument or facet out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    FacetLoadModel(b, facet, image_dev);
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_predict(wg_facet_batch* b, float* vis_out_dev) {
+  return Guard([&] {
+    WG_CHECK(b && vis_out_dev, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    WG_CHECK(b->fb->Prepared(), "wg_facet_batch_prepare has not been called");
+    std::vector<const double*> img(b->img.begin(), b->img.end());
+    b->fb->Predict(img.data(), reinterpret_cast<float2*>(vis_out_dev));
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_virtual_count(wg_facet_batch* b, size_t* n) {
+  return Guard([&] {
+    WG_CHECK(b && n, "null argument");
+    *n = b->fb->NumVirtual();
+  });
+}
+
+int wg_facet_batch_plan_json(wg_facet_batch* b, char* buf, size_t len) {
+  return Guard([&] {
+    WG_CHECK(b && buf && len > 0, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    std::strncpy(buf, b->plan_json.c_str(), len - 1);
+    buf[len - 1] = 0;
   });
 }
 

@@ -23,6 +23,9 @@ struct DevPlan {
   int W, T, ntu, ntv, P0, P, nu, nv;
   int box_u0, box_v0, wrap_u, wrap_v;
   int BU, BV;
+  // box storage: row pitch and plane stride in cells (BU and BU * BV for a
+  // box of its own; larger when the box is a region of a facet atlas)
+  size_t box_pitch, box_plane;
   int nx, ny;
   uint32_t nkeys;     // invalid key; valid keys are below
   uint32_t nbuckets;  // (tile, p0) buckets; key = (bucket * T + dv) * T + du

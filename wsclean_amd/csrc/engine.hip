@@ -49,6 +49,8 @@ DevPlan ToDevPlan(const Plan& p) {
   d.wrap_v = p.wrap_v ? 1 : 0;
   d.BU = p.BU;
   d.BV = p.BV;
+  d.box_pitch = size_t(p.BU);
+  d.box_plane = size_t(p.BU) * size_t(p.BV);
   d.nx = p.img.nx;
   d.ny = p.img.ny;
   d.nkeys = p.nkeys;

@@ -136,6 +136,7 @@ class Engine {
   bool timing = false;
 
  private:
+  friend class FacetBatch;
   const float2* Twiddles(int n);
   void BuildCorrection(VisSet& vs);
   void BuildWorkItems(VisSet& vs);
@@ -149,6 +150,57 @@ class Engine {
 
  public:
   std::vector<PendingTime> pending_;
+};
+
+// Facets gridded and degridded together (facets.hip): one sort, one set of
+// work items and one persistent launch per sub-slab for all facets; the
+// plane FFTs, w screens and corrections per facet.  Facet f's rows are
+// [row_off[f], row_off[f] + n_rows[f]) of the shared uvw / visibility
+// buffers (rows may be shared between facets).  Predict writes the facets'
+// visibilities one after another ("virtual" layout: facet f's block starts
+// at the sum of the earlier facets' n_rows * n_chan).
+// Host-only facet planning: per-facet plans sharing one kernel shape, tile
+// and sub-slab depth (those of the largest facet), the atlas tile origin of
+// every facet (shelf packing) and the atlas plan (tile grid, P0, P, keys,
+// FFT slab).
+struct FacetLayout {
+  std::vector<Plan> plans;
+  std::vector<int> tu0, tv0;
+  Plan atlas;
+};
+FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
+                       const std::vector<VisExtent>& exts, const std::vector<long long>& nvis,
+                       const PlanOptions& opt);
+
+class FacetBatch {
+ public:
+  FacetBatch(const std::vector<ImageGeometry>& facets, const PlanOptions& opt);
+  int NumFacets() const { return int(geoms_.size()); }
+  const ImageGeometry& Geometry(int f) const { return geoms_[f]; }
+  Engine& Eng() { return eng_; }
+  void Prepare(size_t n_chan, const double* freq_host, const double* uvw_dev,
+               const size_t* row_off, const size_t* n_rows);
+  bool Prepared() const { return prepared_; }
+  // img_dev[f] += corrected dirty image of facet f ([x][y], fp64)
+  void Invert(const float2* vis_dev, double* const* img_dev);
+  void Predict(const double* const* img_dev, float2* vis_out_dev);
+  size_t NumVirtual() const { return atlas_.n_vis; }
+  std::string Describe() const;
+
+ private:
+  float2* Region(float2* box, int f) const;
+
+  Engine eng_;
+  std::vector<ImageGeometry> geoms_;
+  std::vector<VisSet> fvs_;  // per facet: plan, device plan, correction
+  VisSet atlas_;             // the shared sorted set and work items
+  std::vector<int> tu0_, tv0_;
+  std::vector<size_t> row_off_, n_rows_;
+  std::vector<unsigned long long> vbase_;
+  size_t n_chan_ = 0;
+  double* scale_ = nullptr;
+  uint32_t* perm_phys_ = nullptr;
+  bool prepared_ = false;
 };
 
 // ---- FFT / plane kernels (fft.hip) ----
@@ -199,6 +251,26 @@ void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
 size_t GridLdsBytes(const DevPlan& dp, int sub_n);
 
 // ---- per-visibility kernels (vis.hip) ----
+// One facet of a facet batch on the device: its plan, the offset of its
+// rows in the shared uvw / visibility buffers, and the atlas tile where its
+// tile (0, 0) sits.
+struct FacetDev {
+  DevPlan dp;
+  unsigned long long row_off;
+  int tu0, tv0;
+};
+// keys[v] / vals[v] = v for every virtual index v of the batch (facet f owns
+// [vbase[f], vbase[f + 1])).
+void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F,
+                     const double* uvw, const double* scale, size_t n_chan, size_t n_virtual,
+                     int atlas_ntu, int atlas_P0, int T, uint32_t* keys, uint32_t* vals,
+                     hipStream_t s);
+// Sorted coordinates / phasors; perm gets the flip bit, perm_phys the
+// physical index (with the flip bit).
+void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vbase, int F,
+                       const double* uvw, const double* scale, size_t n_chan, uint32_t* perm,
+                       uint32_t* perm_phys, size_t n_valid, float4* coord, float2* phasor,
+                       hipStream_t s);
 void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
                    double* partial, double* out, hipStream_t s);
 void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,

@@ -1,0 +1,360 @@
+// Facet batch: many facet images gridded and degridded together.
+//
+// WSClean's BatchInvert (gridding/msgriddermanager.cpp:227-396) reads a
+// measurement-set chunk once and hands it to one gridder per facet
+// (wgridder/wgriddingmsgridder.cpp:97-115); each facet has its own image
+// size and phase-centre shift.  Here the facets of a batch share one sorted
+// visibility set and one set of work items: every facet keeps its own plan
+// (grid size, w planes, uv box, shifts), and its uv box is placed as a
+// region of one "atlas" tile grid.  The sort key of a visibility is its
+// atlas tile, first plane and cell, so a single radix sort orders every
+// facet's visibilities, and each gridding / degridding sub-slab is ONE
+// persistent launch over the work items of all facets.  The plane FFTs,
+// w screens and grid corrections then run per facet on its region of the
+// atlas box.
+//
+// All facets of a batch use one kernel shape (support W, oversampling,
+// beta), one tile size and one sub-slab depth, taken from the plan of the
+// largest facet; their uv boxes are packed into the atlas on shelves, with
+// the W - 1 overhang of every box inside its own region so that no work
+// item's subgrid reaches another facet.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <sstream>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace wg {
+
+FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
+                       const std::vector<VisExtent>& exts, const std::vector<long long>& nvis,
+                       const PlanOptions& opt) {
+  const int F = int(geoms.size());
+  WG_CHECK(F > 0 && int(exts.size()) == F && int(nvis.size()) == F, "facet count mismatch");
+  FacetLayout lay;
+  // the largest facet picks the kernel shape, the tile and the sub-slab depth
+  int big = 0;
+  for (int f = 1; f < F; ++f)
+    if (double(geoms[f].nx) * geoms[f].ny > double(geoms[big].nx) * geoms[big].ny) big = f;
+  PlanOptions o = opt;
+  o.nvis_hint = nvis[big];
+  const Plan lead = MakePlan(geoms[big], exts[big], o);
+  o.force_W = lead.W;
+  o.force_sigma = lead.sigma;
+  o.T = lead.T;
+  o.Sg = lead.Sg;
+  lay.plans.resize(F);
+  for (int f = 0; f < F; ++f) {
+    o.nvis_hint = nvis[f];
+    lay.plans[f] = MakePlan(geoms[f], exts[f], o);
+    const Plan& p = lay.plans[f];
+    WG_CHECK(p.W == lead.W && p.T == lead.T && p.Sg == lead.Sg && p.beta == lead.beta,
+             "facet plans disagree on the kernel shape");
+  }
+
+  // shelf packing of the facets' tile regions: facet f takes ntu + gap by
+  // ntv + gap tiles, the gap holding the W - 1 overhang of its box
+  const int T = lead.T, W = lead.W;
+  const int gap = (W - 1 + T - 1) / T;
+  std::vector<int> tw(F), th(F), order(F);
+  double area = 0;
+  int max_tw = 0;
+  for (int f = 0; f < F; ++f) {
+    tw[f] = lay.plans[f].ntu + gap;
+    th[f] = lay.plans[f].ntv + gap;
+    area += double(tw[f]) * th[f];
+    max_tw = std::max(max_tw, tw[f]);
+  }
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return th[a] > th[b]; });
+  const int shelf_w = std::max(max_tw, int(std::ceil(std::sqrt(area))));
+  lay.tu0.assign(F, 0);
+  lay.tv0.assign(F, 0);
+  int x = 0, y = 0, shelf_h = 0, used_w = 0;
+  for (int f : order) {
+    if (x > 0 && x + tw[f] > shelf_w) {
+      y += shelf_h;
+      x = 0;
+      shelf_h = 0;
+    }
+    lay.tu0[f] = x;
+    lay.tv0[f] = y;
+    x += tw[f];
+    used_w = std::max(used_w, x);
+    shelf_h = std::max(shelf_h, th[f]);
+  }
+  Plan pa = lead;
+  pa.ntu = used_w;
+  pa.ntv = y + shelf_h;
+  pa.BU = pa.ntu * T + W - 1;
+  pa.BV = pa.ntv * T + W - 1;
+  pa.box_u0 = pa.box_v0 = 0;
+  pa.wrap_u = pa.wrap_v = false;
+  pa.P0 = 1;
+  pa.P = 1;
+  for (const Plan& p : lay.plans) {
+    pa.P0 = std::max(pa.P0, p.P0);
+    pa.P = std::max(pa.P, p.P);
+  }
+  const uint64_t nb = uint64_t(pa.ntu) * uint64_t(pa.ntv) * uint64_t(pa.P0);
+  const uint64_t nk = nb * uint64_t(T) * uint64_t(T);
+  WG_CHECK(nk < 0xFFFFFFF0ull,
+           "facet batch too large for one sort key space: split the facets into smaller batches");
+  pa.nbuckets = uint32_t(nb);
+  pa.nkeys = uint32_t(nk);
+  pa.key_bits = 1;
+  while ((uint64_t(1) << pa.key_bits) <= uint64_t(pa.nkeys)) ++pa.key_bits;
+  // FFT slab: whole sub-slabs whose atlas box and largest facet line
+  // buffers fit the slab budget
+  size_t lines = 0;
+  for (const Plan& p : lay.plans) lines = std::max(lines, size_t(p.BV) * p.img.nx);
+  const double per_plane = 8.0 * (double(pa.BU) * pa.BV + 2.0 * double(lines));
+  int sf = int(std::max(1.0, std::floor(double(opt.slab_bytes) / per_plane)));
+  sf = std::max(pa.Sg, (sf / pa.Sg) * pa.Sg);
+  pa.Sf = std::min(sf, ((pa.P + pa.Sg - 1) / pa.Sg) * pa.Sg);
+  lay.atlas = pa;
+  return lay;
+}
+
+FacetBatch::FacetBatch(const std::vector<ImageGeometry>& facets, const PlanOptions& opt)
+    : eng_(facets.empty() ? ImageGeometry{} : facets[0], opt), geoms_(facets) {
+  WG_CHECK(!facets.empty(), "a facet batch needs at least one facet");
+  for (const ImageGeometry& g : facets)
+    WG_CHECK(g.nx > 0 && g.ny > 0, "empty facet image");
+}
+
+void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* uvw_dev,
+                         const size_t* row_off, const size_t* n_rows) {
+  const int F = NumFacets();
+  hipStream_t s = eng_.Stream();
+  DeviceArena& A = eng_.Arena();
+  prepared_ = false;
+  WG_CHECK(n_chan > 0, "n_chan must be positive");
+  row_off_.assign(row_off, row_off + F);
+  n_rows_.assign(n_rows, n_rows + F);
+  n_chan_ = n_chan;
+
+  // ---- shared channel scales ----
+  std::vector<double> scale(n_chan);
+  double smin = 1e300, smax = -1e300;
+  for (size_t c = 0; c < n_chan; ++c) {
+    WG_CHECK(freq_host[c] > 0, "frequencies must be positive");
+    scale[c] = freq_host[c] / kSpeedOfLight;
+    smin = std::min(smin, scale[c]);
+    smax = std::max(smax, scale[c]);
+  }
+  scale_ = static_cast<double*>(A.Get("fb.scale", n_chan * sizeof(double)));
+  WG_HIP(hipMemcpyAsync(scale_, scale.data(), n_chan * sizeof(double), hipMemcpyHostToDevice, s));
+
+  // ---- per-facet extents (one readback for all facets) ----
+  double* part = static_cast<double*>(A.Get("extents", (1024 * 6 + 6) * sizeof(double)));
+  double* ext_dev = static_cast<double*>(A.Get("fb.extents", size_t(F) * 6 * sizeof(double)));
+  LaunchFillZero(ext_dev, size_t(F) * 6 * sizeof(double), s);
+  for (int f = 0; f < F; ++f)
+    if (n_rows_[f] > 0)
+      LaunchExtents(uvw_dev + 3 * row_off_[f], n_rows_[f], smin, smax, part, ext_dev + 6 * f, s);
+  std::vector<double> ext(size_t(F) * 6);
+  WG_HIP(hipMemcpyAsync(ext.data(), ext_dev, ext.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        s));
+  WG_HIP(hipStreamSynchronize(s));
+  for (double v : ext) WG_CHECK(std::isfinite(v), "non-finite uvw coordinates");
+
+  std::vector<VisExtent> exts(F);
+  std::vector<long long> nvis(F);
+  for (int f = 0; f < F; ++f) {
+    const double* e = &ext[6 * size_t(f)];
+    exts[f] = VisExtent{e[0], e[1], e[2], e[3], e[4], e[5]};
+    nvis[f] = (long long)(n_rows_[f] * n_chan);
+  }
+  FacetLayout lay = PlanFacets(geoms_, exts, nvis, eng_.Options());
+  const Plan& pa = lay.atlas;
+  const int T = pa.T;
+  tu0_ = lay.tu0;
+  tv0_ = lay.tv0;
+  fvs_.assign(F, VisSet{});
+  for (int f = 0; f < F; ++f) {
+    fvs_[f].plan = lay.plans[f];
+    fvs_[f].dp = ToDevPlan(lay.plans[f]);
+    fvs_[f].tag = "facet" + std::to_string(f);
+  }
+  atlas_ = VisSet{};
+  atlas_.plan = pa;
+  atlas_.dp = ToDevPlan(pa);
+  atlas_.tag = "atlas";
+  atlas_.n_chan = n_chan;
+  atlas_.scale = scale_;
+  atlas_.uvw = const_cast<double*>(uvw_dev);
+
+  // ---- device facet table ----
+  std::vector<FacetDev> fd(F);
+  vbase_.assign(F + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    fd[f].dp = fvs_[f].dp;
+    // the facet's FFT stages address its region of the atlas box
+    fd[f].dp.box_pitch = size_t(pa.BU);
+    fd[f].dp.box_plane = size_t(pa.BU) * size_t(pa.BV);
+    fd[f].row_off = row_off_[f];
+    fd[f].tu0 = tu0_[f];
+    fd[f].tv0 = tv0_[f];
+    fvs_[f].dp = fd[f].dp;
+    vbase_[f + 1] = vbase_[f] + (unsigned long long)(n_rows_[f] * n_chan);
+  }
+  const size_t n_virtual = size_t(vbase_[F]);
+  WG_CHECK(n_virtual < 0x80000000ull, "at most 2^31 - 1 visibilities per facet batch");
+  FacetDev* dfac = static_cast<FacetDev*>(A.Get("fb.facets", size_t(F) * sizeof(FacetDev)));
+  unsigned long long* dvb =
+      static_cast<unsigned long long*>(A.Get("fb.vbase", size_t(F + 1) * sizeof(uint64_t)));
+  WG_HIP(hipMemcpyAsync(dfac, fd.data(), fd.size() * sizeof(FacetDev), hipMemcpyHostToDevice, s));
+  WG_HIP(hipMemcpyAsync(dvb, vbase_.data(), vbase_.size() * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, s));
+
+  // ---- keys, one stable sort, bucket offsets ----
+  atlas_.n_vis = n_virtual;
+  uint32_t* keys = static_cast<uint32_t*>(A.Get("atlas.keys", n_virtual * 4));
+  uint32_t* vals = static_cast<uint32_t*>(A.Get("atlas.perm", n_virtual * 4));
+  if (n_virtual) {
+    LaunchFacetKeys(dfac, dvb, F, uvw_dev, scale_, n_chan, n_virtual, pa.ntu, pa.P0, T, keys,
+                    vals, s);
+    uint32_t* scratch =
+        static_cast<uint32_t*>(A.Get("sort.scratch", RadixSortScratchWords(n_virtual) * 4));
+    uint32_t* keys2 = static_cast<uint32_t*>(A.Get("atlas.keys2", n_virtual * 4));
+    uint32_t* vals2 = static_cast<uint32_t*>(A.Get("atlas.perm2", n_virtual * 4));
+    if (DeviceRadixSort(keys, vals, keys2, vals2, n_virtual, pa.key_bits, scratch, s)) {
+      keys = keys2;
+      vals = vals2;
+    }
+  }
+  atlas_.perm = vals;
+  atlas_.off = static_cast<uint32_t*>(A.Get("atlas.off", (size_t(pa.nbuckets) + 1) * 4));
+  DeviceBucketOffsets(keys, n_virtual, pa.nbuckets, uint32_t(T) * uint32_t(T), atlas_.off, s);
+  uint32_t nvalid = 0;
+  WG_HIP(hipMemcpyAsync(&nvalid, atlas_.off + pa.nbuckets, 4, hipMemcpyDeviceToHost, s));
+  WG_HIP(hipStreamSynchronize(s));
+  WG_CHECK(nvalid == n_virtual, "facet batch: visibilities outside the atlas key space");
+  atlas_.n_valid = nvalid;
+  atlas_.coord = static_cast<float4*>(A.Get("atlas.coord", nvalid * sizeof(float4)));
+  atlas_.phasor = static_cast<float2*>(A.Get("atlas.phasor", nvalid * sizeof(float2)));
+  perm_phys_ = static_cast<uint32_t*>(A.Get("atlas.perm_phys", nvalid * 4));
+  LaunchFacetCoords(dfac, dvb, F, uvw_dev, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
+                    atlas_.coord, atlas_.phasor, s);
+  eng_.BuildWorkItems(atlas_);
+  for (int f = 0; f < F; ++f) eng_.BuildCorrection(fvs_[f]);
+  prepared_ = true;
+}
+
+float2* FacetBatch::Region(float2* box, int f) const {
+  const DevPlan& a = atlas_.dp;
+  return box + size_t(tv0_[f]) * a.T * size_t(a.BU) + size_t(tu0_[f]) * a.T;
+}
+
+void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
+  WG_CHECK(prepared_, "facet batch not prepared");
+  const int F = NumFacets();
+  const DevPlan& da = atlas_.dp;
+  const Plan& pa = atlas_.plan;
+  hipStream_t s = eng_.Stream();
+  DeviceArena& A = eng_.Arena();
+  if (atlas_.n_valid == 0) return;
+  uint32_t* words = static_cast<uint32_t*>(A.Get("grid.words", 4 * sizeof(uint32_t)));
+  float2* vsorted = static_cast<float2*>(A.Get("vis_sorted", atlas_.n_valid * sizeof(float2)));
+  LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+  std::vector<float*> dirty(F);
+  size_t lines = 0;
+  for (int f = 0; f < F; ++f) {
+    const size_t npix = size_t(geoms_[f].nx) * geoms_[f].ny;
+    dirty[f] = static_cast<float*>(A.Get(fvs_[f].tag + ".dirty", npix * 4));
+    LaunchFillZero(dirty[f], npix * 4, s);
+    lines = std::max(lines, size_t(fvs_[f].plan.BV) * fvs_[f].plan.img.nx);
+  }
+  float2* box =
+      static_cast<float2*>(A.Get("box", size_t(pa.Sf) * da.box_plane * sizeof(float2)));
+  float2* rbuf = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  float2* cbuf = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  float2* scratch = static_cast<float2*>(
+      A.Get("grid.scratch", GridScratchBytes(da, pa.Sg, atlas_.max_items_per_sub)));
+  for (int f0 = 0; f0 < pa.P; f0 += pa.Sf) {
+    const int np = std::min(pa.Sf, pa.P - f0);
+    for (int g = f0 / pa.Sg; g * pa.Sg < f0 + np; ++g) {
+      const int sub_lo = g * pa.Sg;
+      const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
+      // one persistent launch over the work items of every facet
+      LaunchGrid(da, atlas_, vsorted, scratch, words, reinterpret_cast<const float*>(words + 1),
+                 g, sub_lo, sub_n, s);
+      LaunchGatherBox(da, atlas_, box, scratch, g, sub_lo, sub_n, f0, s);
+    }
+    for (int f = 0; f < F; ++f) {
+      const VisSet& v = fvs_[f];
+      const int npf = std::min(np, v.plan.P - f0);
+      if (npf <= 0) continue;
+      const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
+      const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
+      LaunchRowFFTInvert(v.dp, fu, Region(box, f), rbuf, npf, s);
+      LaunchTranspose(rbuf, cbuf, v.dp.BV, v.dp.nx, npf, s);
+      LaunchColFFTInvert(v.dp, fv, cbuf, dirty[f], f0, npf, s);
+    }
+  }
+  for (int f = 0; f < F; ++f)
+    LaunchAccumulateImage(dirty[f], fvs_[f].corr, img_dev[f],
+                          size_t(geoms_[f].nx) * geoms_[f].ny, s);
+}
+
+void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
+  WG_CHECK(prepared_, "facet batch not prepared");
+  const int F = NumFacets();
+  const DevPlan& da = atlas_.dp;
+  const Plan& pa = atlas_.plan;
+  hipStream_t s = eng_.Stream();
+  DeviceArena& A = eng_.Arena();
+  if (atlas_.n_valid == 0) return;
+  float2* acc = static_cast<float2*>(A.Get("vis_acc", atlas_.n_valid * sizeof(float2)));
+  LaunchFillZero(acc, atlas_.n_valid * sizeof(float2), s);
+  size_t lines = 0;
+  for (int f = 0; f < F; ++f)
+    lines = std::max(lines, size_t(fvs_[f].plan.BV) * fvs_[f].plan.img.nx);
+  float2* box =
+      static_cast<float2*>(A.Get("box", size_t(pa.Sf) * da.box_plane * sizeof(float2)));
+  float2* rbuf = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  float2* cbuf = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  uint32_t* queue = static_cast<uint32_t*>(A.Get("grid.words", 4 * sizeof(uint32_t)));
+  for (int f0 = 0; f0 < pa.P; f0 += pa.Sf) {
+    const int np = std::min(pa.Sf, pa.P - f0);
+    // planes of a facet beyond its own P are never read: its work items
+    // only reach planes below P
+    for (int f = 0; f < F; ++f) {
+      const VisSet& v = fvs_[f];
+      const int npf = std::min(np, v.plan.P - f0);
+      if (npf <= 0) continue;
+      const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
+      const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
+      LaunchColFFTPredict(v.dp, fv, img_dev[f], v.corr, cbuf, f0, npf, s);
+      LaunchTranspose(cbuf, rbuf, v.dp.nx, v.dp.BV, npf, s);
+      LaunchRowFFTPredict(v.dp, fu, rbuf, Region(box, f), npf, s);
+    }
+    for (int g = f0 / pa.Sg; g * pa.Sg < f0 + np; ++g) {
+      const int sub_lo = g * pa.Sg;
+      const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
+      LaunchDegrid(da, atlas_, box, acc, queue, g, sub_lo, sub_n, f0, s);
+    }
+  }
+  LaunchScatterVis(atlas_.perm, atlas_.phasor, atlas_.n_valid, acc, vis_out_dev, s);
+}
+
+std::string FacetBatch::Describe() const {
+  std::ostringstream o;
+  o.precision(17);
+  o << "{\"atlas\":" << atlas_.plan.describe() << ",\"n_virtual\":" << atlas_.n_vis
+    << ",\"n_items\":" << atlas_.n_items << ",\"facets\":[";
+  for (int f = 0; f < NumFacets(); ++f) {
+    if (f) o << ",";
+    o << "{\"tu0\":" << tu0_[f] << ",\"tv0\":" << tv0_[f] << ",\"plan\":" << fvs_[f].plan.describe()
+      << "}";
+  }
+  o << "]}";
+  return o.str();
+}
+
+}  // namespace wg

@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(BLOCK) RowFFTInvertKernel(
   float2* twl = lds;
   float2* buf = lds + kTwSlots;
   const int j = blockIdx.x, pl = blockIdx.y;
-  const float2* src = box + (size_t(pl) * dp.BV + j) * dp.BU;
+  const float2* src = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
   LoadTwiddles<BLOCK>(twl, fu);
   ZeroLds<BLOCK>(buf, fu.n);
   __syncthreads();
@@ -405,7 +405,7 @@ __global__ void __launch_bounds__(BLOCK) RowFFTPredictKernel(
   for (int x = threadIdx.x; x < dp.nx; x += BLOCK) buf[Pad(Wrap(dp.nx / 2 - x, fu.n))] = src[x];
   __syncthreads();
   LdsFFT<true, BLOCK, MAXN>(buf, fu, twl);
-  float2* out = box + (size_t(pl) * dp.BV + j) * dp.BU;
+  float2* out = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
   for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Pad(Wrap(dp.box_u0 + i, fu.n))];
 }
 

@@ -392,6 +392,69 @@ __global__ void __launch_bounds__(kBlock) SynthKernel(
   }
 }
 
+
+// ------------------------------------------------------------ facet batches
+// Facet f owns the virtual visibility indices [vbase[f], vbase[f + 1]); its
+// visibility (row r, channel c) is the physical entry
+// (row_off_f + r) * n_chan + c of the shared uvw / visibility buffers.
+__device__ __forceinline__ int FacetOf(const unsigned long long* __restrict__ vbase, int F,
+                                       unsigned long long v) {
+  int lo = 0, hi = F - 1;  // the last facet with vbase <= v
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vbase[mid] <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Keys of a facet batch: every facet maps its visibilities with its own
+// plan, and its tiles are placed at (tu0, tv0) of the atlas tile grid, so
+// one sort and one set of work items covers all facets.
+__global__ void __launch_bounds__(kBlock) FacetKeysKernel(
+    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vbase, int F,
+    const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
+    size_t n_virtual, int atlas_ntu, int atlas_P0, int T, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ vals) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_virtual;
+       i += size_t(gridDim.x) * kBlock) {
+    const int f = FacetOf(vbase, F, i);
+    const size_t local = i - vbase[f];
+    const size_t row = fac[f].row_off + local / n_chan, ch = local % n_chan;
+    const DevPlan& dp = fac[f].dp;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
+    const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
+    const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
+    keys[i] = (bucket * uint32_t(T) + uint32_t(L.dv)) * uint32_t(T) + uint32_t(L.du);
+    vals[i] = uint32_t(i);
+  }
+}
+
+// Sorted coordinates and phasors of a facet batch (as CoordsKernel, with
+// the facet's own plan, phase centre shift and w centre).  perm keeps the
+// virtual index (the predict output layout), perm_phys gets the physical
+// index into the shared visibility buffer; both carry the flip bit.
+__global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
+    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vbase, int F,
+    const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
+    uint32_t* __restrict__ perm, uint32_t* __restrict__ perm_phys, size_t n_valid,
+    float4* __restrict__ coord, float2* __restrict__ phasor) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
+       i += size_t(gridDim.x) * kBlock) {
+    const uint32_t v = perm[i];
+    const int f = FacetOf(vbase, F, v);
+    const size_t local = v - vbase[f];
+    const size_t row = fac[f].row_off + local / n_chan, ch = local % n_chan;
+    const DevPlan& dp = fac[f].dp;
+    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
+    phasor[i] = ShiftPhasor(dp, L, -1.f);
+    const uint32_t flip = L.flip ? kPermFlip : 0u;
+    perm[i] = v | flip;
+    perm_phys[i] = uint32_t(row * n_chan + ch) | flip;
+  }
+}
+
 }  // namespace
 
 void LaunchSynth(const double* uvw, size_t n_rows, size_t n_chan, const double* scale,
@@ -416,6 +479,27 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale, size_
                 hipStream_t s) {
   KeysKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw, scale, n_rows, n_chan, vis,
                                                        keys, vals);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F,
+                     const double* uvw, const double* scale, size_t n_chan, size_t n_virtual,
+                     int atlas_ntu, int atlas_P0, int T, uint32_t* keys, uint32_t* vals,
+                     hipStream_t s) {
+  if (!n_virtual) return;
+  FacetKeysKernel<<<Blocks(n_virtual), kBlock, 0, s>>>(fac, vbase, F, uvw, scale, n_chan,
+                                                        n_virtual, atlas_ntu, atlas_P0, T, keys,
+                                                        vals);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vbase, int F,
+                       const double* uvw, const double* scale, size_t n_chan, uint32_t* perm,
+                       uint32_t* perm_phys, size_t n_valid, float4* coord, float2* phasor,
+                       hipStream_t s) {
+  if (!n_valid) return;
+  FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vbase, F, uvw, scale, n_chan, perm,
+                                                        perm_phys, n_valid, coord, phasor);
   WG_HIP(hipGetLastError());
 }
 
