@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served-stream leg and print its JSON")
+    ap.add_argument("--facet-steps", type=int, default=2,
+                    help="timed steps of the facet-batch leg (0 skips it)")
+    ap.add_argument("--facets-only", action="store_true",
+                    help="run only the facet-batch leg and print its JSON")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--sg", type=int, default=0)
     ap.add_argument("--W", type=int, default=0)
@@ -200,6 +204,9 @@ def main():
     if args.serve_only:
         print(json.dumps(serving_leg(args)))
         return
+    if args.facets_only:
+        print(json.dumps(facets_leg(args)))
+        return
     dev = torch.device("cuda:0")
     t_gen = time.time()
     uvw_h, freq, dx, src = make_inputs(args.channels, args.npix)
@@ -274,13 +281,17 @@ def main():
         "cpu_baseline": None,
         "gen_s": t_gen,
     }
+    line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
     if not args.no_per_channel:
         line["per_channel_gpu"] = per_channel_leg(uvw, vis, freq, n, dx, args.eps, n_rows, value)
     g.close()
     del uvw, vis, out
     torch.cuda.empty_cache()
+    line["graph"]["512"] = small_graph_cycle(args.eps)
     if args.serve_requests > 0:
         line["serving"] = serving_leg(args)
+    if args.facet_steps > 0:
+        line["facets"] = facets_leg(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
     print(json.dumps(line))
@@ -321,6 +332,47 @@ def per_channel_leg(uvw, vis, freq, n, dx, eps, n_rows, batched_value):
             "batched_over_per_channel": batched_value / v}
 
 
+def graph_cycle(g, vis, out, n_vis, steps):
+    """One major cycle on the prepared set (clear, invert, finalize,
+    predict) issued eagerly versus replayed from a captured HIP graph."""
+    import torch
+
+    def eager():
+        g.InitializeInversion()
+        g.AddInversionPrepared(vis)
+        g.FinalizeImage(1.0 / n_vis)
+        g.PredictPrepared(out)
+
+    eager()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eager()
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    g.CaptureMajorCycle(vis, out, 1.0 / n_vis)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.ReplayMajorCycle(steps)
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {"eager_ms": eager_ms, "graph_ms": graph_ms, "cycles": steps}
+
+
+def small_graph_cycle(eps):
+    """The graph leg on a small request: 512^2 image, ~2e6 visibilities."""
+    from wsclean_amd import WGriddingGridder_Simple
+    from wsclean_amd.serve import make_requests
+    r = make_requests(1, 1.0, seed=7, npix_choices=[512], vis_range=(2e6, 2.0001e6), cycles=1,
+                      eps=eps)[0]
+    g = WGriddingGridder_Simple(512, 512, 512, 512, r.pixel_rad, r.pixel_rad, 0.0, 0.0, 1, eps)
+    g.PrepareVisibilities(r.uvw, r.freq, r.n_rows, r.n_chan)
+    res = graph_cycle(g, r.vis, r.out, r.n_vis, 20)
+    res["n_vis"] = r.n_vis
+    g.close()
+    return res
+
+
 def serving_leg(args):
     """Served request stream: seeded Poisson arrivals of requests with
     image sizes 512^2 ... 8192^2 and 1e5 ... 5e7 visibilities from random
@@ -356,6 +408,133 @@ def serving_leg(args):
     del reqs, out
     torch.cuda.empty_cache()
     return s
+
+
+def make_facet_inputs(share=8, seed=1004):
+    """Batched-facets workload: a 12000^2 field at 2 arcsec split into a
+    6 x 6 grid of 36 facets of uneven sizes (each side drawn from
+    1024 ... 3072, multiple of 16), each centred on its grid cell (its own
+    l/m shift).  Facet f takes a log-uniform 1e6 ... 5e7 rows (capped at
+    the track's 23.4e6) of the 128-antenna 10 km, 2880 x 10 s track, as a
+    random contiguous range, at 16 channels over 120-180 MHz.  One GPU takes
+    1/share of every facet's rows (the row shard of an 8-GPU node)."""
+    from wsclean_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rng_arr = np.random.Generator(np.random.PCG64(1002))
+    enu = synth.antennas_logradial(rng_arr, 128, 10000.0)
+    uvw = synth.earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
+    freq = np.linspace(120e6, 180e6, 16)
+    dx = 2.0 * synth.ARCSEC
+    R = uvw.shape[0]
+    facets, row_off, n_rows = [], [], []
+    for i in range(36):
+        gx, gy = i % 6, i // 6
+        nx = int(rng.integers(1024, 3073)) // 16 * 16
+        ny = int(rng.integers(1024, 3073)) // 16 * 16
+        rows = int(np.exp(rng.uniform(np.log(1e6), np.log(5e7))))
+        rows = min(rows, R) // share
+        row_off.append(int(rng.integers(0, R - rows + 1)))
+        n_rows.append(rows)
+        facets.append(dict(width=nx, height=ny, pixel_size_x=dx, pixel_size_y=dx,
+                           l_shift=(6000 - (gx + 0.5) * 2000) * dx,
+                           m_shift=((gy + 0.5) * 2000 - 6000) * dx, epsilon=1e-5))
+    l, m, flux, _, _ = synth.point_sky(rng, 500, 12000, dx)
+    src = np.zeros((500, 5))
+    src[:, 0], src[:, 1], src[:, 3] = l, m, flux
+    src[:, 2] = np.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
+    return uvw, freq, facets, row_off, n_rows, src
+
+
+def facets_leg(args):
+    """Facet batch (all 36 facets in one sort and one gridding / degridding
+    launch per sub-slab) against one gridder per facet on the same rows.
+    A step is prepare (plan + sort) + invert + finalize + predict of every
+    facet; value counts every facet's visibilities once."""
+    import torch
+
+    from wsclean_amd import WGriddingGridder_Simple, _lib
+    from wsclean_amd.facets import FacetBatch
+    dev = torch.device("cuda:0")
+    t = time.time()
+    uvw_h, freq, facets, row_off, n_rows, src = make_facet_inputs()
+    n_chan = len(freq)
+    uvw = torch.from_numpy(uvw_h).to(dev)
+    R = uvw_h.shape[0]
+    vis = torch.empty((R, n_chan, 2), dtype=torch.float32, device=dev)
+    lib = _lib.load()
+    srcc = np.ascontiguousarray(src)
+    _lib.check(lib.wgx_synth_visibilities(
+        ctypes_ptr(uvw), R, n_chan, freq.ctypes.data_as(_c_void_p()), srcc.shape[0],
+        srcc.ctypes.data_as(_c_void_p()), 0.05, 1004, ctypes_ptr(vis)))
+    torch.cuda.synchronize()
+    gen_s = time.time() - t
+    n_virtual = int(sum(n_rows)) * n_chan
+    out = torch.empty((n_virtual, 2), dtype=torch.float32, device=dev)
+    fb = FacetBatch(facets)
+
+    def step():
+        fb.Prepare(uvw, freq, row_off, n_rows)
+        fb.InitializeInversion()
+        fb.AddInversion(vis)
+        for f in range(len(facets)):
+            fb.FinalizeImage(f, 1.0 / max(1, n_rows[f] * n_chan))
+        fb.Predict(out)
+
+    fb.set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    fb.stage_times(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.facet_steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.facet_steps
+    st = fb.stage_times(reset=True)
+    plan = fb.plan()
+    fb.close()
+    del out
+    torch.cuda.empty_cache()
+    # one gridder per facet (WSClean's one-gridder-per-facet structure)
+    gs = []
+    for f, fac in enumerate(facets):
+        g = WGriddingGridder_Simple(fac["width"], fac["height"], fac["width"], fac["height"],
+                                    fac["pixel_size_x"], fac["pixel_size_y"], fac["l_shift"],
+                                    fac["m_shift"], 1, fac["epsilon"])
+        gs.append(g)
+    sub = [(uvw[row_off[f]:row_off[f] + n_rows[f]], vis[row_off[f]:row_off[f] + n_rows[f]])
+           for f in range(len(facets))]
+    outs = [torch.empty_like(v) for _, v in sub]
+
+    def per_facet():
+        for f, g in enumerate(gs):
+            u, v = sub[f]
+            g.PrepareVisibilities(u, freq, n_rows[f], n_chan)
+            g.InitializeInversion()
+            g.AddInversionPrepared(v)
+            g.FinalizeImage(1.0 / max(1, n_rows[f] * n_chan))
+            g.PredictPrepared(outs[f])
+
+    per_facet()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_facet()
+    torch.cuda.synchronize()
+    ms_pf = (time.perf_counter() - t0) * 1e3
+    for g in gs:
+        g.close()
+    del sub, outs, uvw, vis
+    torch.cuda.empty_cache()
+    a = plan["atlas"]
+    return {"workload": "batched facets: 12000^2 field at 2 arcsec, 36 facets 1024-3072 px, "
+                        "log-uniform 1e6-5e7 rows each from the 128-antenna 10 km track at "
+                        "16 channels, one GPU's 1/8 row share, eps 1e-5",
+            "value": n_virtual / (ms / 1e3), "unit": "vis/s", "ms_per_step": ms,
+            "steps": args.facet_steps, "n_facets": len(facets), "n_vis": n_virtual,
+            "per_facet_gridders_ms": ms_pf,
+            "batch_over_per_facet": ms_pf / ms,
+            "atlas": {k: a[k] for k in ("W", "sigma", "T", "Sg", "Sf", "ntu", "ntv", "P", "P0")},
+            "stages_ms": {k: v / args.facet_steps for k, v in st.items()},
+            "n_items": plan["n_items"], "gen_s": gen_s}
 
 
 def ctypes_ptr(t):

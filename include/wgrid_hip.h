@@ -143,6 +143,19 @@ int wg_prepare_visibilities(wg_handle* h, size_t n_rows, size_t n_chan,
 int wg_add_inversion_prepared(wg_handle* h, const float* vis_dev);
 int wg_predict_prepared(wg_handle* h, float* vis_out_dev);
 
+/* A major cycle on the prepared set, captured once as a HIP graph and
+ * replayed without host launch work (WSClean's major cycle,
+ * main/wsclean.cpp:1519-1632, repeats the same invert / predict on the same
+ * data).  what & 1: clear the image, invert vis_dev, multiply by factor
+ * (InitializeInversion + AddInversionData + FinalizeImage); what & 2:
+ * predict vis_out_dev from the handle's image (PredictVisibilities).  The
+ * buffers are bound at capture; a replay reads their current contents.
+ * The capture runs the cycle once eagerly.  wg_prepare_visibilities drops
+ * the graph.  No reference counterpart (launch mechanism). */
+int wg_capture_major_cycle(wg_handle* h, const float* vis_dev, float* vis_out_dev,
+                           double factor, int what);
+int wg_replay_major_cycle(wg_handle* h, int count);
+
 /* Plan of the last call as JSON (W, sigma, nu, nv, P, box, tiles...). */
 int wg_plan_json(const wg_handle* h, char* buf, size_t len);
 /* Per-stage device times (ms, HIP events on the handle's stream) accumulated
@@ -195,6 +208,14 @@ int wg_facet_batch_initialize_prediction_device(wg_facet_batch* b, size_t facet,
 /* PredictVisibilities of every facet from its model */
 int wg_facet_batch_predict(wg_facet_batch* b, float* vis_out_dev);
 int wg_facet_batch_virtual_count(wg_facet_batch* b, size_t* n);
+/* The batch's major cycle as a HIP graph, as wg_capture_major_cycle;
+ * factors: one FinalizeImage factor per facet. */
+int wg_facet_batch_capture_major_cycle(wg_facet_batch* b, const float* vis_dev,
+                                       float* vis_out_dev, const double* factors, int what);
+int wg_facet_batch_replay_major_cycle(wg_facet_batch* b, int count);
+/* Stage timing of the batch, as wg_set_timing / wg_stage_times. */
+int wg_facet_batch_set_timing(wg_facet_batch* b, int enabled);
+int wg_facet_batch_stage_times(wg_facet_batch* b, double* out14, int reset);
 /* Atlas and per-facet plans of the last prepare (JSON). */
 int wg_facet_batch_plan_json(wg_facet_batch* b, char* buf, size_t len);
 

@@ -55,6 +55,9 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_add_inversion_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_predict_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_capture_major_cycle", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
+    ("wg_replay_major_cycle", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("wg_plan_json", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_size]),
     ("wg_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("wg_stage_times", ctypes.c_int, [ctypes.c_void_p, c_dbl_p, ctypes.c_int]),
@@ -85,6 +88,11 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, ctypes.c_void_p]),
     ("wg_facet_batch_predict", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_facet_batch_virtual_count", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
+    ("wg_facet_batch_capture_major_cycle", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    ("wg_facet_batch_replay_major_cycle", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("wg_facet_batch_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("wg_facet_batch_stage_times", ctypes.c_int, [ctypes.c_void_p, c_dbl_p, ctypes.c_int]),
     ("wg_facet_batch_plan_json", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_size]),
     ("wgx_facet_atlas", ctypes.c_int,
      [ctypes.POINTER(WgParams), c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,

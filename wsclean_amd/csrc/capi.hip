@@ -75,6 +75,41 @@ int Guard(F&& f) {
   }
 }
 
+// Runs `body` (device work on the engine stream, no host synchronisation)
+// once eagerly, so that every arena buffer, twiddle table and launch
+// attribute it needs exists, then captures it into an executable HIP graph.
+// The body clears with fill kernels, never with memset nodes.
+template <typename F>
+hipGraphExec_t CaptureGraph(wg::Engine& e, F&& body) {
+  hipStream_t s = e.Stream();
+  body();
+  WG_HIP(hipStreamSynchronize(s));
+  const bool timing = e.timing;
+  e.timing = false;  // no event records inside the graph
+  WG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    e.timing = timing;
+    throw;
+  }
+  e.timing = timing;
+  WG_HIP(hipStreamEndCapture(s, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  WG_HIP(err);
+  return exec;
+}
+
+void DropGraph(hipGraphExec_t& g) {
+  if (g) (void)hipGraphExecDestroy(g);
+  g = nullptr;
+}
+
 }  // namespace
 
 struct wg_handle {
@@ -85,8 +120,10 @@ struct wg_handle {
   wg::VisSet vs_prep;
   bool prepared = false;
   std::string plan_json;
+  hipGraphExec_t cycle = nullptr;  // captured major cycle on vs_prep
   std::mutex mu;
 
+  ~wg_handle() { DropGraph(cycle); }
   size_t npix() const { return prm.width_t * prm.height_t; }
   hipStream_t s() const { return eng->Stream(); }
 };
@@ -96,7 +133,10 @@ struct wg_facet_batch {
   std::unique_ptr<wg::FacetBatch> fb;
   std::vector<double*> img;  // per facet [width_t][height_t] fp64
   std::string plan_json;
+  hipGraphExec_t cycle = nullptr;  // captured major cycle of the batch
   std::mutex mu;
+
+  ~wg_facet_batch() { DropGraph(cycle); }
 
   hipStream_t s() { return fb->Eng().Stream(); }
   size_t npix(size_t f) const { return prm[f].width_t * prm[f].height_t; }
@@ -362,6 +402,7 @@ int wg_prepare_visibilities(wg_handle* h, size_t n_rows, size_t n_chan, const do
     WG_CHECK(h && freq && uvw_dev, "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
     h->prepared = false;
+    DropGraph(h->cycle);
     h->eng->Prepare(h->vs_prep, "prep", n_rows, n_chan, uvw_dev, freq, nullptr);
     h->plan_json = h->vs_prep.plan.describe();
     h->prepared = true;
@@ -384,6 +425,38 @@ int wg_predict_prepared(wg_handle* h, float* vis_out_dev) {
     std::lock_guard<std::mutex> lk(h->mu);
     WG_CHECK(h->prepared, "wg_prepare_visibilities has not been called");
     h->eng->Predict(h->vs_prep, h->img, reinterpret_cast<float2*>(vis_out_dev));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_capture_major_cycle(wg_handle* h, const float* vis_dev, float* vis_out_dev,
+                           double factor, int what) {
+  return Guard([&] {
+    WG_CHECK(h && (what & 3) != 0, "null handle or nothing to capture");
+    WG_CHECK(!(what & 1) || vis_dev, "invert needs vis_dev");
+    WG_CHECK(!(what & 2) || vis_out_dev, "predict needs vis_out_dev");
+    std::lock_guard<std::mutex> lk(h->mu);
+    WG_CHECK(h->prepared, "wg_prepare_visibilities has not been called");
+    DropGraph(h->cycle);
+    h->cycle = CaptureGraph(*h->eng, [&] {
+      if (what & 1) {
+        wg::LaunchFillZero(h->img, h->npix() * sizeof(double), h->s());
+        h->eng->Invert(h->vs_prep, reinterpret_cast<const float2*>(vis_dev), h->img);
+        ScaleKernel<<<Blocks(h->npix()), 256, 0, h->s()>>>(h->img, h->npix(), factor);
+        WG_HIP(hipGetLastError());
+      }
+      if (what & 2)
+        h->eng->Predict(h->vs_prep, h->img, reinterpret_cast<float2*>(vis_out_dev));
+    });
+  });
+}
+
+int wg_replay_major_cycle(wg_handle* h, int count) {
+  return Guard([&] {
+    WG_CHECK(h && count >= 0, "null handle or negative count");
+    std::lock_guard<std::mutex> lk(h->mu);
+    WG_CHECK(h->cycle, "no captured major cycle (wg_capture_major_cycle)");
+    for (int i = 0; i < count; ++i) WG_HIP(hipGraphLaunch(h->cycle, h->s()));
     WG_HIP(hipStreamSynchronize(h->s()));
   });
 }
@@ -614,6 +687,7 @@ int wg_facet_batch_prepare(wg_facet_batch* b, size_t n_chan, const double* freq,
   return Guard([&] {
     WG_CHECK(b && freq && uvw_dev && row_offset && n_rows, "null argument");
     std::lock_guard<std::mutex> lk(b->mu);
+    DropGraph(b->cycle);
     b->fb->Prepare(n_chan, freq, uvw_dev, row_offset, n_rows);
     b->plan_json = b->fb->Describe();
     WG_HIP(hipStreamSynchronize(b->s()));
@@ -716,6 +790,67 @@ int wg_facet_batch_predict(wg_facet_batch* b, float* vis_out_dev) {
     std::vector<const double*> img(b->img.begin(), b->img.end());
     b->fb->Predict(img.data(), reinterpret_cast<float2*>(vis_out_dev));
     WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_capture_major_cycle(wg_facet_batch* b, const float* vis_dev,
+                                       float* vis_out_dev, const double* factors, int what) {
+  return Guard([&] {
+    WG_CHECK(b && (what & 3) != 0, "null handle or nothing to capture");
+    WG_CHECK(!(what & 1) || (vis_dev && factors), "invert needs vis_dev and factors");
+    WG_CHECK(!(what & 2) || vis_out_dev, "predict needs vis_out_dev");
+    std::lock_guard<std::mutex> lk(b->mu);
+    WG_CHECK(b->fb->Prepared(), "wg_facet_batch_prepare has not been called");
+    DropGraph(b->cycle);
+    std::vector<double> fac;
+    if (what & 1) fac.assign(factors, factors + b->img.size());
+    std::vector<const double*> cimg(b->img.begin(), b->img.end());
+    b->cycle = CaptureGraph(b->fb->Eng(), [&] {
+      if (what & 1) {
+        for (size_t f = 0; f < b->img.size(); ++f)
+          wg::LaunchFillZero(b->img[f], b->npix(f) * sizeof(double), b->s());
+        b->fb->Invert(reinterpret_cast<const float2*>(vis_dev), b->img.data());
+        for (size_t f = 0; f < b->img.size(); ++f) {
+          ScaleKernel<<<Blocks(b->npix(f)), 256, 0, b->s()>>>(b->img[f], b->npix(f), fac[f]);
+          WG_HIP(hipGetLastError());
+        }
+      }
+      if (what & 2) b->fb->Predict(cimg.data(), reinterpret_cast<float2*>(vis_out_dev));
+    });
+  });
+}
+
+int wg_facet_batch_replay_major_cycle(wg_facet_batch* b, int count) {
+  return Guard([&] {
+    WG_CHECK(b && count >= 0, "null handle or negative count");
+    std::lock_guard<std::mutex> lk(b->mu);
+    WG_CHECK(b->cycle, "no captured major cycle (wg_facet_batch_capture_major_cycle)");
+    for (int i = 0; i < count; ++i) WG_HIP(hipGraphLaunch(b->cycle, b->s()));
+    WG_HIP(hipStreamSynchronize(b->s()));
+  });
+}
+
+int wg_facet_batch_set_timing(wg_facet_batch* b, int enabled) {
+  return Guard([&] {
+    WG_CHECK(b, "null handle");
+    b->fb->Eng().timing = enabled != 0;
+  });
+}
+
+int wg_facet_batch_stage_times(wg_facet_batch* b, double* out14, int reset) {
+  return Guard([&] {
+    WG_CHECK(b && out14, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    wg::Engine& e = b->fb->Eng();
+    e.CollectTimes();
+    wg::StageTimes& t = e.Times();
+    const double v[14] = {t.plan, t.sort, t.grid, t.fft, t.degrid,
+                          t.plan + t.sort + t.grid + t.fft + t.degrid,
+                          t.grid_kernel_ms, t.grid_launches, t.grid_bytes,
+                          t.degrid_kernel_ms, t.degrid_launches, t.degrid_bytes,
+                          t.grid_passes, t.degrid_passes};
+    std::copy(v, v + 14, out14);
+    if (reset) t = wg::StageTimes{};
   });
 }
 

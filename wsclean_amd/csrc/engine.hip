@@ -8,6 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "device.hpp"
 #include "engine.hpp"
@@ -56,6 +59,44 @@ DevPlan ToDevPlan(const Plan& p) {
   d.nkeys = p.nkeys;
   d.nbuckets = p.nbuckets;
   return d;
+}
+
+// ------------------------------------------------------------ launch queries
+
+namespace {
+std::mutex g_launch_mu;
+std::map<const void*, size_t> g_lds_allowed;
+std::map<std::tuple<const void*, int, size_t>, int> g_resident;
+}  // namespace
+
+void AllowDynamicLds(const void* kernel, size_t bytes) {
+  if (bytes <= 48 * 1024) return;
+  std::lock_guard<std::mutex> lk(g_launch_mu);
+  size_t& have = g_lds_allowed[kernel];
+  if (have >= bytes) return;
+  WG_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  have = bytes;
+}
+
+int ResidentBlocksPerCU(const void* kernel, int block, size_t lds) {
+  std::lock_guard<std::mutex> lk(g_launch_mu);
+  auto key = std::make_tuple(kernel, block, lds);
+  auto it = g_resident.find(key);
+  if (it != g_resident.end()) return it->second;
+  int per_cu = 0;
+  WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds));
+  g_resident[key] = per_cu;
+  return per_cu;
+}
+
+int DeviceCUs() {
+  static const int n = [] {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return cus;
+  }();
+  return n;
 }
 
 // ------------------------------------------------------------ DeviceArena
@@ -141,28 +182,6 @@ const float2* Engine::Twiddles(int n) {
   twiddles_[n] = d;
   return d;
 }
-
-namespace {
-// Records an event pair around a scope on the engine stream; the elapsed
-// time is accumulated into *dst by Engine::CollectTimes().
-struct EventTimer {
-  Engine* e;
-  double* dst;
-  hipEvent_t a = nullptr;
-  EventTimer(Engine& eng, double* out) : e(&eng), dst(out) {
-    if (!e->timing) return;
-    (void)hipEventCreate(&a);
-    (void)hipEventRecord(a, e->Stream());
-  }
-  ~EventTimer() {
-    if (!e->timing || !a) return;
-    hipEvent_t b = nullptr;
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(b, e->Stream());
-    e->pending_.push_back(PendingTime{dst, a, b});
-  }
-};
-}  // namespace
 
 void Engine::CollectTimes() {
   for (PendingTime& t : pending_) {

@@ -159,6 +159,35 @@ class Engine {
 // buffers (rows may be shared between facets).  Predict writes the facets'
 // visibilities one after another ("virtual" layout: facet f's block starts
 // at the sum of the earlier facets' n_rows * n_chan).
+// Launch-configuration queries, cached per kernel so that launches issued
+// during a HIP graph capture make no attribute or occupancy calls once the
+// same configuration has run eagerly: opt a kernel into `bytes` of dynamic
+// LDS (above the 48 KiB default), workgroups of `block` threads resident per
+// CU with `lds` bytes each, and the device's CU count.
+void AllowDynamicLds(const void* kernel, size_t bytes);
+int ResidentBlocksPerCU(const void* kernel, int block, size_t lds);
+int DeviceCUs();
+
+// Records an event pair around a scope on the engine stream when timing is
+// on; Engine::CollectTimes() adds the elapsed time to *dst.
+struct EventTimer {
+  Engine* e;
+  double* dst;
+  hipEvent_t a = nullptr;
+  EventTimer(Engine& eng, double* out) : e(&eng), dst(out) {
+    if (!e->timing) return;
+    (void)hipEventCreate(&a);
+    (void)hipEventRecord(a, e->Stream());
+  }
+  ~EventTimer() {
+    if (!e->timing || !a) return;
+    hipEvent_t b = nullptr;
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(b, e->Stream());
+    e->pending_.push_back(PendingTime{dst, a, b});
+  }
+};
+
 // Host-only facet planning: per-facet plans sharing one kernel shape, tile
 // and sub-slab depth (those of the largest facet), the atlas tile origin of
 // every facet (shelf packing) and the atlas plan (tile grid, P0, P, keys,

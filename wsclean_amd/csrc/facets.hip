@@ -132,6 +132,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   const int F = NumFacets();
   hipStream_t s = eng_.Stream();
   DeviceArena& A = eng_.Arena();
+  EventTimer tm(eng_, &eng_.Times().plan);
   prepared_ = false;
   WG_CHECK(n_chan > 0, "n_chan must be positive");
   row_off_.assign(row_off, row_off + F);
@@ -175,11 +176,16 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   const int T = pa.T;
   tu0_ = lay.tu0;
   tv0_ = lay.tv0;
+  // keep the corrections of the previous prepare: they are rebuilt only
+  // when a facet's geometry, kernel, grid or w axis changed
+  std::vector<std::string> corr_keys(F);
+  for (int f = 0; f < F && f < int(fvs_.size()); ++f) corr_keys[f] = fvs_[f].corr_key;
   fvs_.assign(F, VisSet{});
   for (int f = 0; f < F; ++f) {
     fvs_[f].plan = lay.plans[f];
     fvs_[f].dp = ToDevPlan(lay.plans[f]);
     fvs_[f].tag = "facet" + std::to_string(f);
+    fvs_[f].corr_key = corr_keys[f];
   }
   atlas_ = VisSet{};
   atlas_.plan = pa;
@@ -261,7 +267,11 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
   if (atlas_.n_valid == 0) return;
   uint32_t* words = static_cast<uint32_t*>(A.Get("grid.words", 4 * sizeof(uint32_t)));
   float2* vsorted = static_cast<float2*>(A.Get("vis_sorted", atlas_.n_valid * sizeof(float2)));
-  LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+  {
+    EventTimer tm(eng_, &eng_.Times().sort);
+    LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+  }
+  StageTimes& st = eng_.Times();
   std::vector<float*> dirty(F);
   size_t lines = 0;
   for (int f = 0; f < F; ++f) {
@@ -278,14 +288,28 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
       A.Get("grid.scratch", GridScratchBytes(da, pa.Sg, atlas_.max_items_per_sub)));
   for (int f0 = 0; f0 < pa.P; f0 += pa.Sf) {
     const int np = std::min(pa.Sf, pa.P - f0);
-    for (int g = f0 / pa.Sg; g * pa.Sg < f0 + np; ++g) {
-      const int sub_lo = g * pa.Sg;
-      const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
-      // one persistent launch over the work items of every facet
-      LaunchGrid(da, atlas_, vsorted, scratch, words, reinterpret_cast<const float*>(words + 1),
-                 g, sub_lo, sub_n, s);
-      LaunchGatherBox(da, atlas_, box, scratch, g, sub_lo, sub_n, f0, s);
+    {
+      EventTimer tm(eng_, &st.grid);
+      const size_t E = size_t(da.T + da.W - 1);
+      for (int g = f0 / pa.Sg; g * pa.Sg < f0 + np; ++g) {
+        const int sub_lo = g * pa.Sg;
+        const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
+        {
+          // one persistent launch over the work items of every facet
+          EventTimer tk(eng_, &st.grid_kernel_ms);
+          LaunchGrid(da, atlas_, vsorted, scratch, words,
+                     reinterpret_cast<const float*>(words + 1), g, sub_lo, sub_n, s);
+        }
+        if (eng_.timing) {
+          st.grid_launches += 1;
+          st.grid_passes += atlas_.sub_vis[g];
+          st.grid_bytes += 24.0 * atlas_.sub_vis[g] +
+                           8.0 * double(sub_n) * E * E * double(atlas_.sub_items[g]);
+        }
+        LaunchGatherBox(da, atlas_, box, scratch, g, sub_lo, sub_n, f0, s);
+      }
     }
+    EventTimer tf(eng_, &st.fft);
     for (int f = 0; f < F; ++f) {
       const VisSet& v = fvs_[f];
       const int npf = std::min(np, v.plan.P - f0);
@@ -312,6 +336,7 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
   if (atlas_.n_valid == 0) return;
   float2* acc = static_cast<float2*>(A.Get("vis_acc", atlas_.n_valid * sizeof(float2)));
   LaunchFillZero(acc, atlas_.n_valid * sizeof(float2), s);
+  StageTimes& st = eng_.Times();
   size_t lines = 0;
   for (int f = 0; f < F; ++f)
     lines = std::max(lines, size_t(fvs_[f].plan.BV) * fvs_[f].plan.img.nx);
@@ -324,22 +349,37 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
     const int np = std::min(pa.Sf, pa.P - f0);
     // planes of a facet beyond its own P are never read: its work items
     // only reach planes below P
-    for (int f = 0; f < F; ++f) {
-      const VisSet& v = fvs_[f];
-      const int npf = std::min(np, v.plan.P - f0);
-      if (npf <= 0) continue;
-      const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
-      const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
-      LaunchColFFTPredict(v.dp, fv, img_dev[f], v.corr, cbuf, f0, npf, s);
-      LaunchTranspose(cbuf, rbuf, v.dp.nx, v.dp.BV, npf, s);
-      LaunchRowFFTPredict(v.dp, fu, rbuf, Region(box, f), npf, s);
+    {
+      EventTimer tf(eng_, &st.fft);
+      for (int f = 0; f < F; ++f) {
+        const VisSet& v = fvs_[f];
+        const int npf = std::min(np, v.plan.P - f0);
+        if (npf <= 0) continue;
+        const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
+        const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
+        LaunchColFFTPredict(v.dp, fv, img_dev[f], v.corr, cbuf, f0, npf, s);
+        LaunchTranspose(cbuf, rbuf, v.dp.nx, v.dp.BV, npf, s);
+        LaunchRowFFTPredict(v.dp, fu, rbuf, Region(box, f), npf, s);
+      }
     }
+    EventTimer td(eng_, &st.degrid);
+    const size_t E = size_t(da.T + da.W - 1);
     for (int g = f0 / pa.Sg; g * pa.Sg < f0 + np; ++g) {
       const int sub_lo = g * pa.Sg;
       const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
-      LaunchDegrid(da, atlas_, box, acc, queue, g, sub_lo, sub_n, f0, s);
+      {
+        EventTimer tk(eng_, &st.degrid_kernel_ms);
+        LaunchDegrid(da, atlas_, box, acc, queue, g, sub_lo, sub_n, f0, s);
+      }
+      if (eng_.timing) {
+        st.degrid_launches += 1;
+        st.degrid_passes += atlas_.sub_vis[g];
+        st.degrid_bytes += 32.0 * atlas_.sub_vis[g] +
+                           8.0 * double(sub_n) * E * E * double(atlas_.sub_items[g]);
+      }
     }
   }
+  EventTimer tsc(eng_, &st.sort);
   LaunchScatterVis(atlas_.perm, atlas_.phasor, atlas_.n_valid, acc, vis_out_dev, s);
 }
 

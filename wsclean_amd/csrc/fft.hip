@@ -451,9 +451,7 @@ size_t LineLds(int n) { return size_t(kTwSlots + PaddedLen(n)) * sizeof(float2);
 // Kernels that use more than the default dynamic LDS must opt in once.
 template <typename K>
 void AllowLds(K kernel, size_t bytes) {
-  if (bytes > 48 * 1024)
-    WG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  AllowDynamicLds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 template <typename F>

@@ -786,9 +786,7 @@ __global__ void ItemEmitKernel(DevPlan dp, const uint32_t* __restrict__ off, int
 
 template <typename K>
 void AllowLds(K kernel, size_t bytes) {
-  if (bytes > 48 * 1024)
-    WG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  AllowDynamicLds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 template <typename F>
@@ -805,15 +803,7 @@ void DispatchW(int W, F&& f) {
   }
 }
 
-int GridBlocks() {
-  static int n = [] {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    return cus * 8;
-  }();
-  return n;
-}
+int GridBlocks() { return DeviceCUs() * 8; }
 
 }  // namespace
 
@@ -853,12 +843,9 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
       WG_CHECK(total <= 160 * 1024, "gridding LDS exceeds 160 KiB");
       AllowLds(GridKernelMfma<W>, total);
       // one persistent workgroup per resident slot; they drain the queue
-      int per_cu = 0, dev = 0, cus = 256;
-      WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(GridKernelMfma<W>), 64 * kMfmaWaves, total));
-      WG_HIP(hipGetDevice(&dev));
-      WG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      const int blocks = std::max(1, per_cu) * cus;
+      const int per_cu = ResidentBlocksPerCU(
+          reinterpret_cast<const void*>(GridKernelMfma<W>), 64 * kMfmaWaves, total);
+      const int blocks = std::max(1, per_cu) * DeviceCUs();
       LaunchFillZero(queue, sizeof(uint32_t), s);
       GridKernelMfma<W><<<blocks, 64 * kMfmaWaves, total, s>>>(
           dp, vs.items, vs.item_pos, pos_lo, pos_hi, vs.coord, vis_sorted, scratch, sub_lo, sub_n,
@@ -903,13 +890,10 @@ void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2
       const size_t total = ((lds + 15) & ~size_t(15)) + ring;
       WG_CHECK(total <= 160 * 1024, "degridding LDS exceeds 160 KiB");
       AllowLds(DegridKernelMfma<W>, total);
-      int per_cu = 0, dev = 0, cus = 256;
-      WG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(DegridKernelMfma<W>), 64 * kMfmaWaves, total));
-      WG_HIP(hipGetDevice(&dev));
-      WG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const int per_cu = ResidentBlocksPerCU(
+          reinterpret_cast<const void*>(DegridKernelMfma<W>), 64 * kMfmaWaves, total);
       LaunchFillZero(queue, sizeof(uint32_t), s);
-      DegridKernelMfma<W><<<std::max(1, per_cu) * cus, 64 * kMfmaWaves, total, s>>>(
+      DegridKernelMfma<W><<<std::max(1, per_cu) * DeviceCUs(), 64 * kMfmaWaves, total, s>>>(
           dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
           acc, sub_lo, sub_n, slab_lo, queue);
     } else {

@@ -136,6 +136,31 @@ class FacetBatch:
                           % (vis_out_dev.numel() // 2, n.value))
         _lib.check(self._lib.wg_facet_batch_predict(self._h, _dev(vis_out_dev)))
 
+    def CaptureMajorCycle(self, vis_dev=None, vis_out_dev=None, factors=None):
+        """Capture clear + invert(vis_dev) + FinalizeImage(factors[f]) and / or
+        predict(vis_out_dev) of every facet as one HIP graph."""
+        what = (1 if vis_dev is not None else 0) | (2 if vis_out_dev is not None else 0)
+        fac = np.ascontiguousarray(factors if factors is not None else np.ones(len(self)),
+                                   np.float64)
+        _lib.check(self._lib.wg_facet_batch_capture_major_cycle(
+            self._h, ctypes.c_void_p(vis_dev.data_ptr() if vis_dev is not None else 0),
+            ctypes.c_void_p(vis_out_dev.data_ptr() if vis_out_dev is not None else 0),
+            _ptr(fac), what))
+
+    def ReplayMajorCycle(self, count=1):
+        _lib.check(self._lib.wg_facet_batch_replay_major_cycle(self._h, int(count)))
+
+    def set_timing(self, on=True):
+        _lib.check(self._lib.wg_facet_batch_set_timing(self._h, 1 if on else 0))
+
+    def stage_times(self, reset=True):
+        out = (ctypes.c_double * 14)()
+        _lib.check(self._lib.wg_facet_batch_stage_times(self._h, out, 1 if reset else 0))
+        return dict(zip(["plan", "sort", "grid", "fft", "degrid", "total",
+                         "grid_kernel_ms", "grid_launches", "grid_bytes",
+                         "degrid_kernel_ms", "degrid_launches", "degrid_bytes",
+                         "grid_passes", "degrid_passes"], list(out)))
+
     def plan(self):
         buf = ctypes.create_string_buffer(1 << 20)
         _lib.check(self._lib.wg_facet_batch_plan_json(self._h, buf, 1 << 20))

@@ -195,6 +195,18 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_predict_prepared(self._h,
                                                  ctypes.c_void_p(vis_out_dev.data_ptr())))
 
+    def CaptureMajorCycle(self, vis_dev=None, vis_out_dev=None, factor=1.0):
+        """Capture clear + invert(vis_dev) + FinalizeImage(factor) and / or
+        predict(vis_out_dev) on the prepared set as a HIP graph."""
+        what = (1 if vis_dev is not None else 0) | (2 if vis_out_dev is not None else 0)
+        _lib.check(self._lib.wg_capture_major_cycle(
+            self._h, ctypes.c_void_p(vis_dev.data_ptr() if vis_dev is not None else 0),
+            ctypes.c_void_p(vis_out_dev.data_ptr() if vis_out_dev is not None else 0),
+            float(factor), what))
+
+    def ReplayMajorCycle(self, count=1):
+        _lib.check(self._lib.wg_replay_major_cycle(self._h, int(count)))
+
     # -- introspection ------------------------------------------------------
     def plan(self):
         buf = ctypes.create_string_buffer(4096)
