@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served-stream leg and print its JSON")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner leg")
     ap.add_argument("--facet-steps", type=int, default=2,
                     help="timed steps of the facet-batch leg (0 skips it)")
     ap.add_argument("--facets-only", action="store_true",
@@ -282,6 +283,8 @@ def main():
         "gen_s": t_gen,
     }
     line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
+    if not args.no_autotune:
+        line["autotune"] = autotune_leg(uvw, freq, n, dx, args.eps, n_rows, n_chan, plan)
     if not args.no_per_channel:
         line["per_channel_gpu"] = per_channel_leg(uvw, vis, freq, n, dx, args.eps, n_rows, value)
     g.close()
@@ -330,6 +333,23 @@ def per_channel_leg(uvw, vis, freq, n, dx, eps, n_rows, batched_value):
     torch.cuda.empty_cache()
     return {"value": v, "unit": "vis/s", "ms": dt * 1e3, "calls": n_chan,
             "batched_over_per_channel": batched_value / v}
+
+
+def autotune_leg(uvw, freq, n, dx, eps, n_rows, n_chan, plan):
+    """The autotuner on the headline workload (a 1.3e8-visibility row
+    sample): every trial's extrapolated step time, the choice, and the
+    planner's default (W, sigma, T) beside it."""
+    from wsclean_amd import WGriddingGridder_Simple
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, eps)
+    t = time.time()
+    trials = g.Autotune(uvw, freq, n_rows, n_chan, max_sample_vis=1 << 27)
+    tune_s = time.time() - t
+    g.close()
+    return {"tune_s": tune_s, "sample_vis": 1 << 27,
+            "chosen": {k: trials[0][k] for k in ("W", "sigma", "T", "est_ms")},
+            "default": {"W": plan["W"], "sigma": plan["sigma"], "T": plan["T"]},
+            "trials": [{k: t[k] for k in ("W", "sigma", "T", "P", "nu", "est_ms", "ok")}
+                       for t in trials]}
 
 
 def graph_cycle(g, vis, out, n_vis, steps):

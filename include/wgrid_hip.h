@@ -156,6 +156,23 @@ int wg_capture_major_cycle(wg_handle* h, const float* vis_dev, float* vis_out_de
                            double factor, int what);
 int wg_replay_major_cycle(wg_handle* h, int count);
 
+/* Autotuner (the `tuning` flag of wg_params selects ducc0's tuned
+ * variant in the reference, gridder_simple.h:217-225 via
+ * wgridder/wgriddingmsgridder.cpp:41-54; here it runs this autotuner).
+ * Times plan + sort + invert + predict of every candidate -- per kernel
+ * support W the smallest oversampling meeting epsilon and two larger ones,
+ * 8-cell tiles, and 16-cell tiles for the best two -- on a strided row
+ * sample of at most max_sample_vis visibilities of uvw_dev, extrapolates
+ * to all rows (per-visibility stages scale with the count, plane FFTs do
+ * not), and sets the handle's tuning to the fastest.  The choice is also
+ * kept for the process: a handle created with tuning != 0 runs the
+ * autotuner in its first invert / predict / prepare call for a geometry and
+ * visibility count (rounded to a power of two) and reuses the choice
+ * afterwards.  report (optional): JSON list of the trials,
+ * fastest first. */
+int wg_autotune(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                const double* freq, size_t max_sample_vis, char* report, size_t len);
+
 /* Plan of the last call as JSON (W, sigma, nu, nv, P, box, tiles...). */
 int wg_plan_json(const wg_handle* h, char* buf, size_t len);
 /* Per-stage device times (ms, HIP events on the handle's stream) accumulated

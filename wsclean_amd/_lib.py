@@ -55,6 +55,9 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_add_inversion_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_predict_prepared", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_autotune", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size,
+      ctypes.c_char_p, c_size]),
     ("wg_capture_major_cycle", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
     ("wg_replay_major_cycle", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

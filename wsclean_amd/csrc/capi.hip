@@ -5,8 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <algorithm>
 #include <mutex>
@@ -110,6 +112,41 @@ void DropGraph(hipGraphExec_t& g) {
   g = nullptr;
 }
 
+// Autotuner results of this process ("wisdom"), keyed by the image
+// geometry, epsilon, channel count and the visibility count rounded to a
+// power of two; handles created with tuning on reuse them.
+struct TuneChoice {
+  int W;
+  double sigma;
+  int T;
+};
+std::mutex g_wisdom_mu;
+std::map<std::string, TuneChoice> g_wisdom;
+
+std::string WisdomKey(const wg_params& p, size_t n_rows, size_t n_chan) {
+  char key[256];
+  const double nv = double(n_rows) * double(n_chan);
+  std::snprintf(key, sizeof(key), "%zu %zu %.9g %.9g %.9g %.9g %.3g %zu %d", p.width_t,
+                p.height_t, p.pixel_size_x, p.pixel_size_y, p.l_shift, p.m_shift, p.epsilon,
+                n_chan, nv > 0 ? int(std::lround(std::log2(nv))) : 0);
+  return key;
+}
+
+std::string TrialsJson(const std::vector<wg::Engine::TuneTrial>& t) {
+  std::string o = "[";
+  char buf[512];
+  for (size_t i = 0; i < t.size(); ++i) {
+    std::snprintf(buf, sizeof(buf),
+                  "%s{\"W\":%d,\"sigma\":%.4g,\"T\":%d,\"Sg\":%d,\"P\":%d,\"nu\":%d,"
+                  "\"est_ms\":%.6g,\"vis_ms\":%.6g,\"fixed_ms\":%.6g,\"ok\":%s}",
+                  i ? "," : "", t[i].W, t[i].sigma, t[i].T, t[i].Sg, t[i].P, t[i].nu,
+                  t[i].est_ms < 1e300 ? t[i].est_ms : -1.0, t[i].vis_ms, t[i].fixed_ms,
+                  t[i].error.empty() ? "true" : "false");
+    o += buf;
+  }
+  return o + "]";
+}
+
 }  // namespace
 
 struct wg_handle {
@@ -207,9 +244,34 @@ int wg_initialize_inversion(wg_handle* h) {
   });
 }
 
+// The tuned variant (wg_params.tuning): the autotuner's (W, sigma, tile)
+// for this geometry and visibility count, measured once per process.
+static void ApplyTuning(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                        const double* freq) {
+  if (!h->prm.tuning || n_rows == 0) return;
+  const std::string key = WisdomKey(h->prm, n_rows, n_chan);
+  TuneChoice c{0, 0, 0};
+  {
+    std::lock_guard<std::mutex> wl(g_wisdom_mu);
+    auto it = g_wisdom.find(key);
+    if (it != g_wisdom.end()) c = it->second;
+  }
+  if (c.W == 0) {
+    const auto trials = h->eng->Autotune(n_rows, n_chan, uvw_dev, freq, size_t(1) << 27);
+    c = TuneChoice{trials.front().W, trials.front().sigma, trials.front().T};
+    std::lock_guard<std::mutex> wl(g_wisdom_mu);
+    g_wisdom[key] = c;
+  }
+  wg::PlanOptions& o = h->eng->Options();
+  o.force_W = c.W;
+  o.force_sigma = c.sigma;
+  o.T = c.T;
+}
+
 static void AddInversion(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
                          const double* freq, const float* vis_dev) {
   wg::Engine& e = *h->eng;
+  ApplyTuning(h, n_rows, n_chan, uvw_dev, freq);
   e.Prepare(h->vs_call, "call", n_rows, n_chan, uvw_dev, freq,
             reinterpret_cast<const float2*>(vis_dev));
   h->plan_json = h->vs_call.plan.describe();
@@ -362,6 +424,7 @@ int wg_initialize_prediction_device(wg_handle* h, const float* image_dev) {
 static void Predict(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
                     const double* freq, float* out_dev) {
   wg::Engine& e = *h->eng;
+  ApplyTuning(h, n_rows, n_chan, uvw_dev, freq);
   e.Prepare(h->vs_call, "call", n_rows, n_chan, uvw_dev, freq, nullptr);
   h->plan_json = h->vs_call.plan.describe();
   e.Predict(h->vs_call, h->img, reinterpret_cast<float2*>(out_dev));
@@ -403,6 +466,7 @@ int wg_prepare_visibilities(wg_handle* h, size_t n_rows, size_t n_chan, const do
     std::lock_guard<std::mutex> lk(h->mu);
     h->prepared = false;
     DropGraph(h->cycle);
+    ApplyTuning(h, n_rows, n_chan, uvw_dev, freq);
     h->eng->Prepare(h->vs_prep, "prep", n_rows, n_chan, uvw_dev, freq, nullptr);
     h->plan_json = h->vs_prep.plan.describe();
     h->prepared = true;
@@ -426,6 +490,25 @@ int wg_predict_prepared(wg_handle* h, float* vis_out_dev) {
     WG_CHECK(h->prepared, "wg_prepare_visibilities has not been called");
     h->eng->Predict(h->vs_prep, h->img, reinterpret_cast<float2*>(vis_out_dev));
     WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_autotune(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
+                const double* freq, size_t max_sample_vis, char* report, size_t len) {
+  return Guard([&] {
+    WG_CHECK(h && uvw_dev && freq, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const auto trials = h->eng->Autotune(n_rows, n_chan, uvw_dev, freq, max_sample_vis);
+    {
+      std::lock_guard<std::mutex> wl(g_wisdom_mu);
+      g_wisdom[WisdomKey(h->prm, n_rows, n_chan)] =
+          TuneChoice{trials.front().W, trials.front().sigma, trials.front().T};
+    }
+    if (report && len) {
+      const std::string j = TrialsJson(trials);
+      std::strncpy(report, j.c_str(), len - 1);
+      report[len - 1] = 0;
+    }
   });
 }
 

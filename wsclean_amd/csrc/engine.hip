@@ -463,4 +463,121 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
   }
 }
 
+// ------------------------------------------------------------ autotuner
+
+std::vector<Engine::TuneTrial> Engine::Autotune(size_t n_rows, size_t n_chan,
+                                                const double* uvw_dev, const double* freq_host,
+                                                size_t max_sample_vis) {
+  WG_CHECK(n_rows > 0 && n_chan > 0, "autotune needs visibilities");
+  const size_t n_vis = n_rows * n_chan;
+  const size_t want = std::max<size_t>(n_chan, std::min(n_vis, std::max<size_t>(max_sample_vis, 1)));
+  const size_t stride = std::max<size_t>(1, n_vis / want);
+  const size_t ns = (n_rows + stride - 1) / stride;
+  const double scale_up = double(n_rows) / double(ns);
+  // strided row sample of uvw; the visibility values do not change the work
+  double* suvw = static_cast<double*>(arena_.Get("tune.uvw", ns * 3 * sizeof(double)));
+  LaunchStrideRows(uvw_dev, stride, ns, 3, suvw, stream_);
+  float2* svis = static_cast<float2*>(arena_.Get("tune.vis", ns * n_chan * sizeof(float2)));
+  LaunchFillZero(svis, ns * n_chan * sizeof(float2), stream_);
+  float2* sout = static_cast<float2*>(arena_.Get("tune.out", ns * n_chan * sizeof(float2)));
+  double* simg = static_cast<double*>(
+      arena_.Get("tune.img", size_t(geom_.nx) * geom_.ny * sizeof(double)));
+  LaunchFillZero(simg, size_t(geom_.nx) * geom_.ny * sizeof(double), stream_);
+
+  // candidates: per support W, the smallest oversampling meeting epsilon
+  // and two larger ones (fewer w planes, larger grids)
+  std::vector<TuneTrial> cand;
+  {
+    std::map<int, std::vector<double>> by_w;
+    for (const KernelShape& k : KernelTable()) {
+      if (k.sigma < opt_.sigma_min - 1e-9 || k.sigma > opt_.sigma_max + 1e-9) continue;
+      if (ShapeErrorEstimate(k) > geom_.epsilon) continue;
+      by_w[k.W].push_back(k.sigma);
+    }
+    for (auto& kv : by_w) {
+      std::sort(kv.second.begin(), kv.second.end());
+      const double s0 = kv.second.front();
+      for (double sg : kv.second)
+        if (sg == s0 || std::fabs(sg - (s0 + 0.15)) < 1e-9 || std::fabs(sg - (s0 + 0.3)) < 1e-9) {
+          TuneTrial t;
+          t.W = kv.first;
+          t.sigma = sg;
+          t.T = 8;
+          cand.push_back(t);
+        }
+    }
+  }
+  WG_CHECK(!cand.empty(), "no kernel shape reaches epsilon");
+
+  const PlanOptions saved = opt_;
+  const bool saved_timing = timing;
+  const StageTimes saved_times = times_;
+  auto run = [&](TuneTrial& t) {
+    try {
+      opt_ = saved;
+      opt_.force_W = t.W;
+      opt_.force_sigma = t.sigma;
+      opt_.T = t.T;
+      opt_.Sg = t.Sg;
+      VisSet vs;
+      timing = false;
+      Prepare(vs, "tune", ns, n_chan, suvw, freq_host, nullptr);
+      Invert(vs, svis, simg);
+      Predict(vs, simg, sout);
+      WG_HIP(hipStreamSynchronize(stream_));
+      timing = true;
+      times_ = StageTimes{};
+      Prepare(vs, "tune", ns, n_chan, suvw, freq_host, nullptr);
+      Invert(vs, svis, simg);
+      Predict(vs, simg, sout);
+      WG_HIP(hipStreamSynchronize(stream_));
+      CollectTimes();
+      const StageTimes& st = times_;
+      t.fixed_ms = st.fft;
+      t.vis_ms = st.plan + st.sort + st.grid + st.degrid;
+      t.est_ms = t.vis_ms * scale_up + t.fixed_ms;
+      t.Sg = vs.plan.Sg;
+      t.P = vs.plan.P;
+      t.nu = vs.plan.nu;
+    } catch (const std::exception& e) {
+      (void)hipStreamSynchronize(stream_);
+      pending_.clear();
+      t.est_ms = 1e300;
+      t.error = e.what();
+    }
+  };
+  for (TuneTrial& t : cand) run(t);
+  std::stable_sort(cand.begin(), cand.end(),
+                   [](const TuneTrial& a, const TuneTrial& b) { return a.est_ms < b.est_ms; });
+  // the best two shapes also with 16-cell tiles
+  const size_t base = cand.size();
+  for (size_t i = 0; i < std::min<size_t>(2, base); ++i) {
+    if (cand[i].est_ms >= 1e300) continue;
+    TuneTrial t;
+    t.W = cand[i].W;
+    t.sigma = cand[i].sigma;
+    t.T = 16;
+    run(t);
+    cand.push_back(t);
+  }
+  std::stable_sort(cand.begin(), cand.end(),
+                   [](const TuneTrial& a, const TuneTrial& b) { return a.est_ms < b.est_ms; });
+  timing = saved_timing;
+  times_ = saved_times;
+  opt_ = saved;
+  WG_CHECK(cand.front().est_ms < 1e300, "every autotune candidate failed: " + cand.front().error);
+  opt_.force_W = cand.front().W;
+  opt_.force_sigma = cand.front().sigma;
+  opt_.T = cand.front().T;
+  opt_.Sg = 0;
+  // the trials' buffers; the shared plane buffers are re-sized by the next call
+  for (const char* n : {"box", "rbuf", "cbuf", "grid.scratch",
+                        "tune.uvw", "tune.vis", "tune.out", "tune.img", "tune.keys", "tune.perm",
+                        "tune.keys2", "tune.perm2", "tune.coord", "tune.phasor", "tune.off",
+                        "tune.items", "tune.item_pos", "tune.corr", "tune.scale", "tune.quad",
+                        "tune.phiu", "tune.phiv"})
+    arena_.Release(n);
+  return cand;
+}
+
 }  // namespace wg

@@ -128,6 +128,22 @@ class Engine {
   // vis_out[row][chan] = prediction from model img_dev ([x][y], fp64).
   void Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev);
 
+  // Autotuner: times prepare + invert + predict of every candidate shape
+  // (support W and oversampling meeting epsilon, tile edge) on a strided
+  // row sample of at most max_sample_vis visibilities and extrapolates to
+  // the full set (the per-visibility stages scale with the count, the plane
+  // FFTs do not).  Leaves the fastest candidate in Options() and returns
+  // every trial, fastest first.
+  struct TuneTrial {
+    int W = 0;
+    double sigma = 0;
+    int T = 0, Sg = 0, P = 0, nu = 0;
+    double est_ms = 0, vis_ms = 0, fixed_ms = 0;
+    std::string error;
+  };
+  std::vector<TuneTrial> Autotune(size_t n_rows, size_t n_chan, const double* uvw_dev,
+                                  const double* freq_host, size_t max_sample_vis);
+
   DeviceArena& Arena() { return arena_; }
   StageTimes& Times() { return times_; }
   // Accumulates the recorded event pairs into Times(); the caller must have
@@ -329,6 +345,9 @@ void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+// out row r = in row r * stride, for rows of `row_words` 8-byte words
+void LaunchStrideRows(const void* in, size_t stride, size_t n_out, size_t row_words, void* out,
+                      hipStream_t s);
 // out[row][ch] = Stokes-I collapse of the conjugate-gain (after the forward
 // gain when `forward`) corrected vis[row][ch][0..npol).  gains: cached
 // solutions [n_times][n_chan][n_ant][nparms]; mode in GainMode order

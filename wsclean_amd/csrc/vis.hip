@@ -329,6 +329,16 @@ __global__ void __launch_bounds__(kBlock) CorrectCollapseKernel(
   }
 }
 
+__global__ void StrideRowsKernel(const uint64_t* __restrict__ in, size_t stride, size_t n_out,
+                                 size_t row_words, uint64_t* __restrict__ out) {
+  const size_t n = n_out * row_words;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const size_t r = i / row_words, k = i - r * row_words;
+    out[i] = in[r * stride * row_words + k];
+  }
+}
+
 __global__ void FillZeroKernel(uint32_t* __restrict__ p, size_t nwords) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < nwords;
        i += size_t(gridDim.x) * kBlock)
@@ -559,6 +569,15 @@ void LaunchStridedGather(const uint32_t* in, size_t stride, size_t count, uint32
                          hipStream_t s) {
   if (!count) return;
   StridedGatherKernel<<<unsigned((count + 255) / 256), 256, 0, s>>>(in, stride, count, out);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchStrideRows(const void* in, size_t stride, size_t n_out, size_t row_words, void* out,
+                      hipStream_t s) {
+  const size_t n = n_out * row_words;
+  if (!n) return;
+  StrideRowsKernel<<<Blocks(n), kBlock, 0, s>>>(static_cast<const uint64_t*>(in), stride, n_out,
+                                                 row_words, static_cast<uint64_t*>(out));
   WG_HIP(hipGetLastError());
 }
 

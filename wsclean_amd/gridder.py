@@ -195,6 +195,17 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_predict_prepared(self._h,
                                                  ctypes.c_void_p(vis_out_dev.data_ptr())))
 
+    def Autotune(self, uvw_dev, freq, n_rows, n_chan, max_sample_vis=1 << 27):
+        """Measure the candidate kernel shapes and tiles on a row sample of
+        the device uvw and keep the fastest; returns the trials (fastest
+        first)."""
+        freq = _f64(freq, (n_chan,))
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(self._lib.wg_autotune(self._h, n_rows, n_chan,
+                                         ctypes.c_void_p(uvw_dev.data_ptr()), _ptr(freq),
+                                         int(max_sample_vis), buf, 1 << 16))
+        return json.loads(buf.value.decode())
+
     def CaptureMajorCycle(self, vis_dev=None, vis_out_dev=None, factor=1.0):
         """Capture clear + invert(vis_dev) + FinalizeImage(factor) and / or
         predict(vis_out_dev) on the prepared set as a HIP graph."""
