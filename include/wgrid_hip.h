@@ -237,6 +237,68 @@ int wg_facet_batch_stage_times(wg_facet_batch* b, double* out14, int reset);
 int wg_facet_batch_plan_json(wg_facet_batch* b, char* buf, size_t len);
 
 /* ------------------------------------------------------------------------
+ * Imaging weights: WSClean's ImageWeights (structures/imageweights.h:22-132,
+ * structures/imageweights.cpp:33-390) on the device.  The half-plane grid
+ * of weight sums, [height / 2][width] FP64, lives in HBM.
+ * mode: 0 natural, 1 uniform, 2 Briggs (robustness briggs_robustness);
+ * the other arguments as the constructor (imageweights.cpp:33-48). */
+typedef struct wg_image_weights wg_image_weights;
+int wg_image_weights_create(int mode, double briggs_robustness, size_t width, size_t height,
+                            double pixel_scale_x, double pixel_scale_y, int weights_as_taper,
+                            double super_weight, wg_image_weights** out);
+int wg_image_weights_destroy(wg_image_weights* w);
+/* Grid size after the super-weight division and rounding to even. */
+int wg_image_weights_size(const wg_image_weights* w, size_t* width, size_t* height);
+/* ImageWeights::Grid(MSProvider&, BandData) (imageweights.cpp:76-114) for
+ * one chunk of rows: uvw_dev [rows][3] f64 (metres), weights_dev
+ * [rows][chan][pol] f32 (device), freq [chan] (host). */
+int wg_image_weights_grid(wg_image_weights* w, size_t n_rows, size_t n_chan, size_t n_pol,
+                          const double* uvw_dev, const double* freq, const float* weights_dev);
+/* FinishGridding (imageweights.cpp:116-150); a second call fails. */
+int wg_image_weights_finish(wg_image_weights* w);
+/* SetMinUVRange / SetMaxUVRange / SetTukeyTaper / SetTukeyInnerTaper /
+ * SetEdgeTaper / SetEdgeTukeyTaper / SetGaussianTaper / RankFilter
+ * (imageweights.cpp:152-356) */
+int wg_image_weights_set_min_uv_range(wg_image_weights* w, double min_uv);
+int wg_image_weights_set_max_uv_range(wg_image_weights* w, double max_uv);
+int wg_image_weights_set_tukey_taper(wg_image_weights* w, double transition, double max_uv);
+int wg_image_weights_set_tukey_inner_taper(wg_image_weights* w, double transition,
+                                           double min_uv);
+int wg_image_weights_set_edge_taper(wg_image_weights* w, double size);
+int wg_image_weights_set_edge_tukey_taper(wg_image_weights* w, double transition, double edge);
+int wg_image_weights_set_gaussian_taper(wg_image_weights* w, double beam_size);
+int wg_image_weights_rank_filter(wg_image_weights* w, double rank_limit, size_t window);
+/* GetGrid (imageweights.cpp:270-287): image [height][width] FP64 (host). */
+int wg_image_weights_get_grid(wg_image_weights* w, double* image);
+/* GetWeight(u, v) (imageweights.h:46) for n (u, v) pairs in wavelengths
+ * (host arrays). */
+int wg_image_weights_sample(wg_image_weights* w, size_t n, const double* u, const double* v,
+                            double* out);
+
+/* Caller-side row preparation on the device: MsGridderData::CalculateWeights
+ * + ApplyWeights (gridding/msgridderdata.h:761-789, :815-886) with
+ * RotateVisibilities (:929-941) and CollapseData (:37-52), without gain
+ * corrections (those are wg_add_inversion_data_corrected).  Per (row,
+ * channel, polarization): the point-source visibility exp(+2 pi i (u dl +
+ * v dm + w dn) / lambda) when psf (else vis_dev), minus model_dev when
+ * given; weight zero for unselected channels (selected: host bytes, NULL =
+ * all); weights squared (weighting 1) or set to one (2); times the imaging
+ * weight GetWeight(u / lambda, v / lambda) (image_weights NULL: 1); the
+ * visibility times that weight; then one visibility per channel (2
+ * polarizations: sum; 4: Stokes I, 0.5 (XX + YY)) into out_dev
+ * [rows][chan] complex64, ready for wg_add_inversion_data_device.
+ * stats4 (host): total weight (all polarizations), visibility weight sum,
+ * max gridded weight, gridded visibility count (first polarization), as
+ * MsGridderData accumulates them.  vis_dev, weights_dev, model_dev:
+ * [rows][chan][pol] on the device; uvw_dev [rows][3] f64.  Runs on the
+ * handle's stream. */
+int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
+                    const double* uvw_dev, const double* freq, const float* vis_dev,
+                    const float* weights_dev, const float* model_dev, const uint8_t* selected,
+                    int psf, double psf_dl, double psf_dm, int weighting,
+                    wg_image_weights* image_weights, float* out_dev, double* stats4);
+
+/* ------------------------------------------------------------------------
  * Test entry points (no reference counterpart): expose internals so that
  * the bucket sort and the LDS FFT can be checked bit-for-bit / against a
  * library FFT. */

@@ -1,0 +1,172 @@
+"""Imaging weights and row preparation on the GPU against the oracle
+(oracle/weights.py): the weight grid after gridding and FinishGridding,
+every taper and the rank filter, GetWeight, and the prepared visibilities
+with their weight sums; then the prepared rows gridded end to end against
+the FP64 direct FT of the weighted visibilities."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
+from oracle import weights as ow  # noqa: E402
+
+from wsclean_amd import WGriddingGridder_Simple, synth  # noqa: E402
+from wsclean_amd.weighting import ImageWeights, prepare_rows  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def case(seed=3, n_rows=3000, n_chan=4, n_pol=2):
+    import torch
+    rng = np.random.default_rng(seed)
+    enu = synth.antennas_disc(rng, 24, 1500.0)
+    uvw = synth.earth_rotation_uvw(enu, n_rows // 276 + 1, 60.0, np.deg2rad(-30.0))[:n_rows]
+    freq = np.linspace(120e6, 180e6, n_chan)
+    wts = rng.uniform(0.5, 2.0, (n_rows, n_chan, n_pol)).astype(np.float32)
+    wts[rng.random((n_rows, n_chan, n_pol)) < 0.05] = 0.0
+    dev = torch.device("cuda:0")
+    return uvw, freq, wts, torch.from_numpy(uvw).to(dev), torch.from_numpy(wts).to(dev)
+
+
+@pytest.mark.parametrize("mode,robust,taper", [(ow.NATURAL, 0.0, False),
+                                               (ow.UNIFORM, 0.0, False),
+                                               (ow.UNIFORM, 0.0, True),
+                                               (ow.BRIGGS, 0.5, False),
+                                               (ow.BRIGGS, -1.0, False)])
+def test_weight_grid_matches_oracle(mode, robust, taper):
+    uvw, freq, wts, uvw_d, wts_d = case()
+    psx = 8.0 / 206265.0
+    ref = ow.ImageWeightsOracle(mode, 250, 180, psx, psx, weights_as_taper=taper,
+                                super_weight=1.3, robustness=robust)
+    iw = ImageWeights(mode, 250, 180, psx, psx, weights_as_taper=taper, super_weight=1.3,
+                      briggs_robustness=robust)
+    assert (iw.width, iw.height) == (ref.W, ref.H)
+    # two chunks, as a caller grids chunk by chunk
+    ref.grid_rows(uvw[:1700], freq, wts[:1700])
+    ref.grid_rows(uvw[1700:], freq, wts[1700:])
+    iw.Grid(uvw_d[:1700], freq, wts_d[:1700])
+    iw.Grid(uvw_d[1700:], freq, wts_d[1700:])
+    ref.finish()
+    iw.FinishGridding()
+    got = iw.GetGrid()
+    want = ref.get_grid()
+    assert np.count_nonzero(want) > 100
+    if mode == ow.BRIGGS:
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    else:
+        np.testing.assert_array_equal(got, want)
+    with pytest.raises(Exception):
+        iw.FinishGridding()
+    rng = np.random.default_rng(9)
+    u = rng.uniform(-3e4, 3e4, 2000)
+    v = rng.uniform(-3e4, 3e4, 2000)
+    np.testing.assert_allclose(iw.GetWeight(u, v), ref.get_weight(u, v), rtol=1e-13, atol=0)
+    iw.close()
+
+
+# uv extent of the data: about 1800 wavelengths; grid edge at 0.5 / psx = 1719
+@pytest.mark.parametrize("name,args", [("SetMinUVRange", (300.0,)),
+                                       ("SetMaxUVRange", (1000.0,)),
+                                       ("SetTukeyTaper", (400.0, 1400.0)),
+                                       ("SetTukeyInnerTaper", (300.0, 200.0)),
+                                       ("SetEdgeTaper", (400.0,)),
+                                       ("SetEdgeTukeyTaper", (300.0, 200.0)),
+                                       ("SetGaussianTaper", (40.0 / 206265.0,)),
+                                       ("RankFilter", (1.5, 5))])
+def test_tapers_and_rank_filter_match_oracle(name, args):
+    uvw, freq, wts, uvw_d, wts_d = case(seed=4)
+    psx = 60.0 / 206265.0
+    ref = ow.ImageWeightsOracle(ow.UNIFORM, 128, 96, psx, psx)
+    iw = ImageWeights(ow.UNIFORM, 128, 96, psx, psx)
+    ref.grid_rows(uvw, freq, wts)
+    iw.Grid(uvw_d, freq, wts_d)
+    ref.finish()
+    iw.FinishGridding()
+    oracle_fn = {"SetMinUVRange": ref.set_min_uv_range, "SetMaxUVRange": ref.set_max_uv_range,
+                 "SetTukeyTaper": ref.set_tukey_taper,
+                 "SetTukeyInnerTaper": ref.set_tukey_inner_taper,
+                 "SetEdgeTaper": ref.set_edge_taper,
+                 "SetEdgeTukeyTaper": ref.set_edge_tukey_taper,
+                 "SetGaussianTaper": ref.set_gaussian_taper,
+                 "RankFilter": ref.rank_filter}[name]
+    before = ref.get_grid()
+    oracle_fn(*args)
+    getattr(iw, name)(*args)
+    want = ref.get_grid()
+    assert not np.array_equal(want, before), "the case must exercise the taper"
+    np.testing.assert_allclose(iw.GetGrid(), want, rtol=1e-13, atol=1e-300)
+    iw.close()
+
+
+@pytest.mark.parametrize("n_pol,psf,weighting,model,sel,iwmode", [
+    (1, False, ow.VIS_NORMAL, False, None, None),
+    (2, False, ow.VIS_SQUARED, True, [1, 0, 1, 1], ow.UNIFORM),
+    (4, False, ow.VIS_UNIT, False, None, ow.BRIGGS),
+    (2, True, ow.VIS_NORMAL, False, None, ow.NATURAL),
+])
+def test_prepare_rows_matches_oracle(n_pol, psf, weighting, model, sel, iwmode):
+    import torch
+    uvw, freq, wts, uvw_d, wts_d = case(seed=5, n_rows=2000, n_chan=4, n_pol=n_pol)
+    rng = np.random.default_rng(7)
+    vis = (rng.normal(size=wts.shape) + 1j * rng.normal(size=wts.shape)).astype(np.complex64)
+    mdl = (0.3 * (rng.normal(size=wts.shape) + 1j * rng.normal(size=wts.shape))).astype(
+        np.complex64) if model else None
+    dev = torch.device("cuda:0")
+    as_dev = lambda a: torch.from_numpy(a.view(np.float32).reshape(*a.shape, 2).copy()).to(dev)
+    psx = 8.0 / 206265.0
+    ref_iw = iw = None
+    if iwmode is not None:
+        ref_iw = ow.ImageWeightsOracle(iwmode, 200, 200, psx, psx, robustness=0.0)
+        ref_iw.grid_rows(uvw, freq, wts)
+        ref_iw.finish()
+        iw = ImageWeights(iwmode, 200, 200, psx, psx)
+        iw.Grid(uvw_d, freq, wts_d)
+        iw.FinishGridding()
+    dl, dm = (0.004, -0.006) if psf else (0.0, 0.0)
+    want, wst = ow.prepare_rows(uvw, freq, None if psf else vis, wts, model=mdl,
+                                selected=None if sel is None else np.array(sel, bool), psf=psf,
+                                psf_dl=dl, psf_dm=dm, weighting=weighting, image_weights=ref_iw)
+    g = WGriddingGridder_Simple(64, 64, 64, 64, psx, psx, 0.0, 0.0, 1, 1e-4)
+    out = torch.empty((2000, 4, 2), dtype=torch.float32, device=dev)
+    st = prepare_rows(g, uvw_d, freq, wts_d, out, vis_dev=None if psf else as_dev(vis),
+                      model_dev=as_dev(mdl) if model else None, selected=sel, psf=psf,
+                      psf_dl=dl, psf_dm=dm, weighting=weighting, image_weights=iw)
+    got = out.cpu().numpy().view(np.complex64).reshape(2000, 4)
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-6 * np.abs(want).max())
+    for k in ("total_weight", "visibility_weight_sum", "max_gridded_weight"):
+        assert st[k] == pytest.approx(wst[k], rel=1e-12), k
+    assert st["gridded_visibility_count"] == wst["gridded_visibility_count"]
+    g.close()
+    if iw is not None:
+        iw.close()
+
+
+def test_uniform_weighted_image_matches_direct_ft(oracle):
+    import torch
+    p = synth.small_problem(seed=61, n_ant=16, n_times=30, n_chan=2, npix=96, noise=0.05)
+    n_rows = p["uvw"].shape[0]
+    dev = torch.device("cuda:0")
+    uvw_d = torch.from_numpy(p["uvw"]).to(dev)
+    wts = np.ones((n_rows, 2, 1), np.float32)
+    wts_d = torch.from_numpy(wts).to(dev)
+    vis_d = torch.from_numpy(p["vis"].view(np.float32).reshape(n_rows, 2, 1, 2).copy()).to(dev)
+    iw = ImageWeights(ow.UNIFORM, 96, 96, p["dx"], p["dx"])
+    iw.Grid(uvw_d, p["freq"], wts_d)
+    iw.FinishGridding()
+    g = WGriddingGridder_Simple(96, 96, 96, 96, p["dx"], p["dx"], 0.0, 0.0, 1, 1e-5)
+    prepared = torch.empty((n_rows, 2, 2), dtype=torch.float32, device=dev)
+    st = prepare_rows(g, uvw_d, p["freq"], wts_d, prepared, vis_dev=vis_d, image_weights=iw)
+    g.InitializeInversion()
+    g.AddInversionDataDevice(uvw_d, p["freq"], prepared)
+    g.FinalizeImage(1.0 / st["total_weight"])
+    img = g.RealImage().reshape(96, 96)
+    weighted = prepared.cpu().numpy().view(np.complex64).reshape(n_rows, 2)
+    ref = oracle.dft_invert(96, 96, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
+                            weighted) / st["total_weight"]
+    err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
+    print("uniform-weighted image vs direct FT", err, flush=True)
+    assert err < 1e-5
+    g.close()
+    iw.close()

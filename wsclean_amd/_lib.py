@@ -97,6 +97,35 @@ SIGNATURES = [
     ("wg_facet_batch_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("wg_facet_batch_stage_times", ctypes.c_int, [ctypes.c_void_p, c_dbl_p, ctypes.c_int]),
     ("wg_facet_batch_plan_json", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c_size]),
+    ("wg_image_weights_create", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_double, c_size, c_size, ctypes.c_double, ctypes.c_double,
+      ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wg_image_weights_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_image_weights_size", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.POINTER(c_size)]),
+    ("wg_image_weights_grid", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
+    ("wg_image_weights_finish", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_image_weights_set_min_uv_range", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("wg_image_weights_set_max_uv_range", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("wg_image_weights_set_tukey_taper", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("wg_image_weights_set_tukey_inner_taper", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("wg_image_weights_set_edge_taper", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("wg_image_weights_set_edge_tukey_taper", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    ("wg_image_weights_set_gaussian_taper", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("wg_image_weights_rank_filter", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_size]),
+    ("wg_image_weights_get_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_image_weights_sample", ctypes.c_int,
+     [ctypes.c_void_p, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_prepare_rows", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p]),
     ("wgx_facet_atlas", ctypes.c_int,
      [ctypes.POINTER(WgParams), c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
       c_size]),

@@ -179,6 +179,28 @@ struct wg_facet_batch {
   size_t npix(size_t f) const { return prm[f].width_t * prm[f].height_t; }
 };
 
+struct wg_image_weights {
+  int mode = 0;
+  double robust = 0;
+  bool as_taper = false;
+  bool finished = false;
+  hipStream_t stream = nullptr;
+  wg::DeviceArena arena;
+  wg::WeightGrid g{};
+  double* total_part = nullptr;  // gridded-weight partials (accumulated)
+  double* sums = nullptr;        // total, sum of squares
+  std::mutex mu;
+
+  size_t cells() const { return size_t(g.height / 2) * size_t(g.width); }
+  ~wg_image_weights() {
+    arena.ReleaseAll();
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
 extern "C" {
 
 const char* wg_last_error(void) { return g_last_error.c_str(); }
@@ -950,6 +972,222 @@ int wg_facet_batch_plan_json(wg_facet_batch* b, char* buf, size_t len) {
     std::lock_guard<std::mutex> lk(b->mu);
     std::strncpy(buf, b->plan_json.c_str(), len - 1);
     buf[len - 1] = 0;
+  });
+}
+
+// ------------------------------------------------------------ imaging weights
+
+int wg_image_weights_create(int mode, double briggs_robustness, size_t width, size_t height,
+                            double pixel_scale_x, double pixel_scale_y, int weights_as_taper,
+                            double super_weight, wg_image_weights** out) {
+  return Guard([&] {
+    WG_CHECK(out, "null argument");
+    WG_CHECK(mode >= 0 && mode <= 2, "mode: 0 natural, 1 uniform, 2 Briggs");
+    WG_CHECK(super_weight > 0 && pixel_scale_x > 0 && pixel_scale_y > 0, "bad scale");
+    auto w = std::make_unique<wg_image_weights>();
+    w->mode = mode;
+    w->robust = briggs_robustness;
+    w->as_taper = weights_as_taper != 0;
+    // imageweights.cpp:33-48: sizes divided by the super weight, made even
+    size_t iw = size_t(std::llround(double(width) / super_weight));
+    size_t ih = size_t(std::llround(double(height) / super_weight));
+    if (iw % 2) ++iw;
+    if (ih % 2) ++ih;
+    WG_CHECK(iw >= 2 && ih >= 2 && iw < (1u << 30) && ih < (1u << 30), "bad weight grid size");
+    WG_HIP(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    w->arena.SetStream(w->stream);
+    w->g.width = int(iw);
+    w->g.height = int(ih);
+    w->g.psx = pixel_scale_x;
+    w->g.psy = pixel_scale_y;
+    w->g.grid = static_cast<double*>(w->arena.Get("grid", w->cells() * sizeof(double)));
+    w->total_part =
+        static_cast<double*>(w->arena.Get("total", wg::WeightStatPartials() * sizeof(double)));
+    w->sums = static_cast<double*>(w->arena.Get("sums", 2 * sizeof(double)));
+    wg::LaunchFillZero(w->g.grid, w->cells() * sizeof(double), w->stream);
+    wg::LaunchFillZero(w->total_part, wg::WeightStatPartials() * sizeof(double), w->stream);
+    wg::LaunchFillZero(w->sums, 2 * sizeof(double), w->stream);
+    WG_HIP(hipStreamSynchronize(w->stream));
+    *out = w.release();
+  });
+}
+
+int wg_image_weights_destroy(wg_image_weights* w) {
+  return Guard([&] { delete w; });
+}
+
+int wg_image_weights_size(const wg_image_weights* w, size_t* width, size_t* height) {
+  return Guard([&] {
+    WG_CHECK(w && width && height, "null argument");
+    *width = size_t(w->g.width);
+    *height = size_t(w->g.height);
+  });
+}
+
+static std::vector<double> Wavelengths(const double* freq, size_t n_chan) {
+  std::vector<double> lam(n_chan);
+  for (size_t c = 0; c < n_chan; ++c) {
+    WG_CHECK(freq[c] > 0, "frequencies must be positive");
+    lam[c] = wg::kSpeedOfLight / freq[c];
+  }
+  return lam;
+}
+
+int wg_image_weights_grid(wg_image_weights* w, size_t n_rows, size_t n_chan, size_t n_pol,
+                          const double* uvw_dev, const double* freq, const float* weights_dev) {
+  return Guard([&] {
+    WG_CHECK(w && freq && (n_rows == 0 || (uvw_dev && weights_dev)), "null argument");
+    WG_CHECK(n_pol >= 1 && n_pol <= 4, "1 to 4 polarizations");
+    std::lock_guard<std::mutex> lk(w->mu);
+    WG_CHECK(!w->finished, "Grid() after FinishGridding()");
+    const std::vector<double> lam = Wavelengths(freq, n_chan);
+    double* dl = static_cast<double*>(w->arena.Get("lambda", n_chan * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(dl, lam.data(), n_chan * sizeof(double), hipMemcpyHostToDevice,
+                          w->stream));
+    wg::LaunchGridWeights(w->g, uvw_dev, dl, n_rows, n_chan, n_pol, weights_dev, w->as_taper,
+                          w->total_part, w->stream);
+    WG_HIP(hipStreamSynchronize(w->stream));
+  });
+}
+
+int wg_image_weights_finish(wg_image_weights* w) {
+  return Guard([&] {
+    WG_CHECK(w, "null handle");
+    std::lock_guard<std::mutex> lk(w->mu);
+    WG_CHECK(!w->finished, "FinishGridding() called twice");
+    w->finished = true;
+    double* scratch =
+        static_cast<double*>(w->arena.Get("scratch", wg::WeightStatPartials() * sizeof(double)));
+    wg::LaunchFinishWeights(w->g, w->mode, w->robust, w->total_part, w->sums, scratch, w->stream);
+    WG_HIP(hipStreamSynchronize(w->stream));
+  });
+}
+
+static int Taper(wg_image_weights* w, int kind, double a, double b) {
+  return Guard([&] {
+    WG_CHECK(w, "null handle");
+    std::lock_guard<std::mutex> lk(w->mu);
+    wg::LaunchWeightTaper(w->g, kind, a, b, w->stream);
+    WG_HIP(hipStreamSynchronize(w->stream));
+  });
+}
+
+int wg_image_weights_set_min_uv_range(wg_image_weights* w, double min_uv) {
+  return Taper(w, 0, min_uv, 0);
+}
+int wg_image_weights_set_max_uv_range(wg_image_weights* w, double max_uv) {
+  return Taper(w, 1, max_uv, 0);
+}
+int wg_image_weights_set_tukey_taper(wg_image_weights* w, double transition, double max_uv) {
+  return Taper(w, 2, transition, max_uv);
+}
+int wg_image_weights_set_tukey_inner_taper(wg_image_weights* w, double transition,
+                                           double min_uv) {
+  return Taper(w, 3, transition, min_uv);
+}
+int wg_image_weights_set_edge_taper(wg_image_weights* w, double size) {
+  return Taper(w, 4, size, 0);
+}
+int wg_image_weights_set_edge_tukey_taper(wg_image_weights* w, double transition, double edge) {
+  return Taper(w, 5, transition, edge);
+}
+int wg_image_weights_set_gaussian_taper(wg_image_weights* w, double beam_size) {
+  // imageweights.cpp:333-341: -2 (half-power uv * 2 sqrt(2 ln 2))^2
+  const double half_power_uv = 1.0 / (beam_size * 2.0 * M_PI);
+  const long double sigma_to_hp = 2.0L * sqrtl(2.0L * logl(2.0L));
+  double m2s = half_power_uv * double(sigma_to_hp);
+  m2s *= -2.0 * m2s;
+  return Taper(w, 6, m2s, 0);
+}
+
+int wg_image_weights_rank_filter(wg_image_weights* w, double rank_limit, size_t window) {
+  return Guard([&] {
+    WG_CHECK(w && window > 0, "null handle or empty window");
+    std::lock_guard<std::mutex> lk(w->mu);
+    double* src = static_cast<double*>(w->arena.Get("rank.src", w->cells() * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(src, w->g.grid, w->cells() * sizeof(double), hipMemcpyDeviceToDevice,
+                          w->stream));
+    wg::LaunchRankFilter(w->g, src, rank_limit, int(window), w->stream);
+    WG_HIP(hipStreamSynchronize(w->stream));
+    w->arena.Release("rank.src");
+  });
+}
+
+int wg_image_weights_get_grid(wg_image_weights* w, double* image) {
+  return Guard([&] {
+    WG_CHECK(w && image, "null argument");
+    std::lock_guard<std::mutex> lk(w->mu);
+    const size_t n = size_t(w->g.width) * size_t(w->g.height);
+    double* d = static_cast<double*>(w->arena.Get("image", n * sizeof(double)));
+    wg::LaunchGetWeightGrid(w->g, d, w->stream);
+    WG_HIP(hipMemcpyAsync(image, d, n * sizeof(double), hipMemcpyDeviceToHost, w->stream));
+    WG_HIP(hipStreamSynchronize(w->stream));
+    w->arena.Release("image");
+  });
+}
+
+int wg_image_weights_sample(wg_image_weights* w, size_t n, const double* u, const double* v,
+                            double* out) {
+  return Guard([&] {
+    WG_CHECK(w && (n == 0 || (u && v && out)), "null argument");
+    std::lock_guard<std::mutex> lk(w->mu);
+    if (!n) return;
+    double* d = static_cast<double*>(w->arena.Get("sample", 3 * n * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(d, u, n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+    WG_HIP(hipMemcpyAsync(d + n, v, n * sizeof(double), hipMemcpyHostToDevice, w->stream));
+    wg::LaunchSampleWeights(w->g, d, d + n, n, d + 2 * n, w->stream);
+    WG_HIP(hipMemcpyAsync(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost, w->stream));
+    WG_HIP(hipStreamSynchronize(w->stream));
+  });
+}
+
+int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
+                    const double* uvw_dev, const double* freq, const float* vis_dev,
+                    const float* weights_dev, const float* model_dev, const uint8_t* selected,
+                    int psf, double psf_dl, double psf_dm, int weighting,
+                    wg_image_weights* image_weights, float* out_dev, double* stats4) {
+  return Guard([&] {
+    WG_CHECK(h && freq && stats4, "null argument");
+    WG_CHECK(n_rows == 0 || (uvw_dev && weights_dev && out_dev && (psf || vis_dev)),
+             "null argument");
+    WG_CHECK(n_pol == 1 || n_pol == 2 || n_pol == 4, "1, 2 or 4 polarizations");
+    WG_CHECK(weighting >= 0 && weighting <= 2, "weighting: 0 normal, 1 squared, 2 unit");
+    std::lock_guard<std::mutex> lk(h->mu);
+    wg::DeviceArena& a = h->eng->Arena();
+    const std::vector<double> lam = Wavelengths(freq, n_chan);
+    double* dl = static_cast<double*>(a.Get("prep.lambda", n_chan * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(dl, lam.data(), n_chan * sizeof(double), hipMemcpyHostToDevice,
+                          h->s()));
+    uint8_t* dsel = nullptr;
+    if (selected) {
+      dsel = static_cast<uint8_t*>(a.Get("prep.selected", n_chan));
+      WG_HIP(hipMemcpyAsync(dsel, selected, n_chan, hipMemcpyHostToDevice, h->s()));
+    }
+    double* part =
+        static_cast<double*>(a.Get("prep.part", wg::WeightStatPartials() * sizeof(double)));
+    double* dstats = static_cast<double*>(a.Get("prep.stats", 4 * sizeof(double)));
+    wg::PrepArgs args{};
+    args.n_rows = n_rows;
+    args.n_chan = n_chan;
+    args.n_pol = int(n_pol);
+    args.psf = psf != 0;
+    args.psf_dl = psf_dl;
+    args.psf_dm = psf_dm;
+    args.psf_dn = std::sqrt(1.0 - psf_dl * psf_dl - psf_dm * psf_dm) - 1.0;
+    args.weighting = weighting;
+    std::unique_lock<std::mutex> wl;
+    const wg::WeightGrid* g = nullptr;
+    if (image_weights) {
+      wl = std::unique_lock<std::mutex>(image_weights->mu);
+      WG_CHECK(image_weights->finished, "imaging weights used before FinishGridding()");
+      // the grid is written on the weights' own stream: finished and synchronised
+      g = &image_weights->g;
+    }
+    wg::LaunchPrepareRows(args, uvw_dev, dl, reinterpret_cast<const float2*>(vis_dev),
+                          weights_dev, reinterpret_cast<const float2*>(model_dev), dsel, g,
+                          reinterpret_cast<float2*>(out_dev), part, dstats, h->s());
+    WG_HIP(hipMemcpyAsync(stats4, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
   });
 }
 

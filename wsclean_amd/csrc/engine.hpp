@@ -345,6 +345,48 @@ void LaunchPhiTables(const DevPlan& dp, const double* quad_a, const double* quad
 void LaunchAccumulateImage(const float* dirty, const float* corr, double* img,
                            size_t n, hipStream_t s);
 void LaunchFillZero(void* p, size_t bytes, hipStream_t s);
+// ---- imaging weights and row preparation (weights.hip) ----
+// Half-plane weight grid of ImageWeights: [height / 2][width] FP64.
+struct WeightGrid {
+  double* grid;
+  int width, height;
+  double psx, psy;
+};
+struct PrepArgs {
+  size_t n_rows, n_chan;
+  int n_pol;       // 1, 2 or 4 (collapsed to one visibility per channel)
+  int psf;         // visibilities of a point source instead of vis
+  double psf_dl, psf_dm, psf_dn;  // its position (dn = sqrt(1 - dl^2 - dm^2) - 1)
+  int weighting;   // 0 normal, 1 squared, 2 unit visibility weights
+};
+// Number of FP64 block partials of the weight statistics / gridded total.
+size_t WeightStatPartials();
+// grid[cell] += w for every (row, channel, polarization) weight;
+// total_partial[block] accumulates the gridded sum (kept across calls)
+void LaunchGridWeights(const WeightGrid& g, const double* uvw, const double* lambda,
+                       size_t n_rows, size_t n_chan, size_t n_pol, const float* weights,
+                       bool as_taper, double* total_partial, hipStream_t s);
+// FinishGridding; mode 0 natural, 1 uniform, 2 Briggs.  sums: 2 doubles
+// (total, sum of squares), scratch: WeightStatPartials() doubles.
+void LaunchFinishWeights(const WeightGrid& g, int mode, double robust, const double* total_partial,
+                         double* sums, double* scratch, hipStream_t s);
+// kind: 0 min uv (a), 1 max uv (a), 2 Tukey (a transition, b max uv),
+// 3 inner Tukey (a transition, b min uv), 4 edge (a), 5 edge Tukey
+// (a transition, b edge), 6 Gaussian (a = -2 (half-power uv * 2 sqrt(2 ln 2))^2)
+void LaunchWeightTaper(const WeightGrid& g, int kind, double a, double b, hipStream_t s);
+void LaunchRankFilter(const WeightGrid& g, const double* src, double rank_limit, int win,
+                      hipStream_t s);
+void LaunchGetWeightGrid(const WeightGrid& g, double* img, hipStream_t s);
+void LaunchSampleWeights(const WeightGrid& g, const double* u, const double* v, size_t n,
+                         double* out, hipStream_t s);
+// out[row][chan]: weighted, collapsed visibilities; stats (device, 4
+// doubles): total weight, visibility weight sum, max gridded weight, gridded
+// count.  part: WeightStatPartials() doubles.  g may be null (weights 1).
+void LaunchPrepareRows(const PrepArgs& a, const double* uvw, const double* lambda,
+                       const float2* vis, const float* weights, const float2* model,
+                       const uint8_t* selected, const WeightGrid* g, float2* out, double* part,
+                       double* stats, hipStream_t s);
+
 // out row r = in row r * stride, for rows of `row_words` 8-byte words
 void LaunchStrideRows(const void* in, size_t stride, size_t n_out, size_t row_words, void* out,
                       hipStream_t s);

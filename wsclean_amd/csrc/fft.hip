@@ -151,6 +151,41 @@ __device__ __forceinline__ void Dft<8, false>(float2* v) { Dft8<false>(v); }
 template <>
 __device__ __forceinline__ void Dft<8, true>(float2* v) { Dft8<true>(v); }
 
+// Radix 9 as 3 x 3 (n = 3 n1 + n2, k = k1 + 3 k2): radix-3 DFTs over n1 for
+// each n2, twiddles w9^(n2 k1), radix-3 DFTs over n2 for each k1.
+template <bool INV>
+__device__ __forceinline__ void Dft9(float2* v) {
+  // w9^1, w9^2, w9^4 with w9 = exp(-+ 2 pi i / 9)
+  constexpr float c1 = 0.76604444311897803520f, s1 = 0.64278760968653932632f;
+  constexpr float c2 = 0.17364817766693034885f, s2 = 0.98480775301220805936f;
+  constexpr float c4 = -0.93969262078590838405f, s4 = 0.34202014332566873304f;
+  float2 y[3][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    y[n2][0] = v[n2];
+    y[n2][1] = v[n2 + 3];
+    y[n2][2] = v[n2 + 6];
+    Dft3<INV>(y[n2]);
+  }
+  const float sg = INV ? 1.f : -1.f;
+  y[1][1] = cmul(y[1][1], make_float2(c1, sg * s1));
+  y[1][2] = cmul(y[1][2], make_float2(c2, sg * s2));
+  y[2][1] = cmul(y[2][1], make_float2(c2, sg * s2));
+  y[2][2] = cmul(y[2][2], make_float2(c4, sg * s4));
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    float2 z[3] = {y[0][k1], y[1][k1], y[2][k1]};
+    Dft3<INV>(z);
+    v[k1] = z[0];
+    v[k1 + 3] = z[1];
+    v[k1 + 6] = z[2];
+  }
+}
+template <>
+__device__ __forceinline__ void Dft<9, false>(float2* v) { Dft9<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<9, true>(float2* v) { Dft9<true>(v); }
+
 // LDS line layout: one pad slot after every 16 complex values, so the
 // strided writes of the early Stockham stages spread over the banks.
 __device__ __forceinline__ int Pad(int i) { return i + (i >> 4); }
@@ -219,6 +254,7 @@ __device__ void LdsFFT(float2* buf, const FFTDesc& d, const float2* twl) {
     const uint32_t mg = d.magic[s];
     switch (R) {
       case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      case 9: StockhamStage<9, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
       case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
       case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
       case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;

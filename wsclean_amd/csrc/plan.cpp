@@ -66,20 +66,41 @@ double EsKernelFT(int W, double beta, double t) {
   return 0.5 * W * 0.5 * M_PI * s;
 }
 
+namespace {
+bool Smooth235(int m) {
+  for (int p : {2, 3, 5})
+    while (m % p == 0) m /= p;
+  return m == 1;
+}
+}  // namespace
+
 int GoodFFTSize(int n) {
   if (n < 2) n = 2;
-  for (int m = n + (n & 1);; m += 2) {
-    int r = m;
-    for (int p : {2, 3, 5})
-      while (r % p == 0) r /= p;
-    if (r == 1) return m;
+  int first = n + (n & 1);
+  while (!Smooth235(first)) first += 2;
+  // Among the even 2^a 3^b 5^c sizes up to 25 % above n (at most 16384),
+  // the one of least LDS-FFT work: the plane FFTs touch about m lines of m
+  // points, each in one LDS pass per Stockham stage plus the load, screen
+  // and store, so the cost goes as m^2 (stages + 2).
+  int best = first;
+  double best_cost = 1e300;
+  const int last = std::max(first, std::min(16384, int(1.25 * n)));
+  for (int m = first; m <= last; m += 2) {
+    if (!Smooth235(m)) continue;
+    const double cost = double(m) * double(m) * double(FFTRadices(m).size() + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = m;
+    }
   }
+  return best;
 }
 
 std::vector<int> FFTRadices(int n) {
   std::vector<int> r;
   int m = n;
   while (m % 8 == 0) { r.push_back(8); m /= 8; }
+  while (m % 9 == 0) { r.push_back(9); m /= 9; }
   while (m % 4 == 0) { r.push_back(4); m /= 4; }
   while (m % 2 == 0) { r.push_back(2); m /= 2; }
   while (m % 3 == 0) { r.push_back(3); m /= 3; }
