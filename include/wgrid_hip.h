@@ -299,6 +299,22 @@ int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
                     wg_image_weights* image_weights, float* out_dev, double* stats4);
 
 /* ------------------------------------------------------------------------
+ * Small-inversion resampling (wgridder/wgriddingmsgridder.cpp:290-308 after
+ * inversion, :326-336 before prediction, schaapcommon::math::Resampler):
+ * Fourier interpolation of a float image, on the handle's stream.  Any side
+ * up to 16384 that is 2^a 3^b 5^c, any other side up to 8192.
+ * wg_resample: device [in_h][in_w] -> device [out_h][out_w].
+ * wg_real_image_resampled: RealImage resampled to out_w x out_h (host),
+ * the FinishInversion step.  wg_initialize_prediction_resampled: a host
+ * model of in_w x in_h resampled to the gridder's width x height, then
+ * InitializePrediction, the StartPredict step. */
+int wg_resample(wg_handle* h, const float* in_dev, size_t in_w, size_t in_h, float* out_dev,
+                size_t out_w, size_t out_h);
+int wg_real_image_resampled(wg_handle* h, float* out, size_t out_w, size_t out_h);
+int wg_initialize_prediction_resampled(wg_handle* h, const float* image, size_t in_w,
+                                       size_t in_h);
+
+/* ------------------------------------------------------------------------
  * Test entry points (no reference counterpart): expose internals so that
  * the bucket sort and the LDS FFT can be checked bit-for-bit / against a
  * library FFT. */

@@ -126,6 +126,11 @@ SIGNATURES = [
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p]),
+    ("wg_resample", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, c_size, c_size]),
+    ("wg_real_image_resampled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size]),
+    ("wg_initialize_prediction_resampled", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size]),
     ("wgx_facet_atlas", ctypes.c_int,
      [ctypes.POINTER(WgParams), c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
       c_size]),

@@ -1191,4 +1191,46 @@ int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
   });
 }
 
+// ------------------------------------------------------------ resampling
+
+int wg_resample(wg_handle* h, const float* in_dev, size_t in_w, size_t in_h, float* out_dev,
+                size_t out_w, size_t out_h) {
+  return Guard([&] {
+    WG_CHECK(h && in_dev && out_dev, "null argument");
+    WG_CHECK(in_w <= 16384 && in_h <= 16384 && out_w <= 16384 && out_h <= 16384,
+             "image side above 16384");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->eng->Resample(in_dev, int(in_w), int(in_h), out_dev, int(out_w), int(out_h));
+  });
+}
+
+int wg_real_image_resampled(wg_handle* h, float* out, size_t out_w, size_t out_h) {
+  return Guard([&] {
+    WG_CHECK(h && out, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t n = h->prm.width * h->prm.height;
+    float* d = static_cast<float*>(h->eng->Arena().Get("d2h.image", n * 4));
+    float* r = static_cast<float*>(h->eng->Arena().Get("resampled.image", out_w * out_h * 4));
+    RealImageTo(h, d);
+    h->eng->Resample(d, int(h->prm.width), int(h->prm.height), r, int(out_w), int(out_h));
+    WG_HIP(hipMemcpyAsync(out, r, out_w * out_h * 4, hipMemcpyDeviceToHost, h->s()));
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+int wg_initialize_prediction_resampled(wg_handle* h, const float* image, size_t in_w,
+                                       size_t in_h) {
+  return Guard([&] {
+    WG_CHECK(h && image, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    const size_t n = h->prm.width * h->prm.height;
+    float* src = static_cast<float*>(h->eng->Arena().Get("resample.src", in_w * in_h * 4));
+    float* d = static_cast<float*>(h->eng->Arena().Get("h2d.image", n * 4));
+    WG_HIP(hipMemcpyAsync(src, image, in_w * in_h * 4, hipMemcpyHostToDevice, h->s()));
+    h->eng->Resample(src, int(in_w), int(in_h), d, int(h->prm.width), int(h->prm.height));
+    LoadModelFrom(h, d);
+    WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
 }  // extern "C"

@@ -6,6 +6,7 @@
 
 #include <complex>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <vector>
@@ -144,6 +145,13 @@ class Engine {
   std::vector<TuneTrial> Autotune(size_t n_rows, size_t n_chan, const double* uvw_dev,
                                   const double* freq_host, size_t max_sample_vis);
 
+  // Fourier-interpolation resampling of a float image [hi][wi] (device) to
+  // [ho][wo] (device), on the engine stream (resample.hip).
+  void Resample(const float* in, int wi, int hi, float* out, int wo, int ho);
+  // In-place unnormalised FFT of nlines contiguous lines of length n (any
+  // n <= 16384 that is 2^a 3^b 5^c, any other n <= 8192 by Bluestein).
+  void LineFFTAny(float2* data, size_t nlines, int n, bool inverse);
+
   DeviceArena& Arena() { return arena_; }
   StageTimes& Times() { return times_; }
   // Accumulates the recorded event pairs into Times(); the caller must have
@@ -162,6 +170,7 @@ class Engine {
   hipStream_t stream_ = nullptr;
   DeviceArena arena_;
   std::map<int, float2*> twiddles_;
+  std::set<int> bluestein_ready_;  // line lengths whose chirp tables are built
   StageTimes times_;
 
  public:

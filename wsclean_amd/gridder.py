@@ -175,6 +175,28 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_initialize_prediction_device(
             self._h, ctypes.c_void_p(image_dev.data_ptr())))
 
+    def ResampleDevice(self, in_dev, in_w, in_h, out_dev, out_w, out_h):
+        """Fourier-interpolation resampling of a float32 device image
+        [in_h, in_w] into out_dev [out_h, out_w] on this handle's stream."""
+        if in_dev.numel() != in_w * in_h or out_dev.numel() != out_w * out_h:
+            raise WgError("image sizes do not match the tensors")
+        _lib.check(self._lib.wg_resample(self._h, ctypes.c_void_p(in_dev.data_ptr()), in_w, in_h,
+                                         ctypes.c_void_p(out_dev.data_ptr()), out_w, out_h))
+
+    def RealImageResampled(self, out_w, out_h):
+        """RealImage resampled to out_w x out_h (FinishInversion's resample)."""
+        out = np.empty(out_w * out_h, dtype=np.float32)
+        _lib.check(self._lib.wg_real_image_resampled(self._h, _ptr(out), out_w, out_h))
+        return out
+
+    def InitializePredictionResampled(self, image, in_w, in_h):
+        """InitializePrediction of a model of in_w x in_h, resampled to the
+        gridder's width x height (StartPredict's resample)."""
+        img = np.ascontiguousarray(image, dtype=np.float32).ravel()
+        if img.size != in_w * in_h:
+            raise WgError("image has %d pixels, expected %d" % (img.size, in_w * in_h))
+        _lib.check(self._lib.wg_initialize_prediction_resampled(self._h, _ptr(img), in_w, in_h))
+
     def _check_image(self, t):
         import torch
         if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
