@@ -366,10 +366,17 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   // [0] work-item queue of the gridding kernel, [1] max |component| of the
   // sorted visibilities (sets the gridding's fixed-point scale)
   uint32_t* words = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t)));
+  // one gridding pass per visibility (a single sub-slab): the matrix-core
+  // kernel gathers through the permutation itself
+  const bool fused = dp.W <= 8 && vs.nsub == 1;
   {
     EventTimer tm(*this, &times_.sort);
-    vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
-    LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, words + 1, stream_);
+    if (fused) {
+      LaunchVisMax(vis_dev, vs.n_vis, words + 1, stream_);
+    } else {
+      vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
+      LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, words + 1, stream_);
+    }
   }
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
@@ -395,15 +402,17 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
         {
           EventTimer tk(*this, &times_.grid_kernel_ms);
           LaunchGrid(dp, vs, vsorted, scratch, words, reinterpret_cast<const float*>(words + 1),
-                     g, sub_lo, sub_n, stream_);
+                     g, sub_lo, sub_n, stream_, fused ? vis_dev : nullptr,
+                     fused ? vs.perm : nullptr);
         }
         if (timing) {
-          // coordinates (16 B) and visibility (8 B) per pass, one subgrid
+          // coordinates (16 B) and visibility (8 B) per pass -- fused: also
+          // the permutation word (4 B) and phasor (8 B) -- and one subgrid
           // store per work item
           times_.grid_launches += 1;
           times_.grid_passes += vs.sub_vis[g];
-          times_.grid_bytes +=
-              24.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
+          times_.grid_bytes += (fused ? 36.0 : 24.0) * vs.sub_vis[g] +
+                               8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
         }
         LaunchGatherBox(dp, vs, box, scratch, g, sub_lo, sub_n, f0, stream_);
       }
@@ -420,8 +429,14 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
   const DevPlan& dp = vs.dp;
   const Plan& p = vs.plan;
   if (vs.n_valid == 0) return;
-  float2* acc = static_cast<float2*>(arena_.Get("vis_acc", vs.n_valid * sizeof(float2)));
-  LaunchFillZero(acc, vs.n_valid * sizeof(float2), stream_);
+  // one degridding pass per visibility (a single sub-slab): the
+  // matrix-core kernel writes the caller's output through the permutation
+  const bool fused = dp.W <= 8 && vs.nsub == 1;
+  float2* acc = vis_out_dev;
+  if (!fused) {
+    acc = static_cast<float2*>(arena_.Get("vis_acc", vs.n_valid * sizeof(float2)));
+    LaunchFillZero(acc, vs.n_valid * sizeof(float2), stream_);
+  }
   const FFTDesc fu = MakeFFTDesc(dp.nu, Twiddles(dp.nu));
   const FFTDesc fv = MakeFFTDesc(dp.nv, Twiddles(dp.nv));
   const size_t plane_box = size_t(dp.BU) * dp.BV;
@@ -445,19 +460,21 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
       const int sub_n = std::min(p.Sg, p.P - sub_lo);
       {
         EventTimer tk(*this, &times_.degrid_kernel_ms);
-        LaunchDegrid(dp, vs, box, acc, queue, g, sub_lo, sub_n, f0, stream_);
+        LaunchDegrid(dp, vs, box, acc, queue, g, sub_lo, sub_n, f0, stream_,
+                     fused ? vs.perm : nullptr);
       }
       if (timing) {
         // coordinates (16 B) read and accumulator (8 B) read + written per
-        // pass, one subgrid load per work item
+        // pass -- fused: permutation word (4 B) and phasor (8 B) read and
+        // the output (8 B) written -- and one subgrid load per work item
         times_.degrid_launches += 1;
         times_.degrid_passes += vs.sub_vis[g];
-        times_.degrid_bytes +=
-            32.0 * vs.sub_vis[g] + 8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
+        times_.degrid_bytes += (fused ? 36.0 : 32.0) * vs.sub_vis[g] +
+                               8.0 * double(sub_n) * E * E * double(vs.sub_items[g]);
       }
     }
   }
-  {
+  if (!fused) {
     EventTimer tm(*this, &times_.sort);
     LaunchScatterVis(vs.perm, vs.phasor, vs.n_valid, acc, vis_out_dev, stream_);
   }

@@ -290,18 +290,25 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
 // `queue` is one device word, the work-item counter of the persistent
 // matrix-core kernel (cleared by the launch); `vmax` points to the largest
 // |component| of vis_sorted (from LaunchGatherVis).
+// With vis_raw (matrix-core gridder only), the gather is fused: the kernel
+// reads vis_raw[gather_perm[i]] (flip bit in the word) times phasor[i]
+// instead of vis_sorted[i].
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
                 float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
-                int sub_n, hipStream_t s);
+                int sub_n, hipStream_t s, const float2* vis_raw = nullptr,
+                const uint32_t* gather_perm = nullptr);
 void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
                      const float2* scratch, int sub, int sub_lo, int sub_n,
                      int slab_lo, hipStream_t s);
 size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub);
 // `queue`: one device word, the work-item counter of the persistent
 // matrix-core degridder (cleared by the launch).
+// With scatter_perm (matrix-core degridder only; every visibility in one
+// sub-slab), the scatter is fused: acc is the caller's output, written at
+// scatter_perm[i] with the conjugate phasor and the flip applied.
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box,
                   float2* acc, uint32_t* queue, int sub, int sub_lo, int sub_n,
-                  int slab_lo, hipStream_t s);
+                  int slab_lo, hipStream_t s, const uint32_t* scatter_perm = nullptr);
 size_t GridLdsBytes(const DevPlan& dp, int sub_n);
 
 // ---- per-visibility kernels (vis.hip) ----
@@ -343,6 +350,10 @@ void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,
 void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
                      const float2* vis, float2* vis_sorted, uint32_t* vmax_bits,
                      hipStream_t s);
+// vmax_bits = bits of max |V| over vis[0, n) (a non-negative float; the
+// launch clears the word first): bounds every component of the rotated
+// visibilities, the fused gather's fixed-point scale.
+void LaunchVisMax(const float2* vis, size_t n, uint32_t* vmax_bits, hipStream_t s);
 // vis_out[perm[i]] = flip(acc[i] * conj(phasor[i])).
 void LaunchScatterVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
                       const float2* acc, float2* vis_out, hipStream_t s);

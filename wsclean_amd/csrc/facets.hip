@@ -266,10 +266,19 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
   DeviceArena& A = eng_.Arena();
   if (atlas_.n_valid == 0) return;
   uint32_t* words = static_cast<uint32_t*>(A.Get("grid.words", 4 * sizeof(uint32_t)));
-  float2* vsorted = static_cast<float2*>(A.Get("vis_sorted", atlas_.n_valid * sizeof(float2)));
+  // a single sub-slab: the matrix-core gridder gathers through perm_phys
+  const bool fused = da.W <= 8 && atlas_.nsub == 1;
+  float2* vsorted = nullptr;
   {
     EventTimer tm(eng_, &eng_.Times().sort);
-    LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+    if (fused) {
+      size_t rows_end = 0;
+      for (int f = 0; f < F; ++f) rows_end = std::max(rows_end, row_off_[f] + n_rows_[f]);
+      LaunchVisMax(vis_dev, rows_end * n_chan_, words + 1, s);
+    } else {
+      vsorted = static_cast<float2*>(A.Get("vis_sorted", atlas_.n_valid * sizeof(float2)));
+      LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+    }
   }
   StageTimes& st = eng_.Times();
   std::vector<float*> dirty(F);
@@ -298,12 +307,13 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
           // one persistent launch over the work items of every facet
           EventTimer tk(eng_, &st.grid_kernel_ms);
           LaunchGrid(da, atlas_, vsorted, scratch, words,
-                     reinterpret_cast<const float*>(words + 1), g, sub_lo, sub_n, s);
+                     reinterpret_cast<const float*>(words + 1), g, sub_lo, sub_n, s,
+                     fused ? vis_dev : nullptr, fused ? perm_phys_ : nullptr);
         }
         if (eng_.timing) {
           st.grid_launches += 1;
           st.grid_passes += atlas_.sub_vis[g];
-          st.grid_bytes += 24.0 * atlas_.sub_vis[g] +
+          st.grid_bytes += (fused ? 36.0 : 24.0) * atlas_.sub_vis[g] +
                            8.0 * double(sub_n) * E * E * double(atlas_.sub_items[g]);
         }
         LaunchGatherBox(da, atlas_, box, scratch, g, sub_lo, sub_n, f0, s);
@@ -334,8 +344,13 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
   hipStream_t s = eng_.Stream();
   DeviceArena& A = eng_.Arena();
   if (atlas_.n_valid == 0) return;
-  float2* acc = static_cast<float2*>(A.Get("vis_acc", atlas_.n_valid * sizeof(float2)));
-  LaunchFillZero(acc, atlas_.n_valid * sizeof(float2), s);
+  // a single sub-slab: the degridder writes the output through perm
+  const bool fused = da.W <= 8 && atlas_.nsub == 1;
+  float2* acc = vis_out_dev;
+  if (!fused) {
+    acc = static_cast<float2*>(A.Get("vis_acc", atlas_.n_valid * sizeof(float2)));
+    LaunchFillZero(acc, atlas_.n_valid * sizeof(float2), s);
+  }
   StageTimes& st = eng_.Times();
   size_t lines = 0;
   for (int f = 0; f < F; ++f)
@@ -369,18 +384,21 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
       const int sub_n = std::min(pa.Sg, pa.P - sub_lo);
       {
         EventTimer tk(eng_, &st.degrid_kernel_ms);
-        LaunchDegrid(da, atlas_, box, acc, queue, g, sub_lo, sub_n, f0, s);
+        LaunchDegrid(da, atlas_, box, acc, queue, g, sub_lo, sub_n, f0, s,
+                     fused ? atlas_.perm : nullptr);
       }
       if (eng_.timing) {
         st.degrid_launches += 1;
         st.degrid_passes += atlas_.sub_vis[g];
-        st.degrid_bytes += 32.0 * atlas_.sub_vis[g] +
+        st.degrid_bytes += (fused ? 36.0 : 32.0) * atlas_.sub_vis[g] +
                            8.0 * double(sub_n) * E * E * double(atlas_.sub_items[g]);
       }
     }
   }
-  EventTimer tsc(eng_, &st.sort);
-  LaunchScatterVis(atlas_.perm, atlas_.phasor, atlas_.n_valid, acc, vis_out_dev, s);
+  if (!fused) {
+    EventTimer tsc(eng_, &st.sort);
+    LaunchScatterVis(atlas_.perm, atlas_.phasor, atlas_.n_valid, acc, vis_out_dev, s);
+  }
 }
 
 std::string FacetBatch::Describe() const {

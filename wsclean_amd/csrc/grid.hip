@@ -181,18 +181,50 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
 constexpr int kMfmaWaves = 8;
 constexpr int kStage = 64;
 
-// One staged batch: visibilities [first, first + 64) of the wave's range
-// [.., end), one per lane, zero past the end.
-__device__ __forceinline__ void LoadStage(const float4* __restrict__ coord,
-                                          const float2* __restrict__ vis, uint32_t first,
-                                          uint32_t end, int lane, float4& c, float2& v) {
-  c = make_float4(0.f, 0.f, 0.f, 0.f);
-  v = make_float2(0.f, 0.f);
-  if (first + uint32_t(lane) < end) {
-    c = coord[first + lane];
-    v = vis[first + lane];
+// Source of the visibilities the gridding kernel stages, one batch of 64 in
+// flight.  VisSource<false> reads the gathered (sorted, flipped, shifted)
+// visibilities.  VisSource<true> fuses the gather: it reads the caller's
+// visibility through the sorted permutation and applies the flip and the
+// sorted phasor itself.  The permutation word of a batch is loaded one
+// batch before its visibility, so the dependent random load never waits.
+template <bool FUSED>
+struct VisSource;
+
+template <>
+struct VisSource<false> {
+  const float2* vis;
+  float2 v;
+  __device__ __forceinline__ void Prime(uint32_t, uint32_t, int) {}
+  __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
+    v = first + uint32_t(lane) < end ? vis[first + lane] : make_float2(0.f, 0.f);
   }
-}
+  __device__ __forceinline__ float2 Value() const { return v; }
+};
+
+template <>
+struct VisSource<true> {
+  const float2* vis;
+  const uint32_t* perm;
+  const float2* phasor;
+  uint32_t pw_next, pw;
+  float2 raw, ph;
+  // permutation word of the batch the next Issue() loads
+  __device__ __forceinline__ void Prime(uint32_t first, uint32_t end, int lane) {
+    pw_next = first + uint32_t(lane) < end ? perm[first + lane] : 0u;
+  }
+  __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
+    pw = pw_next;
+    const bool in = first + uint32_t(lane) < end;
+    raw = in ? vis[pw & ~kPermFlip] : make_float2(0.f, 0.f);
+    ph = in ? phasor[first + lane] : make_float2(0.f, 0.f);
+    const uint32_t nxt = first + uint32_t(kStage) + uint32_t(lane);
+    pw_next = nxt < end ? perm[nxt] : 0u;
+  }
+  __device__ __forceinline__ float2 Value() const {
+    const float2 V = make_float2(raw.x, (pw & kPermFlip) ? -raw.y : raw.y);
+    return make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
+  }
+};
 
 // Run key (packed cell and first plane) of the lane's staged visibility;
 // -2 past the end of the range, so the end starts a "run".
@@ -254,7 +286,9 @@ __device__ __forceinline__ float EsKernelAt(float f, float nc2, float xc, float 
   return __builtin_amdgcn_exp2f(__builtin_fmaf(__builtin_amdgcn_sqrtf(a), b2, -b2));
 }
 
-template <int W>
+// FUSED: vis is the caller's [row][chan] buffer, read through perm and
+// rotated by phasor (no gathered copy); otherwise vis is sorted.
+template <int W, bool FUSED>
 // LDS (subgrid + rings) allows two workgroups, i.e. four waves per SIMD: let
 // the register allocator use the registers that occupancy leaves free
 // instead of serialising the staged loads.
@@ -262,7 +296,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     __attribute__((amdgpu_waves_per_eu(4, 4))) GridKernelMfma(
     DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
-    const float2* __restrict__ vis, float2* __restrict__ scratch, int sub_lo, int sub_n,
+    const float2* __restrict__ vis, const uint32_t* __restrict__ perm,
+    const float2* __restrict__ phasor, float2* __restrict__ scratch, int sub_lo, int sub_n,
     uint32_t* __restrict__ queue, const float* __restrict__ vmax_ptr) {
   static_assert(W <= 8 && W % 2 == 0, "matrix-core gridding covers W <= 8");
   __shared__ uint32_t next_item;
@@ -352,17 +387,27 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
       // Run lengths then come from scalar bit scans, so the next group's
       // staged data can be read before this group's matrix ops issue.
       float4 pc;
-      float2 pv;
-      LoadStage(coord, vis, w0, w1, lane, pc, pv);
-      StoreStage(ring, lane, pc, pv);
+      VisSource<FUSED> src;
+      src.vis = vis;
+      if constexpr (FUSED) {
+        src.perm = perm;
+        src.phasor = phasor;
+      }
+      auto load = [&](uint32_t first) {
+        pc = first + uint32_t(lane) < w1 ? coord[first + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        src.Issue(first, w1, lane);
+      };
+      src.Prime(w0, w1, lane);
+      load(w0);
+      StoreStage(ring, lane, pc, src.Value());
       int kb0 = StageKey(w0, w1, lane, pc);
       uint64_t sm0 = RunStarts(kb0, -1, lane);
-      LoadStage(coord, vis, w0 + kStage, w1, lane, pc, pv);
-      StoreStage(ring, kStage + lane, pc, pv);
+      load(w0 + kStage);
+      StoreStage(ring, kStage + lane, pc, src.Value());
       int kb1 = StageKey(w0 + kStage, w1, lane, pc);
       uint64_t sm1 = RunStarts(kb1, __builtin_amdgcn_readlane(kb0, 63), lane);
       uint32_t bbase = w0, bslot = 0;
-      LoadStage(coord, vis, w0 + 2 * kStage, w1, lane, pc, pv);
+      load(w0 + 2 * kStage);
       WaveLdsSync();
       // Each step takes two groups (visibilities pos .. pos + 7) of one run.
       uint32_t pos = w0;
@@ -407,7 +452,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         }
         if (npos - bbase >= uint32_t(kStage) && npos < w1) {
           // the next group starts in batch b + 1: batch b + 2 replaces b
-          StoreStage(ring, bslot * kStage + lane, pc, pv);
+          StoreStage(ring, bslot * kStage + lane, pc, src.Value());
           const int kb2 = StageKey(bbase + 2 * kStage, w1, lane, pc);
           const uint64_t sm2 = RunStarts(kb2, __builtin_amdgcn_readlane(kb1, 63), lane);
           kb0 = kb1;
@@ -416,7 +461,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           sm1 = sm2;
           bbase += kStage;
           bslot ^= 1u;
-          LoadStage(coord, vis, bbase + 2 * kStage, w1, lane, pc, pv);
+          load(bbase + 2 * kStage);
           WaveLdsSync();
         }
 #pragma unroll
@@ -564,12 +609,16 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
 // an LDS ring with run-start masks, as in the gridder.
 constexpr int kDegridGroup = 16;
 
-template <int W>
+// FUSED (every visibility in exactly one sub-slab): the prediction goes
+// straight to the caller's out[row][chan] through perm, rotated back by the
+// conjugate phasor and flipped, instead of into the sorted accumulator.
+template <int W, bool FUSED>
 __global__ void __launch_bounds__(64 * kMfmaWaves)
     __attribute__((amdgpu_waves_per_eu(4, 4))) DegridKernelMfma(
     DevPlan dp, const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_pos,
     int pos_lo, int pos_hi, const float4* __restrict__ coord, const float2* __restrict__ box,
-    float2* __restrict__ acc, int sub_lo, int sub_n, int slab_lo, uint32_t* __restrict__ queue) {
+    float2* __restrict__ acc, const uint32_t* __restrict__ perm, const float2* __restrict__ phasor,
+    int sub_lo, int sub_n, int slab_lo, uint32_t* __restrict__ queue) {
   static_assert(W <= 8 && W % 2 == 0, "matrix-core degridding covers W <= 8");
   __shared__ uint32_t next_item;
   extern __shared__ __attribute__((aligned(16))) float2 degrid_lds[];
@@ -622,7 +671,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         a = make_float2(0.f, 0.f);
         if (first + uint32_t(lane) < w1) {
           c = coord[first + lane];
-          a = acc[first + lane];
+          if constexpr (!FUSED) a = acc[first + lane];
         }
       };
       float4 pc;
@@ -646,6 +695,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
       float4 cv = ring_c[j];
       float2 av = ring_a[j];
       while (pos < w1) {
+        // fused scatter: this group's permutation words and phasors, loaded
+        // now and used after the matrix ops
+        uint32_t pw_f = 0;
+        float2 ph_f = make_float2(0.f, 0.f);
+        if constexpr (FUSED) {
+          if (g == 0 && pos + uint32_t(j) < w1) {
+            pw_f = perm[pos + j];
+            ph_f = phasor[pos + j];
+          }
+        }
         const uint32_t o = pos - bbase;  // < 64
         if ((sm0 >> o) & 1ull) {
           cur = __builtin_amdgcn_readlane(kb0, int(o));
@@ -708,7 +767,15 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         // l halves (lanes L, L ^ 16), then the imaginary part to g = 0
         part += __shfl_xor(part, 16);
         const float im = __shfl_xor(part, 32);
-        if (g == 0 && take) acc[pos + j] = make_float2(av.x + part, av.y + im);
+        if constexpr (FUSED) {
+          if (g == 0 && take) {
+            float2 V = make_float2(part * ph_f.x + im * ph_f.y, im * ph_f.x - part * ph_f.y);
+            if (pw_f & kPermFlip) V.y = -V.y;
+            acc[pw_f & ~kPermFlip] = V;
+          }
+        } else {
+          if (g == 0 && take) acc[pos + j] = make_float2(av.x + part, av.y + im);
+        }
         if (npos - bbase >= uint32_t(kStage) && npos < w1) {
           ring_c[bslot * kStage + lane] = pc;
           ring_a[bslot * kStage + lane] = pa;
@@ -828,7 +895,7 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
 
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* scratch,
                 uint32_t* queue, const float* vmax, int sub, int sub_lo, int sub_n,
-                hipStream_t s) {
+                hipStream_t s, const float2* vis_raw, const uint32_t* gather_perm) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
@@ -841,16 +908,23 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
       static_assert(ring + 16 <= size_t(kGridLdsReserve), "staging rings exceed the LDS reserve");
       const size_t total = ((lds + 15) & ~size_t(15)) + ring;
       WG_CHECK(total <= 160 * 1024, "gridding LDS exceeds 160 KiB");
-      AllowLds(GridKernelMfma<W>, total);
-      // one persistent workgroup per resident slot; they drain the queue
-      const int per_cu = ResidentBlocksPerCU(
-          reinterpret_cast<const void*>(GridKernelMfma<W>), 64 * kMfmaWaves, total);
-      const int blocks = std::max(1, per_cu) * DeviceCUs();
-      LaunchFillZero(queue, sizeof(uint32_t), s);
-      GridKernelMfma<W><<<blocks, 64 * kMfmaWaves, total, s>>>(
-          dp, vs.items, vs.item_pos, pos_lo, pos_hi, vs.coord, vis_sorted, scratch, sub_lo, sub_n,
-          queue, vmax);
+      auto launch = [&](auto kernel, const float2* v, const uint32_t* pm) {
+        AllowLds(kernel, total);
+        // one persistent workgroup per resident slot; they drain the queue
+        const int per_cu =
+            ResidentBlocksPerCU(reinterpret_cast<const void*>(kernel), 64 * kMfmaWaves, total);
+        const int blocks = std::max(1, per_cu) * DeviceCUs();
+        LaunchFillZero(queue, sizeof(uint32_t), s);
+        kernel<<<blocks, 64 * kMfmaWaves, total, s>>>(dp, vs.items, vs.item_pos, pos_lo, pos_hi,
+                                                      vs.coord, v, pm, vs.phasor, scratch, sub_lo,
+                                                      sub_n, queue, vmax);
+      };
+      if (vis_raw)
+        launch(GridKernelMfma<W, true>, vis_raw, gather_perm);
+      else
+        launch(GridKernelMfma<W, false>, vis_sorted, nullptr);
     } else {
+      WG_CHECK(!vis_raw, "the fused gather needs the matrix-core gridder (W <= 8)");
       AllowLds(GridKernel<W>, lds);
       GridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(dp, vs.items, vs.item_pos, pos_lo,
                                                            pos_hi, vs.coord, vis_sorted, scratch,
@@ -877,7 +951,8 @@ size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub) {
 }
 
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc,
-                  uint32_t* queue, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
+                  uint32_t* queue, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s,
+                  const uint32_t* scatter_perm) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
@@ -889,14 +964,21 @@ void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2
       static_assert(ring + 16 <= size_t(kGridLdsReserve), "staging rings exceed the reserve");
       const size_t total = ((lds + 15) & ~size_t(15)) + ring;
       WG_CHECK(total <= 160 * 1024, "degridding LDS exceeds 160 KiB");
-      AllowLds(DegridKernelMfma<W>, total);
-      const int per_cu = ResidentBlocksPerCU(
-          reinterpret_cast<const void*>(DegridKernelMfma<W>), 64 * kMfmaWaves, total);
-      LaunchFillZero(queue, sizeof(uint32_t), s);
-      DegridKernelMfma<W><<<std::max(1, per_cu) * DeviceCUs(), 64 * kMfmaWaves, total, s>>>(
-          dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
-          acc, sub_lo, sub_n, slab_lo, queue);
+      auto launch = [&](auto kernel) {
+        AllowLds(kernel, total);
+        const int per_cu =
+            ResidentBlocksPerCU(reinterpret_cast<const void*>(kernel), 64 * kMfmaWaves, total);
+        LaunchFillZero(queue, sizeof(uint32_t), s);
+        kernel<<<std::max(1, per_cu) * DeviceCUs(), 64 * kMfmaWaves, total, s>>>(
+            dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,
+            acc, scatter_perm, vs.phasor, sub_lo, sub_n, slab_lo, queue);
+      };
+      if (scatter_perm)
+        launch(DegridKernelMfma<W, true>);
+      else
+        launch(DegridKernelMfma<W, false>);
     } else {
+      WG_CHECK(!scatter_perm, "the fused scatter needs the matrix-core degridder (W <= 8)");
       AllowLds(DegridKernel<W>, lds);
       DegridKernel<W><<<GridBlocks(), kGridBlock, lds, s>>>(
           dp, vs.items, vs.item_pos, int(sub * ntiles), int((sub + 1) * ntiles), vs.coord, box,

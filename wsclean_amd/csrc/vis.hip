@@ -154,6 +154,18 @@ __global__ void __launch_bounds__(kBlock) GatherVisKernel(
   if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(vmax));
 }
 
+__global__ void __launch_bounds__(kBlock) VisMaxKernel(const float2* __restrict__ vis, size_t n,
+                                                      uint32_t* __restrict__ vmax_bits) {
+  float vmax = 0.f;
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * kBlock) {
+    const float2 v = vis[i];
+    vmax = fmaxf(vmax, sqrtf(v.x * v.x + v.y * v.y));
+  }
+  for (int m = 32; m >= 1; m >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(vmax));
+}
+
 __global__ void __launch_bounds__(kBlock) ScatterVisKernel(
     const uint32_t* __restrict__ perm, const float2* __restrict__ phasor, size_t n_valid,
     const float2* __restrict__ acc, float2* __restrict__ out) {
@@ -534,6 +546,13 @@ void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
   if (!n_valid) return;
   GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(perm, phasor, n_valid, vis, vis_sorted,
                                                      vmax_bits);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchVisMax(const float2* vis, size_t n, uint32_t* vmax_bits, hipStream_t s) {
+  LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
+  if (!n) return;
+  VisMaxKernel<<<Blocks(n), kBlock, 0, s>>>(vis, n, vmax_bits);
   WG_HIP(hipGetLastError());
 }
 
