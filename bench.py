@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served-stream leg and print its JSON")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner leg")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph leg")
     ap.add_argument("--facet-steps", type=int, default=2,
                     help="timed steps of the facet-batch leg (0 skips it)")
     ap.add_argument("--facets-only", action="store_true",
@@ -108,13 +109,15 @@ def kernel_roofline(st, plan):
     its own HIP events on the engine stream.
 
     For W <= 8 the gridding runs on the FP32 matrix cores
-    (GridKernelMfma<W>): bound "mfma", algorithmic FLOPs = 4 W^3 per
+    (GridKernelMfma<W, fused>): bound "mfma", algorithmic FLOPs = 4 W^3 per
     visibility pass (each of the W^3 footprint cells takes a complex x real
-    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  The HBM
-    view is reported beside it: algorithmic bytes per launch = 24 B per
-    visibility pass (float4 coordinates + complex64 visibility, read once
-    per sub-slab the visibility touches) + 8 B x sub_n x E^2 per work item
-    (its subgrid written once), against 8 TB/s.  traffic = measured HBM
+    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  With one
+    plane sub-slab (P <= Sg) the kernel gathers the visibilities itself
+    (fused).  The HBM view is reported beside it: algorithmic bytes per
+    launch = per visibility pass 16 B coordinates + 8 B visibility (fused:
+    + 4 B permutation word + 8 B phasor), read once per sub-slab the
+    visibility touches, + 8 B x sub_n x E^2 per work item (its subgrid
+    written once), against 8 TB/s.  traffic = measured HBM
     bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the
     copy-kernel calibration) from the committed counter summary of this
     kernel and plan, else None."""
@@ -127,7 +130,9 @@ def kernel_roofline(st, plan):
     per_launch_flops = 4.0 * W ** 3 * st["grid_passes"] / n
     gbs = per_launch_bytes / t / 1e9
     mfma = W <= 8
-    kernel = ("GridKernelMfma<%d>" if mfma else "GridKernel<%d>") % W
+    fused = mfma and plan["P"] <= plan["Sg"]
+    kernel = ("GridKernelMfma<%d, %s>" % (W, "true" if fused else "false")) if mfma else \
+        "GridKernel<%d>" % W
     if mfma:
         achieved = per_launch_flops / t / 1e12
         out = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
@@ -142,7 +147,8 @@ def kernel_roofline(st, plan):
         "flops_per_launch": per_launch_flops, "bytes_per_launch": per_launch_bytes,
         "hbm_view": {"achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": gbs / (HBM_PEAK / 1e9)},
-        "degrid": {"kernel": ("DegridKernelMfma<%d>" if mfma else "DegridKernel<%d>") % W,
+        "degrid": {"kernel": ("DegridKernelMfma<%d, %s>" % (W, "true" if fused else "false"))
+                   if mfma else "DegridKernel<%d>" % W,
                    "launch_ms": dt * 1e3,
                    "achieved_tflops": 4.0 * W ** 3 * st["degrid_passes"]
                    / max(st["degrid_launches"], 1) / max(dt, 1e-12) / 1e12}})
@@ -282,7 +288,8 @@ def main():
         "cpu_baseline": None,
         "gen_s": t_gen,
     }
-    line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
+    if not args.no_graph:
+        line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
     if not args.no_autotune:
         line["autotune"] = autotune_leg(uvw, freq, n, dx, args.eps, n_rows, n_chan, plan)
     if not args.no_per_channel:
@@ -290,7 +297,8 @@ def main():
     g.close()
     del uvw, vis, out
     torch.cuda.empty_cache()
-    line["graph"]["512"] = small_graph_cycle(args.eps)
+    if not args.no_graph:
+        line["graph"]["512"] = small_graph_cycle(args.eps)
     if args.serve_requests > 0:
         line["serving"] = serving_leg(args)
     if args.facet_steps > 0:
