@@ -46,6 +46,8 @@ def parse():
                     help="run only the served-stream leg and print its JSON")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph leg")
+    ap.add_argument("--no-host-leg", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--facet-steps", type=int, default=2,
                     help="timed steps of the facet-batch leg (0 skips it)")
     ap.add_argument("--facets-only", action="store_true",
@@ -290,11 +292,15 @@ def main():
     }
     if not args.no_graph:
         line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
+    # the other legs make handles of their own: free this one's HBM first
+    g.close()
+    torch.cuda.empty_cache()
     if not args.no_autotune:
         line["autotune"] = autotune_leg(uvw, freq, n, dx, args.eps, n_rows, n_chan, plan)
     if not args.no_per_channel:
         line["per_channel_gpu"] = per_channel_leg(uvw, vis, freq, n, dx, args.eps, n_rows, value)
-    g.close()
+    if not args.no_host_leg:
+        line["host_buffers"] = host_buffer_leg(uvw, vis, freq, n, dx, args.eps, n_rows, n_chan)
     del uvw, vis, out
     torch.cuda.empty_cache()
     if not args.no_graph:
@@ -306,6 +312,38 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
     print(json.dumps(line))
+
+
+def host_buffer_leg(uvw, vis, freq, n, dx, eps, n_rows, n_chan):
+    """The reference-shaped call sequence on host buffers: one
+    AddInversionData and one PredictVisibilities with all channels of the
+    workload in one chunk (wgridder/wgriddingmsgridder.cpp:176-178), uvw and
+    visibilities in host memory, so the time includes the PCIe copies both
+    ways and the per-call plan and sort.  One pass, wall clock."""
+    import torch
+
+    from wsclean_amd import WGriddingGridder_Simple
+    uvw_h = uvw.cpu().numpy()
+    vis_h = vis.cpu().numpy().view(np.complex64).reshape(n_rows, n_chan)
+    out_h = np.empty_like(vis_h)
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, eps)
+
+    def once():
+        g.InitializeInversion()
+        g.AddInversionData(n_rows, n_chan, uvw_h, freq, vis_h)
+        g.FinalizeImage(1.0 / (n_rows * n_chan))
+        g.PredictVisibilities(n_rows, n_chan, uvw_h, freq, out_h)
+
+    once()   # buffers, twiddles, kernels
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    once()
+    dt = time.perf_counter() - t0
+    g.close()
+    del vis_h, out_h, uvw_h
+    return {"value": n_rows * n_chan / dt, "unit": "vis/s", "ms": dt * 1e3,
+            "bytes_over_pcie": 2 * (vis.numel() * 4 + uvw.numel() * 8),
+            "calls": "AddInversionData + PredictVisibilities, all channels, host buffers"}
 
 
 def per_channel_leg(uvw, vis, freq, n, dx, eps, n_rows, batched_value):

@@ -167,7 +167,11 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
     double planes = std::floor(wspan / dw) + k.W;
     double band_u = std::min<double>(nu, (e.umax - e.umin) * nu * g.dx + k.W);
     double band_v = std::min<double>(nv, (e.vmax - e.vmin) * nv * g.dy + k.W);
-    double grid_cost = nvis * k.W * k.W * k.W;
+    // per W^3 visibility pass, relative to the FFT units below (measured
+    // on the 8192^2 workload: matrix-core grid + degrid for W <= 8 take
+    // 0.46 FFT units, the VALU kernels for W >= 10 about 2.2)
+    const double grid_weight = k.W <= 8 ? 0.46 : 2.2;
+    double grid_cost = grid_weight * nvis * k.W * k.W * k.W;
     double fft_cost = planes * (band_v * nu * std::log2(nu) +
                                 g.nx * nv * std::log2(nv) + 4.0 * g.nx * g.ny +
                                 band_u * band_v) * 4.0;
