@@ -115,6 +115,17 @@ __device__ __forceinline__ void EsTaps(float f, float b2, float* out) {
   }
 }
 
+// i / d for i, d < 2^31 without an integer division: the FP64 product with
+// the rounded reciprocal is within one of the quotient, and one remainder
+// test corrects it.
+__device__ __forceinline__ uint32_t DivSmall(uint32_t i, uint32_t d, double inv_d) {
+  uint32_t q = uint32_t(double(i) * inv_d);
+  const int64_t r = int64_t(i) - int64_t(q) * int64_t(d);
+  if (r < 0) --q;
+  else if (r >= int64_t(d)) ++q;
+  return q;
+}
+
 // Top bit of a sorted permutation entry: the visibility was flipped to w >= 0
 // (the flat index itself is below 2^31).
 constexpr uint32_t kPermFlip = 0x80000000u;

@@ -3,7 +3,8 @@
 // keys are uint32 bucket ids (tile * P0 + p0; nkeys marks skipped zero
 // visibilities), values are the flat visibility index row * n_chan + chan.
 // The sort is a stable LSD radix sort (digits of up to 9 bits, tiles of
-// 8192 elements reordered in LDS before the global writes), so the
+// 8192 elements reordered in LDS before the global writes, runs of 8 tiles
+// per block), so the
 // permutation is identical to a stable CPU sort (numpy argsort(kind="stable")).
 #include <hip/hip_runtime.h>
 
@@ -25,6 +26,11 @@ constexpr int kRsTile = kRsThreads * kRsPerThread;
 constexpr int kRsMaxBits = 9;
 constexpr int kRsMaxBuckets = 1 << kRsMaxBits;
 static_assert(kRsMaxBuckets <= kRsThreads, "one thread per bucket in the tile scan");
+// Each block sorts a run of kRsSub consecutive tiles (its histogram and its
+// global offsets cover the run), so there are 8x fewer per-block counts to
+// write, scan and read back.
+constexpr int kRsSub = 8;
+constexpr int kRsRun = kRsTile * kRsSub;
 
 constexpr int kScanBlock = 1024;
 constexpr int kScanPer = 8;
@@ -121,8 +127,9 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
   }
 }
 
-// Digit histogram of one tile: counts[d * nblocks + block].  Wave w of the
-// block reads elements [tile + w * 1024, tile + (w + 1) * 1024).
+// Digit histogram of one run of kRsSub tiles: counts[d * nblocks + block].
+// Wave w reads elements [tile + w * 1024, tile + (w + 1) * 1024) of every
+// tile of the run.
 __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
     const uint32_t* __restrict__ keys, size_t n, int shift, uint32_t mask,
     uint32_t* __restrict__ counts, uint32_t nblocks) {
@@ -131,11 +138,14 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   const int R = int(mask) + 1;
   for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
-  const size_t base = size_t(blockIdx.x) * kRsTile + size_t(wave) * (kRsTile / kRsWaves);
+  for (int sub = 0; sub < kRsSub; ++sub) {
+    const size_t base = (size_t(blockIdx.x) * kRsSub + sub) * kRsTile +
+                        size_t(wave) * (kRsTile / kRsWaves);
 #pragma unroll 4
-  for (int it = 0; it < kRsPerThread; ++it) {
-    const size_t i = base + size_t(it) * 64 + lane;
-    if (i < n) atomicAdd(&h[wave][(keys[i] >> shift) & mask], 1u);
+    for (int it = 0; it < kRsPerThread; ++it) {
+      const size_t i = base + size_t(it) * 64 + lane;
+      if (i < n) atomicAdd(&h[wave][(keys[i] >> shift) & mask], 1u);
+    }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kRsThreads) {
@@ -146,98 +156,107 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   }
 }
 
-// Stable scatter of one tile.  Each wave ranks its 1024 elements by digit
-// (lanes with equal digits found by ballots over the digit bits; wave-
-// private running counts in LDS, so the order is element order), the block
-// turns the per-wave counts into tile positions, reorders the keys and then
-// the values through LDS, and writes each digit's run of the tile with
-// consecutive threads to consecutive addresses.
+// Stable scatter of one run of kRsSub tiles, tile by tile in order.  Each
+// wave ranks its 1024 elements of a tile by digit (lanes with equal digits
+// found by ballots over the digit bits; wave-private running counts in LDS,
+// so the order is element order), the block turns the per-wave counts into
+// tile positions, reorders the keys and then the values through LDS, and
+// writes each digit's run of the tile with consecutive threads to
+// consecutive addresses.  run_base[d] (the run's global offset of digit d)
+// advances by each tile's digit count.
 __global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
     int bits, const uint32_t* __restrict__ offs, uint32_t nblocks) {
   __shared__ uint32_t cnt[kRsWaves][kRsMaxBuckets];  // per-wave counts, then offsets
-  __shared__ uint32_t start[kRsMaxBuckets];          // tile-local start of each digit
+  __shared__ uint32_t start[kRsMaxBuckets + 1];      // tile-local start of each digit
   __shared__ uint32_t gbase[kRsMaxBuckets];          // global position - tile position
+  __shared__ uint32_t run_base[kRsMaxBuckets];       // next global position of each digit
   __shared__ uint32_t buf[kRsTile];                  // keys, then values, in tile order
   __shared__ uint16_t dig[kRsTile];                  // digit of each tile position
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t mask = (1u << bits) - 1u;
   const int R = 1 << bits;
-  for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&cnt[0][0])[i] = 0;
-  __syncthreads();
-  const size_t tile = size_t(blockIdx.x) * kRsTile;
-  const size_t base = tile + size_t(wave) * (kRsTile / kRsWaves);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint32_t key[kRsPerThread], pos[kRsPerThread];
-#pragma unroll
-  for (int it = 0; it < kRsPerThread; ++it) {
-    const size_t i = base + size_t(it) * 64 + lane;
-    key[it] = i < n ? kin[i] : 0u;
-  }
-#pragma unroll
-  for (int it = 0; it < kRsPerThread; ++it) {
-    const bool act = base + size_t(it) * 64 + lane < n;
-    const uint32_t d = (key[it] >> shift) & mask;
-    uint64_t peers = __ballot(act);
-    for (int b = 0; b < bits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    }
-    const uint32_t r = uint32_t(__popcll(peers & lt));
-    const uint32_t c = act ? cnt[wave][d] : 0u;
-    pos[it] = c + r;
-    if (act && r == 0) cnt[wave][d] = c + uint32_t(__popcll(peers));
-  }
-  __syncthreads();
-  // wave offsets inside each digit, digit totals, then tile starts
-  uint32_t tot = 0;
-  for (int d = threadIdx.x; d < R; d += kRsThreads) {
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < kRsWaves; ++w) {
-      const uint32_t t = cnt[w][d];
-      cnt[w][d] = run;
-      run += t;
-    }
-    start[d] = run;
-  }
-  __syncthreads();
-  {
-    const int d = threadIdx.x;
-    const uint32_t v = d < R ? start[d] : 0u;
-    uint32_t ex = 0;
-    (void)BlockExclusiveScan(v, ex);
-    tot = ex;
+  for (int d = threadIdx.x; d < R; d += kRsThreads) run_base[d] = offs[size_t(d) * nblocks + blockIdx.x];
+  for (int sub = 0; sub < kRsSub; ++sub) {
+    const size_t tile = (size_t(blockIdx.x) * kRsSub + sub) * kRsTile;
+    if (tile >= n) break;  // uniform over the block
+    for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads)
+      (&cnt[0][0])[i] = 0;
     __syncthreads();
-    if (d < R) {
-      start[d] = tot;
-      gbase[d] = offs[size_t(d) * nblocks + blockIdx.x] - tot;
-    }
-  }
-  __syncthreads();
-  const int count = int(min(size_t(kRsTile), n - tile));
+    const size_t base = tile + size_t(wave) * (kRsTile / kRsWaves);
+    uint32_t key[kRsPerThread], pos[kRsPerThread];
 #pragma unroll
-  for (int it = 0; it < kRsPerThread; ++it) {
-    if (base + size_t(it) * 64 + lane < n) {
+    for (int it = 0; it < kRsPerThread; ++it) {
+      const size_t i = base + size_t(it) * 64 + lane;
+      key[it] = i < n ? kin[i] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kRsPerThread; ++it) {
+      const bool act = base + size_t(it) * 64 + lane < n;
       const uint32_t d = (key[it] >> shift) & mask;
-      const uint32_t p = start[d] + cnt[wave][d] + pos[it];
-      pos[it] = p;
-      buf[p] = key[it];
-      dig[p] = uint16_t(d);
+      uint64_t peers = __ballot(act);
+      for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      const uint32_t r = uint32_t(__popcll(peers & lt));
+      const uint32_t c = act ? cnt[wave][d] : 0u;
+      pos[it] = c + r;
+      if (act && r == 0) cnt[wave][d] = c + uint32_t(__popcll(peers));
     }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < count; i += kRsThreads) kout[gbase[dig[i]] + i] = buf[i];
-  __syncthreads();
+    __syncthreads();
+    // wave offsets inside each digit, digit totals, then tile starts
+    for (int d = threadIdx.x; d < R; d += kRsThreads) {
+      uint32_t run = 0;
 #pragma unroll
-  for (int it = 0; it < kRsPerThread; ++it) {
-    const size_t i = base + size_t(it) * 64 + lane;
-    if (i < n) buf[pos[it]] = vin[i];
+      for (int w = 0; w < kRsWaves; ++w) {
+        const uint32_t t = cnt[w][d];
+        cnt[w][d] = run;
+        run += t;
+      }
+      start[d] = run;
+    }
+    __syncthreads();
+    {
+      const int d = threadIdx.x;
+      const uint32_t v = d < R ? start[d] : 0u;
+      uint32_t ex = 0;
+      const uint32_t total = BlockExclusiveScan(v, ex);
+      if (d < R) {
+        start[d] = ex;
+        gbase[d] = run_base[d] - ex;
+      }
+      if (d == 0) start[R] = total;
+    }
+    __syncthreads();
+    const int count = int(min(size_t(kRsTile), n - tile));
+#pragma unroll
+    for (int it = 0; it < kRsPerThread; ++it) {
+      if (base + size_t(it) * 64 + lane < n) {
+        const uint32_t d = (key[it] >> shift) & mask;
+        const uint32_t p = start[d] + cnt[wave][d] + pos[it];
+        pos[it] = p;
+        buf[p] = key[it];
+        dig[p] = uint16_t(d);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += kRsThreads) kout[gbase[dig[i]] + i] = buf[i];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kRsPerThread; ++it) {
+      const size_t i = base + size_t(it) * 64 + lane;
+      if (i < n) buf[pos[it]] = vin[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += kRsThreads) vout[gbase[dig[i]] + i] = buf[i];
+    // the next tile's digits follow this tile's
+    for (int d = threadIdx.x; d < R; d += kRsThreads) run_base[d] += start[d + 1] - start[d];
+    __syncthreads();
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < count; i += kRsThreads) vout[gbase[dig[i]] + i] = buf[i];
 }
 
 // off[b] = first sorted position whose key >= b * stride, for b in [0, nb].
@@ -273,7 +292,7 @@ void DeviceExclusiveScan(const uint32_t* in, uint32_t* out, size_t n,
 size_t ScanScratchWords(size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 size_t RadixSortScratchWords(size_t n) {
-  const size_t nblocks = (n + kRsTile - 1) / kRsTile;
+  const size_t nblocks = (n + kRsRun - 1) / kRsRun;
   const size_t counts = size_t(kRsMaxBuckets) * std::max<size_t>(nblocks, 1);
   return 2 * counts + ScanScratchWords(counts);
 }
@@ -286,7 +305,7 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
                      size_t n, int key_bits, uint32_t* scratch, hipStream_t s) {
   if (n <= 1) return false;
   WG_CHECK(key_bits >= 1 && key_bits <= 32, "key width out of range");
-  const size_t nblocks = (n + kRsTile - 1) / kRsTile;
+  const size_t nblocks = (n + kRsRun - 1) / kRsRun;
   WG_CHECK(nblocks < (1u << 31), "radix sort too large");
   const int npass = (key_bits + kRsMaxBits - 1) / kRsMaxBits;
   const int digit = (key_bits + npass - 1) / npass;

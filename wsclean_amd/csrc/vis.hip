@@ -80,9 +80,10 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
                                                      uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
   const size_t n = n_rows * n_chan;
+  const double inv_chan = 1.0 / double(n_chan);
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
        i += size_t(gridDim.x) * kBlock) {
-    const size_t row = i / n_chan, ch = i - row * n_chan;
+    const size_t row = DivSmall(uint32_t(i), uint32_t(n_chan), inv_chan), ch = i - row * n_chan;
     uint32_t key = dp.nkeys;
     bool valid = true;
     if (vis) {
@@ -117,10 +118,11 @@ __global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double*
                                                        size_t n_chan, uint32_t* __restrict__ perm,
                                                        size_t n_valid, float4* __restrict__ coord,
                                                        float2* __restrict__ phasor) {
+  const double inv_chan = 1.0 / double(n_chan);
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
     const uint32_t idx = perm[i];
-    const size_t row = idx / n_chan, ch = idx - row * n_chan;
+    const size_t row = DivSmall(idx, uint32_t(n_chan), inv_chan), ch = idx - row * n_chan;
     const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
     coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
     phasor[i] = ShiftPhasor(dp, L, -1.f);
