@@ -220,12 +220,17 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
       for (int r = 0; r < R; ++r) v[b][r] = buf[Pad(j + r * nb)];
       if (Ns > 1) {
         const int k = j - int(__umulhi(uint32_t(j), magic)) * Ns;
+        // w^r for r < R from one table twiddle w = exp(-+2 pi i k tws / n):
+        // products of at most four factors (squarings and one extra factor),
+        // so the rounding stays a few ulp while the LDS reads drop from
+        // 2 (R - 1) to 2 per butterfly
+        float2 w[R];
+        w[1] = Twiddle(twl, k * tws);
+        if (INV) w[1].y = -w[1].y;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = Twiddle(twl, k * r * tws);
-          if (INV) w.y = -w.y;
-          v[b][r] = cmul(v[b][r], w);
-        }
+        for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul(w[r - 1], w[1]) : cmul(w[r / 2], w[r / 2]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
       }
       Dft<R, INV>(v[b]);
     }
