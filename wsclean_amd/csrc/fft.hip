@@ -257,13 +257,18 @@ __device__ void LdsFFT(float2* buf, const FFTDesc& d, const float2* twl) {
   for (int s = 0; s < d.nst; ++s) {
     const int R = d.rad[s];
     const uint32_t mg = d.magic[s];
+    // the line length is re-read per stage through an empty asm statement,
+    // so the compiler cannot hoist the LDS addresses of every radix case
+    // (all loop-invariant) out of the stage loop and hold them in registers
+    int n = d.n;
+    asm volatile("" : "+s"(n));
     switch (R) {
-      case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
-      case 9: StockhamStage<9, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
-      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
-      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
-      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
-      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, d.n, Ns, mg, twl); break;
+      case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      case 9: StockhamStage<9, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
     }
     Ns *= R;
   }
@@ -328,32 +333,104 @@ __device__ __forceinline__ float2 Phasor(double cycles, float sign) {
 
 // ---------------------------------------------------------------- kernels
 
+// Register staging of one line's input for the line kernels: element i =
+// threadIdx.x + k BLOCK (k < PF), zero past len.  A kernel loads the next
+// line into it right after storing the current one to LDS, so those global
+// loads are in flight while the current line is transformed.  Capacity:
+// PF BLOCK elements (half a maximal line by default).
+template <int BLOCK, int MAXN, int PF_ = MAXN / 2 / BLOCK>
+struct LineStage {
+  static constexpr int PF = PF_;
+  float2 v[PF];
+  static __device__ __forceinline__ bool Fits(int len, int n) {
+    return len <= n && len <= PF * BLOCK;
+  }
+  __device__ __forceinline__ void Load(const float2* __restrict__ src, int len) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int i = threadIdx.x + k * BLOCK;
+      v[k] = i < len ? src[i] : make_float2(0.f, 0.f);
+    }
+  }
+  // Writes the whole line: slot (origin + DIR i) mod n holds element i for
+  // i < len and zero for len <= i < n (0 <= origin < n, len <= n), so every
+  // slot is written exactly once and no zeroing pass is needed.
+  template <int DIR>
+  __device__ __forceinline__ void Store(float2* buf, int n, int origin) const {
+    auto slot = [&](int i) {
+      int s = origin + DIR * i;
+      if (DIR > 0)
+        s -= s >= n ? n : 0;
+      else
+        s += s < 0 ? n : 0;
+      return Pad(s);
+    };
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int i = threadIdx.x + k * BLOCK;
+      if (i < n) buf[slot(i)] = v[k];
+    }
+    for (int i = threadIdx.x + PF * BLOCK; i < n; i += BLOCK)
+      buf[slot(i)] = make_float2(0.f, 0.f);
+  }
+};
+
+// (a mod n) for -n < a < 2n
+__device__ __forceinline__ int WrapNear(int a, int n) {
+  return a < 0 ? a + n : (a >= n ? a - n : a);
+}
+
+// Persistent row transforms of the invert pass: line L = (plane L / BV, box
+// row L % BV), u-FFT of the box row folded into the grid period, cropped to
+// the nx image columns.
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) RowFFTInvertKernel(
-    DevPlan dp, FFTDesc fu, const float2* __restrict__ box, float2* __restrict__ rbuf) {
+    DevPlan dp, FFTDesc fu, const float2* __restrict__ box, float2* __restrict__ rbuf,
+    int nlines) {
   extern __shared__ float2 lds[];
   float2* twl = lds;
   float2* buf = lds + kTwSlots;
-  const int j = blockIdx.x, pl = blockIdx.y;
-  const float2* src = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
+  using Stage = LineStage<BLOCK, MAXN>;
+  const bool staged = Stage::Fits(dp.BU, fu.n);
+  auto src_of = [&](int L) {
+    const int j = L % dp.BV, pl = L / dp.BV;
+    return box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
+  };
   LoadTwiddles<BLOCK>(twl, fu);
-  ZeroLds<BLOCK>(buf, fu.n);
-  __syncthreads();
-  FoldLoad<BLOCK>(buf, fu.n, src, dp.BU, dp.box_u0);
-  LdsFFT<false, BLOCK, MAXN>(buf, fu, twl);
-  float2* out = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
-  for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
-    out[x] = buf[Pad(Wrap(dp.nx / 2 - x, fu.n))];
+  Stage st;
+  int L = blockIdx.x;
+  if (staged && L < nlines) st.Load(src_of(L), dp.BU);
+  for (; L < nlines; L += gridDim.x) {
+    if (staged) {
+      st.template Store<1>(buf, fu.n, dp.box_u0);
+      if (L + int(gridDim.x) < nlines) st.Load(src_of(L + gridDim.x), dp.BU);
+      __syncthreads();
+    } else {
+      ZeroLds<BLOCK>(buf, fu.n);
+      __syncthreads();
+      FoldLoad<BLOCK>(buf, fu.n, src_of(L), dp.BU, dp.box_u0);
+    }
+    LdsFFT<false, BLOCK, MAXN>(buf, fu, twl);
+    float2* out = rbuf + size_t(L) * dp.nx;
+    for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
+      out[x] = buf[Pad(WrapNear(dp.nx / 2 - x, fu.n))];
+    // the next line overwrites buf
+    __syncthreads();
+  }
 }
 
-template <int BLOCK, int MAXN>
+// One image column x through all planes of the slab: v-FFT of the column
+// spectrum, w-screen, real part accumulated in registers.  NY = image rows
+// per thread (ny <= NY BLOCK).
+template <int BLOCK, int MAXN, int NY>
 __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
     DevPlan dp, FFTDesc fv, const float2* __restrict__ cbuf, float* __restrict__ dirty,
     int plane_lo, int nplanes) {
   extern __shared__ float2 lds[];
   float2* twl = lds;
   float2* buf = lds + kTwSlots;
-  constexpr int NY = MAXN / BLOCK;
+  // a quarter line of staging: the per-row registers share the budget
+  using Stage = LineStage<BLOCK, MAXN, MAXN / 4 / BLOCK>;
   const int x = blockIdx.x;
   float acc[NY];
   double zz[NY];
@@ -363,18 +440,28 @@ __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
     acc[k] = 0.f;
     zz[k] = y < dp.ny ? PixelZ(dp, x, y) : 0.0;
   }
+  const bool staged = Stage::Fits(dp.BV, fv.n);
+  auto src_of = [&](int pl) { return cbuf + (size_t(pl) * dp.nx + x) * dp.BV; };
   LoadTwiddles<BLOCK>(twl, fv);
+  Stage st;
+  if (staged && nplanes > 0) st.Load(src_of(0), dp.BV);
   for (int pl = 0; pl < nplanes; ++pl) {
-    ZeroLds<BLOCK>(buf, fv.n);
-    __syncthreads();
-    FoldLoad<BLOCK>(buf, fv.n, cbuf + (size_t(pl) * dp.nx + x) * dp.BV, dp.BV, dp.box_v0);
+    if (staged) {
+      st.template Store<1>(buf, fv.n, dp.box_v0);
+      if (pl + 1 < nplanes) st.Load(src_of(pl + 1), dp.BV);
+      __syncthreads();
+    } else {
+      ZeroLds<BLOCK>(buf, fv.n);
+      __syncthreads();
+      FoldLoad<BLOCK>(buf, fv.n, src_of(pl), dp.BV, dp.box_v0);
+    }
     LdsFFT<false, BLOCK, MAXN>(buf, fv, twl);
     const double wp = PlaneW(dp, plane_lo + pl);
 #pragma unroll
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
       if (y < dp.ny) {
-        const float2 val = buf[Pad(Wrap(y - dp.ny / 2, fv.n))];
+        const float2 val = buf[Pad(WrapNear(y - dp.ny / 2, fv.n))];
         const float2 ph = Phasor(wp * zz[k], -1.f);
         acc[k] += val.x * ph.x - val.y * ph.y;
       }
@@ -389,14 +476,16 @@ __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
   }
 }
 
-template <int BLOCK, int MAXN>
+// One image column x through all planes of the slab, predict direction:
+// model * correction * w-screen, inverse v-FFT, the box rows of the column
+// spectrum stored.  NY as in ColFFTInvertKernel.
+template <int BLOCK, int MAXN, int NY>
 __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
     DevPlan dp, FFTDesc fv, const double* __restrict__ img, const float* __restrict__ corr,
     float2* __restrict__ cbuf, int plane_lo, int nplanes) {
   extern __shared__ float2 lds[];
   float2* twl = lds;
   float2* buf = lds + kTwSlots;
-  constexpr int NY = MAXN / BLOCK;
   const int x = blockIdx.x;
   float mv[NY];
   double zz[NY];
@@ -414,15 +503,18 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
   }
   LoadTwiddles<BLOCK>(twl, fv);
   for (int pl = 0; pl < nplanes; ++pl) {
-    ZeroLds<BLOCK>(buf, fv.n);
-    __syncthreads();
     const double wp = PlaneW(dp, plane_lo + pl);
+    // slot (y - ny / 2) mod n for y in [0, n) covers the line once: the
+    // image rows carry the screened model, the rest is zero
 #pragma unroll
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
-      if (y < dp.ny)
-        buf[Pad(Wrap(y - dp.ny / 2, fv.n))] = cscale(Phasor(wp * zz[k], 1.f), mv[k]);
+      if (y < fv.n)
+        buf[Pad(WrapNear(y - dp.ny / 2, fv.n))] =
+            y < dp.ny ? cscale(Phasor(wp * zz[k], 1.f), mv[k]) : make_float2(0.f, 0.f);
     }
+    for (int y = threadIdx.x + NY * BLOCK; y < fv.n; y += BLOCK)
+      buf[Pad(y - dp.ny / 2)] = make_float2(0.f, 0.f);
     __syncthreads();
     LdsFFT<true, BLOCK, MAXN>(buf, fv, twl);
     float2* out = cbuf + (size_t(pl) * dp.nx + x) * dp.BV;
@@ -432,22 +524,41 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
   }
 }
 
+// Persistent row transforms of the predict pass: line L = (plane L / BV, box
+// row L % BV), the nx image columns placed in the grid period, inverse
+// u-FFT, the box row stored.
 template <int BLOCK, int MAXN>
 __global__ void __launch_bounds__(BLOCK) RowFFTPredictKernel(
-    DevPlan dp, FFTDesc fu, const float2* __restrict__ rbuf, float2* __restrict__ box) {
+    DevPlan dp, FFTDesc fu, const float2* __restrict__ rbuf, float2* __restrict__ box,
+    int nlines) {
   extern __shared__ float2 lds[];
   float2* twl = lds;
   float2* buf = lds + kTwSlots;
-  const int j = blockIdx.x, pl = blockIdx.y;
+  using Stage = LineStage<BLOCK, MAXN>;
+  const bool staged = Stage::Fits(dp.nx, fu.n);
   LoadTwiddles<BLOCK>(twl, fu);
-  ZeroLds<BLOCK>(buf, fu.n);
-  __syncthreads();
-  const float2* src = rbuf + (size_t(pl) * dp.BV + j) * dp.nx;
-  for (int x = threadIdx.x; x < dp.nx; x += BLOCK) buf[Pad(Wrap(dp.nx / 2 - x, fu.n))] = src[x];
-  __syncthreads();
-  LdsFFT<true, BLOCK, MAXN>(buf, fu, twl);
-  float2* out = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
-  for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Pad(Wrap(dp.box_u0 + i, fu.n))];
+  Stage st;
+  int L = blockIdx.x;
+  if (staged && L < nlines) st.Load(rbuf + size_t(L) * dp.nx, dp.nx);
+  for (; L < nlines; L += gridDim.x) {
+    if (staged) {
+      // image column x goes to slot (nx / 2 - x) mod n
+      st.template Store<-1>(buf, fu.n, dp.nx / 2);
+      if (L + int(gridDim.x) < nlines) st.Load(rbuf + size_t(L + gridDim.x) * dp.nx, dp.nx);
+    } else {
+      ZeroLds<BLOCK>(buf, fu.n);
+      __syncthreads();
+      const float2* src = rbuf + size_t(L) * dp.nx;
+      for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
+        buf[Pad(WrapNear(dp.nx / 2 - x, fu.n))] = src[x];
+    }
+    __syncthreads();
+    LdsFFT<true, BLOCK, MAXN>(buf, fu, twl);
+    const int j = L % dp.BV, pl = L / dp.BV;
+    float2* out = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
+    for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Pad(Wrap(dp.box_u0 + i, fu.n))];
+    __syncthreads();
+  }
 }
 
 template <int BLOCK, int MAXN>
@@ -537,13 +648,40 @@ std::vector<float2> FFTTwiddleTable(int n) {
   return t;
 }
 
+namespace {
+
+// Grid of a persistent line kernel: the resident workgroups, at most one
+// per line.
+template <typename K>
+int LineGrid(K kernel, int block, size_t lds, int nlines) {
+  const int per_cu = std::max(1, ResidentBlocksPerCU(reinterpret_cast<const void*>(kernel),
+                                                     block, lds));
+  return std::max(1, std::min(nlines, per_cu * DeviceCUs()));
+}
+
+// NY (image rows per thread) of the column kernels: half the line capacity
+// when ny fits, so the per-row registers stay small.
+template <int BLOCK, int MAXN, typename F>
+void DispatchNY(int ny, F&& f) {
+  constexpr int NYF = MAXN / BLOCK;
+  if (ny <= (NYF / 2) * BLOCK)
+    f(std::integral_constant<int, NYF / 2>());
+  else
+    f(std::integral_constant<int, NYF>());
+}
+
+}  // namespace
+
 void LaunchRowFFTInvert(const DevPlan& dp, const FFTDesc& fu, const float2* box,
                         float2* rbuf, int nplanes, hipStream_t s) {
+  const int nlines = dp.BV * nplanes;
+  if (nlines <= 0) return;
   Dispatch(fu.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(RowFFTInvertKernel<BLOCK, MAXN>, LineLds(MAXN));
-    RowFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, LineLds(fu.n), s>>>(
-        dp, fu, box, rbuf);
+    auto k = RowFFTInvertKernel<BLOCK, MAXN>;
+    AllowLds(k, LineLds(MAXN));
+    k<<<LineGrid(k, BLOCK, LineLds(fu.n), nlines), BLOCK, LineLds(fu.n), s>>>(dp, fu, box,
+                                                                            rbuf, nlines);
   });
   WG_HIP(hipGetLastError());
 }
@@ -552,9 +690,12 @@ void LaunchColFFTInvert(const DevPlan& dp, const FFTDesc& fv, const float2* cbuf
                         float* dirty, int plane_lo, int nplanes, hipStream_t s) {
   Dispatch(fv.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(ColFFTInvertKernel<BLOCK, MAXN>, LineLds(MAXN));
-    ColFFTInvertKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
-        dp, fv, cbuf, dirty, plane_lo, nplanes);
+    DispatchNY<BLOCK, MAXN>(dp.ny, [&](auto Y) {
+      constexpr int NY = decltype(Y)::value;
+      AllowLds(ColFFTInvertKernel<BLOCK, MAXN, NY>, LineLds(MAXN));
+      ColFFTInvertKernel<BLOCK, MAXN, NY><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
+          dp, fv, cbuf, dirty, plane_lo, nplanes);
+    });
   });
   WG_HIP(hipGetLastError());
 }
@@ -564,20 +705,26 @@ void LaunchColFFTPredict(const DevPlan& dp, const FFTDesc& fv, const double* img
                          hipStream_t s) {
   Dispatch(fv.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(ColFFTPredictKernel<BLOCK, MAXN>, LineLds(MAXN));
-    ColFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
-        dp, fv, img, corr, cbuf, plane_lo, nplanes);
+    DispatchNY<BLOCK, MAXN>(dp.ny, [&](auto Y) {
+      constexpr int NY = decltype(Y)::value;
+      AllowLds(ColFFTPredictKernel<BLOCK, MAXN, NY>, LineLds(MAXN));
+      ColFFTPredictKernel<BLOCK, MAXN, NY><<<dim3(dp.nx), BLOCK, LineLds(fv.n), s>>>(
+          dp, fv, img, corr, cbuf, plane_lo, nplanes);
+    });
   });
   WG_HIP(hipGetLastError());
 }
 
 void LaunchRowFFTPredict(const DevPlan& dp, const FFTDesc& fu, const float2* rbuf,
                          float2* box, int nplanes, hipStream_t s) {
+  const int nlines = dp.BV * nplanes;
+  if (nlines <= 0) return;
   Dispatch(fu.n, [&](auto B, auto M) {
     constexpr int BLOCK = decltype(B)::value, MAXN = decltype(M)::value;
-    AllowLds(RowFFTPredictKernel<BLOCK, MAXN>, LineLds(MAXN));
-    RowFFTPredictKernel<BLOCK, MAXN><<<dim3(dp.BV, nplanes), BLOCK, LineLds(fu.n), s>>>(
-        dp, fu, rbuf, box);
+    auto k = RowFFTPredictKernel<BLOCK, MAXN>;
+    AllowLds(k, LineLds(MAXN));
+    k<<<LineGrid(k, BLOCK, LineLds(fu.n), nlines), BLOCK, LineLds(fu.n), s>>>(dp, fu, rbuf,
+                                                                            box, nlines);
   });
   WG_HIP(hipGetLastError());
 }
