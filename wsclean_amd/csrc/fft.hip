@@ -186,11 +186,6 @@ __device__ __forceinline__ void Dft<9, false>(float2* v) { Dft9<false>(v); }
 template <>
 __device__ __forceinline__ void Dft<9, true>(float2* v) { Dft9<true>(v); }
 
-// LDS line layout: one pad slot after every 16 complex values, so the
-// strided writes of the early Stockham stages spread over the banks.
-__device__ __forceinline__ int Pad(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int PaddedLen(int n) { return n + (n >> 4) + 1; }
-
 // Twiddle exp(-2 pi i k / n) from the two-level LDS table (lo[k % 128] *
 // hi[k / 128]; FP32 product of correctly rounded factors).
 __device__ __forceinline__ float2 Twiddle(const float2* twl, int k) {
@@ -217,9 +212,13 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
     const int j = threadIdx.x + b * BLOCK;
     if (j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[Pad(j + r * nb)];
-      if (Ns > 1) {
-        const int k = j - int(__umulhi(uint32_t(j), magic)) * Ns;
+      for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nb];
+      // the first stage (Ns = 1, k = 0) multiplies by exactly 1: one
+      // branch-free path keeps the loaded values in place (no register
+      // shuffles between the two paths)
+      {
+        // (magic wraps to 1 for Ns = 1, hence the select)
+        const int k = Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0;
         // w^r for r < R from one table twiddle w = exp(-+2 pi i k tws / n):
         // products of at most four factors (squarings and one extra factor),
         // so the rounding stays a few ulp while the LDS reads drop from
@@ -243,7 +242,7 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
       const int k = Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[Pad(base + r * Ns)] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
     }
   }
   __syncthreads();
@@ -283,7 +282,7 @@ __device__ __forceinline__ void LoadTwiddles(float2* twl, const FFTDesc& d) {
 
 template <int BLOCK>
 __device__ __forceinline__ void ZeroLds(float2* buf, int n) {
-  for (int i = threadIdx.x; i < PaddedLen(n); i += BLOCK) buf[i] = make_float2(0.f, 0.f);
+  for (int i = threadIdx.x; i < n; i += BLOCK) buf[i] = make_float2(0.f, 0.f);
 }
 
 // buf[(origin + i) mod n] += src[i] for i in [0, len), folding periodically.
@@ -294,7 +293,7 @@ __device__ __forceinline__ void FoldLoad(float2* buf, int n, const float2* __res
     const int i0 = max(0, k * n - origin);
     const int i1 = min(len, (k + 1) * n - origin);
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-      float2& d = buf[Pad(origin + i - k * n)];
+      float2& d = buf[origin + i - k * n];
       const float2 a = src[i];
       d.x += a.x;
       d.y += a.y;
@@ -363,7 +362,7 @@ struct LineStage {
         s -= s >= n ? n : 0;
       else
         s += s < 0 ? n : 0;
-      return Pad(s);
+      return s;
     };
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
@@ -413,7 +412,7 @@ __global__ void __launch_bounds__(BLOCK) RowFFTInvertKernel(
     LdsFFT<false, BLOCK, MAXN>(buf, fu, twl);
     float2* out = rbuf + size_t(L) * dp.nx;
     for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
-      out[x] = buf[Pad(WrapNear(dp.nx / 2 - x, fu.n))];
+      out[x] = buf[WrapNear(dp.nx / 2 - x, fu.n)];
     // the next line overwrites buf
     __syncthreads();
   }
@@ -461,7 +460,7 @@ __global__ void __launch_bounds__(BLOCK) ColFFTInvertKernel(
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
       if (y < dp.ny) {
-        const float2 val = buf[Pad(WrapNear(y - dp.ny / 2, fv.n))];
+        const float2 val = buf[WrapNear(y - dp.ny / 2, fv.n)];
         const float2 ph = Phasor(wp * zz[k], -1.f);
         acc[k] += val.x * ph.x - val.y * ph.y;
       }
@@ -510,16 +509,16 @@ __global__ void __launch_bounds__(BLOCK) ColFFTPredictKernel(
     for (int k = 0; k < NY; ++k) {
       const int y = threadIdx.x + k * BLOCK;
       if (y < fv.n)
-        buf[Pad(WrapNear(y - dp.ny / 2, fv.n))] =
+        buf[WrapNear(y - dp.ny / 2, fv.n)] =
             y < dp.ny ? cscale(Phasor(wp * zz[k], 1.f), mv[k]) : make_float2(0.f, 0.f);
     }
     for (int y = threadIdx.x + NY * BLOCK; y < fv.n; y += BLOCK)
-      buf[Pad(y - dp.ny / 2)] = make_float2(0.f, 0.f);
+      buf[y - dp.ny / 2] = make_float2(0.f, 0.f);
     __syncthreads();
     LdsFFT<true, BLOCK, MAXN>(buf, fv, twl);
     float2* out = cbuf + (size_t(pl) * dp.nx + x) * dp.BV;
     for (int j = threadIdx.x; j < dp.BV; j += BLOCK)
-      out[j] = buf[Pad(Wrap(dp.box_v0 + j, fv.n))];
+      out[j] = buf[Wrap(dp.box_v0 + j, fv.n)];
     __syncthreads();
   }
 }
@@ -550,13 +549,13 @@ __global__ void __launch_bounds__(BLOCK) RowFFTPredictKernel(
       __syncthreads();
       const float2* src = rbuf + size_t(L) * dp.nx;
       for (int x = threadIdx.x; x < dp.nx; x += BLOCK)
-        buf[Pad(WrapNear(dp.nx / 2 - x, fu.n))] = src[x];
+        buf[WrapNear(dp.nx / 2 - x, fu.n)] = src[x];
     }
     __syncthreads();
     LdsFFT<true, BLOCK, MAXN>(buf, fu, twl);
     const int j = L % dp.BV, pl = L / dp.BV;
     float2* out = box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch;
-    for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Pad(Wrap(dp.box_u0 + i, fu.n))];
+    for (int i = threadIdx.x; i < dp.BU; i += BLOCK) out[i] = buf[Wrap(dp.box_u0 + i, fu.n)];
     __syncthreads();
   }
 }
@@ -569,13 +568,13 @@ __global__ void __launch_bounds__(BLOCK) LineFFTKernel(float2* __restrict__ data
   float2* buf = lds + kTwSlots;
   float2* line = data + size_t(blockIdx.x) * d.n;
   LoadTwiddles<BLOCK>(twl, d);
-  for (int i = threadIdx.x; i < d.n; i += BLOCK) buf[Pad(i)] = line[i];
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) buf[i] = line[i];
   __syncthreads();
   if (inverse)
     LdsFFT<true, BLOCK, MAXN>(buf, d, twl);
   else
     LdsFFT<false, BLOCK, MAXN>(buf, d, twl);
-  for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[Pad(i)];
+  for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[i];
 }
 
 // out[p][c][r] = in[p][r][c]; 32x32 tiles through LDS.
@@ -597,8 +596,8 @@ __global__ void __launch_bounds__(256) TransposeKernel(const float2* __restrict_
   }
 }
 
-// LDS of one line kernel: twiddle table + padded line of length n.
-size_t LineLds(int n) { return size_t(kTwSlots + PaddedLen(n)) * sizeof(float2); }
+// LDS of one line kernel: twiddle table + line of length n.
+size_t LineLds(int n) { return size_t(kTwSlots + n) * sizeof(float2); }
 
 // Kernels that use more than the default dynamic LDS must opt in once.
 template <typename K>
