@@ -60,6 +60,21 @@ def test_bucket_sort_bit_exact(oracle, eps):
     np.testing.assert_array_equal(perm, ref_perm.astype(np.uint32))
 
 
+@pytest.mark.parametrize("n_ant,n_times,n_chan", [(15, 31, 3), (21, 67, 5)])
+def test_bucket_sort_ragged_counts(oracle, n_ant, n_times, n_chan):
+    # element counts that are odd (not a multiple of the 4-key loads of the
+    # digit histogram) and not a multiple of a radix tile
+    p = synth.small_problem(seed=23, n_ant=n_ant, n_times=n_times, n_chan=n_chan, npix=256,
+                            pixel_arcsec=30.0, radius_m=4000.0, dec_deg=-50.0, dt_s=300.0)
+    n = p["uvw"].shape[0] * n_chan
+    assert n % 4 != 0 and n % 8192 != 0
+    g = make(256, p["dx"], 1e-4)
+    keys, perm, plan = g.sort_keys(p["uvw"], p["freq"], None)
+    ref_keys, _ = oracle.bucket_keys(plan, p["uvw"], p["freq"], None)
+    np.testing.assert_array_equal(keys, ref_keys)
+    np.testing.assert_array_equal(perm, np.argsort(ref_keys, kind="stable").astype(np.uint32))
+
+
 def test_bucket_sort_bit_exact_large(oracle):
     # enough visibilities for several radix blocks and all three digit passes
     p = synth.small_problem(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512,

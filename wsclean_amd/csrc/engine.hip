@@ -241,12 +241,14 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   uint32_t* keys = static_cast<uint32_t*>(arena_.Get(tag + ".keys", n_vis * 4));
   uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
   if (n_vis) {
-    LaunchKeys(dp, uvw_dev, vs.scale, n_rows, n_chan, vis_dev, keys, vals, stream_);
+    // the sort's first pass generates the index values itself
+    LaunchKeys(dp, uvw_dev, vs.scale, n_rows, n_chan, vis_dev, keys, nullptr, stream_);
     uint32_t* scratch =
         static_cast<uint32_t*>(arena_.Get("sort.scratch", RadixSortScratchWords(n_vis) * 4));
     uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
-    if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_)) {
+    if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_,
+                        true)) {
       keys = keys2;
       vals = vals2;
     }

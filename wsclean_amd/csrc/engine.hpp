@@ -22,9 +22,12 @@ void DeviceExclusiveScan(const uint32_t* in, uint32_t* out, size_t n,
 size_t ScanScratchWords(size_t n);
 size_t RadixSortScratchWords(size_t n);
 // Stable sort of (keys, vals) by the low key_bits; keys_alt / vals_alt are
-// the ping-pong buffers.  Returns true when the result is in the alt pair.
+// the ping-pong buffers (all four 16-byte aligned).  index_vals: the input
+// values are the element indices and vals is not read.  Returns true when
+// the result is in the alt pair.
 bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
-                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s);
+                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s,
+                     bool index_vals = false);
 // off[b] = first sorted position with key >= b * stride, b in [0, nbuckets].
 void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
                          uint32_t stride, uint32_t* off, hipStream_t s);
@@ -320,8 +323,8 @@ struct FacetDev {
   unsigned long long row_off;
   int tu0, tv0;
 };
-// keys[v] / vals[v] = v for every virtual index v of the batch (facet f owns
-// [vbase[f], vbase[f + 1])).
+// keys[v] (and vals[v] = v when vals is not null) for every virtual index v
+// of the batch (facet f owns [vbase[f], vbase[f + 1])).
 void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F,
                      const double* uvw, const double* scale, size_t n_chan, size_t n_virtual,
                      int atlas_ntu, int atlas_P0, int T, uint32_t* keys, uint32_t* vals,
@@ -334,6 +337,8 @@ void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vbase, int
                        hipStream_t s);
 void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
                    double* partial, double* out, hipStream_t s);
+// Unsorted sort keys of every visibility (nkeys for zero ones); vals[i] = i
+// when vals is not null.
 void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
                 size_t n_rows, size_t n_chan, const float2* vis, uint32_t* keys,
                 uint32_t* vals, hipStream_t s);

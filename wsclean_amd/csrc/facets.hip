@@ -223,13 +223,14 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   uint32_t* keys = static_cast<uint32_t*>(A.Get("atlas.keys", n_virtual * 4));
   uint32_t* vals = static_cast<uint32_t*>(A.Get("atlas.perm", n_virtual * 4));
   if (n_virtual) {
+    // the sort's first pass generates the index values itself
     LaunchFacetKeys(dfac, dvb, F, uvw_dev, scale_, n_chan, n_virtual, pa.ntu, pa.P0, T, keys,
-                    vals, s);
+                    nullptr, s);
     uint32_t* scratch =
         static_cast<uint32_t*>(A.Get("sort.scratch", RadixSortScratchWords(n_virtual) * 4));
     uint32_t* keys2 = static_cast<uint32_t*>(A.Get("atlas.keys2", n_virtual * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(A.Get("atlas.perm2", n_virtual * 4));
-    if (DeviceRadixSort(keys, vals, keys2, vals2, n_virtual, pa.key_bits, scratch, s)) {
+    if (DeviceRadixSort(keys, vals, keys2, vals2, n_virtual, pa.key_bits, scratch, s, true)) {
       keys = keys2;
       vals = vals2;
     }

@@ -129,7 +129,8 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
 
 // Digit histogram of one run of kRsSub tiles: counts[d * nblocks + block].
 // Wave w reads elements [tile + w * 1024, tile + (w + 1) * 1024) of every
-// tile of the run.
+// tile of the run, four keys per 16-byte load (tiles start 32 KiB apart, so
+// the loads are aligned).
 __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
     const uint32_t* __restrict__ keys, size_t n, int shift, uint32_t mask,
     uint32_t* __restrict__ counts, uint32_t nblocks) {
@@ -138,13 +139,32 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   const int R = int(mask) + 1;
   for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
+  constexpr int kVec = kRsPerThread / 4;
   for (int sub = 0; sub < kRsSub; ++sub) {
     const size_t base = (size_t(blockIdx.x) * kRsSub + sub) * kRsTile +
                         size_t(wave) * (kRsTile / kRsWaves);
-#pragma unroll 4
-    for (int it = 0; it < kRsPerThread; ++it) {
-      const size_t i = base + size_t(it) * 64 + lane;
-      if (i < n) atomicAdd(&h[wave][(keys[i] >> shift) & mask], 1u);
+    if (base >= n) break;
+    uint4 q[kVec];
+#pragma unroll
+    for (int it = 0; it < kVec; ++it) {
+      const size_t i = base + (size_t(it) * 64 + lane) * 4;
+      if (i + 3 < n) {
+        q[it] = *reinterpret_cast<const uint4*>(keys + i);
+      } else {
+        // ragged end: keys past n are not counted
+        q[it].x = i < n ? keys[i] : ~0u;
+        q[it].y = i + 1 < n ? keys[i + 1] : ~0u;
+        q[it].z = i + 2 < n ? keys[i + 2] : ~0u;
+        q[it].w = ~0u;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < kVec; ++it) {
+      const size_t i = base + (size_t(it) * 64 + lane) * 4;
+      const uint32_t kk[4] = {q[it].x, q[it].y, q[it].z, q[it].w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (i + c < n) atomicAdd(&h[wave][(kk[c] >> shift) & mask], 1u);
     }
   }
   __syncthreads();
@@ -164,9 +184,15 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
 // writes each digit's run of the tile with consecutive threads to
 // consecutive addresses.  run_base[d] (the run's global offset of digit d)
 // advances by each tile's digit count.
-__global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
+//
+// IDX: the values are the element indices (vin is not read), which saves
+// writing and reading the identity permutation in the first pass.
+// Two blocks per CU (LDS) need four waves per SIMD, i.e. at most 128 VGPRs.
+template <bool IDX>
+__global__ void __launch_bounds__(kRsThreads)
+    __attribute__((amdgpu_waves_per_eu(4, 4))) RadixScatterKernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
     int bits, const uint32_t* __restrict__ offs, uint32_t nblocks) {
   __shared__ uint32_t cnt[kRsWaves][kRsMaxBuckets];  // per-wave counts, then offsets
   __shared__ uint32_t start[kRsMaxBuckets + 1];      // tile-local start of each digit
@@ -180,21 +206,27 @@ __global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   for (int d = threadIdx.x; d < R; d += kRsThreads) run_base[d] = offs[size_t(d) * nblocks + blockIdx.x];
   for (int sub = 0; sub < kRsSub; ++sub) {
-    const size_t tile = (size_t(blockIdx.x) * kRsSub + sub) * kRsTile;
+    const uint32_t tile = (blockIdx.x * kRsSub + sub) * kRsTile;
     if (tile >= n) break;  // uniform over the block
     for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads)
       (&cnt[0][0])[i] = 0;
     __syncthreads();
-    const size_t base = tile + size_t(wave) * (kRsTile / kRsWaves);
-    uint32_t key[kRsPerThread], pos[kRsPerThread];
+    const uint32_t base = tile + uint32_t(wave) * (kRsTile / kRsWaves);
+    uint32_t key[kRsPerThread], pos[kRsPerThread], val[kRsPerThread];
 #pragma unroll
     for (int it = 0; it < kRsPerThread; ++it) {
-      const size_t i = base + size_t(it) * 64 + lane;
+      const uint32_t i = base + uint32_t(it) * 64 + lane;
       key[it] = i < n ? kin[i] : 0u;
+    }
+    // the values are loaded now, so their latency overlaps the ranking
+#pragma unroll
+    for (int it = 0; it < kRsPerThread; ++it) {
+      const uint32_t i = base + uint32_t(it) * 64 + lane;
+      val[it] = IDX ? i : (i < n ? vin[i] : 0u);
     }
 #pragma unroll
     for (int it = 0; it < kRsPerThread; ++it) {
-      const bool act = base + size_t(it) * 64 + lane < n;
+      const bool act = base + uint32_t(it) * 64 + lane < n;
       const uint32_t d = (key[it] >> shift) & mask;
       uint64_t peers = __ballot(act);
       for (int b = 0; b < bits; ++b) {
@@ -232,10 +264,10 @@ __global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
       if (d == 0) start[R] = total;
     }
     __syncthreads();
-    const int count = int(min(size_t(kRsTile), n - tile));
+    const int count = int(min(uint32_t(kRsTile), n - tile));
 #pragma unroll
     for (int it = 0; it < kRsPerThread; ++it) {
-      if (base + size_t(it) * 64 + lane < n) {
+      if (base + uint32_t(it) * 64 + lane < n) {
         const uint32_t d = (key[it] >> shift) & mask;
         const uint32_t p = start[d] + cnt[wave][d] + pos[it];
         pos[it] = p;
@@ -248,8 +280,8 @@ __global__ void __launch_bounds__(kRsThreads) RadixScatterKernel(
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kRsPerThread; ++it) {
-      const size_t i = base + size_t(it) * 64 + lane;
-      if (i < n) buf[pos[it]] = vin[i];
+      const uint32_t i = base + uint32_t(it) * 64 + lane;
+      if (i < n) buf[pos[it]] = val[it];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < count; i += kRsThreads) vout[gbase[dig[i]] + i] = buf[i];
@@ -297,16 +329,24 @@ size_t RadixSortScratchWords(size_t n) {
   return 2 * counts + ScanScratchWords(counts);
 }
 
-// Sorts (keys, vals) by the low key_bits of keys; stable.  The digit width
+// Sorts (keys, vals) by the low key_bits of keys; stable.  index_vals: vals
+// is not read, the values are the element indices 0 .. n - 1.  The digit width
 // is ceil(key_bits / passes) with passes = ceil(key_bits / 9).  The passes
 // alternate between (keys, vals) and (keys_alt, vals_alt); the return value
 // says which pair holds the result (true: the alt pair).
 bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
-                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s) {
-  if (n <= 1) return false;
+                     size_t n, int key_bits, uint32_t* scratch, hipStream_t s,
+                     bool index_vals) {
+  if (n <= 1) {
+    if (n == 1 && index_vals) LaunchFillZero(vals, sizeof(uint32_t), s);
+    return false;
+  }
   WG_CHECK(key_bits >= 1 && key_bits <= 32, "key width out of range");
+  WG_CHECK(((reinterpret_cast<uintptr_t>(keys) | reinterpret_cast<uintptr_t>(keys_alt)) & 15) == 0,
+           "radix sort keys must be 16-byte aligned");
   const size_t nblocks = (n + kRsRun - 1) / kRsRun;
-  WG_CHECK(nblocks < (1u << 31), "radix sort too large");
+  // 32-bit element indices inside the scatter kernel
+  WG_CHECK(n <= 0xFFFFFFFFull - kRsRun, "radix sort too large");
   const int npass = (key_bits + kRsMaxBits - 1) / kRsMaxBits;
   const int digit = (key_bits + npass - 1) / npass;
   uint32_t* counts = scratch;
@@ -323,8 +363,12 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
     RadixHistKernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
         kin, n, shift, (1u << bits) - 1u, counts, uint32_t(nblocks));
     DeviceExclusiveScan(counts, offs, ncounts, scan_scratch, s, nullptr);
-    RadixScatterKernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
-        kin, vin, ko, vo, n, shift, bits, offs, uint32_t(nblocks));
+    if (index_vals && shift == 0)
+      RadixScatterKernel<true><<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
+          kin, nullptr, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
+    else
+      RadixScatterKernel<false><<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
+          kin, vin, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
     std::swap(kin, ko);
     std::swap(vin, vo);
     in_alt = !in_alt;

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
       key = (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
     }
     keys[i] = key;
-    vals[i] = uint32_t(i);
+    if (vals) vals[i] = uint32_t(i);
   }
 }
 
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) FacetKeysKernel(
     const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
     const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
     keys[i] = (bucket * uint32_t(T) + uint32_t(L.dv)) * uint32_t(T) + uint32_t(L.du);
-    vals[i] = uint32_t(i);
+    if (vals) vals[i] = uint32_t(i);
   }
 }
 
