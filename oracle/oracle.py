@@ -19,6 +19,8 @@ Contents
   (wgridder/gridder_simple.h:217-225; ducc0 itself is absent, so its own
   bucketing is parity unpinned -- the bucket sort is pinned against a stable
   CPU sort of these keys).
+* ``bucket_perm``: the sorted permutation as the device orders ties (rows
+  renumbered in uv order first, then a stable sort of the keys).
 * ``es_phi_hat``: Fourier transform of the ES kernel by adaptive scipy
   quadrature, to pin the planner's table and correction factors.
 * ``cpu_wgridder_*``: the multithreaded C++ CPU w-gridder restatement
@@ -205,6 +207,24 @@ def bucket_keys(plan, uvw, freq, vis=None):
     nkeys = plan["ntu"] * plan["ntv"] * plan["P0"] * T * T
     keys = np.where(valid, keys, nkeys)
     return keys.ravel().astype(np.uint32), valid.ravel()
+
+
+def bucket_perm(plan, uvw, freq, vis=None):
+    """Sorted permutation of the valid visibilities (the caller's flat
+    indices row * n_chan + chan) in the device's order: rows renumbered by
+    a stable sort of their key at the middle channel, then a stable sort of
+    the visibility keys in renumbered (row, channel) order."""
+    uvw = np.asarray(uvw, np.float64).reshape(-1, 3)
+    freq = np.asarray(freq, np.float64).ravel()
+    n_rows, n_chan = uvw.shape[0], freq.size
+    mid = n_chan // 2
+    rkeys, _ = bucket_keys(plan, uvw, freq[mid:mid + 1], None)
+    rperm = np.argsort(rkeys, kind="stable")
+    keys, valid = bucket_keys(plan, uvw, freq, vis)
+    keys = keys.reshape(n_rows, n_chan)[rperm].ravel()
+    order = np.argsort(keys, kind="stable")[: int(valid.sum())]
+    r, ch = order // n_chan, order % n_chan
+    return (rperm[r] * n_chan + ch).astype(np.uint32)
 
 
 def _good_size(n):

@@ -192,3 +192,30 @@ def test_kernel_index_ranges_benchmark_geometry(tune):
     f1 = freq[[0, 31, 63]]
     # the plan above covers the full band; check the extreme channels
     check_plan_bounds(plan, uvw, f1, None)
+
+
+@pytest.mark.parametrize("seed,n_chan", [(3, 4), (11, 5)])
+def test_bucket_perm_order(seed, n_chan):
+    # the device's sorted order: a permutation of the valid visibilities that
+    # sorts the keys; with rows already in the order of their middle-channel
+    # key the row renumbering is the identity and the order reduces to a
+    # stable sort of the keys over the caller's flat index
+    p = synth.small_problem(seed=seed, n_ant=14, n_times=20, n_chan=n_chan, npix=128)
+    uvw, freq = p["uvw"], p["freq"]
+    vis = p["vis"].copy()
+    vis[::5, 0] = 0
+    plan = make_plan(128, p["dx"], 1e-4, uvw, freq)
+    keys, valid = oracle.bucket_keys(plan, uvw, freq, vis)
+    perm = oracle.bucket_perm(plan, uvw, freq, vis)
+    assert np.array_equal(np.sort(perm), np.flatnonzero(valid).astype(np.uint32))
+    assert (np.diff(keys[perm].astype(np.int64)) >= 0).all()
+
+    mid = n_chan // 2
+    rkeys, _ = oracle.bucket_keys(plan, uvw, freq[mid:mid + 1], None)
+    order = np.argsort(rkeys, kind="stable")
+    uvw_s, vis_s = uvw[order], vis[order]
+    keys_s, valid_s = oracle.bucket_keys(plan, uvw_s, freq, vis_s)
+    ref = np.argsort(keys_s, kind="stable")[: int(valid_s.sum())].astype(np.uint32)
+    np.testing.assert_array_equal(oracle.bucket_perm(plan, uvw_s, freq, vis_s), ref)
+    # the renumbering is not trivial for the unsorted rows
+    assert not np.array_equal(order, np.arange(order.size))

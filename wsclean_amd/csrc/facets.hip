@@ -212,11 +212,37 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   const size_t n_virtual = size_t(vbase_[F]);
   WG_CHECK(n_virtual < 0x80000000ull, "at most 2^31 - 1 visibilities per facet batch");
   FacetDev* dfac = static_cast<FacetDev*>(A.Get("fb.facets", size_t(F) * sizeof(FacetDev)));
-  unsigned long long* dvb =
-      static_cast<unsigned long long*>(A.Get("fb.vbase", size_t(F + 1) * sizeof(uint64_t)));
   WG_HIP(hipMemcpyAsync(dfac, fd.data(), fd.size() * sizeof(FacetDev), hipMemcpyHostToDevice, s));
-  WG_HIP(hipMemcpyAsync(dvb, vbase_.data(), vbase_.size() * sizeof(uint64_t),
+
+  // ---- virtual row renumbering ----
+  // Virtual rows are renumbered in the order of their atlas key at the
+  // middle channel (stable), as the single gridder renumbers its rows: the
+  // atlas keys of different facets are disjoint and order each facet's rows
+  // exactly as that facet's own keys do, so the sorted order of every facet
+  // is the one its single gridder produces.
+  const size_t n_vrows = n_virtual / n_chan;
+  std::vector<unsigned long long> vrbase(F + 1);
+  for (int f = 0; f <= F; ++f) vrbase[f] = vbase_[f] / n_chan;
+  unsigned long long* dvrb =
+      static_cast<unsigned long long*>(A.Get("fb.vrbase", size_t(F + 1) * sizeof(uint64_t)));
+  WG_HIP(hipMemcpyAsync(dvrb, vrbase.data(), vrbase.size() * sizeof(uint64_t),
                         hipMemcpyHostToDevice, s));
+  uint32_t* rperm = static_cast<uint32_t*>(A.Get("rows.perm", n_vrows * 4));
+  double* uvw4 = static_cast<double*>(A.Get("uvw4", n_vrows * 4 * sizeof(double)));
+  uint32_t* scratch = static_cast<uint32_t*>(
+      A.Get("sort.scratch", RadixSortScratchWords(std::max(n_virtual, n_vrows)) * 4));
+  if (n_vrows) {
+    uint32_t* rk = static_cast<uint32_t*>(A.Get("rows.keys", n_vrows * 4));
+    uint32_t* rk2 = static_cast<uint32_t*>(A.Get("rows.keys2", n_vrows * 4));
+    uint32_t* rp2 = static_cast<uint32_t*>(A.Get("rows.perm2", n_vrows * 4));
+    uint32_t* rphys = static_cast<uint32_t*>(A.Get("rows.phys", n_vrows * 4));
+    LaunchFacetKeys(dfac, dvrb, F, uvw_dev, scale_ + n_chan / 2, 1, n_vrows, pa.ntu, pa.P0, T,
+                    rk, nullptr, s);
+    if (DeviceRadixSort(rk, rperm, rk2, rp2, n_vrows, pa.key_bits, scratch, s, true))
+      rperm = rp2;
+    LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, s);
+    LaunchPadUvw(uvw_dev, rphys, n_vrows, uvw4, s);
+  }
 
   // ---- keys, one stable sort, bucket offsets ----
   atlas_.n_vis = n_virtual;
@@ -224,10 +250,8 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   uint32_t* vals = static_cast<uint32_t*>(A.Get("atlas.perm", n_virtual * 4));
   if (n_virtual) {
     // the sort's first pass generates the index values itself
-    LaunchFacetKeys(dfac, dvb, F, uvw_dev, scale_, n_chan, n_virtual, pa.ntu, pa.P0, T, keys,
-                    nullptr, s);
-    uint32_t* scratch =
-        static_cast<uint32_t*>(A.Get("sort.scratch", RadixSortScratchWords(n_virtual) * 4));
+    LaunchFacetKeysRenumbered(dfac, dvrb, F, rperm, uvw4, scale_, n_vrows, n_chan, pa.ntu, pa.P0,
+                              T, keys, s);
     uint32_t* keys2 = static_cast<uint32_t*>(A.Get("atlas.keys2", n_virtual * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(A.Get("atlas.perm2", n_virtual * 4));
     if (DeviceRadixSort(keys, vals, keys2, vals2, n_virtual, pa.key_bits, scratch, s, true)) {
@@ -246,7 +270,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   atlas_.coord = static_cast<float4*>(A.Get("atlas.coord", nvalid * sizeof(float4)));
   atlas_.phasor = static_cast<float2*>(A.Get("atlas.phasor", nvalid * sizeof(float2)));
   perm_phys_ = static_cast<uint32_t*>(A.Get("atlas.perm_phys", nvalid * 4));
-  LaunchFacetCoords(dfac, dvb, F, uvw_dev, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
+  LaunchFacetCoords(dfac, dvrb, F, rperm, uvw4, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
                     atlas_.coord, atlas_.phasor, s);
   eng_.BuildWorkItems(atlas_);
   for (int f = 0; f < F; ++f) eng_.BuildCorrection(fvs_[f]);

@@ -129,8 +129,9 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
 
 // Digit histogram of one run of kRsSub tiles: counts[d * nblocks + block].
 // Wave w reads elements [tile + w * 1024, tile + (w + 1) * 1024) of every
-// tile of the run, four keys per 16-byte load (tiles start 32 KiB apart, so
-// the loads are aligned).
+// tile of the run, four keys per 16-byte load in full runs (tiles start
+// 32 KiB apart, so the loads are aligned); the ragged last run reads keys
+// one by one.
 __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
     const uint32_t* __restrict__ keys, size_t n, int shift, uint32_t mask,
     uint32_t* __restrict__ counts, uint32_t nblocks) {
@@ -140,31 +141,42 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
   constexpr int kVec = kRsPerThread / 4;
-  for (int sub = 0; sub < kRsSub; ++sub) {
-    const size_t base = (size_t(blockIdx.x) * kRsSub + sub) * kRsTile +
-                        size_t(wave) * (kRsTile / kRsWaves);
-    if (base >= n) break;
-    uint4 q[kVec];
+  const size_t run0 = size_t(blockIdx.x) * kRsRun + size_t(wave) * (kRsTile / kRsWaves);
+  auto addr = [&](int sub, int it) {
+    return run0 + size_t(sub) * kRsTile + (size_t(it) * 64 + lane) * 4;
+  };
+  if (size_t(blockIdx.x + 1) * kRsRun <= n) {
+    // full run: branch-free, the next tile's keys in flight while the
+    // current ones are counted
+    uint4 q[kVec], qn[kVec];
 #pragma unroll
-    for (int it = 0; it < kVec; ++it) {
-      const size_t i = base + (size_t(it) * 64 + lane) * 4;
-      if (i + 3 < n) {
-        q[it] = *reinterpret_cast<const uint4*>(keys + i);
-      } else {
-        // ragged end: keys past n are not counted
-        q[it].x = i < n ? keys[i] : ~0u;
-        q[it].y = i + 1 < n ? keys[i + 1] : ~0u;
-        q[it].z = i + 2 < n ? keys[i + 2] : ~0u;
-        q[it].w = ~0u;
+    for (int it = 0; it < kVec; ++it) q[it] = *reinterpret_cast<const uint4*>(keys + addr(0, it));
+    for (int sub = 0; sub < kRsSub; ++sub) {
+      if (sub + 1 < kRsSub) {
+#pragma unroll
+        for (int it = 0; it < kVec; ++it)
+          qn[it] = *reinterpret_cast<const uint4*>(keys + addr(sub + 1, it));
       }
+#pragma unroll
+      for (int it = 0; it < kVec; ++it) {
+        atomicAdd(&h[wave][(q[it].x >> shift) & mask], 1u);
+        atomicAdd(&h[wave][(q[it].y >> shift) & mask], 1u);
+        atomicAdd(&h[wave][(q[it].z >> shift) & mask], 1u);
+        atomicAdd(&h[wave][(q[it].w >> shift) & mask], 1u);
+      }
+#pragma unroll
+      for (int it = 0; it < kVec; ++it) q[it] = qn[it];
     }
+  } else {
+    for (int sub = 0; sub < kRsSub; ++sub) {
+      if (addr(sub, 0) - size_t(lane) * 4 >= n) break;
 #pragma unroll
-    for (int it = 0; it < kVec; ++it) {
-      const size_t i = base + (size_t(it) * 64 + lane) * 4;
-      const uint32_t kk[4] = {q[it].x, q[it].y, q[it].z, q[it].w};
+      for (int it = 0; it < kVec; ++it) {
+        const size_t i = addr(sub, it);
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (i + c < n) atomicAdd(&h[wave][(kk[c] >> shift) & mask], 1u);
+        for (int c = 0; c < 4; ++c)
+          if (i + c < n) atomicAdd(&h[wave][(keys[i + c] >> shift) & mask], 1u);
+      }
     }
   }
   __syncthreads();

@@ -73,6 +73,12 @@ __global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nbloc
   }
 }
 
+// Sort key of one visibility: ((tile * P0 + p0) * T + dv) * T + du.
+__device__ __forceinline__ uint32_t VisKey(const VisLoc& L, const DevPlan& dp) {
+  const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
+  return (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
+}
+
 __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* __restrict__ uvw,
                                                      const double* __restrict__ scale,
                                                      size_t n_rows, size_t n_chan,
@@ -90,13 +96,37 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
       const float2 V = vis[i];
       valid = (V.x != 0.f) || (V.y != 0.f);
     }
-    if (valid) {
-      const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
-      const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
-      key = (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
-    }
+    if (valid)
+      key = VisKey(LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp), dp);
     keys[i] = key;
     if (vals) vals[i] = uint32_t(i);
+  }
+}
+
+// Keys in renumbered row order: element e = r * n_chan + ch is channel ch of
+// renumbered row r, i.e. of caller row rperm[r]; uvw4 holds the renumbered
+// rows padded to 32 bytes.  Keys are the same as KeysKernel's for the same
+// visibility, only their order differs.
+__global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
+    DevPlan dp, const double* __restrict__ uvw4, const uint32_t* __restrict__ rperm,
+    const double* __restrict__ scale, size_t n_rows, size_t n_chan,
+    const float2* __restrict__ vis, uint32_t* __restrict__ keys) {
+  const size_t n = n_rows * n_chan;
+  const double inv_chan = 1.0 / double(n_chan);
+  for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * kBlock) {
+    const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
+    uint32_t key = dp.nkeys;
+    bool valid = true;
+    if (vis) {
+      const float2 V = vis[size_t(rperm[r]) * n_chan + ch];
+      valid = (V.x != 0.f) || (V.y != 0.f);
+    }
+    if (valid) {
+      const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+      key = VisKey(LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp), dp);
+    }
+    keys[e] = key;
   }
 }
 
@@ -113,7 +143,14 @@ __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L
 // w-centre phasor exp(-2 pi i (u l_shift + v m_shift + w zc)) and, in the top
 // bit of perm[i], whether (u, v, w, V) was flipped to w >= 0.  The gather
 // and scatter of visibilities then need no uvw reads and no FP64 work.
-__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw,
+//
+// perm enters holding renumbered elements e = r * n_chan + ch (renumbered
+// row r = caller row rperm[r], rows in uvw4 padded to 32 bytes) and leaves
+// holding the caller's flat index rperm[r] * n_chan + ch.  Rows are
+// renumbered in uv order, so the rows read by neighbouring sorted
+// visibilities lie close together in uvw4.
+__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw4,
+                                                       const uint32_t* __restrict__ rperm,
                                                        const double* __restrict__ scale,
                                                        size_t n_chan, uint32_t* __restrict__ perm,
                                                        size_t n_valid, float4* __restrict__ coord,
@@ -121,12 +158,27 @@ __global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double*
   const double inv_chan = 1.0 / double(n_chan);
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
-    const uint32_t idx = perm[i];
-    const size_t row = DivSmall(idx, uint32_t(n_chan), inv_chan), ch = idx - row * n_chan;
-    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    const uint32_t e = perm[i];
+    const size_t r = DivSmall(e, uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
+    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
     coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
     phasor[i] = ShiftPhasor(dp, L, -1.f);
+    const uint32_t idx = rperm[r] * uint32_t(n_chan) + uint32_t(ch);
     perm[i] = idx | (L.flip ? kPermFlip : 0u);
+  }
+}
+
+// uvw4[r] = (uvw[rperm[r]], 0): the renumbered rows padded to 32 bytes.
+__global__ void __launch_bounds__(kBlock) PadUvwKernel(const double* __restrict__ uvw,
+                                                       const uint32_t* __restrict__ rperm,
+                                                       size_t n_rows,
+                                                       double* __restrict__ uvw4) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < 4 * n_rows;
+       i += size_t(gridDim.x) * kBlock) {
+    const size_t row = rperm[i >> 2];
+    const int k = int(i & 3);
+    uvw4[i] = k < 3 ? uvw[3 * row + k] : 0.0;
   }
 }
 
@@ -454,27 +506,70 @@ __global__ void __launch_bounds__(kBlock) FacetKeysKernel(
   }
 }
 
+// Keys of a facet batch in renumbered row order: element e = r * n_chan +
+// ch is channel ch of renumbered virtual row r, i.e. of virtual row
+// rperm[r] (virtual rows of facet f: [vrbase[f], vrbase[f + 1]), physical
+// row row_off_f + local); uvw4 holds the renumbered rows padded to 32 bytes.
+__global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
+    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
+    const uint32_t* __restrict__ rperm, const double* __restrict__ uvw4,
+    const double* __restrict__ scale, size_t n_vrows, size_t n_chan, int atlas_ntu,
+    int atlas_P0, int T, uint32_t* __restrict__ keys) {
+  const size_t n = n_vrows * n_chan;
+  const double inv_chan = 1.0 / double(n_chan);
+  for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * kBlock) {
+    const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
+    const int f = FacetOf(vrbase, F, rperm[r]);
+    const DevPlan& dp = fac[f].dp;
+    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
+    const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
+    const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
+    const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
+    keys[e] = (bucket * uint32_t(T) + uint32_t(L.dv)) * uint32_t(T) + uint32_t(L.du);
+  }
+}
+
+// Physical row of every renumbered virtual row: rphys[r] = row_off_f +
+// (rperm[r] - vrbase[f]).
+__global__ void __launch_bounds__(kBlock) FacetRowMapKernel(
+    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
+    const uint32_t* __restrict__ rperm, size_t n_vrows, uint32_t* __restrict__ rphys) {
+  for (size_t r = size_t(blockIdx.x) * kBlock + threadIdx.x; r < n_vrows;
+       r += size_t(gridDim.x) * kBlock) {
+    const uint32_t vr = rperm[r];
+    const int f = FacetOf(vrbase, F, vr);
+    rphys[r] = uint32_t(fac[f].row_off + (vr - vrbase[f]));
+  }
+}
+
 // Sorted coordinates and phasors of a facet batch (as CoordsKernel, with
-// the facet's own plan, phase centre shift and w centre).  perm keeps the
-// virtual index (the predict output layout), perm_phys gets the physical
-// index into the shared visibility buffer; both carry the flip bit.
+// the facet's own plan, phase centre shift and w centre).  perm enters with
+// sorted renumbered elements; it leaves with the virtual index (the predict
+// output layout) and perm_phys with the physical index into the shared
+// visibility buffer; both carry the flip bit.
 __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
-    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vbase, int F,
-    const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
-    uint32_t* __restrict__ perm, uint32_t* __restrict__ perm_phys, size_t n_valid,
-    float4* __restrict__ coord, float2* __restrict__ phasor) {
+    const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
+    const uint32_t* __restrict__ rperm, const double* __restrict__ uvw4,
+    const double* __restrict__ scale, size_t n_chan, uint32_t* __restrict__ perm,
+    uint32_t* __restrict__ perm_phys, size_t n_valid, float4* __restrict__ coord,
+    float2* __restrict__ phasor) {
+  const double inv_chan = 1.0 / double(n_chan);
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
-    const uint32_t v = perm[i];
-    const int f = FacetOf(vbase, F, v);
-    const size_t local = v - vbase[f];
-    const size_t row = fac[f].row_off + local / n_chan, ch = local % n_chan;
+    const uint32_t e = perm[i];
+    const size_t r = DivSmall(e, uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
+    const uint32_t vr = rperm[r];
+    const int f = FacetOf(vrbase, F, vr);
     const DevPlan& dp = fac[f].dp;
-    const VisLoc L = LocateVis(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], scale[ch], dp);
+    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
     coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
     phasor[i] = ShiftPhasor(dp, L, -1.f);
     const uint32_t flip = L.flip ? kPermFlip : 0u;
-    perm[i] = v | flip;
+    const size_t row = fac[f].row_off + (vr - vrbase[f]);
+    perm[i] = uint32_t(size_t(vr) * n_chan + ch) | flip;
     perm_phys[i] = uint32_t(row * n_chan + ch) | flip;
   }
 }
@@ -517,22 +612,59 @@ void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F
   WG_HIP(hipGetLastError());
 }
 
-void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vbase, int F,
-                       const double* uvw, const double* scale, size_t n_chan, uint32_t* perm,
-                       uint32_t* perm_phys, size_t n_valid, float4* coord, float2* phasor,
-                       hipStream_t s) {
-  if (!n_valid) return;
-  FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vbase, F, uvw, scale, n_chan, perm,
-                                                        perm_phys, n_valid, coord, phasor);
+void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vrbase, int F,
+                               const uint32_t* rperm, const double* uvw4, const double* scale,
+                               size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0, int T,
+                               uint32_t* keys, hipStream_t s) {
+  if (!n_vrows) return;
+  WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
+  FacetKeysRenumberedKernel<<<Blocks(n_vrows * n_chan), kBlock, 0, s>>>(
+      fac, vrbase, F, rperm, uvw4, scale, n_vrows, n_chan, atlas_ntu, atlas_P0, T, keys);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchCoords(const DevPlan& dp, const double* uvw, const double* scale, size_t n_chan,
-                  uint32_t* perm, size_t n_valid, float4* coord, float2* phasor,
-                  hipStream_t s) {
+void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, int F,
+                       const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, hipStream_t s) {
+  if (!n_vrows) return;
+  FacetRowMapKernel<<<Blocks(n_vrows), kBlock, 0, s>>>(fac, vrbase, F, rperm, n_vrows, rphys);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, int F,
+                       const uint32_t* rperm, const double* uvw4, const double* scale,
+                       size_t n_chan, uint32_t* perm, uint32_t* perm_phys, size_t n_valid,
+                       float4* coord, float2* phasor, hipStream_t s) {
   if (!n_valid) return;
-  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw, scale, n_chan, perm, n_valid, coord,
-                                                  phasor);
+  WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
+  FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vrbase, F, rperm, uvw4, scale, n_chan,
+                                                        perm, perm_phys, n_valid, coord, phasor);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchPadUvw(const double* uvw, const uint32_t* rperm, size_t n_rows, double* uvw4,
+                  hipStream_t s) {
+  if (!n_rows) return;
+  PadUvwKernel<<<Blocks(4 * n_rows), kBlock, 0, s>>>(uvw, rperm, n_rows, uvw4);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
+                          const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
+                          uint32_t* keys, hipStream_t s) {
+  if (!n_rows) return;
+  WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
+  KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, rperm, scale, n_rows,
+                                                                  n_chan, vis, keys);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchCoords(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
+                  const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
+                  float4* coord, float2* phasor, hipStream_t s) {
+  if (!n_valid) return;
+  WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
+  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw4, rperm, scale, n_chan, perm, n_valid,
+                                                  coord, phasor);
   WG_HIP(hipGetLastError());
 }
 
