@@ -127,6 +127,22 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
   }
 }
 
+// Adds one to h[d] for every active lane, one LDS atomic per distinct digit
+// of the wave: lanes with equal digits are found by ballots over the digit
+// bits and their lowest lane adds their count.  Keys in uv order put long
+// runs of equal digits into one wave, which would serialise per-lane
+// atomics on one address.
+__device__ __forceinline__ void WaveCount(uint32_t* h, uint32_t d, int bits, uint64_t active,
+                                          int lane) {
+  uint64_t peers = active;
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    peers &= bit ? bb : ~bb;
+  }
+  if (int(__ffsll((long long)peers)) - 1 == lane) atomicAdd(&h[d], uint32_t(__popcll(peers)));
+}
+
 // Digit histogram of one run of kRsSub tiles: counts[d * nblocks + block].
 // Wave w reads elements [tile + w * 1024, tile + (w + 1) * 1024) of every
 // tile of the run, four keys per 16-byte load in full runs (tiles start
@@ -138,6 +154,7 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   __shared__ uint32_t h[kRsWaves][kRsMaxBuckets];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int R = int(mask) + 1;
+  const int bits = __popc(mask);
   for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
   constexpr int kVec = kRsPerThread / 4;
@@ -159,10 +176,10 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
       }
 #pragma unroll
       for (int it = 0; it < kVec; ++it) {
-        atomicAdd(&h[wave][(q[it].x >> shift) & mask], 1u);
-        atomicAdd(&h[wave][(q[it].y >> shift) & mask], 1u);
-        atomicAdd(&h[wave][(q[it].z >> shift) & mask], 1u);
-        atomicAdd(&h[wave][(q[it].w >> shift) & mask], 1u);
+        WaveCount(h[wave], (q[it].x >> shift) & mask, bits, ~0ull, lane);
+        WaveCount(h[wave], (q[it].y >> shift) & mask, bits, ~0ull, lane);
+        WaveCount(h[wave], (q[it].z >> shift) & mask, bits, ~0ull, lane);
+        WaveCount(h[wave], (q[it].w >> shift) & mask, bits, ~0ull, lane);
       }
 #pragma unroll
       for (int it = 0; it < kVec; ++it) q[it] = qn[it];
@@ -174,8 +191,11 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
       for (int it = 0; it < kVec; ++it) {
         const size_t i = addr(sub, it);
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (i + c < n) atomicAdd(&h[wave][(keys[i + c] >> shift) & mask], 1u);
+        for (int c = 0; c < 4; ++c) {
+          const bool act = i + c < n;
+          const uint32_t d = act ? (keys[i + c] >> shift) & mask : 0u;
+          WaveCount(h[wave], d, bits, __ballot(act), lane);
+        }
       }
     }
   }
