@@ -32,8 +32,8 @@ def invert(p, eps, ls=0.0, ms=0.0, n=None):
 
 # ------------------------------------------------------------------- FFT
 
-@pytest.mark.parametrize("n", [16, 60, 81, 96, 162, 1000, 1458, 2048, 2560, 3000, 4096, 10240,
-                               13122, 13500, 13824, 14580, 16384])
+@pytest.mark.parametrize("n", [16, 18, 60, 81, 96, 162, 486, 1000, 1458, 2048, 2560, 3000, 4096,
+                               4374, 10240, 13122, 13500, 13824, 14580, 16384])
 def test_lds_fft_matches_numpy(n):
     rng = np.random.default_rng(n)
     x = (rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))).astype(np.complex64)

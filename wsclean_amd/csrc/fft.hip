@@ -1,4 +1,4 @@
-// Hand-written LDS Stockham FFT (radix 2/3/4/5, FP32) and the per-plane
+// Hand-written LDS Stockham FFT (radix 2/3/4/5/8/9/18, FP32) and the per-plane
 // line kernels of the w-gridder:
 //
 //   invert : box rows --FFT(u)--> crop columns --transpose--> image columns
@@ -181,10 +181,44 @@ __device__ __forceinline__ void Dft9(float2* v) {
     v[k1 + 6] = z[2];
   }
 }
+// Radix 18 as 2 x 9 (n = 9 n1 + n2, k = k1 + 2 k2): radix-2 DFTs over n1
+// for each n2, twiddles w18^(n2 k1), radix-9 DFTs over n2 for each k1.
+template <bool INV>
+__device__ __forceinline__ void Dft18(float2* v) {
+  // cos, sin of 2 pi n2 / 18 for n2 = 0 .. 8
+  constexpr float c[9] = {1.0f, 0.93969262078590838405f, 0.76604444311897803520f, 0.5f,
+                          0.17364817766693034885f, -0.17364817766693034885f, -0.5f,
+                          -0.76604444311897803520f, -0.93969262078590838405f};
+  constexpr float s[9] = {0.0f, 0.34202014332566873304f, 0.64278760968653932632f,
+                          0.86602540378443864676f, 0.98480775301220805936f,
+                          0.98480775301220805936f, 0.86602540378443864676f,
+                          0.64278760968653932632f, 0.34202014332566873304f};
+  const float sg = INV ? 1.f : -1.f;
+  float2 a[9], b[9];
+  a[0] = cadd(v[0], v[9]);
+  b[0] = csub(v[0], v[9]);
+#pragma unroll
+  for (int n2 = 1; n2 < 9; ++n2) {
+    a[n2] = cadd(v[n2], v[9 + n2]);
+    b[n2] = cmul(csub(v[n2], v[9 + n2]), make_float2(c[n2], sg * s[n2]));
+  }
+  Dft9<INV>(a);
+  Dft9<INV>(b);
+#pragma unroll
+  for (int k2 = 0; k2 < 9; ++k2) {
+    v[2 * k2] = a[k2];
+    v[2 * k2 + 1] = b[k2];
+  }
+}
+
 template <>
 __device__ __forceinline__ void Dft<9, false>(float2* v) { Dft9<false>(v); }
 template <>
 __device__ __forceinline__ void Dft<9, true>(float2* v) { Dft9<true>(v); }
+template <>
+__device__ __forceinline__ void Dft<18, false>(float2* v) { Dft18<false>(v); }
+template <>
+__device__ __forceinline__ void Dft<18, true>(float2* v) { Dft18<true>(v); }
 
 // Twiddle exp(-2 pi i k / n) from the two-level LDS table (lo[k % 128] *
 // hi[k / 128]; FP32 product of correctly rounded factors).
@@ -223,13 +257,20 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
         // products of at most four factors (squarings and one extra factor),
         // so the rounding stays a few ulp while the LDS reads drop from
         // 2 (R - 1) to 2 per butterfly
-        float2 w[R];
+        constexpr int RW = R == 18 ? 9 : R;  // radix 18: w^(9 + r) = w^9 w^r
+        float2 w[RW];
         w[1] = Twiddle(twl, k * tws);
         if (INV) w[1].y = -w[1].y;
 #pragma unroll
-        for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul(w[r - 1], w[1]) : cmul(w[r / 2], w[r / 2]);
+        for (int r = 2; r < RW; ++r) w[r] = (r & 1) ? cmul(w[r - 1], w[1]) : cmul(w[r / 2], w[r / 2]);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
+        for (int r = 1; r < RW; ++r) v[b][r] = cmul(v[b][r], w[r]);
+        if constexpr (R == 18) {
+          const float2 w9 = cmul(w[8], w[1]);
+          v[b][9] = cmul(v[b][9], w9);
+#pragma unroll
+          for (int r = 1; r < 9; ++r) v[b][9 + r] = cmul(cmul(v[b][9 + r], w9), w[r]);
+        }
       }
       Dft<R, INV>(v[b]);
     }
@@ -264,6 +305,7 @@ __device__ void LdsFFT(float2* buf, const FFTDesc& d, const float2* twl) {
     switch (R) {
       case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
       case 9: StockhamStage<9, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
+      case 18: StockhamStage<18, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
       case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
       case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
       case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;

@@ -106,6 +106,15 @@ std::vector<int> FFTRadices(int n) {
   while (m % 3 == 0) { r.push_back(3); m /= 3; }
   while (m % 5 == 0) { r.push_back(5); m /= 5; }
   WG_CHECK(m == 1, "FFT length " + std::to_string(n) + " is not 2^a 3^b 5^c");
+  // a lone factor 2 and a 9 become one radix-18 stage (placed last): one
+  // LDS pass less
+  auto two = std::find(r.begin(), r.end(), 2);
+  auto nine = std::find(r.begin(), r.end(), 9);
+  if (two != r.end() && nine != r.end()) {
+    r.erase(two);
+    r.erase(std::find(r.begin(), r.end(), 9));
+    r.push_back(18);
+  }
   return r;
 }
 
