@@ -219,107 +219,135 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
 //
 // IDX: the values are the element indices (vin is not read), which saves
 // writing and reading the identity permutation in the first pass.
+// Shared memory of one scatter block.
+struct ScatterLds {
+  uint32_t cnt[kRsWaves][kRsMaxBuckets];  // per-wave counts, then offsets
+  uint32_t start[kRsMaxBuckets + 1];      // tile-local start of each digit
+  uint32_t gbase[kRsMaxBuckets];          // global position - tile position
+  uint32_t run_base[kRsMaxBuckets];       // next global position of each digit
+  uint32_t buf[kRsTile];                  // keys, then values, in tile order
+  uint16_t dig[kRsTile];                  // digit of each tile position
+};
+
+// One tile of the scatter (see RadixScatterKernel).  FULL: the tile holds
+// kRsTile elements, so no element needs a bounds test.  BITS: the digit
+// width when known at compile time (0: the runtime `bits`).
+template <bool IDX, bool FULL, int BITS>
+__device__ __forceinline__ void ScatterTile(ScatterLds& L, const uint32_t* __restrict__ kin,
+                                            const uint32_t* __restrict__ vin,
+                                            uint32_t* __restrict__ kout,
+                                            uint32_t* __restrict__ vout, uint32_t n,
+                                            uint32_t tile, int shift, int bits_rt) {
+  const int bits = BITS > 0 ? BITS : bits_rt;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t mask = (1u << bits) - 1u;
+  const int R = 1 << bits;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads)
+    (&L.cnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t base = tile + uint32_t(wave) * (kRsTile / kRsWaves);
+  auto in = [&](int it) { return FULL || base + uint32_t(it) * 64 + lane < n; };
+  uint32_t key[kRsPerThread], pos[kRsPerThread], val[kRsPerThread];
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const uint32_t i = base + uint32_t(it) * 64 + lane;
+    key[it] = in(it) ? kin[i] : 0u;
+  }
+  // the values are loaded now, so their latency overlaps the ranking
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const uint32_t i = base + uint32_t(it) * 64 + lane;
+    val[it] = IDX ? i : (in(it) ? vin[i] : 0u);
+  }
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    const bool act = in(it);
+    const uint32_t d = (key[it] >> shift) & mask;
+    uint64_t peers = FULL ? ~0ull : __ballot(act);
+    auto narrow = [&](int b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    };
+    if constexpr (BITS > 0) {
+#pragma unroll
+      for (int b = 0; b < BITS; ++b) narrow(b);
+    } else {
+      for (int b = 0; b < bits; ++b) narrow(b);
+    }
+    const uint32_t r = uint32_t(__popcll(peers & lt));
+    const uint32_t c = act ? L.cnt[wave][d] : 0u;
+    pos[it] = c + r;
+    if (act && r == 0) L.cnt[wave][d] = c + uint32_t(__popcll(peers));
+  }
+  __syncthreads();
+  // wave offsets inside each digit, digit totals, then tile starts
+  for (int d = threadIdx.x; d < R; d += kRsThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kRsWaves; ++w) {
+      const uint32_t t = L.cnt[w][d];
+      L.cnt[w][d] = run;
+      run += t;
+    }
+    L.start[d] = run;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;
+    const uint32_t v = d < R ? L.start[d] : 0u;
+    uint32_t ex = 0;
+    const uint32_t total = BlockExclusiveScan(v, ex);
+    if (d < R) {
+      L.start[d] = ex;
+      L.gbase[d] = L.run_base[d] - ex;
+    }
+    if (d == 0) L.start[R] = total;
+  }
+  __syncthreads();
+  const int count = FULL ? kRsTile : int(min(uint32_t(kRsTile), n - tile));
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it) {
+    if (in(it)) {
+      const uint32_t d = (key[it] >> shift) & mask;
+      const uint32_t p = L.start[d] + L.cnt[wave][d] + pos[it];
+      pos[it] = p;
+      L.buf[p] = key[it];
+      L.dig[p] = uint16_t(d);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < count; i += kRsThreads) kout[L.gbase[L.dig[i]] + i] = L.buf[i];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kRsPerThread; ++it)
+    if (in(it)) L.buf[pos[it]] = val[it];
+  __syncthreads();
+  for (int i = threadIdx.x; i < count; i += kRsThreads) vout[L.gbase[L.dig[i]] + i] = L.buf[i];
+  // the next tile's digits follow this tile's
+  for (int d = threadIdx.x; d < R; d += kRsThreads) L.run_base[d] += L.start[d + 1] - L.start[d];
+  __syncthreads();
+}
+
 // Two blocks per CU (LDS) need four waves per SIMD, i.e. at most 128 VGPRs.
-template <bool IDX>
+template <bool IDX, int BITS>
 __global__ void __launch_bounds__(kRsThreads)
     __attribute__((amdgpu_waves_per_eu(4, 4))) RadixScatterKernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
     int bits, const uint32_t* __restrict__ offs, uint32_t nblocks) {
-  __shared__ uint32_t cnt[kRsWaves][kRsMaxBuckets];  // per-wave counts, then offsets
-  __shared__ uint32_t start[kRsMaxBuckets + 1];      // tile-local start of each digit
-  __shared__ uint32_t gbase[kRsMaxBuckets];          // global position - tile position
-  __shared__ uint32_t run_base[kRsMaxBuckets];       // next global position of each digit
-  __shared__ uint32_t buf[kRsTile];                  // keys, then values, in tile order
-  __shared__ uint16_t dig[kRsTile];                  // digit of each tile position
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t mask = (1u << bits) - 1u;
-  const int R = 1 << bits;
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int d = threadIdx.x; d < R; d += kRsThreads) run_base[d] = offs[size_t(d) * nblocks + blockIdx.x];
+  __shared__ ScatterLds L;
+  const int R = 1 << (BITS > 0 ? BITS : bits);
+  for (int d = threadIdx.x; d < R; d += kRsThreads)
+    L.run_base[d] = offs[size_t(d) * nblocks + blockIdx.x];
   for (int sub = 0; sub < kRsSub; ++sub) {
     const uint32_t tile = (blockIdx.x * kRsSub + sub) * kRsTile;
     if (tile >= n) break;  // uniform over the block
-    for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads)
-      (&cnt[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t base = tile + uint32_t(wave) * (kRsTile / kRsWaves);
-    uint32_t key[kRsPerThread], pos[kRsPerThread], val[kRsPerThread];
-#pragma unroll
-    for (int it = 0; it < kRsPerThread; ++it) {
-      const uint32_t i = base + uint32_t(it) * 64 + lane;
-      key[it] = i < n ? kin[i] : 0u;
-    }
-    // the values are loaded now, so their latency overlaps the ranking
-#pragma unroll
-    for (int it = 0; it < kRsPerThread; ++it) {
-      const uint32_t i = base + uint32_t(it) * 64 + lane;
-      val[it] = IDX ? i : (i < n ? vin[i] : 0u);
-    }
-#pragma unroll
-    for (int it = 0; it < kRsPerThread; ++it) {
-      const bool act = base + uint32_t(it) * 64 + lane < n;
-      const uint32_t d = (key[it] >> shift) & mask;
-      uint64_t peers = __ballot(act);
-      for (int b = 0; b < bits; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        peers &= bit ? bb : ~bb;
-      }
-      const uint32_t r = uint32_t(__popcll(peers & lt));
-      const uint32_t c = act ? cnt[wave][d] : 0u;
-      pos[it] = c + r;
-      if (act && r == 0) cnt[wave][d] = c + uint32_t(__popcll(peers));
-    }
-    __syncthreads();
-    // wave offsets inside each digit, digit totals, then tile starts
-    for (int d = threadIdx.x; d < R; d += kRsThreads) {
-      uint32_t run = 0;
-#pragma unroll
-      for (int w = 0; w < kRsWaves; ++w) {
-        const uint32_t t = cnt[w][d];
-        cnt[w][d] = run;
-        run += t;
-      }
-      start[d] = run;
-    }
-    __syncthreads();
-    {
-      const int d = threadIdx.x;
-      const uint32_t v = d < R ? start[d] : 0u;
-      uint32_t ex = 0;
-      const uint32_t total = BlockExclusiveScan(v, ex);
-      if (d < R) {
-        start[d] = ex;
-        gbase[d] = run_base[d] - ex;
-      }
-      if (d == 0) start[R] = total;
-    }
-    __syncthreads();
-    const int count = int(min(uint32_t(kRsTile), n - tile));
-#pragma unroll
-    for (int it = 0; it < kRsPerThread; ++it) {
-      if (base + uint32_t(it) * 64 + lane < n) {
-        const uint32_t d = (key[it] >> shift) & mask;
-        const uint32_t p = start[d] + cnt[wave][d] + pos[it];
-        pos[it] = p;
-        buf[p] = key[it];
-        dig[p] = uint16_t(d);
-      }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < count; i += kRsThreads) kout[gbase[dig[i]] + i] = buf[i];
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < kRsPerThread; ++it) {
-      const uint32_t i = base + uint32_t(it) * 64 + lane;
-      if (i < n) buf[pos[it]] = val[it];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < count; i += kRsThreads) vout[gbase[dig[i]] + i] = buf[i];
-    // the next tile's digits follow this tile's
-    for (int d = threadIdx.x; d < R; d += kRsThreads) run_base[d] += start[d + 1] - start[d];
-    __syncthreads();
+    if (n - tile >= uint32_t(kRsTile))
+      ScatterTile<IDX, true, BITS>(L, kin, vin, kout, vout, n, tile, shift, bits);
+    else
+      ScatterTile<IDX, false, BITS>(L, kin, vin, kout, vout, n, tile, shift, bits);
   }
 }
 
@@ -395,12 +423,15 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
     RadixHistKernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
         kin, n, shift, (1u << bits) - 1u, counts, uint32_t(nblocks));
     DeviceExclusiveScan(counts, offs, ncounts, scan_scratch, s, nullptr);
-    if (index_vals && shift == 0)
-      RadixScatterKernel<true><<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
-          kin, nullptr, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
+    const bool idx = index_vals && shift == 0;
+    auto launch = [&](auto kernel) {
+      kernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
+          kin, idx ? nullptr : vin, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
+    };
+    if (bits == kRsMaxBits)
+      idx ? launch(RadixScatterKernel<true, kRsMaxBits>) : launch(RadixScatterKernel<false, kRsMaxBits>);
     else
-      RadixScatterKernel<false><<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
-          kin, vin, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
+      idx ? launch(RadixScatterKernel<true, 0>) : launch(RadixScatterKernel<false, 0>);
     std::swap(kin, ko);
     std::swap(vin, vo);
     in_alt = !in_alt;
