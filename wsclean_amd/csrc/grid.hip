@@ -264,18 +264,23 @@ __device__ __forceinline__ void WaveLdsSync() {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
+// Lane exchanges of the gridding step.  Every source lane of a quad
+// broadcast or of a rotation inside a 16-lane row is a valid lane, and the
+// step runs with all 64 lanes active, so bound_ctrl (zero for an invalid
+// source) never applies; it is set because it lets the compiler fold the
+// move into the VALU instruction that consumes it (one DPP-modified op).
 __device__ __forceinline__ float DppQuadBcast(float v, int m) {
   // quad_perm [m, m, m, m]
   switch (m) {
-    case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xf, 0xf, false));
-    case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xf, 0xf, false));
-    case 2: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xaa, 0xf, 0xf, false));
-    default: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xff, 0xf, 0xf, false));
+    case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xf, 0xf, true));
+    case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xf, 0xf, true));
+    case 2: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xaa, 0xf, 0xf, true));
+    default: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xff, 0xf, 0xf, true));
   }
 }
 
 __device__ __forceinline__ float DppRowRor8(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, true));
 }
 
 // ES kernel at x = xc - f * c2 (xc = tap abscissa * c2, nc2 = -c2), with the
@@ -444,12 +449,6 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         const float a1 = take1 ? (hi ? k11 : k11o) * c1.z : 0.f;
         const float u0 = take0 ? (hi ? k10o : k10) : 0.f;
         const float u1 = take1 ? (hi ? k11o : k11) : 0.f;
-        float b0[4], b1[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          b0[m] = u0 * DppQuadBcast(kw0, m);
-          b1[m] = u1 * DppQuadBcast(kw1, m);
-        }
         if (npos - bbase >= uint32_t(kStage) && npos < w1) {
           // the next group starts in batch b + 1: batch b + 2 replaces b
           StoreStage(ring, bslot * kStage + lane, pc, src.Value());
@@ -464,13 +463,17 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           load(bbase + 2 * kStage);
           WaveLdsSync();
         }
+        // B operands next to their use, in the same basic block, so that
+        // each quad broadcast folds into its multiply (one DPP-modified op)
 #pragma unroll
         for (int m = 0; m < 4; ++m)
-          acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0[m], acc[m], 0, 0, 0);
+          acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, DppQuadBcast(kw0, m) * u0, acc[m], 0,
+                                                        0, 0);
         if (lead > 4) {
 #pragma unroll
           for (int m = 0; m < 4; ++m)
-            acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[m], acc[m], 0, 0, 0);
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, DppQuadBcast(kw1, m) * u1, acc[m],
+                                                          0, 0, 0);
         }
         pos = npos;
       };
