@@ -364,11 +364,41 @@ __device__ __forceinline__ double PlaneW(const DevPlan& dp, int p) {
   return dp.w_lo + double(p - (dp.W / 2 - 1)) * dp.dw;
 }
 
+// sin(pi x), cos(pi x) for |x| <= 1: x = q / 2 + t with q = rint(2 x) and
+// |t| <= 1/4, the Taylor series of sin(pi t) (to t^9) and cos(pi t) (to
+// t^10) -- truncation below 2e-9, FP32 Horner evaluation within 1e-7 --
+// and the quadrant q mod 4 rotating (sin, cos).  The general sincospif
+// spends about twice the instructions on range handling this argument does
+// not need.
+__device__ __forceinline__ void SinCosPiSmall(float x, float& s, float& c) {
+  const float q = rintf(2.0f * x);
+  const float t = fmaf(q, -0.5f, x);
+  const float u = t * t;
+  float sp = 0.082145886611128191f;
+  sp = fmaf(sp, u, -0.59926452932079188f);
+  sp = fmaf(sp, u, 2.5501640398773451f);
+  sp = fmaf(sp, u, -5.1677127800499694f);
+  sp = fmaf(sp, u, 3.1415926535897931f);
+  const float st = sp * t;
+  float cp = -0.025806891390014051f;
+  cp = fmaf(cp, u, 0.23533063035889312f);
+  cp = fmaf(cp, u, -1.3352627688545893f);
+  cp = fmaf(cp, u, 4.0587121264167676f);
+  cp = fmaf(cp, u, -4.934802200544679f);
+  cp = fmaf(cp, u, 1.0f);
+  // q = 0: (st, cp); 1: (cp, -st); 2: (-st, -cp); 3: (-cp, st)
+  const int qi = int(q) & 3;
+  const float ss = (qi & 1) ? cp : st;
+  const float cc = (qi & 1) ? st : cp;
+  s = (qi & 2) ? -ss : ss;
+  c = ((qi + 1) & 2) ? -cc : cc;
+}
+
 // exp(sign * 2 pi i * cycles) with the integer part removed in FP64.
 __device__ __forceinline__ float2 Phasor(double cycles, float sign) {
   const double f = cycles - rint(cycles);
   float s, c;
-  sincospif(2.0f * float(f), &s, &c);
+  SinCosPiSmall(2.0f * float(f), s, c);
   return make_float2(c, sign * s);
 }
 
