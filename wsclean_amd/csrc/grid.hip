@@ -598,16 +598,19 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
 //
 // Inside a run the visibilities read the same W x W x W block G of the box:
 //   V_j = sum_t kw_j[t] sum_l kv_j[l] sum_k G[t][l][k] ku_j[k].
-// The u contraction is a GEMM with the block as the A operand: for a group
-// of 16 visibilities and each plane t, two FP32 MFMA 16x16x4 steps (k = 0..3,
-// 4..7) give  D_t[(l,c)][j] = sum_k G[t][l][k].c ku_j[k].  Lane L holds
-// D_t[4 (L/16) + r][L % 16], i.e. visibility j = L % 16, component
-// c = L / 32 and taps l = 4 (L/16 % 2) + r; it finishes the v and w
-// contractions in registers and the four lanes of a visibility combine
-// their parts with lane exchanges.  The 16 A registers are reloaded from the
-// LDS box block only when the run changes.  Kernel values: each lane
-// evaluates two u taps (its B operand), two v taps and two w taps and gets
-// the other taps its sums need from the lanes of the same visibility.
+// The u and v contractions together are one GEMM with the block as the A
+// operand: for a group of 16 visibilities, 16 FP32 MFMA 16x16x4 steps over
+// K = (k, l) give  D[(t,c)][j] = sum_{k,l} G[t][l][k].c ku_j[k] kv_j[l],
+// with A[(t,c)][(k,l)] = G[t][l][k].c and B[(k,l)][j] = ku_j[k] kv_j[l].
+// Step s covers (k, l) = (4 (s % 2) + g, s / 2) for g = 0..3.  Lane L holds
+// D[4 (L/16) + r][L % 16]: visibility j = L % 16 and rows (t, c) =
+// (2 (L/16) + r/2, r % 2), so only the w contraction (two planes per lane)
+// remains in registers, and the four lanes of a visibility add their parts
+// with lane exchanges.  The 16 A registers are reloaded from the LDS box
+// block only when the run changes.  Kernel values: each lane evaluates two
+// u taps (g, 4 + g), two v taps and the two w taps (2g, 2g + 1) of its own
+// rows, and gets the other six v taps of its visibility from the lanes of
+// the same visibility.
 // Coordinates and the accumulated visibilities are staged per wave through
 // an LDS ring with run-start masks, as in the gridder.
 constexpr int kDegridGroup = 16;
@@ -637,19 +640,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
                        degrid_lds + ((ncell + 1) & ~1)) + kMfmaWaves * (2 * kStage)) +
                    wave * (2 * kStage);
   const int j = lane & 15;        // visibility of the group (B column, D column)
-  const int g = lane >> 4;        // B row k = 4 s + g; D rows 4 g + r
-  const int cc = g >> 1;          // D component of the lane: 0 re, 1 im
-  const int lh = g & 1;           // D v taps l = 4 lh + r
+  const int g = lane >> 4;        // B row (k, l) = (4 (s % 2) + g, s / 2); D rows 4 g + r
   const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
   // abscissae (times 2/W) of the taps this lane evaluates
   const float xu0 = float(g - W / 2 + 1) * c2, xu1 = float(4 + g - W / 2 + 1) * c2;
-  const int lv0 = 4 * lh + 2 * (g >> 1);       // v taps lv0, lv0 + 1
-  const float xv0 = float(lv0 - W / 2 + 1) * c2, xv1 = float(lv0 + 1 - W / 2 + 1) * c2;
-  const float xw0 = float(2 * g - W / 2 + 1) * c2, xw1 = float(2 * g + 1 - W / 2 + 1) * c2;
-  const bool u0_ok = g < W, u1_ok = 4 + g < W, v0_ok = lv0 < W, v1_ok = lv0 + 1 < W;
-  const bool w0_ok = 2 * g < W, w1_ok = 2 * g + 1 < W;
-  // A operand lane mapping: A[i = L % 16][kk = L / 16] = G[t][l = i % 8][4 s + kk].(i / 8)
-  const int a_l = lane & 7, a_c = (lane >> 3) & 1, a_k = lane >> 4;
+  // v and w taps 2 g, 2 g + 1
+  const float x20 = float(2 * g - W / 2 + 1) * c2, x21 = float(2 * g + 1 - W / 2 + 1) * c2;
+  const bool u0_ok = g < W, u1_ok = 4 + g < W, t0_ok = 2 * g < W, t1_ok = 2 * g + 1 < W;
+  // A operand lane mapping: A[i = L % 16][g] of step s =
+  // G[t = i / 2][l = s / 2][k = 4 (s % 2) + g].(i % 2)
+  const int a_t = (lane & 15) >> 1, a_c = lane & 1;
   const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
   for (;;) {
     if (threadIdx.x == 0) next_item = atomicAdd(queue, 1u);
@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
       uint32_t bbase = w0, bslot = 0;
       stage(w0 + 2 * kStage, pc, pa);
       WaveLdsSync();
-      float areg[8][2];  // A operands of the current run: [t][s]
+      float areg[16];  // A operands of the current run, one per step
       int cur = -1;
       uint32_t pos = w0;
       float4 cv = ring_c[j];
@@ -713,18 +713,17 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           cur = __builtin_amdgcn_readlane(kb0, int(o));
           const int du = cur & 255, dv = (cur >> 8) & 255;
           const int q0 = ((cur >> 16) & 0x7fff) - sub_lo;
+          const int q = q0 + a_t;
+          const bool q_ok = a_t < W && q >= 0 && q < sub_n;
 #pragma unroll
-          for (int t = 0; t < 8; ++t) {
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-              const int q = q0 + t, k = 4 * s + a_k;
-              float v = 0.f;
-              if (t < W && a_l < W && k < W && q >= 0 && q < sub_n) {
-                const float2 gv = sub[(q * E + dv + a_l) * E + du + k];
-                v = a_c ? gv.y : gv.x;
-              }
-              areg[t][s] = v;
+          for (int st = 0; st < 16; ++st) {
+            const int l = st >> 1, k = 4 * (st & 1) + g;
+            float v = 0.f;
+            if (q_ok && l < W && k < W) {
+              const float2 gv = sub[(q * E + dv + l) * E + du + k];
+              v = a_c ? gv.y : gv.x;
             }
+            areg[st] = v;
           }
         }
         // bit d - 1 of win: visibility pos + d starts a run
@@ -736,40 +735,42 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         const float4 cn = ring_c[ns];
         const float2 an = ring_a[ns];
         const bool take = j < lead;
-        // B operands: u taps 4 s + g of visibility j
+        // B operands: u taps g (even steps) and 4 + g (odd steps) of
+        // visibility j, times its v tap of the step
         const float bu0 = (take && u0_ok) ? EsKernelAt(cv.x, nc2, xu0, b2) : 0.f;
         const float bu1 = (take && u1_ok) ? EsKernelAt(cv.x, nc2, xu1, b2) : 0.f;
-        // v taps lv0, lv0 + 1 here; the other pair of this lane's half from
-        // lane L ^ 32
-        const float v0 = v0_ok ? EsKernelAt(cv.y, nc2, xv0, b2) : 0.f;
-        const float v1 = v1_ok ? EsKernelAt(cv.y, nc2, xv1, b2) : 0.f;
-        const float v2 = __shfl_xor(v0, 32), v3 = __shfl_xor(v1, 32);
-        // kv for l = 4 lh + r, r = 0..3 (lanes with g < 2 hold the first pair)
-        const float kv0 = g < 2 ? v0 : v2, kv1 = g < 2 ? v1 : v3;
-        const float kv2 = g < 2 ? v2 : v0, kv3 = g < 2 ? v3 : v1;
-        // w taps 2 g, 2 g + 1 here; all eight from the four lanes of j
-        // (lane 16 t + j holds taps 2 t, 2 t + 1 of visibility j)
-        float kw[8];
+        // v taps 2 g, 2 g + 1 here; all eight from the four lanes of j
+        // (lane 16 p + j holds taps 2 p, 2 p + 1 of visibility j)
+        float kv[8];
         {
-          const float w0v = w0_ok ? EsKernelAt(cv.z, nc2, xw0, b2) : 0.f;
-          const float w1v = w1_ok ? EsKernelAt(cv.z, nc2, xw1, b2) : 0.f;
+          const float v0 = t0_ok ? EsKernelAt(cv.y, nc2, x20, b2) : 0.f;
+          const float v1 = t1_ok ? EsKernelAt(cv.y, nc2, x21, b2) : 0.f;
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            kw[2 * t] = __shfl(w0v, 16 * t + j);
-            kw[2 * t + 1] = __shfl(w1v, 16 * t + j);
+          for (int pp = 0; pp < 4; ++pp) {
+            kv[2 * pp] = __shfl(v0, 16 * pp + j);
+            kv[2 * pp + 1] = __shfl(v1, 16 * pp + j);
           }
         }
-        float part = 0.f;
+        // w taps of this lane's rows: planes 2 g, 2 g + 1
+        const float kw0 = t0_ok ? EsKernelAt(cv.z, nc2, x20, b2) : 0.f;
+        const float kw1 = t1_ok ? EsKernelAt(cv.z, nc2, x21, b2) : 0.f;
+        // four accumulators (steps mod 4): four independent chains
+        mfma_f4 dq[4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          mfma_f4 d = mfma_f4{0.f, 0.f, 0.f, 0.f};
-          d = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[t][0], bu0, d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[t][1], bu1, d, 0, 0, 0);
-          part += kw[t] * (d[0] * kv0 + d[1] * kv1 + d[2] * kv2 + d[3] * kv3);
-        }
-        // l halves (lanes L, L ^ 16), then the imaginary part to g = 0
+        for (int c = 0; c < 4; ++c) dq[c] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 16; ++st)
+          dq[st & 3] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              areg[st], ((st & 1) ? bu1 : bu0) * kv[st >> 1], dq[st & 3], 0, 0, 0);
+        const mfma_f4 d = (dq[0] + dq[1]) + (dq[2] + dq[3]);
+        // rows 4 g + r = (t, c) = (2 g + r / 2, r % 2)
+        float part = kw0 * d[0] + kw1 * d[2];
+        float im = kw0 * d[1] + kw1 * d[3];
+        // the four lanes of visibility j (L, L ^ 16, L ^ 32, L ^ 48)
         part += __shfl_xor(part, 16);
-        const float im = __shfl_xor(part, 32);
+        im += __shfl_xor(im, 16);
+        part += __shfl_xor(part, 32);
+        im += __shfl_xor(im, 32);
         if constexpr (FUSED) {
           if (g == 0 && take) {
             float2 V = make_float2(part * ph_f.x + im * ph_f.y, im * ph_f.x - part * ph_f.y);
