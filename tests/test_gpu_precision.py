@@ -75,3 +75,39 @@ def test_precision_study_per_channel(study, eps):
              np.mean(err_p)), flush=True)
     assert max(err_i) < eps
     assert max(err_p) < eps
+
+
+def test_plain_invert_predict_2048(oracle):
+    # The plain CPU-runnable configuration: 2048^2 at 1 arcsec, one channel
+    # at 150 MHz, 64 antennas in a 3 km disc (2016 baselines), 300 x 10 s at
+    # dec -30 deg, 20 power-law point sources plus noise 0.01, eps 1e-4.
+    # Invert on 2000 random pixels and predict on 2e4 random rows against
+    # the FP64 direct FT.
+    n, eps = 2048, 1e-4
+    rng = np.random.Generator(np.random.PCG64(2048))
+    enu = synth.antennas_disc(rng, 64, 3000.0)
+    uvw = synth.earth_rotation_uvw(enu, 300, 10.0, np.deg2rad(-30.0))
+    assert uvw.shape[0] == 2016 * 300
+    freq = np.array([150e6])
+    l, m, flux, xs, ys = synth.point_sky(rng, 20, n, DX, on_pixel=True)
+    vis = synth.add_noise(rng, synth.predict_sources(uvw, freq, l, m, flux), 0.01)
+    vis = vis.astype(np.complex64)
+    g = WGriddingGridder_Simple(n, n, n, n, DX, DX, 0.0, 0.0, 1, eps)
+    g.InitializeInversion()
+    g.AddInversionData(uvw.shape[0], 1, uvw, freq, vis)
+    g.FinalizeImage(1.0)
+    img = g.RealImage().reshape(n, n)
+    px, py = rng.integers(0, n, 2000), rng.integers(0, n, 2000)
+    ref = oracle.dft_invert_pixels(n, n, DX, DX, 0.0, 0.0, uvw, freq, vis, px, py)
+    err_i = rel_l2(img[py, px], ref)
+    model = np.zeros((n, n))
+    model[ys, xs] = flux
+    g.InitializePrediction(model.astype(np.float32).ravel())
+    out = np.zeros((uvw.shape[0], 1), np.complex64)
+    g.PredictVisibilities(uvw.shape[0], 1, uvw, freq, out)
+    sel = rng.choice(uvw.shape[0], 20000, replace=False)
+    err_p = rel_l2(out[sel, 0], oracle.dft_predict(n, n, DX, DX, 0.0, 0.0, model, uvw, freq,
+                                                    sel=sel))
+    print("2048^2 plain: invert %.2e predict %.2e plan %s" % (err_i, err_p, g.plan()), flush=True)
+    assert err_i < eps
+    assert err_p < eps
