@@ -264,11 +264,16 @@ __device__ __forceinline__ void WaveLdsSync() {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// Lane exchanges of the gridding step.  Every source lane of a quad
-// broadcast or of a rotation inside a 16-lane row is a valid lane, and the
-// step runs with all 64 lanes active, so bound_ctrl (zero for an invalid
-// source) never applies; it is set because it lets the compiler fold the
-// move into the VALU instruction that consumes it (one DPP-modified op).
+// Lane exchanges of the gridding step.  The step runs with all 64 lanes
+// active, and every exchange must be executed by all of them: a DPP move
+// reads its source lane whatever that lane is doing, and a source lane
+// masked off by EXEC reads as invalid.  A per-lane select between an
+// exchanged and an own value can be turned by the compiler into a masked
+// branch around the move, so the lane choice is made by the bank mask of
+// update_dpp instead (lanes of banks outside the mask keep their own
+// value).  Every source lane read is a valid lane; bound_ctrl is set
+// because it lets the compiler fold a quad broadcast into the multiply that
+// consumes it.
 __device__ __forceinline__ float DppQuadBcast(float v, int m) {
   // quad_perm [m, m, m, m]
   switch (m) {
@@ -279,8 +284,13 @@ __device__ __forceinline__ float DppQuadBcast(float v, int m) {
   }
 }
 
-__device__ __forceinline__ float DppRowRor8(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, true));
+// The lanes of the banks (quads of a 16-lane row) in BANKS take the value
+// moved by CTRL, the others keep `own`.  CTRL 0x128 rotates a row by 8,
+// 0x114 shifts it right by 4 (lane i reads i - 4), 0x104 left by 4 (i + 4).
+template <int CTRL, int BANKS>
+__device__ __forceinline__ float DppMerge(float own) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(own), __float_as_int(own),
+                                                    CTRL, 0xf, BANKS, true));
 }
 
 // ES kernel at x = xc - f * c2 (xc = tap abscissa * c2, nc2 = -c2), with the
@@ -327,6 +337,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
   const bool hi = r16 >= 8;         // A: imaginary part; first evaluation: v
   const int tsig = 2 * (r16 & 3) + (r16 >> 3);  // w tap this lane evaluates
   const bool tap_ok = tap < W, w_ok = tsig < W;
+  const bool q_hi = (r16 & 4) != 0;  // quad 1 or 3 of the 16-lane row
   const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
   const float xc1 = float(tap - W / 2 + 1) * c2;
   const float xcw = float(tsig - W / 2 + 1) * c2;
@@ -440,15 +451,23 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         // kernel values: lanes r16 < 8 evaluate ku[tap], r16 >= 8 kv[tap]
         const float k10 = EsKernelAt(c0.x, nc2, xc1, b2);
         const float k11 = EsKernelAt(c1.x, nc2, xc1, b2);
-        const float kw0 = w_ok ? EsKernelAt(c0.y, nc2, xcw, b2) : 0.f;
-        const float kw1 = w_ok ? EsKernelAt(c1.y, nc2, xcw, b2) : 0.f;
-        const float k10o = DppRowRor8(k10), k11o = DppRowRor8(k11);
+        // w taps of both groups from one evaluation: quads 0 and 2 of a
+        // 16-lane row (r16 & 4 clear) take group 0, quads 1 and 3 group 1
+        // (the quads of a pair hold the same taps); each lane then gets the
+        // other group's value from the neighbouring quad
+        const float kwx = w_ok ? EsKernelAt(q_hi ? c1.y : c0.y, nc2, xcw, b2) : 0.f;
+        const float kw0 = DppMerge<0x114, 0xA>(kwx);  // quads 1, 3 from quads 0, 2
+        const float kw1 = DppMerge<0x104, 0x5>(kwx);  // quads 0, 2 from quads 1, 3
+        // A factor: kv (lanes r16 >= 8 own, the others from the rotated
+        // row); B factor: ku (lanes r16 < 8 own)
+        const float fa0 = DppMerge<0x128, 0x3>(k10), fa1 = DppMerge<0x128, 0x3>(k11);
+        const float fb0 = DppMerge<0x128, 0xC>(k10), fb1 = DppMerge<0x128, 0xC>(k11);
         // visibilities past the run contribute zero rows and columns
         const bool take0 = tap_ok && j < lead, take1 = tap_ok && j + 4 < lead;
-        const float a0 = take0 ? (hi ? k10 : k10o) * c0.z : 0.f;
-        const float a1 = take1 ? (hi ? k11 : k11o) * c1.z : 0.f;
-        const float u0 = take0 ? (hi ? k10o : k10) : 0.f;
-        const float u1 = take1 ? (hi ? k11o : k11) : 0.f;
+        const float a0 = take0 ? fa0 * c0.z : 0.f;
+        const float a1 = take1 ? fa1 * c1.z : 0.f;
+        const float u0 = take0 ? fb0 : 0.f;
+        const float u1 = take1 ? fb1 : 0.f;
         if (npos - bbase >= uint32_t(kStage) && npos < w1) {
           // the next group starts in batch b + 1: batch b + 2 replaces b
           StoreStage(ring, bslot * kStage + lane, pc, src.Value());
