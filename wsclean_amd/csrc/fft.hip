@@ -234,7 +234,8 @@ __device__ __forceinline__ float2 Twiddle(const float2* twl, int k) {
 // j < n / R <= 8192 used here.  Every thread first reads all its
 // butterflies into registers, then (after a barrier) writes them, so a
 // single LDS buffer suffices.
-template <int R, bool INV, int BLOCK, int MAXN>
+// FIRST: the first stage (Ns = 1), whose twiddles are all 1 and skipped.
+template <int R, bool INV, int BLOCK, int MAXN, bool FIRST>
 __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32_t magic,
                                               const float2* twl) {
   constexpr int NB = (MAXN / R + BLOCK - 1) / BLOCK;
@@ -247,10 +248,7 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
     if (j < nb) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * nb];
-      // the first stage (Ns = 1, k = 0) multiplies by exactly 1: one
-      // branch-free path keeps the loaded values in place (no register
-      // shuffles between the two paths)
-      {
+      if constexpr (!FIRST) {
         // (magic wraps to 1 for Ns = 1, hence the select)
         const int k = Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0;
         // w^r for r < R from one table twiddle w = exp(-+2 pi i k tws / n):
@@ -280,10 +278,10 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
   for (int b = 0; b < NB; ++b) {
     const int j = threadIdx.x + b * BLOCK;
     if (j < nb) {
-      const int k = Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0;
+      const int k = FIRST ? 0 : (Ns > 1 ? j - int(__umulhi(uint32_t(j), magic)) * Ns : 0);
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[base + r * (FIRST ? 1 : Ns)] = v[b][r];
     }
   }
   __syncthreads();
@@ -291,27 +289,43 @@ __device__ __forceinline__ void StockhamStage(float2* buf, int n, int Ns, uint32
 
 // Full in-LDS FFT; unnormalised, forward = exp(-2 pi i jk/n).  Caller must
 // have synchronised after filling buf and the twiddle table twl.
+template <bool INV, int BLOCK, int MAXN, bool FIRST>
+__device__ __forceinline__ void StageOfRadix(int R, float2* buf, int n, int Ns, uint32_t mg,
+                                             const float2* twl) {
+  if constexpr (FIRST) {
+    // twiddle-free variants for the usual first radices only (less code);
+    // the others take the general stage with Ns = 1
+    switch (R) {
+      case 8: StockhamStage<8, INV, BLOCK, MAXN, true>(buf, n, Ns, mg, twl); return;
+      case 9: StockhamStage<9, INV, BLOCK, MAXN, true>(buf, n, Ns, mg, twl); return;
+      default: StageOfRadix<INV, BLOCK, MAXN, false>(R, buf, n, Ns, mg, twl); return;
+    }
+  }
+  switch (R) {
+    case 8: StockhamStage<8, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    case 9: StockhamStage<9, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    case 18: StockhamStage<18, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    case 4: StockhamStage<4, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    case 2: StockhamStage<2, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    case 3: StockhamStage<3, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+    default: StockhamStage<5, INV, BLOCK, MAXN, FIRST>(buf, n, Ns, mg, twl); break;
+  }
+}
+
 template <bool INV, int BLOCK, int MAXN>
 __device__ void LdsFFT(float2* buf, const FFTDesc& d, const float2* twl) {
   int Ns = 1;
   for (int s = 0; s < d.nst; ++s) {
-    const int R = d.rad[s];
-    const uint32_t mg = d.magic[s];
     // the line length is re-read per stage through an empty asm statement,
     // so the compiler cannot hoist the LDS addresses of every radix case
     // (all loop-invariant) out of the stage loop and hold them in registers
     int n = d.n;
     asm volatile("" : "+s"(n));
-    switch (R) {
-      case 8: StockhamStage<8, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      case 9: StockhamStage<9, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      case 18: StockhamStage<18, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      case 4: StockhamStage<4, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      case 2: StockhamStage<2, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      case 3: StockhamStage<3, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-      default: StockhamStage<5, INV, BLOCK, MAXN>(buf, n, Ns, mg, twl); break;
-    }
-    Ns *= R;
+    if (s == 0)  // the first stage without twiddles
+      StageOfRadix<INV, BLOCK, MAXN, true>(d.rad[s], buf, n, Ns, d.magic[s], twl);
+    else
+      StageOfRadix<INV, BLOCK, MAXN, false>(d.rad[s], buf, n, Ns, d.magic[s], twl);
+    Ns *= d.rad[s];
   }
 }
 
