@@ -106,6 +106,14 @@ FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
            "facet batch too large for one sort key space: split the facets into smaller batches");
   pa.nbuckets = uint32_t(nb);
   pa.nkeys = uint32_t(nk);
+  {
+    // the atlas takes as many planes as its deepest facet; when they fit
+    // one sub-slab of the full LDS, one sub-slab lets the slab gather and
+    // scatter fuse into the FFT passes (the same rule as a single plan)
+    const size_t E = size_t(pa.T + pa.W - 1);
+    const int max_sg = int((160 * 1024 - kGridLdsReserve) / (E * E * 8));
+    if (opt.Sg <= 0 && pa.P > pa.Sg && pa.P <= max_sg && pa.P <= kMaxSubSlab) pa.Sg = pa.P;
+  }
   pa.key_bits = 1;
   while ((uint64_t(1) << pa.key_bits) <= uint64_t(pa.nkeys)) ++pa.key_bits;
   // FFT slab: whole sub-slabs whose atlas box and largest facet line

@@ -268,6 +268,10 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
     const int max_sg = int((160 * 1024 - kGridLdsReserve) / (E * E * 8));
     WG_CHECK(max_sg >= 1, "tile too large for LDS");
     p.Sg = std::max(1, std::min(p.Sg, max_sg));
+    // when every plane fits one sub-slab of the full LDS, take one sub-slab
+    // at one workgroup per CU: the slab gather and scatter then fuse into
+    // the FFT passes, which saves more than the second workgroup gains
+    if (opt.Sg <= 0 && p.P > p.Sg && p.P <= max_sg && p.P <= kMaxSubSlab) p.Sg = p.P;
   }
   const double per_plane = 8.0 * p.BV * (double(p.BU) + 2.0 * g.nx);
   int sf = int(std::max(1.0, std::floor(double(opt.slab_bytes) / per_plane)));
