@@ -239,9 +239,10 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
 
   // ---- row renumbering ----
   // Rows are renumbered in the order of their key at the middle channel
-  // (stable), i.e. in uv order, and copied padded to 32 bytes: the
-  // coordinates kernel then reads the rows of neighbouring sorted
-  // visibilities from neighbouring addresses instead of at random.
+  // (stable), i.e. in uv order, and copied padded to 32 bytes with the
+  // caller row in the pad word: the coordinates kernel then reads the rows
+  // of neighbouring sorted visibilities from neighbouring addresses instead
+  // of at random, and each row's caller index from the same cache line.
   uint32_t* rperm = static_cast<uint32_t*>(arena_.Get("rows.perm", n_rows * 4));
   double* uvw4 = static_cast<double*>(arena_.Get("uvw4", n_rows * 4 * sizeof(double)));
   uint32_t* scratch = static_cast<uint32_t*>(
@@ -253,7 +254,7 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
     LaunchKeys(dp, uvw_dev, vs.scale + n_chan / 2, n_rows, 1, nullptr, rk, nullptr, stream_);
     if (DeviceRadixSort(rk, rperm, rk2, rp2, n_rows, vs.plan.key_bits, scratch, stream_, true))
       rperm = rp2;
-    LaunchPadUvw(uvw_dev, rperm, n_rows, uvw4, stream_);
+    LaunchPadUvw(uvw_dev, rperm, rperm, n_rows, uvw4, stream_);
   }
 
   // ---- keys + stable sort ----
@@ -261,7 +262,7 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
   if (n_vis) {
     // the sort's first pass generates the index values itself
-    LaunchKeysRenumbered(dp, uvw4, rperm, vs.scale, n_rows, n_chan, vis_dev, keys, stream_);
+    LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_);
     uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
     if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_,
@@ -281,7 +282,7 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
 
   vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", vs.n_valid * sizeof(float4)));
   vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", vs.n_valid * sizeof(float2)));
-  LaunchCoords(dp, uvw4, rperm, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, vs.phasor,
+  LaunchCoords(dp, uvw4, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, vs.phasor,
                stream_);
   BuildWorkItems(vs);
   BuildCorrection(vs);

@@ -333,7 +333,7 @@ void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F
 // ch of virtual row rperm[r]; facet f owns virtual rows [vrbase[f],
 // vrbase[f + 1])); uvw4 = the renumbered rows padded to 32 bytes.
 void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                               const uint32_t* rperm, const double* uvw4, const double* scale,
+                               const double* uvw4, const double* scale,
                                size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0, int T,
                                uint32_t* keys, hipStream_t s);
 // rphys[r] = physical row of renumbered virtual row r.
@@ -343,7 +343,7 @@ void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, in
 // and leaves with the virtual index, perm_phys gets the physical index
 // (both with the flip bit).
 void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                       const uint32_t* rperm, const double* uvw4, const double* scale,
+                       const double* uvw4, const double* scale,
                        size_t n_chan, uint32_t* perm, uint32_t* perm_phys, size_t n_valid,
                        float4* coord, float2* phasor, hipStream_t s);
 void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
@@ -353,19 +353,19 @@ void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
 void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
                 size_t n_rows, size_t n_chan, const float2* vis, uint32_t* keys,
                 uint32_t* vals, hipStream_t s);
-// uvw4[r][4] = (u, v, w, 0) of caller row rperm[r] (renumbered rows).
-void LaunchPadUvw(const double* uvw, const uint32_t* rperm, size_t n_rows, double* uvw4,
-                  hipStream_t s);
+// uvw4[r][4] = (u, v, w, tag) of row rfetch[r] (renumbered rows), the
+// fourth word holding the bits of rtag[r] (the row number the coordinate and
+// key kernels report for renumbered row r).
+void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rtag, size_t n_rows,
+                  double* uvw4, hipStream_t s);
 // Keys of the visibilities in renumbered row order (element r * n_chan + ch
-// = channel ch of caller row rperm[r]); zero visibilities get nkeys.
-void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
-                          const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
+// = channel ch of caller row tag(r)); zero visibilities get nkeys.
+void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
                           uint32_t* keys, hipStream_t s);
 // Sorted coordinates and phasors from the renumbered padded rows; perm
 // enters with sorted renumbered elements and leaves with the caller's flat
 // indices and the flip bit (kPermFlip).
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
-                  const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
+void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
                   float4* coord, float2* phasor, hipStream_t s);
 // out[i] = in[i] & mask
 void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,

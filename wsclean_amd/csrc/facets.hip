@@ -249,7 +249,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
     if (DeviceRadixSort(rk, rperm, rk2, rp2, n_vrows, pa.key_bits, scratch, s, true))
       rperm = rp2;
     LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, s);
-    LaunchPadUvw(uvw_dev, rphys, n_vrows, uvw4, s);
+    LaunchPadUvw(uvw_dev, rphys, rperm, n_vrows, uvw4, s);
   }
 
   // ---- keys, one stable sort, bucket offsets ----
@@ -258,7 +258,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   uint32_t* vals = static_cast<uint32_t*>(A.Get("atlas.perm", n_virtual * 4));
   if (n_virtual) {
     // the sort's first pass generates the index values itself
-    LaunchFacetKeysRenumbered(dfac, dvrb, F, rperm, uvw4, scale_, n_vrows, n_chan, pa.ntu, pa.P0,
+    LaunchFacetKeysRenumbered(dfac, dvrb, F, uvw4, scale_, n_vrows, n_chan, pa.ntu, pa.P0,
                               T, keys, s);
     uint32_t* keys2 = static_cast<uint32_t*>(A.Get("atlas.keys2", n_virtual * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(A.Get("atlas.perm2", n_virtual * 4));
@@ -278,7 +278,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   atlas_.coord = static_cast<float4*>(A.Get("atlas.coord", nvalid * sizeof(float4)));
   atlas_.phasor = static_cast<float2*>(A.Get("atlas.phasor", nvalid * sizeof(float2)));
   perm_phys_ = static_cast<uint32_t*>(A.Get("atlas.perm_phys", nvalid * 4));
-  LaunchFacetCoords(dfac, dvrb, F, rperm, uvw4, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
+  LaunchFacetCoords(dfac, dvrb, F, uvw4, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
                     atlas_.coord, atlas_.phasor, s);
   eng_.BuildWorkItems(atlas_);
   for (int f = 0; f < F; ++f) eng_.BuildCorrection(fvs_[f]);

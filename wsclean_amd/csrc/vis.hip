@@ -74,6 +74,12 @@ __global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nbloc
 }
 
 // Sort key of one visibility: ((tile * P0 + p0) * T + dv) * T + du.
+// Caller row of a renumbered row: the bits of the fourth word of its padded
+// uvw4 entry (w, tag).
+__device__ __forceinline__ uint32_t RowTag(double2 wt) {
+  return uint32_t(__double_as_longlong(wt.y));
+}
+
 __device__ __forceinline__ uint32_t VisKey(const VisLoc& L, const DevPlan& dp) {
   const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
   return (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
@@ -104,11 +110,11 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
 }
 
 // Keys in renumbered row order: element e = r * n_chan + ch is channel ch of
-// renumbered row r, i.e. of caller row rperm[r]; uvw4 holds the renumbered
-// rows padded to 32 bytes.  Keys are the same as KeysKernel's for the same
-// visibility, only their order differs.
+// renumbered row r, i.e. of caller row RowTag(uvw4, r); uvw4 holds the
+// renumbered rows padded to 32 bytes.  Keys are the same as KeysKernel's for
+// the same visibility, only their order differs.
 __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
-    DevPlan dp, const double* __restrict__ uvw4, const uint32_t* __restrict__ rperm,
+    DevPlan dp, const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_rows, size_t n_chan,
     const float2* __restrict__ vis, uint32_t* __restrict__ keys) {
   const size_t n = n_rows * n_chan;
@@ -118,14 +124,13 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
     uint32_t key = dp.nkeys;
     bool valid = true;
+    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
     if (vis) {
-      const float2 V = vis[size_t(rperm[r]) * n_chan + ch];
+      const float2 V = vis[size_t(RowTag(wt)) * n_chan + ch];
       valid = (V.x != 0.f) || (V.y != 0.f);
     }
-    if (valid) {
-      const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-      key = VisKey(LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp), dp);
-    }
+    if (valid) key = VisKey(LocateVis(uv.x, uv.y, wt.x, scale[ch], dp), dp);
     keys[e] = key;
   }
 }
@@ -145,12 +150,11 @@ __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L
 // and scatter of visibilities then need no uvw reads and no FP64 work.
 //
 // perm enters holding renumbered elements e = r * n_chan + ch (renumbered
-// row r = caller row rperm[r], rows in uvw4 padded to 32 bytes) and leaves
-// holding the caller's flat index rperm[r] * n_chan + ch.  Rows are
+// row r = caller row RowTag(uvw4, r), rows in uvw4 padded to 32 bytes) and
+// leaves holding the caller's flat index RowTag * n_chan + ch.  Rows are
 // renumbered in uv order, so the rows read by neighbouring sorted
 // visibilities lie close together in uvw4.
 __global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw4,
-                                                       const uint32_t* __restrict__ rperm,
                                                        const double* __restrict__ scale,
                                                        size_t n_chan, uint32_t* __restrict__ perm,
                                                        size_t n_valid, float4* __restrict__ coord,
@@ -161,24 +165,29 @@ __global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double*
     const uint32_t e = perm[i];
     const size_t r = DivSmall(e, uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
     const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
+    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
+    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
     coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
     phasor[i] = ShiftPhasor(dp, L, -1.f);
-    const uint32_t idx = rperm[r] * uint32_t(n_chan) + uint32_t(ch);
+    const uint32_t idx = RowTag(wt) * uint32_t(n_chan) + uint32_t(ch);
     perm[i] = idx | (L.flip ? kPermFlip : 0u);
   }
 }
 
-// uvw4[r] = (uvw[rperm[r]], 0): the renumbered rows padded to 32 bytes.
+// uvw4[r] = (uvw[rfetch[r]], tag): the renumbered rows padded to 32 bytes,
+// the fourth word holding rtag[r] (read back by RowTag), so that kernels
+// reading a renumbered row find its caller row in the same cache line.
 __global__ void __launch_bounds__(kBlock) PadUvwKernel(const double* __restrict__ uvw,
-                                                       const uint32_t* __restrict__ rperm,
+                                                       const uint32_t* __restrict__ rfetch,
+                                                       const uint32_t* __restrict__ rtag,
                                                        size_t n_rows,
                                                        double* __restrict__ uvw4) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < 4 * n_rows;
        i += size_t(gridDim.x) * kBlock) {
-    const size_t row = rperm[i >> 2];
+    const size_t r = i >> 2;
     const int k = int(i & 3);
-    uvw4[i] = k < 3 ? uvw[3 * row + k] : 0.0;
+    uvw4[i] = k < 3 ? uvw[3 * size_t(rfetch[r]) + k]
+                    : __longlong_as_double((long long)(rtag[r]));
   }
 }
 
@@ -537,11 +546,12 @@ __global__ void __launch_bounds__(kBlock) FacetKeysKernel(
 
 // Keys of a facet batch in renumbered row order: element e = r * n_chan +
 // ch is channel ch of renumbered virtual row r, i.e. of virtual row
-// rperm[r] (virtual rows of facet f: [vrbase[f], vrbase[f + 1]), physical
-// row row_off_f + local); uvw4 holds the renumbered rows padded to 32 bytes.
+// RowTag(uvw4, r) (virtual rows of facet f: [vrbase[f], vrbase[f + 1]),
+// physical row row_off_f + local); uvw4 holds the renumbered rows padded to
+// 32 bytes.
 __global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
-    const uint32_t* __restrict__ rperm, const double* __restrict__ uvw4,
+    const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_vrows, size_t n_chan, int atlas_ntu,
     int atlas_P0, int T, uint32_t* __restrict__ keys) {
   const size_t n = n_vrows * n_chan;
@@ -549,10 +559,11 @@ __global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
   for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
        e += size_t(gridDim.x) * kBlock) {
     const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
-    const int f = FacetOf(vrbase, F, rperm[r]);
-    const DevPlan& dp = fac[f].dp;
     const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
+    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
+    const int f = FacetOf(vrbase, F, RowTag(wt));
+    const DevPlan& dp = fac[f].dp;
+    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
     const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
     const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
     const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
@@ -580,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) FacetRowMapKernel(
 // visibility buffer; both carry the flip bit.
 __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
-    const uint32_t* __restrict__ rperm, const double* __restrict__ uvw4,
+    const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_chan, uint32_t* __restrict__ perm,
     uint32_t* __restrict__ perm_phys, size_t n_valid, float4* __restrict__ coord,
     float2* __restrict__ phasor) {
@@ -589,11 +600,12 @@ __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
        i += size_t(gridDim.x) * kBlock) {
     const uint32_t e = perm[i];
     const size_t r = DivSmall(e, uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
-    const uint32_t vr = rperm[r];
+    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
+    const uint32_t vr = RowTag(wt);
     const int f = FacetOf(vrbase, F, vr);
     const DevPlan& dp = fac[f].dp;
-    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-    const VisLoc L = LocateVis(uv.x, uv.y, uvw4[4 * r + 2], scale[ch], dp);
+    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
     coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
     phasor[i] = ShiftPhasor(dp, L, -1.f);
     const uint32_t flip = L.flip ? kPermFlip : 0u;
@@ -642,13 +654,13 @@ void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F
 }
 
 void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                               const uint32_t* rperm, const double* uvw4, const double* scale,
+                               const double* uvw4, const double* scale,
                                size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0, int T,
                                uint32_t* keys, hipStream_t s) {
   if (!n_vrows) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   FacetKeysRenumberedKernel<<<Blocks(n_vrows * n_chan), kBlock, 0, s>>>(
-      fac, vrbase, F, rperm, uvw4, scale, n_vrows, n_chan, atlas_ntu, atlas_P0, T, keys);
+      fac, vrbase, F, uvw4, scale, n_vrows, n_chan, atlas_ntu, atlas_P0, T, keys);
   WG_HIP(hipGetLastError());
 }
 
@@ -660,39 +672,37 @@ void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, in
 }
 
 void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                       const uint32_t* rperm, const double* uvw4, const double* scale,
+                       const double* uvw4, const double* scale,
                        size_t n_chan, uint32_t* perm, uint32_t* perm_phys, size_t n_valid,
                        float4* coord, float2* phasor, hipStream_t s) {
   if (!n_valid) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
-  FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vrbase, F, rperm, uvw4, scale, n_chan,
+  FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vrbase, F, uvw4, scale, n_chan,
                                                         perm, perm_phys, n_valid, coord, phasor);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchPadUvw(const double* uvw, const uint32_t* rperm, size_t n_rows, double* uvw4,
-                  hipStream_t s) {
+void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rtag, size_t n_rows,
+                  double* uvw4, hipStream_t s) {
   if (!n_rows) return;
-  PadUvwKernel<<<Blocks(4 * n_rows), kBlock, 0, s>>>(uvw, rperm, n_rows, uvw4);
+  PadUvwKernel<<<Blocks(4 * n_rows), kBlock, 0, s>>>(uvw, rfetch, rtag, n_rows, uvw4);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
-                          const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
+void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
                           uint32_t* keys, hipStream_t s) {
   if (!n_rows) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
-  KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, rperm, scale, n_rows,
-                                                                  n_chan, vis, keys);
+  KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, scale, n_rows, n_chan,
+                                                                  vis, keys);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const uint32_t* rperm,
-                  const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
+void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
                   float4* coord, float2* phasor, hipStream_t s) {
   if (!n_valid) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
-  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw4, rperm, scale, n_chan, perm, n_valid,
+  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw4, scale, n_chan, perm, n_valid,
                                                   coord, phasor);
   WG_HIP(hipGetLastError());
 }
