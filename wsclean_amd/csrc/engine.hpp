@@ -360,7 +360,8 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
                   double* uvw4, hipStream_t s);
 // Keys of the visibilities in renumbered row order (element r * n_chan + ch
 // = channel ch of caller row tag(r)); zero visibilities get nkeys.
-void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
+void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
+                          size_t n_rows, size_t n_chan, const float2* vis,
                           uint32_t* keys, hipStream_t s);
 // Sorted coordinates and phasors from the renumbered padded rows; perm
 // enters with sorted renumbered elements and leaves with the caller's flat

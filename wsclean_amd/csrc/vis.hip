@@ -689,7 +689,8 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
   WG_HIP(hipGetLastError());
 }
 
-void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_rows, size_t n_chan, const float2* vis,
+void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
+                          size_t n_rows, size_t n_chan, const float2* vis,
                           uint32_t* keys, hipStream_t s) {
   if (!n_rows) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
