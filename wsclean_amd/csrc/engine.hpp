@@ -366,7 +366,8 @@ void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* s
 // Sorted coordinates and phasors from the renumbered padded rows; perm
 // enters with sorted renumbered elements and leaves with the caller's flat
 // indices and the flip bit (kPermFlip).
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
+void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan,
+                  uint32_t* perm, size_t n_valid,
                   float4* coord, float2* phasor, hipStream_t s);
 // out[i] = in[i] & mask
 void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,

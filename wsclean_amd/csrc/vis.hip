@@ -699,7 +699,8 @@ void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* s
   WG_HIP(hipGetLastError());
 }
 
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan, uint32_t* perm, size_t n_valid,
+void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan,
+                  uint32_t* perm, size_t n_valid,
                   float4* coord, float2* phasor, hipStream_t s) {
   if (!n_valid) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
