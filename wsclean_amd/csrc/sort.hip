@@ -127,20 +127,25 @@ __global__ void ScanDownKernel(const uint32_t* __restrict__ in, size_t n,
   }
 }
 
-// Adds one to h[d] for every active lane, one LDS atomic per distinct digit
-// of the wave: lanes with equal digits are found by ballots over the digit
-// bits and their lowest lane adds their count.  Keys in uv order put long
-// runs of equal digits into one wave, which would serialise per-lane
-// atomics on one address.
-__device__ __forceinline__ void WaveCount(uint32_t* h, uint32_t d, int bits, uint64_t active,
-                                          int lane) {
-  uint64_t peers = active;
-  for (int b = 0; b < bits; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bb = __ballot(bit);
-    peers &= bit ? bb : ~bb;
+// Adds one to h[d] for every active lane, one LDS atomic per run of equal
+// digits in consecutive active lanes: the run's first lane adds its length.
+// Equal digits in separate runs add separately.  Keys in uv order put long
+// runs of equal digits into a wave, which per-lane atomics on one address
+// would serialise; finding the runs takes a neighbour exchange and one
+// ballot, where grouping all equal digits of a wave takes a ballot per
+// digit bit and a 64-bit select per lane and bit.
+__device__ __forceinline__ void WaveCountRuns(uint32_t* h, uint32_t d, uint64_t active,
+                                              int lane) {
+  const uint32_t prev = __shfl_up(d, 1);
+  const bool on = (active >> lane) & 1ull;
+  const bool prev_on = lane > 0 && ((active >> (lane - 1)) & 1ull);
+  const bool head = on && (!prev_on || prev != d);
+  // a run ends before the next head or the next inactive lane
+  const uint64_t stop = __ballot(head) | ~active;
+  if (head) {
+    const uint64_t after = lane < 63 ? (stop >> (lane + 1)) | (1ull << (63 - lane)) : 1ull;
+    atomicAdd(&h[d], uint32_t(__builtin_ctzll(after)) + 1u);
   }
-  if (int(__ffsll((long long)peers)) - 1 == lane) atomicAdd(&h[d], uint32_t(__popcll(peers)));
 }
 
 // Digit histogram of one run of kRsSub tiles: counts[d * nblocks + block].
@@ -154,7 +159,6 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
   __shared__ uint32_t h[kRsWaves][kRsMaxBuckets];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int R = int(mask) + 1;
-  const int bits = __popc(mask);
   for (int i = threadIdx.x; i < kRsWaves * kRsMaxBuckets; i += kRsThreads) (&h[0][0])[i] = 0;
   __syncthreads();
   constexpr int kVec = kRsPerThread / 4;
@@ -176,10 +180,10 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
       }
 #pragma unroll
       for (int it = 0; it < kVec; ++it) {
-        WaveCount(h[wave], (q[it].x >> shift) & mask, bits, ~0ull, lane);
-        WaveCount(h[wave], (q[it].y >> shift) & mask, bits, ~0ull, lane);
-        WaveCount(h[wave], (q[it].z >> shift) & mask, bits, ~0ull, lane);
-        WaveCount(h[wave], (q[it].w >> shift) & mask, bits, ~0ull, lane);
+        WaveCountRuns(h[wave], (q[it].x >> shift) & mask, ~0ull, lane);
+        WaveCountRuns(h[wave], (q[it].y >> shift) & mask, ~0ull, lane);
+        WaveCountRuns(h[wave], (q[it].z >> shift) & mask, ~0ull, lane);
+        WaveCountRuns(h[wave], (q[it].w >> shift) & mask, ~0ull, lane);
       }
 #pragma unroll
       for (int it = 0; it < kVec; ++it) q[it] = qn[it];
@@ -194,7 +198,7 @@ __global__ void __launch_bounds__(kRsThreads) RadixHistKernel(
         for (int c = 0; c < 4; ++c) {
           const bool act = i + c < n;
           const uint32_t d = act ? (keys[i + c] >> shift) & mask : 0u;
-          WaveCount(h[wave], d, bits, __ballot(act), lane);
+          WaveCountRuns(h[wave], d, __ballot(act), lane);
         }
       }
     }
