@@ -231,6 +231,7 @@ struct ScatterLds {
   uint32_t run_base[kRsMaxBuckets];       // next global position of each digit
   uint32_t buf[kRsTile];                  // keys, then values, in tile order
   uint16_t dig[kRsTile];                  // digit of each tile position
+  uint8_t tag[kRsWaves][kRsMaxBuckets];   // lane of a run head, per digit
 };
 
 // One tile of the scatter (see RadixScatterKernel).  FULL: the tile holds
@@ -264,26 +265,58 @@ __device__ __forceinline__ void ScatterTile(ScatterLds& L, const uint32_t* __res
     const uint32_t i = base + uint32_t(it) * 64 + lane;
     val[it] = IDX ? i : (in(it) ? vin[i] : 0u);
   }
+  // Rank of each element among the wave's equal digits (r), and at the
+  // digit's first lane (lead) their number (tot).  Fast path: when every
+  // digit of the wave forms one run of consecutive lanes (keys in uv order
+  // mostly do), the runs give both: run heads from a neighbour exchange,
+  // and a digit with two runs is detected by its heads writing their lane
+  // to tag[d] and reading it back.  Otherwise the equal digits are found by
+  // one ballot per digit bit.
 #pragma unroll
   for (int it = 0; it < kRsPerThread; ++it) {
     const bool act = in(it);
     const uint32_t d = (key[it] >> shift) & mask;
-    uint64_t peers = FULL ? ~0ull : __ballot(act);
-    auto narrow = [&](int b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
-    };
-    if constexpr (BITS > 0) {
-#pragma unroll
-      for (int b = 0; b < BITS; ++b) narrow(b);
+    const uint64_t active = FULL ? ~0ull : __ballot(act);
+    const uint32_t prev = __shfl_up(d, 1);
+    const bool prev_on = lane > 0 && ((active >> (lane - 1)) & 1ull);
+    const bool head = act && (!prev_on || prev != d);
+    const uint64_t heads = __ballot(head);
+    if (head) L.tag[wave][d] = uint8_t(lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool clash = head && L.tag[wave][d] != uint8_t(lane);
+    uint32_t r, tot;
+    bool lead;
+    if (__ballot(clash) == 0ull) {
+      // this lane's run starts at the highest head at or below it
+      const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+      r = uint32_t(lane - (63 - __builtin_clzll(upto | 1ull)));
+      // a run ends before the next head or the next inactive lane
+      const uint64_t stop = heads | ~active;
+      const uint64_t after = lane < 63 ? (stop >> (lane + 1)) | (1ull << (63 - lane)) : 1ull;
+      tot = uint32_t(__builtin_ctzll(after)) + 1u;
+      lead = head;
     } else {
-      for (int b = 0; b < bits; ++b) narrow(b);
+      uint64_t peers = active;
+      auto narrow = [&](int b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      };
+      if constexpr (BITS > 0) {
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) narrow(b);
+      } else {
+        for (int b = 0; b < bits; ++b) narrow(b);
+      }
+      r = uint32_t(__popcll(peers & lt));
+      tot = uint32_t(__popcll(peers));
+      lead = r == 0;
     }
-    const uint32_t r = uint32_t(__popcll(peers & lt));
     const uint32_t c = act ? L.cnt[wave][d] : 0u;
     pos[it] = c + r;
-    if (act && r == 0) L.cnt[wave][d] = c + uint32_t(__popcll(peers));
+    if (act && lead) L.cnt[wave][d] = c + tot;
   }
   __syncthreads();
   // wave offsets inside each digit, digit totals, then tile starts
