@@ -326,6 +326,11 @@ int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan,
                   char* plan_json, size_t plan_len);
 /* In-place batched complex FFT of nlines lines of length n (host data). */
 int wgx_fft(int n, int nlines, int inverse, float* data);
+/* The device radix sort on host data: keys_out / perm_out = the n keys and
+ * their element indices in the stable order of the low key_bits of the
+ * keys (the sort behind the bucket index). */
+int wgx_radix_sort(size_t n, int key_bits, const uint32_t* keys_in, uint32_t* keys_out,
+                   uint32_t* perm_out);
 /* Host-only: the plan (JSON) the library makes for an image geometry and
  * flipped-coordinate extents {umin, umax, vmin, vmax, wmin, wmax} (in
  * wavelengths), with the tuning overrides of wg_set_tuning.  Needs no GPU;

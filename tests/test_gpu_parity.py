@@ -88,6 +88,35 @@ def test_bucket_sort_bit_exact_large(oracle):
     np.testing.assert_array_equal(perm, oracle.bucket_perm(plan, p["uvw"], p["freq"], None))
 
 
+def _sort_patterns(rng, n, key_bits):
+    top = 1 << key_bits
+    runs = np.repeat(rng.integers(0, top, max(1, n // 7 + 1), dtype=np.uint64),
+                     7)[:n]
+    yield "random", rng.integers(0, top, n, dtype=np.uint64)
+    yield "equal", np.full(n, top - 1, dtype=np.uint64)
+    yield "runs", runs
+    yield "sorted", np.sort(rng.integers(0, top, n, dtype=np.uint64))
+    # equal digits in separate runs of one wave
+    yield "alternating", (np.arange(n, dtype=np.uint64) % 2) * (top - 1)
+    yield "few", rng.integers(0, 3, n, dtype=np.uint64) << np.uint64(max(0, key_bits - 2))
+
+
+@pytest.mark.parametrize("n,key_bits", [(1, 9), (2, 5), (100, 9), (8192, 18), (8193, 27),
+                                        (65536 * 3 + 17, 27), (300001, 21)])
+def test_radix_sort_patterns(n, key_bits):
+    # the device sort against numpy's stable argsort, on key patterns that
+    # take the run-based ranking (runs, sorted, equal) and the per-bit
+    # ballot ranking (random, alternating) of the scatter
+    from wsclean_amd.gridder import radix_sort
+    rng = np.random.default_rng(n + key_bits)
+    for name, k in _sort_patterns(rng, n, key_bits):
+        k = k.astype(np.uint32)
+        ko, po = radix_sort(k, key_bits)
+        ref = np.argsort(k, kind="stable")
+        np.testing.assert_array_equal(po, ref, err_msg=name)
+        np.testing.assert_array_equal(ko, k[ref], err_msg=name)
+
+
 # ---------------------------------------------------------------- invert
 
 @pytest.mark.parametrize("eps", [1e-3, 1e-4, 1e-5])

@@ -70,6 +70,8 @@ SIGNATURES = [
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),
     ("wgx_fft", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    ("wgx_radix_sort", ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
     ("wgx_make_plan", ctypes.c_int,
      [ctypes.POINTER(WgParams), ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_size]),

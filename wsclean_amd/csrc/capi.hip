@@ -726,6 +726,28 @@ int wgx_fft(int n, int nlines, int inverse, float* data) {
   });
 }
 
+int wgx_radix_sort(size_t n, int key_bits, const uint32_t* keys_in, uint32_t* keys_out,
+                   uint32_t* perm_out) {
+  return Guard([&] {
+    WG_CHECK(keys_in && keys_out && perm_out, "null argument");
+    WG_CHECK(key_bits >= 1 && key_bits <= 32 && n < 0x7FFFFFFFull, "bad arguments");
+    if (n == 0) return;
+    uint32_t *k = nullptr, *v = nullptr, *k2 = nullptr, *v2 = nullptr, *scr = nullptr;
+    const size_t bytes = n * sizeof(uint32_t);
+    WG_HIP(hipMalloc(&k, bytes));
+    WG_HIP(hipMalloc(&v, bytes));
+    WG_HIP(hipMalloc(&k2, bytes));
+    WG_HIP(hipMalloc(&v2, bytes));
+    WG_HIP(hipMalloc(&scr, wg::RadixSortScratchWords(n) * sizeof(uint32_t)));
+    WG_HIP(hipMemcpy(k, keys_in, bytes, hipMemcpyHostToDevice));
+    const bool alt = wg::DeviceRadixSort(k, v, k2, v2, n, key_bits, scr, nullptr, true);
+    WG_HIP(hipDeviceSynchronize());
+    WG_HIP(hipMemcpy(keys_out, alt ? k2 : k, bytes, hipMemcpyDeviceToHost));
+    WG_HIP(hipMemcpy(perm_out, alt ? v2 : v, bytes, hipMemcpyDeviceToHost));
+    for (uint32_t* q : {k, v, k2, v2, scr}) (void)hipFree(q);
+  });
+}
+
 int wgx_synth_visibilities(const double* uvw_dev, size_t n_rows, size_t n_chan,
                            const double* freq, int n_src, const double* src, double noise_sigma,
                            uint64_t seed, float* vis_dev) {

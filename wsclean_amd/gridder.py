@@ -280,6 +280,17 @@ class WGriddingGridder_Simple:
         return keys, perm[: nv.value], json.loads(buf.value.decode())
 
 
+def radix_sort(keys, key_bits):
+    """Test hook: the device radix sort of uint32 keys by their low key_bits
+    (stable); returns (sorted keys, element indices in sorted order)."""
+    lib = _lib.load()
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    ko = np.empty_like(k)
+    po = np.empty_like(k)
+    _lib.check(lib.wgx_radix_sort(k.size, int(key_bits), _ptr(k), _ptr(ko), _ptr(po)))
+    return ko, po
+
+
 def fft_lines(data, inverse=False):
     """Test hook: batched LDS FFT of each row of a complex64 2-D array."""
     lib = _lib.load()
