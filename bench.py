@@ -75,20 +75,7 @@ def copy_probe(mib):
 
 def make_inputs(n_chan, npix, seed=1002):
     from wsclean_amd import synth
-    rng = np.random.Generator(np.random.PCG64(seed))
-    enu = synth.antennas_logradial(rng, 128, 10000.0)
-    uvw = synth.earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
-    freq = np.linspace(120e6, 180e6, 64)[:n_chan] if n_chan <= 64 else \
-        np.linspace(120e6, 180e6, n_chan)
-    dx = 2.0 * synth.ARCSEC
-    l, m, flux, _, _ = synth.point_sky(rng, 500, npix, dx)
-    gl, gm, gflux, _, _ = synth.point_sky(rng, 5, npix, dx)
-    gsig = rng.uniform(10.0, 60.0, 5) * synth.ARCSEC
-    src = np.zeros((505, 5))
-    src[:500, 0], src[:500, 1], src[:500, 3] = l, m, flux
-    src[500:, 0], src[500:, 1], src[500:, 3], src[500:, 4] = gl, gm, gflux, gsig
-    src[:, 2] = np.sqrt(1.0 - src[:, 0] ** 2 - src[:, 1] ** 2) - 1.0
-    return uvw, freq, dx, src
+    return synth.large_wgridding_inputs(n_chan, npix, seed)
 
 
 def roofline_bytes(plan, n_vis, n_rows, nx, ny):
@@ -477,38 +464,8 @@ def serving_leg(args):
 
 
 def make_facet_inputs(share=8, seed=1004):
-    """Batched-facets workload: a 12000^2 field at 2 arcsec split into a
-    6 x 6 grid of 36 facets of uneven sizes (each side drawn from
-    1024 ... 3072, multiple of 16), each centred on its grid cell (its own
-    l/m shift).  Facet f takes a log-uniform 1e6 ... 5e7 rows (capped at
-    the track's 23.4e6) of the 128-antenna 10 km, 2880 x 10 s track, as a
-    random contiguous range, at 16 channels over 120-180 MHz.  One GPU takes
-    1/share of every facet's rows (the row shard of an 8-GPU node)."""
     from wsclean_amd import synth
-    rng = np.random.Generator(np.random.PCG64(seed))
-    rng_arr = np.random.Generator(np.random.PCG64(1002))
-    enu = synth.antennas_logradial(rng_arr, 128, 10000.0)
-    uvw = synth.earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
-    freq = np.linspace(120e6, 180e6, 16)
-    dx = 2.0 * synth.ARCSEC
-    R = uvw.shape[0]
-    facets, row_off, n_rows = [], [], []
-    for i in range(36):
-        gx, gy = i % 6, i // 6
-        nx = int(rng.integers(1024, 3073)) // 16 * 16
-        ny = int(rng.integers(1024, 3073)) // 16 * 16
-        rows = int(np.exp(rng.uniform(np.log(1e6), np.log(5e7))))
-        rows = min(rows, R) // share
-        row_off.append(int(rng.integers(0, R - rows + 1)))
-        n_rows.append(rows)
-        facets.append(dict(width=nx, height=ny, pixel_size_x=dx, pixel_size_y=dx,
-                           l_shift=(6000 - (gx + 0.5) * 2000) * dx,
-                           m_shift=((gy + 0.5) * 2000 - 6000) * dx, epsilon=1e-5))
-    l, m, flux, _, _ = synth.point_sky(rng, 500, 12000, dx)
-    src = np.zeros((500, 5))
-    src[:, 0], src[:, 1], src[:, 3] = l, m, flux
-    src[:, 2] = np.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
-    return uvw, freq, facets, row_off, n_rows, src
+    return synth.batched_facets_workload(share, seed)
 
 
 def facets_leg(args):

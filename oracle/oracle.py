@@ -92,6 +92,29 @@ def dft_invert_pixels(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, xs, ys, 
     return out
 
 
+def dft_invert_pixels_fast(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, xs, ys,
+                           nthreads=0, return_mode=False):
+    """dft_invert_pixels for large visibility sets (threads over rows, the
+    per-channel phasors of evenly spaced channels by recurrence).  Pinned
+    against dft_invert_pixels by tests/test_oracle.py.  With return_mode,
+    also returns whether the recurrence was used."""
+    lib = load()
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex64)
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    out = np.zeros(xs.size, np.float64)
+    used = ctypes.c_int(0)
+    f = lib.oracle_dft_invert_pixels_fast
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
+        [ctypes.c_size_t, ctypes.c_size_t] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t] + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p]
+    f(nx, ny, dx, dy, l_shift, m_shift, uvw.size // 3, freq.size, _p(uvw), _p(freq), _p(vis),
+      xs.size, _p(xs), _p(ys), _p(out), _nthreads(nthreads), ctypes.byref(used))
+    return (out, bool(used.value)) if return_mode else out
+
+
 def dft_predict(nx, ny, dx, dy, l_shift, m_shift, image, uvw, freq, sel=None, nthreads=0):
     """Visibilities from image [ny, nx]; complex128 [n_rows, n_chan] (or [len(sel)])."""
     lib = load()

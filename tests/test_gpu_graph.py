@@ -88,3 +88,43 @@ def test_facet_batch_graph_replays_match_eager():
             np.testing.assert_array_equal(fb.RealImage(f), imgs[f])
         assert torch.equal(out_g, out_e), "replay %d" % rep
     fb.close()
+
+
+def test_graph_refuses_replay_after_buffers_move():
+    # an eager call that needs larger shared buffers (box, line buffers,
+    # subgrid scratch) than the captured cycle reallocates them: the replay
+    # then refuses instead of touching freed memory, and a new capture works
+    import torch
+    dev = torch.device("cuda:0")
+    p = synth.small_problem(seed=41, n_ant=16, n_times=30, n_chan=3, npix=128,
+                            pixel_arcsec=15.0, noise=0.05)
+    n_rows, n_chan = p["uvw"].shape[0], len(p["freq"])
+    uvw = torch.from_numpy(p["uvw"]).to(dev)
+    vis = torch.from_numpy(p["vis"].view(np.float32).reshape(n_rows, n_chan, 2).copy()).to(dev)
+    g = WGriddingGridder_Simple(128, 128, 128, 128, p["dx"], p["dx"], 0.0, 0.0, 1, 1e-5)
+    g.PrepareVisibilities(uvw, p["freq"], n_rows, n_chan)
+    out = torch.empty_like(vis)
+    g.CaptureMajorCycle(vis, out, 1.0)
+    img_e = g.RealImage()
+    out_e = out.clone()
+    # same-size eager calls on the handle keep the graph
+    g.InitializeInversion()
+    g.AddInversionData(n_rows, n_chan, p["uvw"], p["freq"], p["vis"])
+    g.ReplayMajorCycle(1)
+    np.testing.assert_array_equal(g.RealImage(), img_e)
+    # a wider uv box (longer baselines, finer tiling) grows the shared buffers
+    big = synth.small_problem(seed=42, n_ant=24, n_times=40, n_chan=3, npix=128,
+                              pixel_arcsec=15.0, radius_m=6000.0)
+    g.set_tuning(sub_slab=2)
+    g.InitializeInversion()
+    g.AddInversionData(big["uvw"].shape[0], 3, big["uvw"], big["freq"], big["vis"])
+    g.set_tuning()
+    with pytest.raises(WgError):
+        g.ReplayMajorCycle(1)
+    with pytest.raises(WgError):   # the graph was dropped
+        g.ReplayMajorCycle(1)
+    g.CaptureMajorCycle(vis, out, 1.0)
+    g.ReplayMajorCycle(2)
+    np.testing.assert_array_equal(g.RealImage(), img_e)
+    assert torch.equal(out, out_e)
+    g.close()

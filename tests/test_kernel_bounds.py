@@ -219,3 +219,47 @@ def test_bucket_perm_order(seed, n_chan):
     np.testing.assert_array_equal(oracle.bucket_perm(plan, uvw_s, freq, vis_s), ref)
     # the renumbering is not trivial for the unsorted rows
     assert not np.array_equal(order, np.arange(order.size))
+
+
+_P96 = dict(seed=1001, n_ant=16, n_times=24, n_chan=2, npix=96, noise=0.01)
+_P512 = dict(seed=5, n_ant=20, n_times=40, n_chan=2, npix=512, pixel_arcsec=30.0,
+             radius_m=6000.0, n_src=8, dec_deg=-70.0, dt_s=600.0)
+_P512B = dict(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512, pixel_arcsec=30.0,
+              radius_m=6000.0, dec_deg=-70.0, dt_s=600.0)
+
+
+@pytest.mark.parametrize("kw,eps,tune,one_fft_slab", [
+    # forced VALU shapes (W >= 10), several sub-slabs and FFT slabs
+    (_P96, 1e-5, dict(W=10, sigma=1.5), False),
+    (_P96, 1e-5, dict(W=16, sigma=1.8), False),
+    (_P512, 1e-5, dict(W=10, sigma=1.5), False),
+    (_P512, 1e-5, dict(W=12, sigma=1.6), False),
+    (_P512, 1e-5, dict(W=14, sigma=1.7), False),
+    (_P512, 1e-5, dict(W=16, sigma=1.8), False),
+    (_P512, 1e-4, dict(sub_slab=4), False),
+    (_P512B, 1e-5, dict(sub_slab=4), False),
+    (_P512B, 1e-5, dict(sub_slab=4), True),
+    (_P512, 1e-5, dict(W=10, sigma=1.5, sub_slab=4), False),
+    (_P512, 1e-5, dict(W=12, sigma=1.6, sub_slab=3), True),
+    (_P96, 2e-6, {}, False),
+    (_P96, 1e-6, {}, False),
+    (_P96, 7e-7, {}, False),
+])
+def test_kernel_index_ranges_wide_paths(kw, eps, tune, one_fft_slab):
+    p = synth.small_problem(**kw)
+    n = kw["npix"]
+    plan = make_plan(n, p["dx"], eps, p["uvw"], p["freq"], **tune)
+    if one_fft_slab:
+        plan["Sf"] = plan["Sg"]   # the smallest FFT slab a byte budget can give
+    check_plan_bounds(plan, p["uvw"], p["freq"], p["vis"])
+
+
+def test_kernel_index_ranges_wide_field_default():
+    # 4096^2 at 20 arcsec, 128 antennas in a 2.5 km disc (W 10, 2 sub-slabs)
+    rng = np.random.Generator(np.random.PCG64(77))
+    enu = synth.antennas_disc(rng, 128, 2500.0)
+    uvw = synth.earth_rotation_uvw(enu, 60, 120.0, np.deg2rad(-30.0))
+    freq = np.array([150e6])
+    plan = make_plan(4096, 20.0 * synth.ARCSEC, 1e-5, uvw, freq)
+    assert plan["W"] >= 10 and plan["P"] > plan["Sg"]
+    check_plan_bounds(plan, uvw, freq, None)

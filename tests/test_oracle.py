@@ -123,6 +123,30 @@ def test_pixel_subset_and_vis_subset(oracle):
     np.testing.assert_allclose(Vs, V.ravel()[sel], rtol=1e-12)
 
 
+@pytest.mark.parametrize("uniform", [True, False])
+def test_fast_pixel_dft_matches_direct(oracle, uniform):
+    # the row-parallel recurrence form against the per-term cos / sin form:
+    # 64 evenly spaced channels (the recurrence's longest chain) and, with
+    # one channel moved, the per-channel fallback; a wrong step phasor or a
+    # dropped channel fails at far more than 1e-11
+    p = synth.small_problem(seed=41, n_ant=12, n_times=12, n_chan=64, npix=128,
+                            pixel_arcsec=30.0, radius_m=4000.0, df=0.9e6, noise=0.1)
+    freq = p["freq"].copy()
+    if not uniform:
+        freq[17] += 0.3e6
+    rng = np.random.default_rng(2)
+    xs, ys = rng.integers(0, 128, 40), rng.integers(0, 128, 40)
+    vis = p["vis"].copy()
+    vis[3, 5] = 0
+    ref = oracle.dft_invert_pixels(128, 128, p["dx"], p["dx"], 0.01, -0.02, p["uvw"], freq,
+                                   vis, xs, ys)
+    got, used = oracle.dft_invert_pixels_fast(128, 128, p["dx"], p["dx"], 0.01, -0.02,
+                                              p["uvw"], freq, vis, xs, ys, nthreads=3,
+                                              return_mode=True)
+    assert used == uniform
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
 def test_golden_fixture(oracle):
     g = np.load(os.path.join(HERE, "golden", "dft_small.npz"))
     img = oracle.dft_invert(int(g["nx"]), int(g["ny"]), float(g["dx"]), float(g["dy"]),

@@ -58,6 +58,22 @@ __global__ void LoadModelKernel(const float* __restrict__ image, double* __restr
   }
 }
 
+// Device allocation owned for the duration of one entry point: freed on
+// every exit path, exceptions included.
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) { WG_HIP(hipMalloc(&p, bytes ? bytes : 16)); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
 unsigned Blocks(size_t n) {
   size_t b = (n + 255) / 256;
   return unsigned(b < 65536 ? (b ? b : 1) : 65536);
@@ -158,6 +174,7 @@ struct wg_handle {
   bool prepared = false;
   std::string plan_json;
   hipGraphExec_t cycle = nullptr;  // captured major cycle on vs_prep
+  uint64_t cycle_gen = 0;          // arena generation the graph's pointers belong to
   std::mutex mu;
 
   ~wg_handle() { DropGraph(cycle); }
@@ -171,6 +188,7 @@ struct wg_facet_batch {
   std::vector<double*> img;  // per facet [width_t][height_t] fp64
   std::string plan_json;
   hipGraphExec_t cycle = nullptr;  // captured major cycle of the batch
+  uint64_t cycle_gen = 0;          // arena generation the graph's pointers belong to
   std::mutex mu;
 
   ~wg_facet_batch() { DropGraph(cycle); }
@@ -553,6 +571,7 @@ int wg_capture_major_cycle(wg_handle* h, const float* vis_dev, float* vis_out_de
       if (what & 2)
         h->eng->Predict(h->vs_prep, h->img, reinterpret_cast<float2*>(vis_out_dev));
     });
+    h->cycle_gen = h->eng->Arena().Generation();
   });
 }
 
@@ -561,6 +580,13 @@ int wg_replay_major_cycle(wg_handle* h, int count) {
     WG_CHECK(h && count >= 0, "null handle or negative count");
     std::lock_guard<std::mutex> lk(h->mu);
     WG_CHECK(h->cycle, "no captured major cycle (wg_capture_major_cycle)");
+    if (h->eng->Arena().Generation() != h->cycle_gen) {
+      // a later call on this handle grew or released a buffer the graph
+      // addresses: replaying would touch freed device memory
+      DropGraph(h->cycle);
+      throw wg::Error("a later call reallocated device buffers of the captured major cycle; "
+                      "capture it again (wg_capture_major_cycle)");
+    }
     for (int i = 0; i < count; ++i) WG_HIP(hipGraphLaunch(h->cycle, h->s()));
     WG_HIP(hipStreamSynchronize(h->s()));
   });
@@ -712,17 +738,14 @@ int wgx_fft(int n, int nlines, int inverse, float* data) {
   return Guard([&] {
     WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");
     const std::vector<float2> tw = wg::FFTTwiddleTable(n);
-    float2 *dtw = nullptr, *dd = nullptr;
     const size_t bytes = size_t(n) * nlines * sizeof(float2);
-    WG_HIP(hipMalloc(&dtw, tw.size() * sizeof(float2)));
-    WG_HIP(hipMalloc(&dd, bytes));
-    WG_HIP(hipMemcpy(dtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
-    WG_HIP(hipMemcpy(dd, data, bytes, hipMemcpyHostToDevice));
-    wg::LaunchLineFFT(dd, nlines, wg::MakeFFTDesc(n, dtw), inverse != 0, nullptr);
+    DevBuf dtw(tw.size() * sizeof(float2)), dd(bytes);
+    WG_HIP(hipMemcpy(dtw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    WG_HIP(hipMemcpy(dd.p, data, bytes, hipMemcpyHostToDevice));
+    wg::LaunchLineFFT(dd.as<float2>(), nlines, wg::MakeFFTDesc(n, dtw.as<float2>()),
+                      inverse != 0, nullptr);
     WG_HIP(hipDeviceSynchronize());
-    WG_HIP(hipMemcpy(data, dd, bytes, hipMemcpyDeviceToHost));
-    (void)hipFree(dd);
-    (void)hipFree(dtw);
+    WG_HIP(hipMemcpy(data, dd.p, bytes, hipMemcpyDeviceToHost));
   });
 }
 
@@ -732,19 +755,16 @@ int wgx_radix_sort(size_t n, int key_bits, const uint32_t* keys_in, uint32_t* ke
     WG_CHECK(keys_in && keys_out && perm_out, "null argument");
     WG_CHECK(key_bits >= 1 && key_bits <= 32 && n < 0x7FFFFFFFull, "bad arguments");
     if (n == 0) return;
-    uint32_t *k = nullptr, *v = nullptr, *k2 = nullptr, *v2 = nullptr, *scr = nullptr;
     const size_t bytes = n * sizeof(uint32_t);
-    WG_HIP(hipMalloc(&k, bytes));
-    WG_HIP(hipMalloc(&v, bytes));
-    WG_HIP(hipMalloc(&k2, bytes));
-    WG_HIP(hipMalloc(&v2, bytes));
-    WG_HIP(hipMalloc(&scr, wg::RadixSortScratchWords(n) * sizeof(uint32_t)));
-    WG_HIP(hipMemcpy(k, keys_in, bytes, hipMemcpyHostToDevice));
-    const bool alt = wg::DeviceRadixSort(k, v, k2, v2, n, key_bits, scr, nullptr, true);
+    DevBuf k(bytes), v(bytes), k2(bytes), v2(bytes);
+    DevBuf scr(wg::RadixSortScratchWords(n) * sizeof(uint32_t));
+    WG_HIP(hipMemcpy(k.p, keys_in, bytes, hipMemcpyHostToDevice));
+    const bool alt = wg::DeviceRadixSort(k.as<uint32_t>(), v.as<uint32_t>(), k2.as<uint32_t>(),
+                                         v2.as<uint32_t>(), n, key_bits, scr.as<uint32_t>(),
+                                         nullptr, true);
     WG_HIP(hipDeviceSynchronize());
-    WG_HIP(hipMemcpy(keys_out, alt ? k2 : k, bytes, hipMemcpyDeviceToHost));
-    WG_HIP(hipMemcpy(perm_out, alt ? v2 : v, bytes, hipMemcpyDeviceToHost));
-    for (uint32_t* q : {k, v, k2, v2, scr}) (void)hipFree(q);
+    WG_HIP(hipMemcpy(keys_out, alt ? k2.p : k.p, bytes, hipMemcpyDeviceToHost));
+    WG_HIP(hipMemcpy(perm_out, alt ? v2.p : v.p, bytes, hipMemcpyDeviceToHost));
   });
 }
 
@@ -755,17 +775,13 @@ int wgx_synth_visibilities(const double* uvw_dev, size_t n_rows, size_t n_chan,
     WG_CHECK(uvw_dev && freq && vis_dev && (n_src == 0 || src), "null argument");
     std::vector<double> scale(n_chan);
     for (size_t c = 0; c < n_chan; ++c) scale[c] = freq[c] / wg::kSpeedOfLight;
-    double *dscale = nullptr, *dsrc = nullptr;
-    WG_HIP(hipMalloc(&dscale, n_chan * sizeof(double)));
-    WG_HIP(hipMalloc(&dsrc, (size_t(n_src) * 5 + 1) * sizeof(double)));
-    WG_HIP(hipMemcpy(dscale, scale.data(), n_chan * sizeof(double), hipMemcpyHostToDevice));
+    DevBuf dscale(n_chan * sizeof(double)), dsrc((size_t(n_src) * 5 + 1) * sizeof(double));
+    WG_HIP(hipMemcpy(dscale.p, scale.data(), n_chan * sizeof(double), hipMemcpyHostToDevice));
     if (n_src)
-      WG_HIP(hipMemcpy(dsrc, src, size_t(n_src) * 5 * sizeof(double), hipMemcpyHostToDevice));
-    wg::LaunchSynth(uvw_dev, n_rows, n_chan, dscale, dsrc, n_src, noise_sigma, seed,
-                    reinterpret_cast<float2*>(vis_dev), nullptr);
+      WG_HIP(hipMemcpy(dsrc.p, src, size_t(n_src) * 5 * sizeof(double), hipMemcpyHostToDevice));
+    wg::LaunchSynth(uvw_dev, n_rows, n_chan, dscale.as<double>(), dsrc.as<double>(), n_src,
+                    noise_sigma, seed, reinterpret_cast<float2*>(vis_dev), nullptr);
     WG_HIP(hipDeviceSynchronize());
-    (void)hipFree(dscale);
-    (void)hipFree(dsrc);
   });
 }
 
@@ -944,6 +960,7 @@ int wg_facet_batch_capture_major_cycle(wg_facet_batch* b, const float* vis_dev,
       }
       if (what & 2) b->fb->Predict(cimg.data(), reinterpret_cast<float2*>(vis_out_dev));
     });
+    b->cycle_gen = b->fb->Eng().Arena().Generation();
   });
 }
 
@@ -952,6 +969,11 @@ int wg_facet_batch_replay_major_cycle(wg_facet_batch* b, int count) {
     WG_CHECK(b && count >= 0, "null handle or negative count");
     std::lock_guard<std::mutex> lk(b->mu);
     WG_CHECK(b->cycle, "no captured major cycle (wg_facet_batch_capture_major_cycle)");
+    if (b->fb->Eng().Arena().Generation() != b->cycle_gen) {
+      DropGraph(b->cycle);
+      throw wg::Error("a later call reallocated device buffers of the captured major cycle; "
+                      "capture it again (wg_facet_batch_capture_major_cycle)");
+    }
     for (int i = 0; i < count; ++i) WG_HIP(hipGraphLaunch(b->cycle, b->s()));
     WG_HIP(hipStreamSynchronize(b->s()));
   });

@@ -45,15 +45,24 @@ struct KernelShape {
 };
 
 // Expected relative RMS error of one shape: aliasing in 3 dimensions with a
-// factor-3 margin, plus FP32 rounding of grid, FFT and accumulation
-// amplified by the grid correction in u, v and w (calibrated against the
-// FP64 direct FT: 6e-9 * amp_rms^3 measured, 1e-8 used).  The aliasing
-// margin covers weighted data: uniformly weighted visibilities (dirty
-// beams with strong far sidelobes) were measured at up to 4.4 x err1d
-// against the direct FT, naturally weighted ones at up to 0.5 x err1d.
+// factor-3 margin, plus the FP32 rounding of grid, FFT and accumulation
+// amplified by the grid correction in u, v and w, plus a floor of FP32
+// rounding that no shape removes.  The aliasing margin covers weighted
+// data: uniformly weighted visibilities (dirty beams with strong far
+// sidelobes) were measured at up to 4.4 x err1d against the direct FT,
+// naturally weighted ones at up to 0.5 x err1d.  The FP32 terms are fitted
+// to the worst of invert and predict against the FP64 direct FT on a 96^2
+// and a 4 degree 512^2 field: W 10 .. 16 at amp_rms^3 = 330 .. 555 measured
+// 3e-6 .. 1.25e-5 (2.3e-8 amp^3 at most), W 10 at sigma 1.65 / 1.8 measured
+// 1.16e-6 / 8.3e-7, W 8 at sigma 2 measured 9.2e-7 on the 4096^2 study.
+constexpr double kFp32AmpTerm = 4e-8;
+constexpr double kFp32Floor = 3e-7;
 inline double ShapeErrorEstimate(const KernelShape& k) {
-  return 3.0 * k.err1d + 1e-8 * k.amp_rms * k.amp_rms * k.amp_rms;
+  return 3.0 * k.err1d + kFp32AmpTerm * k.amp_rms * k.amp_rms * k.amp_rms + kFp32Floor;
 }
+
+// Smallest epsilon any shape of the table reaches with FP32 grids.
+double Fp32EpsilonFloor();
 
 const std::vector<KernelShape>& KernelTable();
 

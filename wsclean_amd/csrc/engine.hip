@@ -124,6 +124,7 @@ void* DeviceArena::Get(const std::string& name, size_t bytes) {
   Slot& s = slots_[name];
   if (bytes == 0) bytes = 256;
   if (s.bytes < bytes) {
+    if (s.ptr) ++generation_;  // a new slot leaves existing pointers valid
     Free(s.ptr);
     s.ptr = nullptr;
     s.bytes = 0;
@@ -141,11 +142,13 @@ void* DeviceArena::Get(const std::string& name, size_t bytes) {
 void DeviceArena::Release(const std::string& name) {
   auto it = slots_.find(name);
   if (it == slots_.end()) return;
+  ++generation_;
   Free(it->second.ptr);
   slots_.erase(it);
 }
 
 void DeviceArena::ReleaseAll() {
+  if (!slots_.empty()) ++generation_;
   for (auto& kv : slots_) Free(kv.second.ptr);
   slots_.clear();
 }

@@ -53,6 +53,10 @@ class DeviceArena {
   void Release(const std::string& name);
   void ReleaseAll();
   size_t Allocated() const;
+  // Bumped whenever a slot is freed or moved to a new allocation: device
+  // pointers taken from the arena (for example by a captured HIP graph)
+  // stay valid exactly while the generation is unchanged.
+  uint64_t Generation() const { return generation_; }
 
  private:
   struct Slot {
@@ -63,6 +67,7 @@ class DeviceArena {
   std::map<std::string, Slot> slots_;
   hipStream_t stream_ = nullptr;
   bool async_ = false;  // device supports stream-ordered allocation
+  uint64_t generation_ = 0;
 };
 
 // Per-call timing of the pipeline stages (ms, measured with HIP events).

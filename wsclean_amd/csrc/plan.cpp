@@ -50,6 +50,12 @@ const std::vector<KernelShape>& KernelTable() {
   return t;
 }
 
+double Fp32EpsilonFloor() {
+  double m = 1e300;
+  for (const KernelShape& k : KernelTable()) m = std::min(m, ShapeErrorEstimate(k));
+  return m;
+}
+
 double EsKernelFT(int W, double beta, double t) {
   // Phi(t) = (W/2) * int_{-1}^{1} phi(x) cos(pi W x t) dx with x = sin(th):
   // the integrand exp(beta (cos th - 1)) cos(pi W t sin th) cos th is analytic.
@@ -190,7 +196,17 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
       chosen = k;
     }
   }
-  WG_CHECK(chosen.W > 0, "no kernel shape reaches epsilon " + std::to_string(g.epsilon));
+  if (chosen.W == 0) {
+    char msg[320];
+    std::snprintf(msg, sizeof(msg),
+                  "epsilon %.3g is below what the FP32 grids and FFT of this library reach "
+                  "(%.2g with any kernel shape); the reference switches to FP64 gridding "
+                  "there, which this library does not provide",
+                  g.epsilon, Fp32EpsilonFloor());
+    WG_CHECK(g.epsilon >= Fp32EpsilonFloor(), msg);
+    WG_CHECK(false, "no kernel shape reaches epsilon " + std::to_string(g.epsilon) +
+                        " within the oversampling limits");
+  }
   p.W = chosen.W;
   p.sigma = chosen.sigma;
   p.beta = chosen.beta;

@@ -298,3 +298,17 @@ def fft_lines(data, inverse=False):
     nlines, n = a.shape
     _lib.check(lib.wgx_fft(n, nlines, 1 if inverse else 0, _ptr(a)))
     return a
+
+
+def synth_visibilities(uvw_dev, freq, src, noise_sigma, seed, vis_dev):
+    """Benchmark / test input hook (wgx_synth_visibilities): fills the
+    device tensor vis_dev [n_rows, n_chan, 2] float32 with the visibilities
+    of the sources src (n_src x {l, m, n - 1, S, gaussian sigma}) at the
+    device uvw [n_rows, 3] (metres) plus seeded Gaussian noise."""
+    lib = _lib.load()
+    freq = _f64(freq)
+    src = np.ascontiguousarray(src, dtype=np.float64).reshape(-1, 5)
+    n_rows = int(uvw_dev.shape[0])
+    _lib.check(lib.wgx_synth_visibilities(
+        ctypes.c_void_p(uvw_dev.data_ptr()), n_rows, freq.size, _ptr(freq), src.shape[0],
+        _ptr(src), float(noise_sigma), int(seed), ctypes.c_void_p(vis_dev.data_ptr())))

@@ -51,6 +51,10 @@ class Request:
     start: float = 0.0
     done: float = 0.0
     image: np.ndarray = field(default=None, repr=False)
+    # keep the last cycle's residual visibilities (device tensor) in
+    # `residual`, so that the last dirty image can be checked
+    keep_residual: bool = False
+    residual: object = field(default=None, repr=False)
 
     @property
     def n_rows(self):
@@ -121,6 +125,8 @@ def run_request(req):
             if c + 1 < req.cycles:
                 torch.sub(req.vis, req.out, out=residual)
                 stream.synchronize()
+        if req.keep_residual:
+            req.residual = residual
         return img.view(n, n).cpu().numpy()
     finally:
         g.close()
@@ -227,14 +233,15 @@ def latency_stats(requests):
 
 
 def make_requests(n_requests, rate_hz, seed=1005, device="cuda:0", npix_choices=None,
-                  vis_range=(1e5, 5e7), cycles=5, eps=1e-5):
+                  vis_range=(1e5, 5e7), cycles=5, eps=1e-5, specs=None):
     """Seeded served-stream workload: Poisson arrivals at ``rate_hz``; image
     size drawn from ``npix_choices`` (512^2 ... 8192^2), 1 arcsec pixels; a
     random array of 32-256 antennas uniform in a 3 km disc, 1-64 channels
     over 120-180 MHz, 10 s time steps at dec -30 deg until a visibility
     count drawn log-uniformly from ``vis_range`` is reached; 20 power-law
     point sources plus Gaussian noise (sigma 0.01), generated on the
-    device."""
+    device.  ``specs`` (optional): one (npix, visibility count) pair per
+    request in place of the random image size and count."""
     import torch
     rng = np.random.Generator(np.random.PCG64(seed))
     npix_choices = npix_choices or [512, 1024, 2048, 4096, 8192]
@@ -249,6 +256,8 @@ def make_requests(n_requests, rate_hz, seed=1005, device="cuda:0", npix_choices=
         n_ant = int(rng.integers(32, 257))
         n_chan = int(rng.integers(1, 65))
         target = float(np.exp(rng.uniform(np.log(vis_range[0]), np.log(vis_range[1]))))
+        if specs is not None:
+            npix, target = int(specs[rid][0]), float(specs[rid][1])
         n_bl = n_ant * (n_ant - 1) // 2
         n_times = max(1, int(round(target / (n_bl * n_chan))))
         enu = synth.antennas_disc(rng, n_ant, 1500.0)

@@ -115,3 +115,65 @@ def small_problem(seed=1001, n_ant=12, n_times=20, n_chan=2, npix=64, pixel_arcs
         vis = add_noise(rng, vis, noise)
     return dict(uvw=uvw, freq=freq, vis=vis.astype(np.complex64), npix=npix, dx=dx,
                 l=l, m=m, flux=flux, xs=xs, ys=ys, seed=seed)
+
+
+def large_wgridding_inputs(n_chan=64, npix=8192, seed=1002):
+    """The large w-gridding workload: 128 antennas in a seeded 10 km
+    log-radial layout (8128 baselines), 2880 x 10 s at dec -30 deg, n_chan
+    channels over 120-180 MHz (64: 1.498e9 visibilities), 2 arcsec pixels,
+    a sky of 500 power-law point sources and 5 circular Gaussians (10-60
+    arcsec).  Returns (uvw [rows, 3] m, freq [n_chan] Hz, pixel size (rad),
+    src [505, 5] = {l, m, n - 1, S, gaussian sigma}); the visibilities come
+    from the device generator (gridder.synth_visibilities, noise 0.05)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    enu = antennas_logradial(rng, 128, 10000.0)
+    uvw = earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
+    freq = np.linspace(120e6, 180e6, 64)[:n_chan] if n_chan <= 64 else \
+        np.linspace(120e6, 180e6, n_chan)
+    dx = 2.0 * ARCSEC
+    l, m, flux, _, _ = point_sky(rng, 500, npix, dx)
+    gl, gm, gflux, _, _ = point_sky(rng, 5, npix, dx)
+    gsig = rng.uniform(10.0, 60.0, 5) * ARCSEC
+    src = np.zeros((505, 5))
+    src[:500, 0], src[:500, 1], src[:500, 3] = l, m, flux
+    src[500:, 0], src[500:, 1], src[500:, 3], src[500:, 4] = gl, gm, gflux, gsig
+    src[:, 2] = np.sqrt(1.0 - src[:, 0] ** 2 - src[:, 1] ** 2) - 1.0
+    return uvw, freq, dx, src
+
+
+def batched_facets_workload(share=8, seed=1004, max_rows=None):
+    """The batched-facets workload: a 12000^2 field at 2 arcsec split into a
+    6 x 6 grid of 36 facets of uneven sizes (each side drawn from 1024 ...
+    3072, a multiple of 16), each centred on its grid cell (its own l/m
+    shift).  Facet f takes a log-uniform 1e6 ... 5e7 rows (capped at the
+    track's 23.4e6, then at max_rows) of the 128-antenna 10 km, 2880 x 10 s
+    track of large_wgridding_inputs, as a random contiguous range, at 16
+    channels over 120-180 MHz; 1/share of those rows (the row shard of a
+    `share`-GPU node).  Returns (uvw, freq, facets (dicts of constructor
+    arguments), row_off, n_rows, src [500, 5])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rng_arr = np.random.Generator(np.random.PCG64(1002))
+    enu = antennas_logradial(rng_arr, 128, 10000.0)
+    uvw = earth_rotation_uvw(enu, 2880, 10.0, np.deg2rad(-30.0))
+    freq = np.linspace(120e6, 180e6, 16)
+    dx = 2.0 * ARCSEC
+    R = uvw.shape[0]
+    facets, row_off, n_rows = [], [], []
+    for i in range(36):
+        gx, gy = i % 6, i // 6
+        nx = int(rng.integers(1024, 3073)) // 16 * 16
+        ny = int(rng.integers(1024, 3073)) // 16 * 16
+        rows = int(np.exp(rng.uniform(np.log(1e6), np.log(5e7))))
+        rows = min(rows, R) // share
+        if max_rows is not None:
+            rows = min(rows, int(max_rows))
+        row_off.append(int(rng.integers(0, R - rows + 1)))
+        n_rows.append(rows)
+        facets.append(dict(width=nx, height=ny, pixel_size_x=dx, pixel_size_y=dx,
+                           l_shift=(6000 - (gx + 0.5) * 2000) * dx,
+                           m_shift=((gy + 0.5) * 2000 - 6000) * dx, epsilon=1e-5))
+    l, m, flux, _, _ = point_sky(rng, 500, 12000, dx)
+    src = np.zeros((500, 5))
+    src[:, 0], src[:, 1], src[:, 3] = l, m, flux
+    src[:, 2] = np.sqrt(1.0 - l ** 2 - m ** 2) - 1.0
+    return uvw, freq, facets, row_off, n_rows, src
