@@ -338,6 +338,11 @@ int wgx_radix_sort(size_t n, int key_bits, const uint32_t* keys_in, uint32_t* ke
 int wgx_make_plan(const wg_params* params, const double* extents6, long long nvis,
                   int W, double sigma, int tile, int sub_slab, char* plan_json,
                   size_t plan_len);
+/* Host-only: the gridding kernels' polynomial form of the ES kernel for
+ * support W and shape beta: coef[16][12] (row k: coefficients of g^0 .. g^d
+ * of tap k, g = 2 f - 1; rows k >= W are zero) and the degree d.  Needs no
+ * GPU. */
+int wgx_es_poly(int W, double beta, float* coef, int* degree);
 /* Host-only: the facet-batch layout (JSON: atlas plan, per-facet tile
  * origin and plan) for facets with flipped-coordinate extents extents6
  * (6 per facet) and visibility counts nvis.  Needs no GPU. */

@@ -243,7 +243,8 @@ _P512B = dict(seed=17, n_ant=40, n_times=60, n_chan=8, npix=512, pixel_arcsec=30
     (_P512, 1e-5, dict(W=12, sigma=1.6, sub_slab=3), True),
     (_P96, 2e-6, {}, False),
     (_P96, 1e-6, {}, False),
-    (_P96, 7e-7, {}, False),
+    (_P96, 5e-7, {}, False),
+    (_P96, 3e-7, {}, False),
 ])
 def test_kernel_index_ranges_wide_paths(kw, eps, tune, one_fft_slab):
     p = synth.small_problem(**kw)

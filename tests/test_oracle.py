@@ -289,3 +289,37 @@ def test_correct_and_collapse_unitary_round_trip(oracle):
     toff = rng.integers(0, n_times, n_rows)
     out = oracle.correct_and_collapse(vis, ant, toff, gains, n_ant, 4, oracle.GAIN_FULL, True)
     np.testing.assert_allclose(out, 0.5 * (vis[..., 0] + vis[..., 3]), atol=1e-12)
+
+
+@pytest.mark.parametrize("W,beta", [(4, 8.8), (6, 12.48), (8, 16.0), (8, 17.68), (10, 20.65),
+                                    (12, 25.08), (14, 29.54), (16, 34.32)])
+def test_es_kernel_polynomial_against_closed_form(W, beta):
+    # the gridding kernels' per-tap polynomial of the ES kernel, evaluated as
+    # the device does (FP32 Horner in g = 2 f - 1), against phi(x) =
+    # exp(beta (sqrt(1 - x^2) - 1)) in FP64 on 4e5 fractional coordinates:
+    # phi-weighted RMS relative error below 1e-7 (W >= 8; 2e-7 at W 6, where
+    # the support edge's e^-beta = 4e-6 limits the fit) and every tap's
+    # largest error below e^-beta plus 3e-7 of the peak; taps >= W are zero
+    import ctypes
+    from wsclean_amd import _lib
+    lib = _lib.load()
+    coef = np.zeros((16, 12), np.float32)
+    deg = ctypes.c_int()
+    _lib.check(lib.wgx_es_poly(W, beta, coef.ctypes.data_as(ctypes.c_void_p), ctypes.byref(deg)))
+    D = deg.value
+    assert np.all(coef[W:] == 0) and np.all(coef[:, D + 1:] == 0)
+    f = np.random.default_rng(W).random(400000).astype(np.float32)
+    g = (np.float32(2) * f - np.float32(1)).astype(np.float32)
+    num = den = 0.0
+    for k in range(W):
+        acc = np.full_like(g, coef[k, D])
+        for d in range(D - 1, -1, -1):
+            acc = (acc * g + coef[k, d]).astype(np.float32)
+        x = (k - W / 2 + 1 - f.astype(np.float64)) * 2.0 / W
+        ref = np.exp(beta * (np.sqrt(np.maximum(0.0, 1.0 - x * x)) - 1.0))
+        err = acc.astype(np.float64) - ref
+        assert np.abs(err).max() < np.exp(-beta) + 3e-7, (k, np.abs(err).max())
+        num += (err ** 2).sum()
+        den += (ref ** 2).sum()
+    rms = np.sqrt(num / den)
+    assert rms < (1e-7 if W >= 8 else 2e-7 if W == 6 else 2e-5), rms

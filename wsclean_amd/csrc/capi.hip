@@ -700,6 +700,16 @@ int wgx_make_plan(const wg_params* params, const double* e, long long nvis, int 
   });
 }
 
+int wgx_es_poly(int W, double beta, float* coef, int* degree) {
+  return Guard([&] {
+    WG_CHECK(coef && degree, "null argument");
+    float t[wg::kEsPolyTaps][wg::kEsPolyCoefs];
+    wg::EsPolyCoefficients(W, beta, t);
+    std::memcpy(coef, t, sizeof(t));
+    *degree = wg::EsPolyDegree(W);
+  });
+}
+
 int wgx_facet_atlas(const wg_params* facets, size_t n_facets, const double* extents6,
                     const long long* nvis, char* json, size_t len) {
   return Guard([&] {

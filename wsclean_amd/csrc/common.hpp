@@ -34,6 +34,19 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::wg::Error(std::string(msg)); \
   } while (0)
 
+// Piecewise-polynomial form of the ES kernel used by the gridding kernels:
+// tap k of W is phi(x) at x = (k - W/2 + 1 - f) 2 / W as a polynomial of
+// degree EsPolyDegree(W) in g = 2 f - 1 (f in [0, 1), the fractional grid
+// coordinate).  Degrees are the smallest whose interpolation error is below
+// 3e-7 of phi's peak (W <= 8: the sqrt singularity at the support edge
+// limits it to ~e^-beta) or 1e-8 (W >= 10).
+constexpr int kEsPolyTaps = 16;
+constexpr int kEsPolyCoefs = 12;
+constexpr int EsPolyDegree(int W) { return W <= 8 ? 7 : (W <= 10 ? 8 : (W <= 12 ? 9 : 10)); }
+// out[k][d]: coefficient of g^d of tap k (rows k >= W and d > degree are 0),
+// from Chebyshev interpolation of phi in FP64, rounded to FP32.
+void EsPolyCoefficients(int W, double beta, float (*out)[kEsPolyCoefs]);
+
 // One row of the ES kernel parameter table (gen_es_table.py).
 struct KernelShape {
   int W;          // support in cells (even)
@@ -52,11 +65,12 @@ struct KernelShape {
 // sidelobes) were measured at up to 4.4 x err1d against the direct FT,
 // naturally weighted ones at up to 0.5 x err1d.  The FP32 terms are fitted
 // to the worst of invert and predict against the FP64 direct FT on a 96^2
-// and a 4 degree 512^2 field: W 10 .. 16 at amp_rms^3 = 330 .. 555 measured
-// 3e-6 .. 1.25e-5 (2.3e-8 amp^3 at most), W 10 at sigma 1.65 / 1.8 measured
-// 1.16e-6 / 8.3e-7, W 8 at sigma 2 measured 9.2e-7 on the 4096^2 study.
-constexpr double kFp32AmpTerm = 4e-8;
-constexpr double kFp32Floor = 3e-7;
+// field, a 4 degree 512^2 field and the 4096^2 precision study, with the
+// kernel evaluated as its per-tap polynomial: W 10 .. 16 at amp_rms^3 =
+// 100 .. 140 measured 4e-7 .. 3.2e-6 (2.3e-8 amp^3 at most), W 10 at sigma
+// 1.8 / 1.9 measured 1.8e-7 .. 3.6e-7, W 8 at sigma 1.9 measured 2.8e-7.
+constexpr double kFp32AmpTerm = 2.5e-8;
+constexpr double kFp32Floor = 1e-7;
 inline double ShapeErrorEstimate(const KernelShape& k) {
   return 3.0 * k.err1d + kFp32AmpTerm * k.amp_rms * k.amp_rms * k.amp_rms + kFp32Floor;
 }

@@ -36,8 +36,7 @@ DevPlan ToDevPlan(const Plan& p) {
   d.dw = p.dw;
   d.dx_rad = p.img.dx;
   d.dy_rad = p.img.dy;
-  d.beta2 = float(p.beta * 1.4426950408889634);
-  d.two_over_w = 2.0f / float(p.W);
+  EsPolyCoefficients(p.W, p.beta, d.es_poly);
   d.W = p.W;
   d.T = p.T;
   d.ntu = p.ntu;

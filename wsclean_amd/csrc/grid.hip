@@ -55,9 +55,11 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int kcol = lane % W, lrow = lane / W;
-  // tap abscissa of the lane's column; lane t (< W) also evaluates the w
-  // weight of tap t (plane p0 + t)
-  const float xk = float(kcol - W / 2 + 1);
+  // the lane evaluates tap kcol of u, v and w; lane t (< W, row 0) holds
+  // the v and w weights of tap t (plane p0 + t)
+  constexpr int D = EsPolyDegree(W);
+  float ck[D + 1];
+  EsPolyLoad<D>(dp, kcol, ck);
   const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
   for (uint32_t it = it0 + blockIdx.x; it < it1; it += gridDim.x) {
     const WorkItem wi = items[it];
@@ -123,15 +125,17 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
           cur = pk;
         }
         // per-tap w weights times the visibility, one tap per lane
-        const float kwl = EsKernel((xk - fw) * dp.two_over_w, dp.beta2);
+        const float kwl = EsPoly<D>(ck, fw);
         const float wrl = vr * kwl, wil = vi * kwl;
-        const float ku = EsKernel((xk - fu) * dp.two_over_w, dp.beta2);
+        const float ku = EsPoly<D>(ck, fu);
+        const float kvc = EsPoly<D>(ck, fv);
         float kv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int l = lrow + r * RPP;
-          kv[r] = (l < W) ? ku * EsKernel((float(l - W / 2 + 1) - fv) * dp.two_over_w, dp.beta2)
-                          : 0.f;
+          // lane l (< W) evaluated v tap l
+          const float kvl = __shfl(kvc, l < W ? l : 0);
+          kv[r] = (l < W) ? ku * kvl : 0.f;
         }
 #pragma unroll
         for (int t = 0; t < W; ++t) {
@@ -293,14 +297,6 @@ __device__ __forceinline__ float DppMerge(float own) {
                                                     CTRL, 0xf, BANKS, true));
 }
 
-// ES kernel at x = xc - f * c2 (xc = tap abscissa * c2, nc2 = -c2), with the
-// scale folded into FMAs: exp2(b2 (sqrt(1 - x^2) - 1)).
-__device__ __forceinline__ float EsKernelAt(float f, float nc2, float xc, float b2) {
-  const float x = __builtin_fmaf(f, nc2, xc);
-  const float a = fmaxf(__builtin_fmaf(-x, x, 1.0f), 0.0f);
-  return __builtin_amdgcn_exp2f(__builtin_fmaf(__builtin_amdgcn_sqrtf(a), b2, -b2));
-}
-
 // FUSED: vis is the caller's [row][chan] buffer, read through perm and
 // rotated by phasor (no gathered copy); otherwise vis is sorted.
 template <int W, bool FUSED>
@@ -338,9 +334,11 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
   const int tsig = 2 * (r16 & 3) + (r16 >> 3);  // w tap this lane evaluates
   const bool tap_ok = tap < W, w_ok = tsig < W;
   const bool q_hi = (r16 & 4) != 0;  // quad 1 or 3 of the 16-lane row
-  const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
-  const float xc1 = float(tap - W / 2 + 1) * c2;
-  const float xcw = float(tsig - W / 2 + 1) * c2;
+  // kernel polynomials of the lane's u / v tap and of its w tap
+  constexpr int D = EsPolyDegree(W);
+  float cu[D + 1], cw[D + 1];
+  EsPolyLoad<D>(dp, tap, cu);
+  EsPolyLoad<D>(dp, tsig, cw);
   // flush coordinates of the lane's D entries: D_m[4 (L/16) + r][r16]
   const int f_c = (lane >> 4) >> 1;                 // row i = 4 (L/16) + r -> c = i / 8
   const int f_k = r16 & 7, f_tp = r16 >> 3;          // column -> k, t'
@@ -449,13 +447,13 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         n0 = ReadStage(view, int(ns & uint32_t(2 * kStage - 1)));
         n1 = ReadStage(view, int((ns + 4) & uint32_t(2 * kStage - 1)));
         // kernel values: lanes r16 < 8 evaluate ku[tap], r16 >= 8 kv[tap]
-        const float k10 = EsKernelAt(c0.x, nc2, xc1, b2);
-        const float k11 = EsKernelAt(c1.x, nc2, xc1, b2);
+        const float k10 = EsPoly<D>(cu, c0.x);
+        const float k11 = EsPoly<D>(cu, c1.x);
         // w taps of both groups from one evaluation: quads 0 and 2 of a
         // 16-lane row (r16 & 4 clear) take group 0, quads 1 and 3 group 1
         // (the quads of a pair hold the same taps); each lane then gets the
         // other group's value from the neighbouring quad
-        const float kwx = w_ok ? EsKernelAt(q_hi ? c1.y : c0.y, nc2, xcw, b2) : 0.f;
+        const float kwx = w_ok ? EsPoly<D>(cw, q_hi ? c1.y : c0.y) : 0.f;
         const float kw0 = DppMerge<0x114, 0xA>(kwx);  // quads 1, 3 from quads 0, 2
         const float kw1 = DppMerge<0x104, 0x5>(kwx);  // quads 0, 2 from quads 1, 3
         // A factor: kv (lanes r16 >= 8 own, the others from the rotated
@@ -578,13 +576,18 @@ __global__ void __launch_bounds__(kGridBlock) DegridKernel(
       const uint32_t pk = __float_as_uint(c.w);
       const int du = pk & 255u, dv = (pk >> 8) & 255u, p0 = (pk >> 16) & 0x7fffu;
       float ku[W], kv[W];
-      EsTaps<W>(c.x, dp.beta2, ku);
-      EsTaps<W>(c.y, dp.beta2, kv);
-      const int plo = max(p0, sub_lo), phi = min(p0 + W, sub_lo + sub_n);
+      EsTaps<W>(dp, c.x, ku);
+      EsTaps<W>(dp, c.y, kv);
       float sx = 0.f, sy = 0.f;
-      for (int p = plo; p < phi; ++p) {
-        const float kw =
-            EsKernel((float(p - p0 - W / 2 + 1) - c.z) * dp.two_over_w, dp.beta2);
+      // w taps in a wave-uniform loop, so each tap's coefficients are
+      // scalar loads; lanes skip the planes outside the sub-slab
+#pragma unroll 1
+      for (int t = 0; t < W; ++t) {
+        const int p = p0 + t;
+        if (p < sub_lo || p >= sub_lo + sub_n) continue;
+        float cw[EsPolyDegree(W) + 1];
+        EsPolyLoad<EsPolyDegree(W)>(dp, t, cw);
+        const float kw = EsPoly<EsPolyDegree(W)>(cw, c.z);
         const float2* base = sub + ((p - sub_lo) * E + dv) * E + du;
         float px = 0.f, py = 0.f;
 #pragma unroll
@@ -660,12 +663,15 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
                    wave * (2 * kStage);
   const int j = lane & 15;        // visibility of the group (B column, D column)
   const int g = lane >> 4;        // B row (k, l) = (4 (s % 2) + g, s / 2); D rows 4 g + r
-  const float c2 = dp.two_over_w, b2 = dp.beta2, nc2 = -c2;
-  // abscissae (times 2/W) of the taps this lane evaluates
-  const float xu0 = float(g - W / 2 + 1) * c2, xu1 = float(4 + g - W / 2 + 1) * c2;
-  // v and w taps 2 g, 2 g + 1
-  const float x20 = float(2 * g - W / 2 + 1) * c2, x21 = float(2 * g + 1 - W / 2 + 1) * c2;
-  const bool u0_ok = g < W, u1_ok = 4 + g < W, t0_ok = 2 * g < W, t1_ok = 2 * g + 1 < W;
+  // the lane evaluates u taps g and 4 + g, and v and w taps 2 g and
+  // 2 g + 1 (the coefficient rows of taps >= W are zero, so those taps
+  // weigh 0)
+  constexpr int D = EsPolyDegree(W);
+  float ca[D + 1], cb[D + 1], c2a[D + 1], c2b[D + 1];
+  EsPolyLoad<D>(dp, g, ca);
+  EsPolyLoad<D>(dp, 4 + g, cb);
+  EsPolyLoad<D>(dp, 2 * g, c2a);
+  EsPolyLoad<D>(dp, 2 * g + 1, c2b);
   // A operand lane mapping: A[i = L % 16][g] of step s =
   // G[t = i / 2][l = s / 2][k = 4 (s % 2) + g].(i % 2)
   const int a_t = (lane & 15) >> 1, a_c = lane & 1;
@@ -756,14 +762,13 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         const bool take = j < lead;
         // B operands: u taps g (even steps) and 4 + g (odd steps) of
         // visibility j, times its v tap of the step
-        const float bu0 = (take && u0_ok) ? EsKernelAt(cv.x, nc2, xu0, b2) : 0.f;
-        const float bu1 = (take && u1_ok) ? EsKernelAt(cv.x, nc2, xu1, b2) : 0.f;
+        const float bu0 = take ? EsPoly<D>(ca, cv.x) : 0.f;
+        const float bu1 = take ? EsPoly<D>(cb, cv.x) : 0.f;
         // v taps 2 g, 2 g + 1 here; all eight from the four lanes of j
         // (lane 16 p + j holds taps 2 p, 2 p + 1 of visibility j)
         float kv[8];
         {
-          const float v0 = t0_ok ? EsKernelAt(cv.y, nc2, x20, b2) : 0.f;
-          const float v1 = t1_ok ? EsKernelAt(cv.y, nc2, x21, b2) : 0.f;
+          const float v0 = EsPoly<D>(c2a, cv.y), v1 = EsPoly<D>(c2b, cv.y);
 #pragma unroll
           for (int pp = 0; pp < 4; ++pp) {
             kv[2 * pp] = __shfl(v0, 16 * pp + j);
@@ -771,8 +776,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           }
         }
         // w taps of this lane's rows: planes 2 g, 2 g + 1
-        const float kw0 = t0_ok ? EsKernelAt(cv.z, nc2, x20, b2) : 0.f;
-        const float kw1 = t1_ok ? EsKernelAt(cv.z, nc2, x21, b2) : 0.f;
+        const float kw0 = EsPoly<D>(c2a, cv.z), kw1 = EsPoly<D>(c2b, cv.z);
         // four accumulators (steps mod 4): four independent chains
         mfma_f4 dq[4];
 #pragma unroll

@@ -50,6 +50,40 @@ const std::vector<KernelShape>& KernelTable() {
   return t;
 }
 
+void EsPolyCoefficients(int W, double beta, float (*out)[kEsPolyCoefs]) {
+  WG_CHECK(W >= 2 && W <= kEsPolyTaps, "kernel support out of range");
+  const int D = EsPolyDegree(W), N = D + 1;
+  for (int k = 0; k < kEsPolyTaps; ++k)
+    for (int d = 0; d < kEsPolyCoefs; ++d) out[k][d] = 0.f;
+  for (int k = 0; k < W; ++k) {
+    // values at the N Chebyshev points of the first kind in g
+    std::vector<double> y(N), a(N, 0.0);
+    for (int i = 0; i < N; ++i) {
+      const double g = std::cos(M_PI * (i + 0.5) / N);
+      const double f = 0.5 * (g + 1.0);
+      const double x = (double(k - W / 2 + 1) - f) * 2.0 / double(W);
+      y[i] = std::fabs(x) <= 1.0 ? std::exp(beta * (std::sqrt(1.0 - x * x) - 1.0)) : 0.0;
+    }
+    // Chebyshev coefficients of the interpolant
+    for (int j = 0; j < N; ++j) {
+      double sum = 0;
+      for (int i = 0; i < N; ++i) sum += y[i] * std::cos(M_PI * j * (i + 0.5) / N);
+      a[j] = (j == 0 ? 1.0 : 2.0) * sum / N;
+    }
+    // power basis in g: T_0 = 1, T_1 = g, T_{j+1} = 2 g T_j - T_{j-1}
+    std::vector<std::vector<double>> T(N, std::vector<double>(N, 0.0));
+    T[0][0] = 1.0;
+    if (N > 1) T[1][1] = 1.0;
+    for (int j = 2; j < N; ++j)
+      for (int d = 0; d < N; ++d)
+        T[j][d] = (d > 0 ? 2.0 * T[j - 1][d - 1] : 0.0) - T[j - 2][d];
+    std::vector<double> pw(N, 0.0);
+    for (int j = 0; j < N; ++j)
+      for (int d = 0; d < N; ++d) pw[d] += a[j] * T[j][d];
+    for (int d = 0; d < N; ++d) out[k][d] = float(pw[d]);
+  }
+}
+
 double Fp32EpsilonFloor() {
   double m = 1e300;
   for (const KernelShape& k : KernelTable()) m = std::min(m, ShapeErrorEstimate(k));
