@@ -92,6 +92,14 @@ int wg_predict_visibilities(wg_handle* h, size_t n_rows, size_t n_chan,
 /* Thread-local message of the last failed call on this thread. */
 const char* wg_last_error(void);
 
+/* Polarization codes (the gridder's polarization, WSClean's
+ * aocommon::PolarizationEnum; values of the FITS STOKES axis). */
+enum {
+  WG_POL_I = 1, WG_POL_Q = 2, WG_POL_U = 3, WG_POL_V = 4,
+  WG_POL_RR = -1, WG_POL_LL = -2, WG_POL_RL = -3, WG_POL_LR = -4,
+  WG_POL_XX = -5, WG_POL_YY = -6, WG_POL_XY = -7, WG_POL_YX = -8
+};
+
 /* WGriddingGridderBase::AddInversionDataWithCorrectionCallback
  * (wgridder/gridder_simple.h:29-35; VisibilityCallbackBuffer,
  * wgridder/callback_implementation.h:34-136).  The reference reaches the
@@ -104,14 +112,18 @@ const char* wg_last_error(void);
  * GainMode order: 0 xx, 1 yy, 2 trace (n_pol 1), 3 2-vis diagonal (n_pol 2),
  * 4 full (n_pol 4).  Each visibility gets the forward gain (apply_forward,
  * direction-dependent PSF mode) and then the conjugate gain, the
- * polarizations are collapsed (2: sum; 4: Stokes I = (xx + yy) / 2) and the
- * result is gridded as by wg_add_inversion_data. */
+ * polarizations are collapsed as CollapseData (gridding/msgridderdata.h:
+ * 37-52; wgridder/callback_implementation.h:92-95): 2 are summed, 4 linear
+ * correlations (XX, XY, YX, YY) are converted to `polarization` (a WG_POL_
+ * code: Stokes I = (XX + YY) / 2, Q, U, V, the linear products, or the
+ * circular RR, LL, RL, LR), and the result is gridded as by
+ * wg_add_inversion_data. */
 int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, size_t n_rows,
                                     const double* uvw, const double* freq, size_t n_chan,
                                     const uint32_t* antennas, const float* vis,
                                     const uint64_t* time_offsets, const float* gains,
                                     size_t n_times, size_t n_antennas, size_t nparms,
-                                    int apply_forward);
+                                    int apply_forward, int polarization);
 
 /* ------------------------------------------------------------------------
  * Device-resident variants.  Same semantics as the calls above, but uvw,
@@ -285,7 +297,8 @@ int wg_image_weights_sample(wg_image_weights* w, size_t n, const double* u, cons
  * all); weights squared (weighting 1) or set to one (2); times the imaging
  * weight GetWeight(u / lambda, v / lambda) (image_weights NULL: 1); the
  * visibility times that weight; then one visibility per channel (2
- * polarizations: sum; 4: Stokes I, 0.5 (XX + YY)) into out_dev
+ * polarizations: sum; 4 linear correlations XX, XY, YX, YY: converted to
+ * `polarization`, a WG_POL_ code, as CollapseData does) into out_dev
  * [rows][chan] complex64, ready for wg_add_inversion_data_device.
  * stats4 (host): total weight (all polarizations), visibility weight sum,
  * max gridded weight, gridded visibility count (first polarization), as
@@ -295,7 +308,7 @@ int wg_image_weights_sample(wg_image_weights* w, size_t n, const double* u, cons
 int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
                     const double* uvw_dev, const double* freq, const float* vis_dev,
                     const float* weights_dev, const float* model_dev, const uint8_t* selected,
-                    int psf, double psf_dl, double psf_dm, int weighting,
+                    int psf, double psf_dl, double psf_dm, int weighting, int polarization,
                     wg_image_weights* image_weights, float* out_dev, double* stats4);
 
 /* ------------------------------------------------------------------------

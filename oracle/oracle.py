@@ -317,6 +317,43 @@ def es_phi_hat(W, beta, t):
 
 
 # ---------------------------------------------------------------------------
+# Polarization codes: the values of the FITS STOKES axis, which
+# aocommon::PolarizationEnum uses (aocommon is absent from the reference
+# mount: the numbering and the conventions below are parity unpinned).
+POL_I, POL_Q, POL_U, POL_V = 1, 2, 3, 4
+POL_RR, POL_LL, POL_RL, POL_LR = -1, -2, -3, -4
+POL_XX, POL_YY, POL_XY, POL_YX = -5, -6, -7, -8
+ALL_POLS = (POL_I, POL_Q, POL_U, POL_V, POL_RR, POL_LL, POL_RL, POL_LR, POL_XX, POL_YY,
+            POL_XY, POL_YX)
+_SIGMA = [np.array(m, np.complex128) for m in
+          ([[1, 0], [0, 1]], [[1, 0], [0, -1]], [[0, 1], [1, 0]], [[0, -1j], [1j, 0]])]
+
+
+def stokes_from_linear(v):
+    """(I, Q, U, V) of linear correlations v[..., 4] = XX, XY, YX, YY, from
+    the coherency matrix C = [[XX, XY], [YX, YY]] = I s0 + Q s3 + U s1 - V s2
+    (Pauli matrices s; the IAU convention XY = U + iV): S = tr(C s) / 2, with
+    the sign of s2 for V."""
+    v = np.asarray(v, np.complex128)
+    C = v.reshape(v.shape[:-1] + (2, 2))
+    tr = [np.einsum("...ij,ji->...", C, s) / 2 for s in _SIGMA]
+    return tr[0], tr[1], tr[2], -tr[3]
+
+
+def convert_from_linear(v, pol):
+    """One visibility of polarization pol from v[..., 4] = XX, XY, YX, YY:
+    the linear products themselves, Stokes parameters, or circular products
+    RR = I + V, LL = I - V, RL = Q + iU, LR = Q - iU
+    (gridding/msgridderdata.h:45-49, aocommon ConvertFromLinear)."""
+    v = np.asarray(v, np.complex128)
+    lin = {POL_XX: 0, POL_XY: 1, POL_YX: 2, POL_YY: 3}
+    if pol in lin:
+        return v[..., lin[pol]]
+    I, Q, U, V = stokes_from_linear(v)
+    return {POL_I: I, POL_Q: Q, POL_U: U, POL_V: V, POL_RR: I + V, POL_LL: I - V,
+            POL_RL: Q + 1j * U, POL_LR: Q - 1j * U}[pol]
+
+
 # Per-antenna gain corrections and polarization collapse (the correction
 # callback of WGriddingGridder_Simple, wgridder/callback_implementation.h:
 # 78-97).  Gain modes in the order of GainMode (gridding/gainmode.h:16-31).
@@ -371,16 +408,16 @@ def _apply_gain(mode, v, g1, g2, conjugated):
     return R.reshape(4)
 
 
-def correct_and_collapse(vis, antennas, time_offsets, gains, n_ant, nparms, mode, forward):
-    """vis [rows, chan, n_pol] complex -> corrected Stokes-I [rows, chan].
+def correct_and_collapse(vis, antennas, time_offsets, gains, n_ant, nparms, mode, forward,
+                         pol=POL_I):
+    """vis [rows, chan, n_pol] complex -> corrected, collapsed [rows, chan].
 
     gains: cached solutions [n_times, n_chan, n_ant, nparms] complex, indexed
     (time_offset * n_chan + ch) * n_ant * nparms
     (gridding/visibilitymodifier.h:550-553).  forward applies the forward
     gain first (direction-dependent PSF mode, :559-562).  Collapse:
-    2 polarizations -> v0 + v1 (gridding/msgridderdata.h:40-44), 4 -> Stokes
-    I = (xx + yy) / 2 (aocommon Polarization::ConvertFromLinear; aocommon is
-    absent from the reference mount, parity unpinned)."""
+    2 polarizations -> v0 + v1 (gridding/msgridderdata.h:40-44), 4 ->
+    convert_from_linear(v, pol) (:45-49)."""
     vis = np.asarray(vis)
     n_rows, n_chan, npol = vis.shape
     assert npol == GAIN_NPOL[mode]
@@ -400,5 +437,5 @@ def correct_and_collapse(vis, antennas, time_offsets, gains, n_ant, nparms, mode
             elif npol == 2:
                 out[r, c] = v[0] + v[1]
             else:
-                out[r, c] = 0.5 * (v[0] + v[3])
+                out[r, c] = convert_from_linear(v, pol)
     return out

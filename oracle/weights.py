@@ -217,7 +217,7 @@ class ImageWeightsOracle:
 
 
 def prepare_rows(uvw, freq, vis, weights, model=None, selected=None, psf=False, psf_dl=0.0,
-                 psf_dm=0.0, weighting=VIS_NORMAL, image_weights=None):
+                 psf_dm=0.0, weighting=VIS_NORMAL, image_weights=None, polarization=1):
     """Returns (out [rows][chan] complex64, stats dict).  vis, weights,
     model: [rows][chan][pol] (complex64 / float32).  Sequential in row,
     channel, polarization order, float32 products as the reference."""
@@ -270,5 +270,6 @@ def prepare_rows(uvw, freq, vis, weights, model=None, selected=None, psf=False, 
     elif n_pol == 2:
         out = v[:, :, 0] + v[:, :, 1]
     else:
-        out = (np.float32(0.5) * (v[:, :, 0] + v[:, :, 3])).astype(np.complex64)
+        from .oracle import convert_from_linear
+        out = convert_from_linear(v, polarization).astype(np.complex64)
     return out.astype(np.complex64), stats

@@ -339,9 +339,13 @@ def test_errors_leave_handle_usable():
 
 # ------------------------------------------------ correction callback (f3)
 
-@pytest.mark.parametrize("mode,nparms,forward", [(2, 2, False), (3, 2, False), (4, 4, False),
-                                                 (4, 4, True), (0, 4, False)])
-def test_corrected_inversion_matches_host_correction(oracle, mode, nparms, forward):
+@pytest.mark.parametrize("mode,nparms,forward,pol", [
+    (2, 2, False, 1), (3, 2, False, 1), (4, 4, False, 1), (4, 4, True, 1), (0, 4, False, 1),
+    # the 4-polarization collapse to the other polarizations (CollapseData<4>,
+    # gridding/msgridderdata.h:45-49; wgridder/callback_implementation.h:92-95)
+    (4, 4, False, 2), (4, 4, False, 3), (4, 4, True, 4), (4, 4, False, -1), (4, 4, False, -3),
+    (4, 4, False, -4), (4, 4, False, -7), (4, 4, False, -6)])
+def test_corrected_inversion_matches_host_correction(oracle, mode, nparms, forward, pol):
     # GPU gain correction + collapse + gridding against the oracle's
     # correction on the host followed by the plain inversion; the two paths
     # differ only by FP32 rounding of the correction (tolerance 1e-5)
@@ -360,7 +364,7 @@ def test_corrected_inversion_matches_host_correction(oracle, mode, nparms, forwa
     g = g.astype(np.complex64)
     ant = rng.integers(0, n_ant, (n_rows, 2)).astype(np.uint32)
     toff = rng.integers(0, n_times, n_rows).astype(np.uint64)
-    host = oracle.correct_and_collapse(vis, ant, toff, g, n_ant, nparms, mode, forward)
+    host = oracle.correct_and_collapse(vis, ant, toff, g, n_ant, nparms, mode, forward, pol)
     ref_g = make(96, p["dx"], 1e-5)
     ref_g.InitializeInversion()
     ref_g.AddInversionData(n_rows, n_chan, p["uvw"], p["freq"], host.astype(np.complex64))
@@ -369,6 +373,7 @@ def test_corrected_inversion_matches_host_correction(oracle, mode, nparms, forwa
     gg = make(96, p["dx"], 1e-5)
     gg.InitializeInversion()
     gg.AddInversionDataWithCorrectionCallback(mode, npol, n_rows, p["uvw"], p["freq"], n_chan,
-                                              ant, vis, toff, g, apply_forward=forward)
+                                              ant, vis, toff, g, apply_forward=forward,
+                                              polarization=pol)
     gg.FinalizeImage(1.0)
     assert rel_l2(gg.RealImage(), ref) < 1e-5

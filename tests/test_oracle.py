@@ -323,3 +323,40 @@ def test_es_kernel_polynomial_against_closed_form(W, beta):
         den += (ref ** 2).sum()
     rms = np.sqrt(num / den)
     assert rms < (1e-7 if W >= 8 else 2e-7 if W == 6 else 2e-5), rms
+
+
+def test_polarization_conversion_closed_forms(oracle):
+    # The oracle's conversion from linear correlations (coherency-matrix
+    # traces with the Pauli matrices) against the closed forms of the IAU
+    # definitions XX = I + Q, YY = I - Q, XY = U + iV, YX = U - iV, the
+    # circular identities RR + LL = 2 I, RR - LL = 2 V, RL + LR = 2 Q,
+    # RL - LR = 2 i U, and the physics of a feed rotation (I and V invariant,
+    # Q + iU turns by 2 theta); a swapped XY / YX or a wrong V sign fails
+    rng = np.random.default_rng(4)
+    I, Q, U, V = (rng.normal(size=50) + 1j * rng.normal(size=50) for _ in range(4))
+    lin = np.stack([I + Q, U + 1j * V, U - 1j * V, I - Q], axis=-1)
+    got = [oracle.convert_from_linear(lin, p) for p in (1, 2, 3, 4)]
+    for a, b in zip(got, (I, Q, U, V)):
+        np.testing.assert_allclose(a, b, atol=1e-12)
+    for p, k in ((-5, 0), (-7, 1), (-8, 2), (-6, 3)):
+        np.testing.assert_array_equal(oracle.convert_from_linear(lin, p), lin[:, k])
+    rr, ll, rl, lr = (oracle.convert_from_linear(lin, p) for p in (-1, -2, -3, -4))
+    np.testing.assert_allclose(rr + ll, 2 * I, atol=1e-12)
+    np.testing.assert_allclose(rr - ll, 2 * V, atol=1e-12)
+    np.testing.assert_allclose(rl + lr, 2 * Q, atol=1e-12)
+    np.testing.assert_allclose(rl - lr, 2j * U, atol=1e-12)
+    # a real sky (I, Q, U, V real) seen through feeds rotated by theta:
+    # C' = R C R^T with R the rotation of the (X, Y) feed pair
+    Ir, Qr, Ur, Vr = 1.0, 0.3, -0.2, 0.1
+    C = np.array([[Ir + Qr, Ur + 1j * Vr], [Ur - 1j * Vr, Ir - Qr]])
+    th = 0.37
+    R = np.array([[np.cos(th), np.sin(th)], [-np.sin(th), np.cos(th)]])
+    Cr = (R @ C @ R.T).reshape(4)
+    I2, Q2, U2, V2 = oracle.stokes_from_linear(Cr)
+    assert abs(I2 - Ir) < 1e-12 and abs(V2 - Vr) < 1e-12
+    np.testing.assert_allclose(Q2 + 1j * U2, (Qr + 1j * Ur) * np.exp(-2j * th), atol=1e-12)
+    # a physical coherency (C = J J^H) is positive semi-definite: I >= |(Q, U, V)|
+    for _ in range(20):
+        J = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        s = [np.real(x) for x in oracle.stokes_from_linear((J @ J.conj().T).reshape(4))]
+        assert s[0] >= np.sqrt(s[1] ** 2 + s[2] ** 2 + s[3] ** 2) - 1e-12

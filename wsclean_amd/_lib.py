@@ -46,7 +46,7 @@ SIGNATURES = [
     ("wg_add_inversion_data_corrected", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, c_size,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, c_size,
-      ctypes.c_int]),
+      ctypes.c_int, ctypes.c_int]),
     ("wg_add_inversion_data_device", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_predict_visibilities_device", ctypes.c_int,
@@ -126,8 +126,8 @@ SIGNATURES = [
     ("wg_prepare_rows", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
-      ctypes.c_void_p]),
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_resample", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, c_size, c_size]),
     ("wg_real_image_resampled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size]),

@@ -340,11 +340,12 @@ int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, s
                                     const uint32_t* antennas, const float* vis,
                                     const uint64_t* time_offsets, const float* gains,
                                     size_t n_times, size_t n_antennas, size_t nparms,
-                                    int apply_forward) {
+                                    int apply_forward, int polarization) {
   return Guard([&] {
     WG_CHECK(h && freq && (n_rows == 0 || (uvw && antennas && vis && time_offsets && gains)),
              "null argument");
     WG_CHECK(gain_mode >= 0 && gain_mode <= 4, "unknown gain mode");
+    WG_CHECK(wg::ValidPolCode(polarization), "unknown polarization code");
     static const size_t kNPol[5] = {1, 1, 1, 2, 4};
     WG_CHECK(n_pol == kNPol[gain_mode], "polarization count does not match the gain mode");
     WG_CHECK(nparms == 2 || nparms == 4, "solutions have 2 or 4 values per antenna");
@@ -374,7 +375,7 @@ int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, s
                           s));
     WG_HIP(hipMemcpyAsync(dgain, gains, ngain * sizeof(float2), hipMemcpyHostToDevice, s));
     wg::LaunchCorrectCollapse(dpol, dant, dtoff, dgain, n_rows, n_chan, n_antennas, int(nparms),
-                              gain_mode, int(n_pol), apply_forward != 0, dvis, s);
+                              gain_mode, int(n_pol), apply_forward != 0, polarization, dvis, s);
     AddInversion(h, n_rows, n_chan, duvw, freq, reinterpret_cast<const float*>(dvis));
   });
 }
@@ -1198,9 +1199,10 @@ int wg_image_weights_sample(wg_image_weights* w, size_t n, const double* u, cons
 int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
                     const double* uvw_dev, const double* freq, const float* vis_dev,
                     const float* weights_dev, const float* model_dev, const uint8_t* selected,
-                    int psf, double psf_dl, double psf_dm, int weighting,
+                    int psf, double psf_dl, double psf_dm, int weighting, int polarization,
                     wg_image_weights* image_weights, float* out_dev, double* stats4) {
   return Guard([&] {
+    WG_CHECK(wg::ValidPolCode(polarization), "unknown polarization code");
     WG_CHECK(h && freq && stats4, "null argument");
     WG_CHECK(n_rows == 0 || (uvw_dev && weights_dev && out_dev && (psf || vis_dev)),
              "null argument");
@@ -1229,6 +1231,7 @@ int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
     args.psf_dm = psf_dm;
     args.psf_dn = std::sqrt(1.0 - psf_dl * psf_dl - psf_dm * psf_dm) - 1.0;
     args.weighting = weighting;
+    args.polarization = polarization;
     std::unique_lock<std::mutex> wl;
     const wg::WeightGrid* g = nullptr;
     if (image_weights) {

@@ -140,6 +140,43 @@ __device__ __forceinline__ uint32_t DivSmall(uint32_t i, uint32_t d, double inv_
   return q;
 }
 
+// Polarizations, with the values of the FITS STOKES axis (the values of
+// aocommon::PolarizationEnum): Stokes I, Q, U, V, circular RR, LL, RL, LR
+// and linear XX, YY, XY, YX.
+enum PolCode : int {
+  kPolI = 1, kPolQ = 2, kPolU = 3, kPolV = 4,
+  kPolRR = -1, kPolLL = -2, kPolRL = -3, kPolLR = -4,
+  kPolXX = -5, kPolYY = -6, kPolXY = -7, kPolYX = -8
+};
+inline bool ValidPolCode(int p) { return (p >= 1 && p <= 4) || (p >= -8 && p <= -1); }
+
+// One visibility of polarization `pol` from the four linear correlations
+// v = (XX, XY, YX, YY) (gridding/msgridderdata.h:45-49; aocommon's
+// ConvertFromLinear, restated from the IAU definitions XX = I + Q,
+// YY = I - Q, XY = U + iV, YX = U - iV and RR = I + V, LL = I - V,
+// RL = Q + iU, LR = Q - iU).
+__host__ __device__ __forceinline__ float2 FromLinear(const float2* v, int pol) {
+  const float ix = 0.5f * (v[0].x + v[3].x), iy = 0.5f * (v[0].y + v[3].y);  // I
+  const float qx = 0.5f * (v[0].x - v[3].x), qy = 0.5f * (v[0].y - v[3].y);  // Q
+  const float ux = 0.5f * (v[1].x + v[2].x), uy = 0.5f * (v[1].y + v[2].y);  // U
+  // V = -i (XY - YX) / 2
+  const float vx = 0.5f * (v[1].y - v[2].y), vy = -0.5f * (v[1].x - v[2].x);
+  switch (pol) {
+    case kPolXX: return v[0];
+    case kPolXY: return v[1];
+    case kPolYX: return v[2];
+    case kPolYY: return v[3];
+    case kPolQ: return make_float2(qx, qy);
+    case kPolU: return make_float2(ux, uy);
+    case kPolV: return make_float2(vx, vy);
+    case kPolRR: return make_float2(ix + vx, iy + vy);
+    case kPolLL: return make_float2(ix - vx, iy - vy);
+    case kPolRL: return make_float2(qx - uy, qy + ux);  // Q + iU
+    case kPolLR: return make_float2(qx + uy, qy - ux);  // Q - iU
+    default: return make_float2(ix, iy);                // Stokes I
+  }
+}
+
 // Top bit of a sorted permutation entry: the visibility was flipped to w >= 0
 // (the flat index itself is below 2^31).
 constexpr uint32_t kPermFlip = 0x80000000u;

@@ -411,6 +411,7 @@ struct PrepArgs {
   int psf;         // visibilities of a point source instead of vis
   double psf_dl, psf_dm, psf_dn;  // its position (dn = sqrt(1 - dl^2 - dm^2) - 1)
   int weighting;   // 0 normal, 1 squared, 2 unit visibility weights
+  int polarization;  // PolCode the 4 linear correlations are converted to
 };
 // Number of FP64 block partials of the weight statistics / gridded total.
 size_t WeightStatPartials();
@@ -443,14 +444,15 @@ void LaunchPrepareRows(const PrepArgs& a, const double* uvw, const double* lambd
 // out row r = in row r * stride, for rows of `row_words` 8-byte words
 void LaunchStrideRows(const void* in, size_t stride, size_t n_out, size_t row_words, void* out,
                       hipStream_t s);
-// out[row][ch] = Stokes-I collapse of the conjugate-gain (after the forward
-// gain when `forward`) corrected vis[row][ch][0..npol).  gains: cached
-// solutions [n_times][n_chan][n_ant][nparms]; mode in GainMode order
-// (xx, yy, trace, 2-vis diagonal, full).
+// out[row][ch] = collapse of the conjugate-gain (after the forward gain
+// when `forward`) corrected vis[row][ch][0..npol): 2 polarizations summed,
+// 4 converted to polarization `pol` (PolCode).  gains: cached solutions
+// [n_times][n_chan][n_ant][nparms]; mode in GainMode order (xx, yy, trace,
+// 2-vis diagonal, full).
 void LaunchCorrectCollapse(const float2* vis, const uint32_t* antennas,
                            const uint64_t* time_offsets, const float2* gains, size_t n_rows,
                            size_t n_chan, size_t n_ant, int nparms, int mode, int npol,
-                           bool forward, float2* out, hipStream_t s);
+                           bool forward, int pol, float2* out, hipStream_t s);
 // out[k] = in[k * stride] for k in [0, count)
 void LaunchStridedGather(const uint32_t* in, size_t stride, size_t count, uint32_t* out,
                          hipStream_t s);

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) CorrectCollapseKernel(
     const float2* __restrict__ vis, const uint32_t* __restrict__ antennas,
     const uint64_t* __restrict__ time_offsets, const float2* __restrict__ gains,
     size_t n_rows, size_t n_chan, size_t n_ant, int nparms, int mode, int npol, int forward,
-    float2* __restrict__ out) {
+    int pol, float2* __restrict__ out) {
   const size_t n = n_rows * n_chan;
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
        i += size_t(gridDim.x) * kBlock) {
@@ -426,9 +426,11 @@ __global__ void __launch_bounds__(kBlock) CorrectCollapseKernel(
     for (int p = 0; p < npol; ++p) v[p] = vis[i * npol + p];
     if (forward) ApplyJones(mode, false, v, a, b);
     ApplyJones(mode, true, v, a, b);
+    // collapse (gridding/msgridderdata.h:37-52): 2 -> sum, 4 -> the
+    // gridder's polarization from the linear correlations
     float2 r = v[0];
     if (npol == 2) r = ca(v[0], v[1]);
-    if (npol == 4) r = make_float2(0.5f * (v[0].x + v[3].x), 0.5f * (v[0].y + v[3].y));
+    if (npol == 4) r = FromLinear(v, pol);
     out[i] = r;
   }
 }
@@ -791,12 +793,12 @@ void LaunchFillZero(void* p, size_t bytes, hipStream_t s) {
 void LaunchCorrectCollapse(const float2* vis, const uint32_t* antennas,
                            const uint64_t* time_offsets, const float2* gains, size_t n_rows,
                            size_t n_chan, size_t n_ant, int nparms, int mode, int npol,
-                           bool forward, float2* out, hipStream_t s) {
+                           bool forward, int pol, float2* out, hipStream_t s) {
   const size_t n = n_rows * n_chan;
   if (!n) return;
   CorrectCollapseKernel<<<Blocks(n), kBlock, 0, s>>>(vis, antennas, time_offsets, gains, n_rows,
                                                      n_chan, n_ant, nparms, mode, npol,
-                                                     forward ? 1 : 0, out);
+                                                     forward ? 1 : 0, pol, out);
   WG_HIP(hipGetLastError());
 }
 

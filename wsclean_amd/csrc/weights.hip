@@ -312,13 +312,14 @@ __global__ void __launch_bounds__(kBlock) PrepareRowsKernel(
       tot += double(cum);
       v[p] = make_float2(v[p].x * cum, v[p].y * cum);
     }
-    // polarization collapse (msgridderdata.h:37-52): 2 -> sum, 4 -> Stokes I
+    // polarization collapse (msgridderdata.h:37-52): 2 -> sum, 4 -> the
+    // gridder's polarization from the linear correlations
     if (npol == 1) {
       acc = v[0];
     } else if (npol == 2) {
       acc = make_float2(v[0].x + v[1].x, v[0].y + v[1].y);
     } else {
-      acc = make_float2(0.5f * (v[0].x + v[3].x), 0.5f * (v[0].y + v[3].y));
+      acc = FromLinear(v, a.polarization);
     }
     out[i] = acc;
   }

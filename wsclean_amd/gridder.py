@@ -24,6 +24,12 @@ from . import _lib
 from ._lib import WgError
 
 
+# Polarization codes (aocommon::PolarizationEnum values, the FITS STOKES axis)
+POL_I, POL_Q, POL_U, POL_V = 1, 2, 3, 4
+POL_RR, POL_LL, POL_RL, POL_LR = -1, -2, -3, -4
+POL_XX, POL_YY, POL_XY, POL_YX = -5, -6, -7, -8
+
+
 def _f64(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if shape is not None and a.size != int(np.prod(shape)):
@@ -102,14 +108,16 @@ class WGriddingGridder_Simple:
 
     def AddInversionDataWithCorrectionCallback(self, gain_mode, n_polarizations, n_rows, uvws,
                                                frequencies, n_channels, antennas, visibilities,
-                                               time_offsets, gains, apply_forward=False):
+                                               time_offsets, gains, apply_forward=False,
+                                               polarization=POL_I):
         """Mirror of AddInversionDataWithCorrectionCallback
         (wgridder/gridder_simple.h:29-35).  The MsGridder callback state is
         passed explicitly: antennas [n_rows, 2] antenna pairs, time_offsets
         [n_rows] rows of `gains`, gains [n_times, n_chan, n_antennas, nparms]
         complex64 (the cached solution table), visibilities [n_rows, n_chan,
         n_pol] complex64.  gain_mode: 0 xx, 1 yy, 2 trace, 3 2-vis diagonal,
-        4 full (GainMode order)."""
+        4 full (GainMode order).  polarization: the gridder's polarization
+        (POL_* codes) that 4 linear correlations are converted to."""
         uvw = np.ascontiguousarray(uvws, np.float64).reshape(-1, 3)
         freq = np.ascontiguousarray(frequencies, np.float64)
         ant = np.ascontiguousarray(antennas, np.uint32).reshape(-1, 2)
@@ -122,7 +130,7 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_add_inversion_data_corrected(
             self._h, int(gain_mode), int(n_polarizations), int(n_rows), _ptr(uvw), _ptr(freq),
             int(n_channels), _ptr(ant), _ptr(vis), _ptr(toff), _ptr(g), int(n_times), int(n_ant),
-            int(nparms), 1 if apply_forward else 0))
+            int(nparms), 1 if apply_forward else 0, int(polarization)))
 
     def FinalizeImage(self, multiplication_factor):
         _lib.check(self._lib.wg_finalize_image(self._h, float(multiplication_factor)))

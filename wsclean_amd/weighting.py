@@ -122,12 +122,14 @@ class ImageWeights:
 
 def prepare_rows(gridder, uvw_dev, freq, weights_dev, out_dev, vis_dev=None, model_dev=None,
                  selected=None, psf=False, psf_dl=0.0, psf_dm=0.0, weighting=VIS_NORMAL,
-                 image_weights=None):
+                 image_weights=None, polarization=1):
     """Weighted, collapsed visibilities [rows, chan] (complex64 as float32
     [..., 2]) into out_dev on the gridder handle's stream; returns the
     weight sums (total_weight, visibility_weight_sum, max_gridded_weight,
     gridded_visibility_count).  weights_dev: float32 [rows, chan, pol];
-    vis_dev / model_dev: float32 [rows, chan, pol, 2]."""
+    vis_dev / model_dev: float32 [rows, chan, pol, 2].  polarization: the
+    gridder's polarization (gridder.POL_* codes) 4 linear correlations are
+    converted to."""
     freq = np.ascontiguousarray(freq, np.float64)
     n_rows, n_chan, n_pol = (int(x) for x in weights_dev.shape)
     if n_chan != freq.size or out_dev.numel() != 2 * n_rows * n_chan:
@@ -144,7 +146,7 @@ def prepare_rows(gridder, uvw_dev, freq, weights_dev, out_dev, vis_dev=None, mod
     _lib.check(gridder._lib.wg_prepare_rows(
         gridder._h, n_rows, n_chan, n_pol, _dev(uvw_dev), _ptr(freq), _dev(vis_dev),
         _dev(weights_dev), _dev(model_dev), _ptr(sel) if sel is not None else None,
-        1 if psf else 0, float(psf_dl), float(psf_dm), int(weighting),
+        1 if psf else 0, float(psf_dl), float(psf_dm), int(weighting), int(polarization),
         image_weights.handle if image_weights is not None else None, _dev(out_dev),
         _ptr(stats)))
     return dict(total_weight=float(stats[0]), visibility_weight_sum=float(stats[1]),
