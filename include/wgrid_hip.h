@@ -311,6 +311,22 @@ int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
                     int psf, double psf_dl, double psf_dm, int weighting, int polarization,
                     wg_image_weights* image_weights, float* out_dev, double* stats4);
 
+/* wg_prepare_rows without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as wg_prepare_rows; its scratch (statistics partials, channel
+ * wavelengths and selection flags) is a fixed ~620 KB device array of the
+ * library, one per GPU, held until the process ends, so n_chan is limited to
+ * 65536 (larger counts fail with an error).  Calls on one GPU share that
+ * array; the library serialises them per device.  The current device must
+ * be the handle's. */
+int wg_prepare_rows_noalloc(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
+                            const double* uvw_dev, const double* freq, const float* vis_dev,
+                            const float* weights_dev, const float* model_dev,
+                            const uint8_t* selected, int psf, double psf_dl, double psf_dm,
+                            int weighting, int polarization, wg_image_weights* image_weights,
+                            float* out_dev, double* stats4);
+
 /* ------------------------------------------------------------------------
  * Small-inversion resampling (wgridder/wgriddingmsgridder.cpp:290-308 after
  * inversion, :326-336 before prediction, schaapcommon::math::Resampler):

@@ -128,6 +128,11 @@ SIGNATURES = [
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_prepare_rows_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_resample", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, c_size, c_size]),
     ("wg_real_image_resampled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size]),

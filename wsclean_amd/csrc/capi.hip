@@ -175,6 +175,7 @@ struct wg_handle {
   std::string plan_json;
   hipGraphExec_t cycle = nullptr;  // captured major cycle on vs_prep
   uint64_t cycle_gen = 0;          // arena generation the graph's pointers belong to
+  int device = 0;                  // device current at creation (the stream's)
   std::mutex mu;
 
   ~wg_handle() { DropGraph(cycle); }
@@ -219,6 +220,36 @@ struct wg_image_weights {
   }
 };
 
+// Device scratch of wg_prepare_rows_noalloc, one copy per GPU for the life
+// of the process: the statistics partials of the row kernel, the four
+// statistics, and the wavelengths and selection flags of at most
+// kNoAllocMaxChan channels.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device.
+constexpr size_t kNoAllocMaxChan = 65536;
+constexpr size_t kNoAllocPartWords = 4 * 1024;  // == wg::WeightStatPartials()
+constexpr size_t kNoAllocStatsOff = kNoAllocPartWords * sizeof(double);
+constexpr size_t kNoAllocLambdaOff = kNoAllocStatsOff + 256;
+constexpr size_t kNoAllocSelOff = kNoAllocLambdaOff + kNoAllocMaxChan * sizeof(double);
+constexpr size_t kNoAllocScratchBytes = kNoAllocSelOff + kNoAllocMaxChan;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu[kNoAllocMaxDevices];
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// lambda[c] = c / freq[c] in place (the FP64 division is correctly rounded,
+// as the host's)
+__global__ void WavelengthsKernel(double* __restrict__ lam, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    lam[i] = wg::kSpeedOfLight / lam[i];
+}
+
+}  // namespace
+
+
 extern "C" {
 
 const char* wg_last_error(void) { return g_last_error.c_str(); }
@@ -241,6 +272,7 @@ int wg_create(const wg_params* params, wg_handle** out) {
     g.m_shift = params->m_shift;
     g.epsilon = params->epsilon;
     wg::PlanOptions opt;
+    WG_HIP(hipGetDevice(&h->device));
     h->eng = std::make_unique<wg::Engine>(g, opt);
     h->img = static_cast<double*>(h->eng->Arena().Get("image.acc", h->npix() * sizeof(double)));
     WG_HIP(hipMemsetAsync(h->img, 0, h->npix() * sizeof(double), h->s()));
@@ -1245,6 +1277,75 @@ int wg_prepare_rows(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
                           reinterpret_cast<float2*>(out_dev), part, dstats, h->s());
     WG_HIP(hipMemcpyAsync(stats4, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, h->s()));
     WG_HIP(hipStreamSynchronize(h->s()));
+  });
+}
+
+// ------------------------------------------------ allocation-free row prep
+
+int wg_prepare_rows_noalloc(wg_handle* h, size_t n_rows, size_t n_chan, size_t n_pol,
+                            const double* uvw_dev, const double* freq, const float* vis_dev,
+                            const float* weights_dev, const float* model_dev,
+                            const uint8_t* selected, int psf, double psf_dl, double psf_dm,
+                            int weighting, int polarization, wg_image_weights* image_weights,
+                            float* out_dev, double* stats4) {
+  return Guard([&] {
+    WG_CHECK(h && freq && stats4, "null argument");
+    WG_CHECK(n_rows == 0 || (uvw_dev && weights_dev && out_dev && (psf || vis_dev)),
+             "null argument");
+    WG_CHECK(n_pol == 1 || n_pol == 2 || n_pol == 4, "1, 2 or 4 polarizations");
+    WG_CHECK(weighting >= 0 && weighting <= 2, "weighting: 0 normal, 1 squared, 2 unit");
+    WG_CHECK(wg::ValidPolCode(polarization), "unknown polarization code");
+    WG_CHECK(n_chan >= 1 && n_chan <= kNoAllocMaxChan,
+             "wg_prepare_rows_noalloc takes 1 to 65536 channels (its fixed device scratch)");
+    WG_CHECK(wg::WeightStatPartials() == kNoAllocPartWords, "statistics partials size changed");
+    for (size_t c = 0; c < n_chan; ++c) WG_CHECK(freq[c] > 0, "frequencies must be positive");
+    int dev = -1;
+    WG_HIP(hipGetDevice(&dev));
+    WG_CHECK(dev == h->device, "the current device is not the handle's device");
+    WG_CHECK(dev >= 0 && dev < kNoAllocMaxDevices, "device index out of range");
+    // one call at a time per device: calls share the device's scratch
+    std::lock_guard<std::mutex> sl(g_scratch_mu[dev]);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      WG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      WG_CHECK(p, "no device address for the row-preparation scratch");
+      g_scratch_ptr[dev] = p;
+    }
+    unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    double* part = reinterpret_cast<double*>(base);
+    double* dstats = reinterpret_cast<double*>(base + kNoAllocStatsOff);
+    double* dl = reinterpret_cast<double*>(base + kNoAllocLambdaOff);
+    uint8_t* dsel = selected ? base + kNoAllocSelOff : nullptr;
+    std::lock_guard<std::mutex> lk(h->mu);
+    hipStream_t s = h->s();
+    WG_HIP(hipMemcpyAsync(dl, freq, n_chan * sizeof(double), hipMemcpyHostToDevice, s));
+    WavelengthsKernel<<<1, 256, 0, s>>>(dl, n_chan);
+    WG_HIP(hipGetLastError());
+    if (selected) WG_HIP(hipMemcpyAsync(dsel, selected, n_chan, hipMemcpyHostToDevice, s));
+    wg::PrepArgs args{};
+    args.n_rows = n_rows;
+    args.n_chan = n_chan;
+    args.n_pol = int(n_pol);
+    args.psf = psf != 0;
+    args.psf_dl = psf_dl;
+    args.psf_dm = psf_dm;
+    args.psf_dn = std::sqrt(1.0 - psf_dl * psf_dl - psf_dm * psf_dm) - 1.0;
+    args.weighting = weighting;
+    args.polarization = polarization;
+    std::unique_lock<std::mutex> wl;
+    const wg::WeightGrid* g = nullptr;
+    if (image_weights) {
+      wl = std::unique_lock<std::mutex>(image_weights->mu);
+      WG_CHECK(image_weights->finished, "imaging weights used before FinishGridding()");
+      g = &image_weights->g;
+    }
+    // the row kernel writes every partial and the statistics kernel all
+    // four statistics, so nothing of the previous call's contents is read
+    wg::LaunchPrepareRows(args, uvw_dev, dl, reinterpret_cast<const float2*>(vis_dev),
+                          weights_dev, reinterpret_cast<const float2*>(model_dev), dsel, g,
+                          reinterpret_cast<float2*>(out_dev), part, dstats, s);
+    WG_HIP(hipMemcpyAsync(stats4, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    WG_HIP(hipStreamSynchronize(s));
   });
 }
 

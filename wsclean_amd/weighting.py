@@ -122,14 +122,15 @@ class ImageWeights:
 
 def prepare_rows(gridder, uvw_dev, freq, weights_dev, out_dev, vis_dev=None, model_dev=None,
                  selected=None, psf=False, psf_dl=0.0, psf_dm=0.0, weighting=VIS_NORMAL,
-                 image_weights=None, polarization=1):
+                 image_weights=None, polarization=1, no_alloc=False):
     """Weighted, collapsed visibilities [rows, chan] (complex64 as float32
     [..., 2]) into out_dev on the gridder handle's stream; returns the
     weight sums (total_weight, visibility_weight_sum, max_gridded_weight,
     gridded_visibility_count).  weights_dev: float32 [rows, chan, pol];
     vis_dev / model_dev: float32 [rows, chan, pol, 2].  polarization: the
     gridder's polarization (gridder.POL_* codes) 4 linear correlations are
-    converted to."""
+    converted to.  no_alloc: use wg_prepare_rows_noalloc (no device
+    allocation; at most 65536 channels)."""
     freq = np.ascontiguousarray(freq, np.float64)
     n_rows, n_chan, n_pol = (int(x) for x in weights_dev.shape)
     if n_chan != freq.size or out_dev.numel() != 2 * n_rows * n_chan:
@@ -143,7 +144,8 @@ def prepare_rows(gridder, uvw_dev, freq, weights_dev, out_dev, vis_dev=None, mod
         if sel.size != n_chan:
             raise WgError("one selection flag per channel")
     stats = np.zeros(4, np.float64)
-    _lib.check(gridder._lib.wg_prepare_rows(
+    fn = gridder._lib.wg_prepare_rows_noalloc if no_alloc else gridder._lib.wg_prepare_rows
+    _lib.check(fn(
         gridder._h, n_rows, n_chan, n_pol, _dev(uvw_dev), _ptr(freq), _dev(vis_dev),
         _dev(weights_dev), _dev(model_dev), _ptr(sel) if sel is not None else None,
         1 if psf else 0, float(psf_dl), float(psf_dm), int(weighting), int(polarization),
