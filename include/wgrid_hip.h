@@ -50,10 +50,14 @@ int wg_create(const wg_params* params, wg_handle** out);
 int wg_destroy(wg_handle* h);
 
 /* WGriddingGridderBase::ConstantMemoryUsage (gridder_simple.h:23,
- * gridder_simple_implementation.h:52-60) */
+ * gridder_simple_implementation.h:52-60): the device bytes that do not grow
+ * with the visibility count (images, the FFT-slab budget, tables). */
 int wg_constant_memory_usage(const wg_handle* h, size_t* bytes);
 /* WGriddingGridderBase::PerVisibilityMemoryUsage (gridder_simple.h:24,
- * gridder_simple_implementation.h:62-68) */
+ * gridder_simple_implementation.h:62-68): device bytes per visibility of
+ * one call.  Calls larger than 2^31 - 1 visibilities or than the free
+ * device memory holds are split into row chunks by the library, so a
+ * caller's chunk of any size works. */
 int wg_per_visibility_memory_usage(const wg_handle* h, size_t* bytes);
 
 /* WGriddingGridderBase::InitializeInversion (gridder_simple.h:25,
@@ -367,6 +371,13 @@ int wgx_radix_sort(size_t n, int key_bits, const uint32_t* keys_in, uint32_t* ke
 int wgx_make_plan(const wg_params* params, const double* extents6, long long nvis,
                   int W, double sigma, int tile, int sub_slab, char* plan_json,
                   size_t plan_len);
+/* Largest number of visibilities the drop-in calls (AddInversionData,
+ * PredictVisibilities and their device / corrected variants) hand to one
+ * engine pass; larger calls are split into row chunks.  0 (the default):
+ * 2^31 - 1 or what 80 % of the free device memory holds. */
+int wgx_set_max_chunk_vis(wg_handle* h, size_t max_vis);
+/* Device bytes the handle holds (its buffer arena). */
+int wgx_device_bytes(wg_handle* h, size_t* bytes);
 /* Host-only: the gridding kernels' polynomial form of the ES kernel for
  * support W and shape beta: coef[16][12] (row k: coefficients of g^0 .. g^d
  * of tap k, g = 2 f - 1; rows k >= W are zero) and the degree d.  Needs no

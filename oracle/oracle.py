@@ -187,8 +187,10 @@ def dft_invert_reference_wrap(width, height, dx, dy, l_shift, m_shift, uvw, freq
     return image.reshape(height, width)
 
 
-def bucket_keys(plan, uvw, freq, vis=None):
-    """(keys, valid) per flat visibility index, FP64 like LocateVis."""
+def bucket_locations(plan, uvw, freq):
+    """(tile, p0, du, dv) per flat visibility index [rows, chan], FP64 like
+    LocateVis: the visibility's uv tile, first w plane and first kernel cell
+    relative to the tile origin."""
     uvw = np.asarray(uvw, np.float64).reshape(-1, 3)
     freq = np.asarray(freq, np.float64)
     W, T = plan["W"], plan["T"]
@@ -221,13 +223,24 @@ def bucket_keys(plan, uvw, freq, vis=None):
     else:
         lv = a0v - plan["box_v0"]
     tile = (lv // T) * plan["ntu"] + (lu // T)
+    return tile, p0, lu % T, lv % T
+
+
+def bucket_keys(plan, uvw, freq, vis=None):
+    """(keys, valid) per flat visibility index, FP64 like LocateVis."""
+    T = plan["T"]
+    tile, p0, du, dv = bucket_locations(plan, uvw, freq)
     bucket = tile * plan["P0"] + p0
-    keys = ((bucket * T + (lv % T)) * T + (lu % T)).astype(np.int64)
+    # the in-bucket cell order, coarsened by key_shift when (bucket, cell)
+    # keys would exceed 32 bits
+    ks = plan.get("key_shift", 0)
+    kt = ((T - 1) >> ks) + 1
+    keys = ((bucket * kt + (dv >> ks)) * kt + (du >> ks)).astype(np.int64)
     valid = np.ones(keys.shape, bool)
     if vis is not None:
         vis = np.asarray(vis).reshape(keys.shape)
         valid = vis != 0
-    nkeys = plan["ntu"] * plan["ntv"] * plan["P0"] * T * T
+    nkeys = plan["ntu"] * plan["ntv"] * plan["P0"] * kt * kt
     keys = np.where(valid, keys, nkeys)
     return keys.ravel().astype(np.uint32), valid.ravel()
 

@@ -77,7 +77,7 @@ def test_atlas_regions_disjoint_and_plans_shared(seed, n_facets):
     assert occupied.max() == 1
     assert a["BU"] == a["ntu"] * T + W - 1 and a["BV"] == a["ntv"] * T + W - 1
     assert a["nbuckets"] == a["ntu"] * a["ntv"] * a["P0"]
-    assert a["nkeys"] == a["nbuckets"] * T * T < 2 ** 32
+    assert a["nkeys"] == a["nbuckets"] * a["key_T"] ** 2 < 2 ** 32
     # shelf packing wastes at most a modest share of the atlas
     used = sum((fl["plan"]["ntu"] + gap) * (fl["plan"]["ntv"] + gap) for fl in lay["facets"])
     assert used >= 0.5 * a["ntu"] * a["ntv"]

@@ -186,3 +186,40 @@ def test_epsilon_below_fp32_floor_is_refused():
     with pytest.raises(WgError, match="FP32"):
         g.AddInversionData(p["uvw"].shape[0], 2, p["uvw"], p["freq"], p["vis"])
     g.close()
+
+
+def coarse_key_problem():
+    # the 128-antenna 10 km track, one of every 256 time steps, at the band
+    # edges 120 / 180 MHz, imaged at 8192^2 with 6 arcsec pixels: the uv box
+    # times the w planes needs more than 2^32 (tile, plane, cell) keys
+    uvw, _, _, _ = synth.large_wgridding_inputs(64, 8192)
+    sub = np.ascontiguousarray(uvw.reshape(-1, 8128, 3)[::256].reshape(-1, 3))
+    freq = np.array([120e6, 180e6])
+    rng = np.random.Generator(np.random.PCG64(66))
+    dx = 6.0 * synth.ARCSEC
+    l, m, flux, _, _ = synth.point_sky(rng, 25, 8192, dx)
+    vis = synth.predict_sources(sub, freq, l, m, flux).astype(np.complex64)
+    return dict(uvw=sub, freq=freq, vis=vis, npix=8192, dx=dx)
+
+
+def test_coarse_sort_keys_bit_exact(oracle):
+    p = coarse_key_problem()
+    g = WGriddingGridder_Simple(8192, 8192, 8192, 8192, p["dx"], p["dx"], 0.0, 0.0, 1, 1e-5)
+    keys, perm, plan = g.sort_keys(p["uvw"], p["freq"], p["vis"])
+    print("coarse keys: key_shift %d key_T %d nkeys %d" % (plan["key_shift"], plan["key_T"],
+                                                           plan["nkeys"]), flush=True)
+    assert plan["key_shift"] >= 1
+    ref_keys, _ = oracle.bucket_keys(plan, p["uvw"], p["freq"], p["vis"])
+    np.testing.assert_array_equal(keys, ref_keys)
+    np.testing.assert_array_equal(perm, oracle.bucket_perm(plan, p["uvw"], p["freq"], p["vis"]))
+    g.close()
+
+
+def test_coarse_sort_keys_match_direct_ft(oracle):
+    p = coarse_key_problem()
+    e_inv, e_pred, plan = run_pair(oracle, p, 1e-5, None, n_pix_sample=300, n_vis_sample=3000)
+    print("coarse keys plan W %d P %d Sg %d key_shift %d: invert %.2e predict %.2e"
+          % (plan["W"], plan["P"], plan["Sg"], plan["key_shift"], e_inv, e_pred), flush=True)
+    assert plan["key_shift"] >= 1
+    assert e_inv < 1e-5
+    assert e_pred < 1e-5

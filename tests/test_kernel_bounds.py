@@ -55,7 +55,9 @@ def check_plan_bounds(plan, uvw, freq, vis=None):
     E = T + W - 1
     ntiles = ntu * ntv
     nbuckets = ntiles * P0
-    assert plan["nbuckets"] == nbuckets and plan["nkeys"] == nbuckets * T * T
+    kt = plan["key_T"]
+    assert kt == ((T - 1) >> plan["key_shift"]) + 1
+    assert plan["nbuckets"] == nbuckets and plan["nkeys"] == nbuckets * kt * kt < 2 ** 32
     assert plan["nu"] <= MAX_FFT and plan["nv"] <= MAX_FFT
     assert Sf % Sg == 0 and 1 <= Sg <= 32
     assert Sg * E * E * 8 <= 160 * 1024                      # LDS subgrid
@@ -72,13 +74,21 @@ def check_plan_bounds(plan, uvw, freq, vis=None):
     assert kv.size == 0 or (kv.min() >= 0 and kv.max() < plan["nkeys"])
     sk = np.sort(keys, kind="stable").astype(np.int64)
     n_valid = int(valid.sum())
-    off = np.searchsorted(sk, np.arange(nbuckets + 1, dtype=np.int64) * T * T, side="left")
+    off = np.searchsorted(sk, np.arange(nbuckets + 1, dtype=np.int64) * kt * kt, side="left")
     assert off[-1] == n_valid
 
     # decode the sorted keys exactly as the kernels unpack them
     s = sk[:n_valid]
-    du, dv = s % T, (s // T) % T
-    bucket = s // (T * T)
+    # the visibilities' locations in sorted order (the keys hold the cell
+    # only when key_shift is 0)
+    order = np.argsort(keys, kind="stable")[:n_valid]
+    loc_tile, loc_p0, loc_du, loc_dv = (a.ravel()[order] for a in
+                                        oracle.bucket_locations(plan, uvw, freq))
+    du, dv = loc_du, loc_dv
+    bucket = s // (kt * kt)
+    assert (bucket == loc_tile * P0 + loc_p0).all()
+    if plan["key_shift"] == 0:
+        assert (du == s % T).all() and (dv == (s // T) % T).all()
     tile, p0 = bucket // P0, bucket % P0
     assert (p0 < P0).all() and (tile < ntiles).all() and (p0 < 32768).all()
 
@@ -264,3 +274,16 @@ def test_kernel_index_ranges_wide_field_default():
     plan = make_plan(4096, 20.0 * synth.ARCSEC, 1e-5, uvw, freq)
     assert plan["W"] >= 10 and plan["P"] > plan["Sg"]
     check_plan_bounds(plan, uvw, freq, None)
+
+
+def test_kernel_index_ranges_coarse_sort_keys():
+    # 8192^2 at 6 arcsec on the benchmark's 128-antenna track: (tile, plane,
+    # cell) keys would exceed 32 bits, so the in-bucket cell order is
+    # coarsened (key_shift 1); one channel of every 32nd time step at both
+    # band edges
+    uvw, freq, _, _ = synth.large_wgridding_inputs(64, 8192)
+    sub = uvw.reshape(-1, 8128, 3)[::32].reshape(-1, 3)
+    plan = make_plan(8192, 6.0 * synth.ARCSEC, 1e-5, sub, freq)
+    assert plan["key_shift"] >= 1 and plan["nkeys"] < 2 ** 32
+    assert plan["nbuckets"] * plan["key_T"] ** 2 == plan["nkeys"]
+    check_plan_bounds(plan, sub[::4], freq[[0, 63]], None)

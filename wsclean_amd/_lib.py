@@ -141,6 +141,8 @@ SIGNATURES = [
     ("wgx_facet_atlas", ctypes.c_int,
      [ctypes.POINTER(WgParams), c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p,
       c_size]),
+    ("wgx_set_max_chunk_vis", ctypes.c_int, [ctypes.c_void_p, c_size]),
+    ("wgx_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
     ("wgx_es_poly", ctypes.c_int,
      [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     ("wgx_synth_visibilities", ctypes.c_int,

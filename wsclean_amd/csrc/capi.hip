@@ -176,6 +176,7 @@ struct wg_handle {
   hipGraphExec_t cycle = nullptr;  // captured major cycle on vs_prep
   uint64_t cycle_gen = 0;          // arena generation the graph's pointers belong to
   int device = 0;                  // device current at creation (the stream's)
+  size_t max_chunk_vis = 0;        // test hook: largest engine call (0: automatic)
   std::mutex mu;
 
   ~wg_handle() { DropGraph(cycle); }
@@ -288,22 +289,37 @@ int wg_destroy(wg_handle* h) {
   });
 }
 
+// Device memory of the gridder.  Per image pixel: the FP64 image, the FP32
+// dirty accumulator and correction, and the host-transfer images; plus the
+// FFT slab (w planes of the occupied uv box and two line buffers, budgeted
+// by the plan at slab_bytes) and tables.  Per visibility of one engine call:
+// sort keys and permutation with their ping-pong copies, sorted coordinates
+// and phasors, sorted or accumulated visibilities and the staging of host
+// calls, grown by the arena's 1/8 rounding; per row the staged and
+// renumbered uvw and the row sort.
+constexpr size_t kDevBytesPerPixel = 28;
+constexpr size_t kDevBytesTables = size_t(64) << 20;
+constexpr size_t kDevBytesPerVis = 64;
+constexpr size_t kDevBytesPerRow = 96;
+
 int wg_constant_memory_usage(const wg_handle* h, size_t* bytes) {
   return Guard([&] {
     WG_CHECK(h && bytes, "null argument");
-    // Same accounting as gridder_simple_implementation.h:53-60.
-    const size_t num_size = h->prm.epsilon <= 1.01e-5 ? sizeof(double) : sizeof(float);
-    size_t c = size_t(kSigmaMax * kSigmaMax * double(h->prm.width_t) * double(h->prm.height_t) *
-                      8.0);
-    c += h->prm.width_t * h->prm.height_t * num_size;
-    *bytes = c;
+    // The reference counts its padded grid and image on the host
+    // (gridder_simple_implementation.h:53-60); this library's constant
+    // footprint is in device memory.
+    const size_t npix = h->prm.width_t * h->prm.height_t;
+    *bytes = kDevBytesPerPixel * npix + h->eng->Options().slab_bytes + kDevBytesTables;
   });
 }
 
 int wg_per_visibility_memory_usage(const wg_handle* h, size_t* bytes) {
   return Guard([&] {
     WG_CHECK(h && bytes, "null argument");
-    *bytes = 8;  // gridder_simple_implementation.h:62-68
+    // device bytes per visibility of one call (the reference: 8 host bytes,
+    // gridder_simple_implementation.h:62-68); calls larger than the device
+    // holds are split by the library
+    *bytes = kDevBytesPerVis;
   });
 }
 
@@ -340,6 +356,23 @@ static void ApplyTuning(wg_handle* h, size_t n_rows, size_t n_chan, const double
   o.T = c.T;
 }
 
+// Rows per engine call: at most 2^31 - 1 visibilities (the index range of
+// the sorted permutation) and what 80 % of the free device memory (plus what
+// this handle's arena already holds, which is reused) takes at
+// kDevBytesPerVis / kDevBytesPerRow; wgx_set_max_chunk_vis lowers it.
+static size_t ChunkRows(wg_handle* h, size_t n_rows, size_t n_chan) {
+  size_t cap = size_t(0x7FFFFFFF) / n_chan;
+  if (h->max_chunk_vis) cap = std::min(cap, std::max<size_t>(1, h->max_chunk_vis / n_chan));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    const double budget = 0.8 * double(free_b + h->eng->Arena().Allocated());
+    const double per_row = double(kDevBytesPerVis) * double(n_chan) + double(kDevBytesPerRow);
+    cap = std::min(cap, std::max<size_t>(1, size_t(budget / per_row)));
+  }
+  WG_CHECK(cap >= 1, "no rows fit the device");
+  return std::min(cap, n_rows);
+}
+
 static void AddInversion(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw_dev,
                          const double* freq, const float* vis_dev) {
   wg::Engine& e = *h->eng;
@@ -356,14 +389,20 @@ int wg_add_inversion_data(wg_handle* h, size_t n_rows, size_t n_chan, const doub
   return Guard([&] {
     WG_CHECK(h && freq && (n_rows == 0 || (uvw && vis)), "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (n_rows == 0) return;
+    if (n_rows == 0 || n_chan == 0) return;
     wg::DeviceArena& a = h->eng->Arena();
-    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
-    float* dvis = static_cast<float*>(a.Get("h2d.vis", n_rows * n_chan * 8));
-    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice,
-                          h->s()));
-    WG_HIP(hipMemcpyAsync(dvis, vis, n_rows * n_chan * 8, hipMemcpyHostToDevice, h->s()));
-    AddInversion(h, n_rows, n_chan, duvw, freq, dvis);
+    // chunks of rows: the inversion of a chunk adds to the image
+    const size_t step = ChunkRows(h, n_rows, n_chan);
+    for (size_t r0 = 0; r0 < n_rows; r0 += step) {
+      const size_t nr = std::min(step, n_rows - r0);
+      double* duvw = static_cast<double*>(a.Get("h2d.uvw", nr * 3 * sizeof(double)));
+      float* dvis = static_cast<float*>(a.Get("h2d.vis", nr * n_chan * 8));
+      WG_HIP(hipMemcpyAsync(duvw, uvw + 3 * r0, nr * 3 * sizeof(double), hipMemcpyHostToDevice,
+                            h->s()));
+      WG_HIP(hipMemcpyAsync(dvis, vis + 2 * r0 * n_chan, nr * n_chan * 8, hipMemcpyHostToDevice,
+                            h->s()));
+      AddInversion(h, nr, n_chan, duvw, freq, dvis);
+    }
   });
 }
 
@@ -383,32 +422,40 @@ int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, s
     WG_CHECK(nparms == 2 || nparms == 4, "solutions have 2 or 4 values per antenna");
     WG_CHECK(n_antennas > 0 && n_times > 0, "empty solution table");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (n_rows == 0) return;
+    if (n_rows == 0 || n_chan == 0) return;
     for (size_t r = 0; r < n_rows; ++r) {
       WG_CHECK(antennas[2 * r] < n_antennas && antennas[2 * r + 1] < n_antennas,
                "antenna index out of range");
       WG_CHECK(time_offsets[r] < n_times, "time offset out of range");
     }
     wg::DeviceArena& a = h->eng->Arena();
-    const size_t nvis = n_rows * n_chan;
     const size_t ngain = n_times * n_chan * n_antennas * nparms;
-    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
-    float2* dpol = static_cast<float2*>(a.Get("h2d.polvis", nvis * n_pol * sizeof(float2)));
-    uint32_t* dant = static_cast<uint32_t*>(a.Get("h2d.ant", n_rows * 2 * sizeof(uint32_t)));
-    uint64_t* dtoff = static_cast<uint64_t*>(a.Get("h2d.toff", n_rows * sizeof(uint64_t)));
     float2* dgain = static_cast<float2*>(a.Get("h2d.gains", ngain * sizeof(float2)));
-    float2* dvis = static_cast<float2*>(a.Get("h2d.vis", nvis * sizeof(float2)));
     hipStream_t s = h->s();
-    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice, s));
-    WG_HIP(hipMemcpyAsync(dpol, vis, nvis * n_pol * sizeof(float2), hipMemcpyHostToDevice, s));
-    WG_HIP(hipMemcpyAsync(dant, antennas, n_rows * 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                          s));
-    WG_HIP(hipMemcpyAsync(dtoff, time_offsets, n_rows * sizeof(uint64_t), hipMemcpyHostToDevice,
-                          s));
     WG_HIP(hipMemcpyAsync(dgain, gains, ngain * sizeof(float2), hipMemcpyHostToDevice, s));
-    wg::LaunchCorrectCollapse(dpol, dant, dtoff, dgain, n_rows, n_chan, n_antennas, int(nparms),
-                              gain_mode, int(n_pol), apply_forward != 0, polarization, dvis, s);
-    AddInversion(h, n_rows, n_chan, duvw, freq, reinterpret_cast<const float*>(dvis));
+    // chunks of rows (the polarized input takes n_pol visibilities per row
+    // and channel)
+    const size_t step = std::max<size_t>(1, ChunkRows(h, n_rows, n_chan * (n_pol + 1)));
+    for (size_t r0 = 0; r0 < n_rows; r0 += step) {
+      const size_t nr = std::min(step, n_rows - r0);
+      const size_t nvis = nr * n_chan;
+      double* duvw = static_cast<double*>(a.Get("h2d.uvw", nr * 3 * sizeof(double)));
+      float2* dpol = static_cast<float2*>(a.Get("h2d.polvis", nvis * n_pol * sizeof(float2)));
+      uint32_t* dant = static_cast<uint32_t*>(a.Get("h2d.ant", nr * 2 * sizeof(uint32_t)));
+      uint64_t* dtoff = static_cast<uint64_t*>(a.Get("h2d.toff", nr * sizeof(uint64_t)));
+      float2* dvis = static_cast<float2*>(a.Get("h2d.vis", nvis * sizeof(float2)));
+      WG_HIP(hipMemcpyAsync(duvw, uvw + 3 * r0, nr * 3 * sizeof(double), hipMemcpyHostToDevice,
+                            s));
+      WG_HIP(hipMemcpyAsync(dpol, vis + 2 * r0 * n_chan * n_pol, nvis * n_pol * sizeof(float2),
+                            hipMemcpyHostToDevice, s));
+      WG_HIP(hipMemcpyAsync(dant, antennas + 2 * r0, nr * 2 * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s));
+      WG_HIP(hipMemcpyAsync(dtoff, time_offsets + r0, nr * sizeof(uint64_t),
+                            hipMemcpyHostToDevice, s));
+      wg::LaunchCorrectCollapse(dpol, dant, dtoff, dgain, nr, n_chan, n_antennas, int(nparms),
+                                gain_mode, int(n_pol), apply_forward != 0, polarization, dvis, s);
+      AddInversion(h, nr, n_chan, duvw, freq, reinterpret_cast<const float*>(dvis));
+    }
   });
 }
 
@@ -418,8 +465,11 @@ int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
   return Guard([&] {
     WG_CHECK(h && freq && (n_rows == 0 || (uvw_dev && vis_dev)), "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (n_rows == 0) return;
-    AddInversion(h, n_rows, n_chan, uvw_dev, freq, vis_dev);
+    if (n_rows == 0 || n_chan == 0) return;
+    const size_t step = ChunkRows(h, n_rows, n_chan);
+    for (size_t r0 = 0; r0 < n_rows; r0 += step)
+      AddInversion(h, std::min(step, n_rows - r0), n_chan, uvw_dev + 3 * r0, freq,
+                   vis_dev + 2 * r0 * n_chan);
   });
 }
 
@@ -509,15 +559,20 @@ int wg_predict_visibilities(wg_handle* h, size_t n_rows, size_t n_chan, const do
   return Guard([&] {
     WG_CHECK(h && freq && (n_rows == 0 || (uvw && vis_out)), "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (n_rows == 0) return;
+    if (n_rows == 0 || n_chan == 0) return;
     wg::DeviceArena& a = h->eng->Arena();
-    double* duvw = static_cast<double*>(a.Get("h2d.uvw", n_rows * 3 * sizeof(double)));
-    float* dout = static_cast<float*>(a.Get("h2d.vis", n_rows * n_chan * 8));
-    WG_HIP(hipMemcpyAsync(duvw, uvw, n_rows * 3 * sizeof(double), hipMemcpyHostToDevice,
-                          h->s()));
-    Predict(h, n_rows, n_chan, duvw, freq, dout);
-    WG_HIP(hipMemcpyAsync(vis_out, dout, n_rows * n_chan * 8, hipMemcpyDeviceToHost, h->s()));
-    WG_HIP(hipStreamSynchronize(h->s()));
+    const size_t step = ChunkRows(h, n_rows, n_chan);
+    for (size_t r0 = 0; r0 < n_rows; r0 += step) {
+      const size_t nr = std::min(step, n_rows - r0);
+      double* duvw = static_cast<double*>(a.Get("h2d.uvw", nr * 3 * sizeof(double)));
+      float* dout = static_cast<float*>(a.Get("h2d.vis", nr * n_chan * 8));
+      WG_HIP(hipMemcpyAsync(duvw, uvw + 3 * r0, nr * 3 * sizeof(double), hipMemcpyHostToDevice,
+                            h->s()));
+      Predict(h, nr, n_chan, duvw, freq, dout);
+      WG_HIP(hipMemcpyAsync(vis_out + 2 * r0 * n_chan, dout, nr * n_chan * 8,
+                            hipMemcpyDeviceToHost, h->s()));
+      WG_HIP(hipStreamSynchronize(h->s()));
+    }
   });
 }
 
@@ -527,8 +582,11 @@ int wg_predict_visibilities_device(wg_handle* h, size_t n_rows, size_t n_chan,
   return Guard([&] {
     WG_CHECK(h && freq && (n_rows == 0 || (uvw_dev && vis_out_dev)), "null argument");
     std::lock_guard<std::mutex> lk(h->mu);
-    if (n_rows == 0) return;
-    Predict(h, n_rows, n_chan, uvw_dev, freq, vis_out_dev);
+    if (n_rows == 0 || n_chan == 0) return;
+    const size_t step = ChunkRows(h, n_rows, n_chan);
+    for (size_t r0 = 0; r0 < n_rows; r0 += step)
+      Predict(h, std::min(step, n_rows - r0), n_chan, uvw_dev + 3 * r0, freq,
+              vis_out_dev + 2 * r0 * n_chan);
   });
 }
 
@@ -730,6 +788,22 @@ int wgx_make_plan(const wg_params* params, const double* e, long long nvis, int 
     const wg::Plan p = wg::MakePlan(g, wg::VisExtent{e[0], e[1], e[2], e[3], e[4], e[5]}, o);
     std::strncpy(plan_json, p.describe().c_str(), plan_len - 1);
     plan_json[plan_len - 1] = 0;
+  });
+}
+
+int wgx_set_max_chunk_vis(wg_handle* h, size_t max_vis) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->max_chunk_vis = max_vis;
+  });
+}
+
+int wgx_device_bytes(wg_handle* h, size_t* bytes) {
+  return Guard([&] {
+    WG_CHECK(h && bytes, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    *bytes = h->eng->Arena().Allocated();
   });
 }
 

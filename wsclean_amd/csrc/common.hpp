@@ -114,10 +114,14 @@ struct Plan {
   bool wrap_u = false, wrap_v = false;
   int ntu = 0, ntv = 0;        // tiles along u / v
   int BU = 0, BV = 0;          // box storage extent (ntu*T + W - 1, ...)
-  // sort
+  // sort: key = (bucket * key_T + (dv >> key_shift)) * key_T + (du >> key_shift);
+  // key_shift > 0 coarsens the first-cell order inside a bucket when the
+  // (bucket, cell) keys would not fit 32 bits
   uint32_t nbuckets = 0;       // (tile, first plane) buckets
   uint32_t nkeys = 0;          // valid keys are [0, nkeys); nkeys = invalid
   int key_bits = 0;
+  int key_shift = 0;
+  int key_T = 0;               // ((T - 1) >> key_shift) + 1
   // plane batching
   int Sg = 4;                  // planes per gridding sub-slab (LDS bound)
   int Sf = 0;                  // planes per FFT slab (HBM bound)
@@ -143,6 +147,10 @@ constexpr int kMaxSubSlab = 32;
 constexpr int kGridLdsReserve = 33 * 1024;
 // Default LDS per gridding workgroup, so that two are resident per CU.
 constexpr int kGridLdsTarget = 80 * 1024;
+
+// Sets nbuckets, nkeys, key_bits, key_shift and key_T of a plan whose tile
+// grid (ntu, ntv, T) and P0 are set.
+void PlanSortKeys(Plan& p);
 
 int GoodFFTSize(int n);        // even 2^a 3^b 5^c >= n of least LDS-FFT work
 std::vector<int> FFTRadices(int n);  // Stockham stage radices for n

@@ -29,8 +29,17 @@ struct DevPlan {
   size_t box_pitch, box_plane;
   int nx, ny;
   uint32_t nkeys;     // invalid key; valid keys are below
-  uint32_t nbuckets;  // (tile, p0) buckets; key = (bucket * T + dv) * T + du
+  uint32_t nbuckets;  // (tile, p0) buckets
+  // key = (bucket * key_T + (dv >> key_shift)) * key_T + (du >> key_shift)
+  int key_shift, key_T;
 };
+
+// Sort key of a visibility in bucket `bucket` with first cell (du, dv).
+__device__ __forceinline__ uint32_t SortKey(uint32_t bucket, int du, int dv, int key_shift,
+                                            int key_T) {
+  return (bucket * uint32_t(key_T) + uint32_t(dv >> key_shift)) * uint32_t(key_T) +
+         uint32_t(du >> key_shift);
+}
 
 DevPlan ToDevPlan(const Plan& p);
 

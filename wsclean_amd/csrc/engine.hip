@@ -57,6 +57,8 @@ DevPlan ToDevPlan(const Plan& p) {
   d.ny = p.img.ny;
   d.nkeys = p.nkeys;
   d.nbuckets = p.nbuckets;
+  d.key_shift = p.key_shift;
+  d.key_T = p.key_T;
   return d;
 }
 
@@ -201,8 +203,11 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   EventTimer tm(*this, &times_.plan);
   WG_CHECK(n_chan > 0, "n_chan must be positive");
   const size_t n_vis = n_rows * n_chan;
-  // flat indices live in the low 31 bits of the sorted permutation
-  WG_CHECK(n_vis < 0x80000000ull, "at most 2^31 - 1 visibilities per call");
+  // flat indices live in the low 31 bits of the sorted permutation; the
+  // drop-in calls split larger inputs into row chunks below this
+  WG_CHECK(n_vis < 0x80000000ull,
+           "a prepared visibility set holds at most 2^31 - 1 visibilities "
+           "(wg_add_inversion_data / wg_predict_visibilities split larger calls)");
   vs.tag = tag;
   vs.n_rows = n_rows;
   vs.n_chan = n_chan;
@@ -275,7 +280,7 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   }
   vs.perm = vals;
   vs.off = static_cast<uint32_t*>(arena_.Get(tag + ".off", (size_t(dp.nbuckets) + 1) * 4));
-  DeviceBucketOffsets(keys, n_vis, dp.nbuckets, uint32_t(dp.T) * uint32_t(dp.T), vs.off,
+  DeviceBucketOffsets(keys, n_vis, dp.nbuckets, uint32_t(dp.key_T) * uint32_t(dp.key_T), vs.off,
                       stream_);
   uint32_t nvalid = 0;
   WG_HIP(hipMemcpyAsync(&nvalid, vs.off + dp.nbuckets, 4, hipMemcpyDeviceToHost, stream_));

@@ -332,15 +332,15 @@ struct FacetDev {
 // of the batch (facet f owns [vbase[f], vbase[f + 1])).
 void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F,
                      const double* uvw, const double* scale, size_t n_chan, size_t n_virtual,
-                     int atlas_ntu, int atlas_P0, int T, uint32_t* keys, uint32_t* vals,
-                     hipStream_t s);
+                     int atlas_ntu, int atlas_P0, int key_shift, int key_T, uint32_t* keys,
+                     uint32_t* vals, hipStream_t s);
 // Keys in renumbered virtual row order (element r * n_chan + ch = channel
 // ch of virtual row rperm[r]; facet f owns virtual rows [vrbase[f],
 // vrbase[f + 1])); uvw4 = the renumbered rows padded to 32 bytes.
 void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vrbase, int F,
                                const double* uvw4, const double* scale,
-                               size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0, int T,
-                               uint32_t* keys, hipStream_t s);
+                               size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0,
+                               int key_shift, int key_T, uint32_t* keys, hipStream_t s);
 // rphys[r] = physical row of renumbered virtual row r.
 void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, int F,
                        const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, hipStream_t s);

@@ -100,12 +100,7 @@ FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
     pa.P0 = std::max(pa.P0, p.P0);
     pa.P = std::max(pa.P, p.P);
   }
-  const uint64_t nb = uint64_t(pa.ntu) * uint64_t(pa.ntv) * uint64_t(pa.P0);
-  const uint64_t nk = nb * uint64_t(T) * uint64_t(T);
-  WG_CHECK(nk < 0xFFFFFFF0ull,
-           "facet batch too large for one sort key space: split the facets into smaller batches");
-  pa.nbuckets = uint32_t(nb);
-  pa.nkeys = uint32_t(nk);
+  PlanSortKeys(pa);
   {
     // the atlas takes as many planes as its deepest facet; when they fit
     // one sub-slab of the full LDS, one sub-slab lets the slab gather and
@@ -114,8 +109,6 @@ FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
     const int max_sg = int((160 * 1024 - kGridLdsReserve) / (E * E * 8));
     if (opt.Sg <= 0 && pa.P > pa.Sg && pa.P <= max_sg && pa.P <= kMaxSubSlab) pa.Sg = pa.P;
   }
-  pa.key_bits = 1;
-  while ((uint64_t(1) << pa.key_bits) <= uint64_t(pa.nkeys)) ++pa.key_bits;
   // FFT slab: whole sub-slabs whose atlas box and largest facet line
   // buffers fit the slab budget
   size_t lines = 0;
@@ -244,8 +237,8 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
     uint32_t* rk2 = static_cast<uint32_t*>(A.Get("rows.keys2", n_vrows * 4));
     uint32_t* rp2 = static_cast<uint32_t*>(A.Get("rows.perm2", n_vrows * 4));
     uint32_t* rphys = static_cast<uint32_t*>(A.Get("rows.phys", n_vrows * 4));
-    LaunchFacetKeys(dfac, dvrb, F, uvw_dev, scale_ + n_chan / 2, 1, n_vrows, pa.ntu, pa.P0, T,
-                    rk, nullptr, s);
+    LaunchFacetKeys(dfac, dvrb, F, uvw_dev, scale_ + n_chan / 2, 1, n_vrows, pa.ntu, pa.P0,
+                    pa.key_shift, pa.key_T, rk, nullptr, s);
     if (DeviceRadixSort(rk, rperm, rk2, rp2, n_vrows, pa.key_bits, scratch, s, true))
       rperm = rp2;
     LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, s);
@@ -259,7 +252,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   if (n_virtual) {
     // the sort's first pass generates the index values itself
     LaunchFacetKeysRenumbered(dfac, dvrb, F, uvw4, scale_, n_vrows, n_chan, pa.ntu, pa.P0,
-                              T, keys, s);
+                              pa.key_shift, pa.key_T, keys, s);
     uint32_t* keys2 = static_cast<uint32_t*>(A.Get("atlas.keys2", n_virtual * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(A.Get("atlas.perm2", n_virtual * 4));
     if (DeviceRadixSort(keys, vals, keys2, vals2, n_virtual, pa.key_bits, scratch, s, true)) {
@@ -269,7 +262,8 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   }
   atlas_.perm = vals;
   atlas_.off = static_cast<uint32_t*>(A.Get("atlas.off", (size_t(pa.nbuckets) + 1) * 4));
-  DeviceBucketOffsets(keys, n_virtual, pa.nbuckets, uint32_t(T) * uint32_t(T), atlas_.off, s);
+  DeviceBucketOffsets(keys, n_virtual, pa.nbuckets, uint32_t(pa.key_T) * uint32_t(pa.key_T),
+                      atlas_.off, s);
   uint32_t nvalid = 0;
   WG_HIP(hipMemcpyAsync(&nvalid, atlas_.off + pa.nbuckets, 4, hipMemcpyDeviceToHost, s));
   WG_HIP(hipStreamSynchronize(s));

@@ -291,15 +291,7 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   box(e.vmin, e.vmax, p.vfac, p.nv, p.box_v0, p.wrap_v, p.ntv, p.BV);
   WG_CHECK(p.BU <= 2 * p.nu && p.BV <= 2 * p.nv, "uv box larger than twice the grid");
 
-  // sort key = ((tile * P0 + p0) * T + dv) * T + du: buckets of one tile and
-  // first plane, ordered by the exact first cell inside the bucket
-  const uint64_t nb = uint64_t(p.ntu) * uint64_t(p.ntv) * uint64_t(p.P0);
-  const uint64_t nk = nb * uint64_t(p.T) * uint64_t(p.T);
-  WG_CHECK(nk < 0xFFFFFFF0ull, "too many (tile, plane, cell) keys: use a smaller tile");
-  p.nbuckets = uint32_t(nb);
-  p.nkeys = uint32_t(nk);
-  p.key_bits = 1;
-  while ((uint64_t(1) << p.key_bits) <= uint64_t(p.nkeys)) ++p.key_bits;
+  PlanSortKeys(p);
 
   // ---- plane batching ----
   // default: as many planes as keep a gridding workgroup (subgrid + staging)
@@ -330,6 +322,30 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   return p;
 }
 
+void PlanSortKeys(Plan& p) {
+  // sort key = ((tile * P0 + p0) * T + dv) * T + du: buckets of one tile and
+  // first plane, ordered by the exact first cell inside the bucket (long
+  // runs of one cell for the gridding kernels).  When those keys exceed 32
+  // bits, the cell order inside a bucket is coarsened to (dv >> s, du >> s)
+  // cells: still one bucket per (tile, first plane), only the runs of one
+  // exact cell get shorter.
+  const uint64_t nb = uint64_t(p.ntu) * uint64_t(p.ntv) * uint64_t(p.P0);
+  WG_CHECK(nb < 0xFFFFFFF0ull,
+           "more than 2^32 (uv tile, w plane) buckets: use a larger tile (wg_set_tuning)");
+  int s = 0;
+  uint64_t kt = uint64_t(p.T);
+  while (nb * kt * kt >= 0xFFFFFFF0ull) {
+    ++s;
+    kt = uint64_t((p.T - 1) >> s) + 1;
+  }
+  p.key_shift = s;
+  p.key_T = int(kt);
+  p.nbuckets = uint32_t(nb);
+  p.nkeys = uint32_t(nb * kt * kt);
+  p.key_bits = 1;
+  while ((uint64_t(1) << p.key_bits) <= uint64_t(p.nkeys)) ++p.key_bits;
+}
+
 std::string Plan::describe() const {
   std::ostringstream s;
   s.precision(17);
@@ -344,7 +360,8 @@ std::string Plan::describe() const {
     << ",\"wrap_v\":" << (wrap_v ? 1 : 0) << ",\"key_bits\":" << key_bits
     << ",\"Sg\":" << Sg << ",\"Sf\":" << Sf << ",\"dx\":" << img.dx
     << ",\"dy\":" << img.dy << ",\"zc\":" << zc << ",\"nkeys\":" << nkeys
-    << ",\"nbuckets\":" << nbuckets << "}";
+    << ",\"nbuckets\":" << nbuckets << ",\"key_shift\":" << key_shift
+    << ",\"key_T\":" << key_T << "}";
   return s.str();
 }
 

@@ -73,7 +73,8 @@ __global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nbloc
   }
 }
 
-// Sort key of one visibility: ((tile * P0 + p0) * T + dv) * T + du.
+// Sort key of one visibility: ((tile * P0 + p0) * key_T + cell row) * key_T +
+// cell column (SortKey).
 // Caller row of a renumbered row: the bits of the fourth word of its padded
 // uvw4 entry (w, tag).
 __device__ __forceinline__ uint32_t RowTag(double2 wt) {
@@ -82,7 +83,7 @@ __device__ __forceinline__ uint32_t RowTag(double2 wt) {
 
 __device__ __forceinline__ uint32_t VisKey(const VisLoc& L, const DevPlan& dp) {
   const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
-  return (bucket * uint32_t(dp.T) + uint32_t(L.dv)) * uint32_t(dp.T) + uint32_t(L.du);
+  return SortKey(bucket, L.du, L.dv, dp.key_shift, dp.key_T);
 }
 
 __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* __restrict__ uvw,
@@ -529,8 +530,8 @@ __device__ __forceinline__ int FacetOf(const unsigned long long* __restrict__ vb
 __global__ void __launch_bounds__(kBlock) FacetKeysKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vbase, int F,
     const double* __restrict__ uvw, const double* __restrict__ scale, size_t n_chan,
-    size_t n_virtual, int atlas_ntu, int atlas_P0, int T, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ vals) {
+    size_t n_virtual, int atlas_ntu, int atlas_P0, int key_shift, int key_T,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_virtual;
        i += size_t(gridDim.x) * kBlock) {
     const int f = FacetOf(vbase, F, i);
@@ -541,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) FacetKeysKernel(
     const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
     const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
     const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
-    keys[i] = (bucket * uint32_t(T) + uint32_t(L.dv)) * uint32_t(T) + uint32_t(L.du);
+    keys[i] = SortKey(bucket, L.du, L.dv, key_shift, key_T);
     if (vals) vals[i] = uint32_t(i);
   }
 }
@@ -555,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
     const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_vrows, size_t n_chan, int atlas_ntu,
-    int atlas_P0, int T, uint32_t* __restrict__ keys) {
+    int atlas_P0, int key_shift, int key_T, uint32_t* __restrict__ keys) {
   const size_t n = n_vrows * n_chan;
   const double inv_chan = 1.0 / double(n_chan);
   for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
@@ -569,7 +570,7 @@ __global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
     const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
     const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
     const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
-    keys[e] = (bucket * uint32_t(T) + uint32_t(L.dv)) * uint32_t(T) + uint32_t(L.du);
+    keys[e] = SortKey(bucket, L.du, L.dv, key_shift, key_T);
   }
 }
 
@@ -646,23 +647,23 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale, size_
 
 void LaunchFacetKeys(const FacetDev* fac, const unsigned long long* vbase, int F,
                      const double* uvw, const double* scale, size_t n_chan, size_t n_virtual,
-                     int atlas_ntu, int atlas_P0, int T, uint32_t* keys, uint32_t* vals,
-                     hipStream_t s) {
+                     int atlas_ntu, int atlas_P0, int key_shift, int key_T, uint32_t* keys,
+                     uint32_t* vals, hipStream_t s) {
   if (!n_virtual) return;
   FacetKeysKernel<<<Blocks(n_virtual), kBlock, 0, s>>>(fac, vbase, F, uvw, scale, n_chan,
-                                                        n_virtual, atlas_ntu, atlas_P0, T, keys,
-                                                        vals);
+                                                        n_virtual, atlas_ntu, atlas_P0,
+                                                        key_shift, key_T, keys, vals);
   WG_HIP(hipGetLastError());
 }
 
 void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vrbase, int F,
                                const double* uvw4, const double* scale,
-                               size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0, int T,
-                               uint32_t* keys, hipStream_t s) {
+                               size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0,
+                               int key_shift, int key_T, uint32_t* keys, hipStream_t s) {
   if (!n_vrows) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   FacetKeysRenumberedKernel<<<Blocks(n_vrows * n_chan), kBlock, 0, s>>>(
-      fac, vrbase, F, uvw4, scale, n_vrows, n_chan, atlas_ntu, atlas_P0, T, keys);
+      fac, vrbase, F, uvw4, scale, n_vrows, n_chan, atlas_ntu, atlas_P0, key_shift, key_T, keys);
   WG_HIP(hipGetLastError());
 }
 

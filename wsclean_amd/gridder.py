@@ -266,6 +266,16 @@ class WGriddingGridder_Simple:
                          "degrid_kernel_ms", "degrid_launches", "degrid_bytes",
                          "grid_passes", "degrid_passes"], list(out)))
 
+    def set_max_chunk_vis(self, max_vis=0):
+        """Test hook: largest engine pass of the drop-in calls (0: automatic)."""
+        _lib.check(self._lib.wgx_set_max_chunk_vis(self._h, int(max_vis)))
+
+    def device_bytes(self):
+        """Device bytes the handle holds."""
+        v = ctypes.c_size_t()
+        _lib.check(self._lib.wgx_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_tuning(self, W=0, sigma=0.0, tile=0, sub_slab=0, slab_bytes=0):
         _lib.check(self._lib.wg_set_tuning(self._h, int(W), float(sigma), int(tile),
                                            int(sub_slab), int(slab_bytes)))
