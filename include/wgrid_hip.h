@@ -332,6 +332,42 @@ int wg_prepare_rows_noalloc(wg_handle* h, size_t n_rows, size_t n_chan, size_t n
                             float* out_dev, double* stats4);
 
 /* ------------------------------------------------------------------------
+ * Reordered measurement-set parts: the temporary files WSClean's
+ * ReorderedMsProvider reads (msproviders/reordering.h:22-60,
+ * msproviders/reorderedmsprovider.cpp:58-97, msproviders/msreaders/
+ * reorderedmsreader.cpp:6-150).  meta_path: the spectral window's
+ * "<prefix>-spw<d>-parted-meta.tmp"; data_path: the part's "<part
+ * prefix>.tmp"; weight_path ("<part prefix>-w.tmp") and model_path ("<part
+ * prefix>-m.tmp") may be NULL (weights 1, no model); n_pol: visibilities
+ * per row and channel in the files (1 for a single polarization). */
+typedef struct wg_part_reader wg_part_reader;
+int wg_part_open(const char* meta_path, const char* data_path, const char* weight_path,
+                 const char* model_path, size_t n_pol, wg_part_reader** out);
+int wg_part_close(wg_part_reader* p);
+/* info6 = rows, channel count, first channel, data description id, has
+ * model, polarizations per file; start time (s) and the measurement-set path
+ * of the meta header (optional). */
+int wg_part_info(const wg_part_reader* p, uint64_t* info6, double* start_time, char* ms_path,
+                 size_t ms_path_len);
+/* Rows [row0, row0 + n) into host buffers (each may be NULL): uvw [n][3]
+ * (metres), antennas [n][2], times [n], field ids [n] (ReadMeta), vis and
+ * model [n][chan][pol] complex<float> (ReadData, ReadModel), weights
+ * [n][chan][pol] float (ReadWeights). */
+int wg_part_read(const wg_part_reader* p, size_t row0, size_t n, double* uvw, uint16_t* antennas,
+                 double* times, uint16_t* fields, float* vis, float* weights, float* model);
+/* GridMeasurementSet over a whole part (wgridder/wgriddingmsgridder.cpp:
+ * 117-183 on a ReorderedMsProvider): chunks of chunk_rows rows (0: automatic)
+ * are read from the files into page-locked host buffers (the next chunk on a
+ * worker thread while the device grids the current one), weighted and
+ * collapsed on the device as wg_prepare_rows does (weighting, polarization,
+ * image_weights as there) and added to the handle's image as
+ * AddInversionData.  freq: the part's channel frequencies.  stats4: the
+ * weight sums of wg_prepare_rows over the part. */
+int wg_part_grid(wg_part_reader* p, wg_handle* h, const double* freq, size_t chunk_rows,
+                 int weighting, int polarization, wg_image_weights* image_weights,
+                 double* stats4);
+
+/* ------------------------------------------------------------------------
  * Small-inversion resampling (wgridder/wgriddingmsgridder.cpp:290-308 after
  * inversion, :326-336 before prediction, schaapcommon::math::Resampler):
  * Fourier interpolation of a float image, on the handle's stream.  Any side

@@ -133,6 +133,19 @@ SIGNATURES = [
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_part_open", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_size,
+      ctypes.POINTER(ctypes.c_void_p)]),
+    ("wg_part_close", ctypes.c_int, [ctypes.c_void_p]),
+    ("wg_part_info", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p,
+      c_size]),
+    ("wg_part_read", ctypes.c_int,
+     [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("wg_part_grid", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_size, ctypes.c_int, ctypes.c_int,
+      ctypes.c_void_p, ctypes.c_void_p]),
     ("wg_resample", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, c_size, c_size]),
     ("wg_real_image_resampled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_size, c_size]),

@@ -18,6 +18,9 @@
 #include "../../include/wgrid_hip.h"
 #include "device.hpp"
 #include "engine.hpp"
+#include "partfile.hpp"
+
+#include <future>
 
 namespace {
 
@@ -247,6 +250,30 @@ __global__ void WavelengthsKernel(double* __restrict__ lam, size_t n) {
        i += size_t(gridDim.x) * blockDim.x)
     lam[i] = wg::kSpeedOfLight / lam[i];
 }
+
+}  // namespace
+
+
+struct wg_part_reader {
+  std::unique_ptr<wg::PartReader> r;
+};
+
+namespace {
+
+// Page-locked host memory owned for the duration of one entry point.
+struct PinnedBuf {
+  void* p = nullptr;
+  explicit PinnedBuf(size_t bytes) { WG_HIP(hipHostMalloc(&p, bytes ? bytes : 16)); }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
 
 }  // namespace
 
@@ -1420,6 +1447,139 @@ int wg_prepare_rows_noalloc(wg_handle* h, size_t n_rows, size_t n_chan, size_t n
                           reinterpret_cast<float2*>(out_dev), part, dstats, s);
     WG_HIP(hipMemcpyAsync(stats4, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
     WG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ------------------------------------------------------------ reordered parts
+
+int wg_part_open(const char* meta_path, const char* data_path, const char* weight_path,
+                 const char* model_path, size_t n_pol, wg_part_reader** out) {
+  return Guard([&] {
+    WG_CHECK(meta_path && data_path && out, "null argument");
+    auto pr = std::make_unique<wg_part_reader>();
+    pr->r = std::make_unique<wg::PartReader>(meta_path, data_path, weight_path ? weight_path : "",
+                                             model_path ? model_path : "", n_pol);
+    *out = pr.release();
+  });
+}
+
+int wg_part_close(wg_part_reader* p) {
+  return Guard([&] { delete p; });
+}
+
+int wg_part_info(const wg_part_reader* p, uint64_t* info6, double* start_time, char* ms_path,
+                 size_t ms_path_len) {
+  return Guard([&] {
+    WG_CHECK(p && info6, "null argument");
+    const wg::PartReader& r = *p->r;
+    const uint64_t v[6] = {r.Rows(), r.Channels(), r.ChannelStart(), r.DataDescId(),
+                           r.HasModel() ? 1u : 0u, r.Polarizations()};
+    std::copy(v, v + 6, info6);
+    if (start_time) *start_time = r.StartTime();
+    if (ms_path && ms_path_len) {
+      std::strncpy(ms_path, r.MsPath().c_str(), ms_path_len - 1);
+      ms_path[ms_path_len - 1] = 0;
+    }
+  });
+}
+
+int wg_part_read(const wg_part_reader* p, size_t row0, size_t n, double* uvw, uint16_t* antennas,
+                 double* times, uint16_t* fields, float* vis, float* weights, float* model) {
+  return Guard([&] {
+    WG_CHECK(p, "null argument");
+    const wg::PartReader& r = *p->r;
+    WG_CHECK(row0 <= r.Rows() && n <= r.Rows() - row0, "rows beyond the part");
+    if (uvw || antennas || times || fields) r.ReadMeta(row0, n, uvw, antennas, times, fields);
+    if (vis) r.ReadData(row0, n, vis);
+    if (weights) r.ReadWeights(row0, n, weights);
+    if (model) r.ReadModel(row0, n, model);
+  });
+}
+
+int wg_part_grid(wg_part_reader* p, wg_handle* h, const double* freq, size_t chunk_rows,
+                 int weighting, int polarization, wg_image_weights* image_weights,
+                 double* stats4) {
+  return Guard([&] {
+    WG_CHECK(p && h && freq && stats4, "null argument");
+    WG_CHECK(weighting >= 0 && weighting <= 2, "weighting: 0 normal, 1 squared, 2 unit");
+    WG_CHECK(wg::ValidPolCode(polarization), "unknown polarization code");
+    const wg::PartReader& r = *p->r;
+    const size_t R = r.Rows(), C = r.Channels(), P = r.Polarizations();
+    std::fill(stats4, stats4 + 4, 0.0);
+    for (size_t c = 0; c < C; ++c) WG_CHECK(freq[c] > 0, "frequencies must be positive");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (R == 0) return;
+    std::unique_lock<std::mutex> wl;
+    const wg::WeightGrid* g = nullptr;
+    if (image_weights) {
+      wl = std::unique_lock<std::mutex>(image_weights->mu);
+      WG_CHECK(image_weights->finished, "imaging weights used before FinishGridding()");
+      g = &image_weights->g;
+    }
+    // rows per chunk: at most 2^24 visibilities (the page-locked staging of
+    // two chunks), and what one engine pass takes
+    size_t step = chunk_rows ? chunk_rows : std::max<size_t>(1, (size_t(1) << 24) / (C * P));
+    step = std::min({step, R, ChunkRows(h, R, C * (P + 1))});
+    // two page-locked staging sets: the disk read of the next chunk runs on
+    // a worker thread while the device grids the current one
+    struct Stage {
+      std::unique_ptr<PinnedBuf> rec, uvw, vis, wts;
+    } st[2];
+    for (Stage& b : st) {
+      b.rec = std::make_unique<PinnedBuf>(step * wg::kMetaRecordBytes);
+      b.uvw = std::make_unique<PinnedBuf>(step * 3 * sizeof(double));
+      b.vis = std::make_unique<PinnedBuf>(step * C * P * 8);
+      b.wts = std::make_unique<PinnedBuf>(step * C * P * 4);
+    }
+    PinnedBuf hstats(4 * sizeof(double));
+    auto read_chunk = [&r, &st, step, R](size_t k) {
+      const size_t r0 = k * step, nr = std::min(step, R - r0);
+      Stage& b = st[k & 1];
+      r.ReadMetaRaw(r0, nr, b.rec->p);
+      wg::PartReader::UnpackUvw(b.rec->p, nr, b.uvw->as<double>());
+      r.ReadData(r0, nr, b.vis->as<float>());
+      r.ReadWeights(r0, nr, b.wts->as<float>());
+    };
+    wg::DeviceArena& a = h->eng->Arena();
+    hipStream_t s = h->s();
+    const std::vector<double> lam = Wavelengths(freq, C);
+    double* dlam = static_cast<double*>(a.Get("part.lambda", C * sizeof(double)));
+    WG_HIP(hipMemcpyAsync(dlam, lam.data(), C * sizeof(double), hipMemcpyHostToDevice, s));
+    WG_HIP(hipStreamSynchronize(s));
+    double* dpart =
+        static_cast<double*>(a.Get("part.partials", wg::WeightStatPartials() * sizeof(double)));
+    double* dstats = static_cast<double*>(a.Get("part.stats", 4 * sizeof(double)));
+    double* duvw = static_cast<double*>(a.Get("part.uvw", step * 3 * sizeof(double)));
+    float2* dvis = static_cast<float2*>(a.Get("part.vis", step * C * P * sizeof(float2)));
+    float* dwts = static_cast<float*>(a.Get("part.wts", step * C * P * sizeof(float)));
+    float2* dout = static_cast<float2*>(a.Get("part.out", step * C * sizeof(float2)));
+    const size_t nchunks = (R + step - 1) / step;
+    std::future<void> next = std::async(std::launch::async, read_chunk, size_t(0));
+    for (size_t k = 0; k < nchunks; ++k) {
+      next.get();  // chunk k is in staging set k & 1 (rethrows a read error)
+      const size_t r0 = k * step, nr = std::min(step, R - r0);
+      const Stage& b = st[k & 1];
+      WG_HIP(hipMemcpyAsync(duvw, b.uvw->p, nr * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+      WG_HIP(hipMemcpyAsync(dvis, b.vis->p, nr * C * P * 8, hipMemcpyHostToDevice, s));
+      WG_HIP(hipMemcpyAsync(dwts, b.wts->p, nr * C * P * 4, hipMemcpyHostToDevice, s));
+      WG_HIP(hipStreamSynchronize(s));  // staging set k & 1 is free again
+      if (k + 1 < nchunks) next = std::async(std::launch::async, read_chunk, k + 1);
+      wg::PrepArgs args{};
+      args.n_rows = nr;
+      args.n_chan = C;
+      args.n_pol = int(P);
+      args.weighting = weighting;
+      args.polarization = polarization;
+      wg::LaunchPrepareRows(args, duvw, dlam, dvis, dwts, nullptr, nullptr, g, dout, dpart, dstats,
+                            s);
+      WG_HIP(hipMemcpyAsync(hstats.p, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+      AddInversion(h, nr, C, duvw, freq, reinterpret_cast<const float*>(dout));
+      const double* q = hstats.as<double>();
+      stats4[0] += q[0];
+      stats4[1] += q[1];
+      stats4[2] = std::max(stats4[2], q[2]);
+      stats4[3] += q[3];
+    }
   });
 }
 
