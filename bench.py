@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-per-channel", action="store_true",
                     help="skip the one-call-per-channel GPU comparison leg")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-stride", type=int, default=200,
+                    help="CPU baseline sample: every n-th small-w row of the track")
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests in the served-stream leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrivals per second")
@@ -46,6 +47,8 @@ def parse():
                     help="run only the served-stream leg and print its JSON")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph leg")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the direct-FT error of the timed results")
     ap.add_argument("--no-host-leg", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--facet-steps", type=int, default=2,
@@ -93,94 +96,121 @@ def roofline_bytes(plan, n_vis, n_rows, nx, ny):
 MFMA_F32_PEAK = 157.3e12   # MI355X dense FP32 matrix FLOP/s
 
 
-def kernel_roofline(st, plan):
-    """Roofline of the dominant kernel, the ES-kernel gridding kernel, from
-    its own HIP events on the engine stream.
-
-    For W <= 8 the gridding runs on the FP32 matrix cores
-    (GridKernelMfma<W, fused>): bound "mfma", algorithmic FLOPs = 4 W^3 per
-    visibility pass (each of the W^3 footprint cells takes a complex x real
-    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  With one
-    plane sub-slab (P <= Sg) the kernel gathers the visibilities itself
-    (fused).  The HBM view is reported beside it: algorithmic bytes per
-    launch = per visibility pass 16 B coordinates + 8 B visibility (fused:
-    + 4 B permutation word + 8 B phasor), read once per sub-slab the
-    visibility touches, + 8 B x sub_n x E^2 per work item (its subgrid
-    written once), against 8 TB/s.  traffic = measured HBM
-    bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the
-    copy-kernel calibration) from the committed counter summary of this
-    kernel and plan, else None."""
-    n = st["grid_launches"]
-    if not n or st["grid_kernel_ms"] <= 0:
-        return None
-    W = plan["W"]
-    t = st["grid_kernel_ms"] / n / 1e3
-    per_launch_bytes = st["grid_bytes"] / n
-    per_launch_flops = 4.0 * W ** 3 * st["grid_passes"] / n
-    gbs = per_launch_bytes / t / 1e9
-    mfma = W <= 8
-    fused = mfma and plan["P"] <= plan["Sg"]
-    kernel = ("GridKernelMfma<%d, %s>" % (W, "true" if fused else "false")) if mfma else \
-        "GridKernel<%d>" % W
-    if mfma:
-        achieved = per_launch_flops / t / 1e12
-        out = {"bound": "mfma", "kernel": kernel, "achieved": achieved,
-               "peak": MFMA_F32_PEAK / 1e12, "unit": "TFLOP/s",
-               "frac": achieved / (MFMA_F32_PEAK / 1e12)}
+def _kernel_entry(name, ms, launches, flops, nbytes, W):
+    """Roofline entry of one gridding kernel from its HIP-event time."""
+    t = ms / launches / 1e3
+    gbs = nbytes / launches / t / 1e9
+    out = {"kernel": name, "launch_ms": t * 1e3, "launches": int(launches),
+           "flops_per_launch": flops / launches, "bytes_per_launch": nbytes / launches,
+           "hbm_view": {"achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": gbs / (HBM_PEAK / 1e9)}, "traffic": None}
+    if W <= 8:
+        ach = flops / launches / t / 1e12
+        out.update({"bound": "mfma", "achieved": ach, "peak": MFMA_F32_PEAK / 1e12,
+                    "unit": "TFLOP/s", "frac": ach / (MFMA_F32_PEAK / 1e12)})
     else:
-        out = {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK / 1e9,
-               "unit": "GB/s", "frac": gbs / (HBM_PEAK / 1e9)}
-    dt = st["degrid_kernel_ms"] / max(st["degrid_launches"], 1) / 1e3
-    out.update({
-        "traffic": None, "launch_ms": t * 1e3, "launches": int(n),
-        "flops_per_launch": per_launch_flops, "bytes_per_launch": per_launch_bytes,
-        "hbm_view": {"achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": gbs / (HBM_PEAK / 1e9)},
-        "degrid": {"kernel": ("DegridKernelMfma<%d, %s>" % (W, "true" if fused else "false"))
-                   if mfma else "DegridKernel<%d>" % W,
-                   "launch_ms": dt * 1e3,
-                   "achieved_tflops": 4.0 * W ** 3 * st["degrid_passes"]
-                   / max(st["degrid_launches"], 1) / max(dt, 1e-12) / 1e12}})
-    path = os.path.join(ROOT, "profiles", "gridkernel_8192_traffic.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            tr = json.load(f)
-        if tr.get("kernel") == kernel and \
-                tr.get("plan") == {k: plan[k] for k in tr.get("plan", {})}:
-            out["traffic"] = tr["bytes_per_dispatch"]
+        out.update({"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": gbs / (HBM_PEAK / 1e9)})
     return out
 
 
-def cpu_baseline(uvw_h, freq, dx, src, npix, plan, max_seconds):
+def kernel_roofline(st, plan):
+    """Rooflines of the two gridding kernels from their own HIP events on
+    the engine stream; the slower one is the dominant kernel.
+
+    For W <= 8 both run on the FP32 matrix cores (GridKernelMfma<W, fused>,
+    DegridKernelMfma<W, fused>): bound "mfma", algorithmic FLOPs = 4 W^3 per
+    visibility pass (each of the W^3 footprint cells takes a complex x real
+    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  With one
+    plane sub-slab (P <= Sg) the kernels gather / scatter the visibilities
+    themselves (fused).  The HBM view is reported beside it: algorithmic
+    bytes per launch = per visibility pass 16 B coordinates + 8 B visibility
+    (fused: + 4 B permutation word + 8 B phasor), read once per sub-slab the
+    visibility touches, + 8 B x sub_n x E^2 per work item (its subgrid
+    written or read once), against 8 TB/s.  traffic = measured HBM bytes per
+    launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    calibration) from the committed counter summary of that kernel and plan,
+    else None."""
+    if not st["grid_launches"] or not st["degrid_launches"]:
+        return None
+    W = plan["W"]
+    mfma = W <= 8
+    fused = mfma and plan["P"] <= plan["Sg"]
+    fl = "true" if fused else "false"
+    names = (("GridKernelMfma<%d, %s>" % (W, fl), "DegridKernelMfma<%d, %s>" % (W, fl)) if mfma
+             else ("GridKernel<%d>" % W, "DegridKernel<%d>" % W))
+    grid = _kernel_entry(names[0], st["grid_kernel_ms"], st["grid_launches"],
+                         4.0 * W ** 3 * st["grid_passes"], st["grid_bytes"], W)
+    degrid = _kernel_entry(names[1], st["degrid_kernel_ms"], st["degrid_launches"],
+                           4.0 * W ** 3 * st["degrid_passes"], st["degrid_bytes"], W)
+    for e, stem in ((grid, "gridkernel"), (degrid, "degridkernel")):
+        path = os.path.join(ROOT, "profiles", "%s_8192_traffic.json" % stem)
+        if os.path.exists(path):
+            with open(path) as f:
+                tr = json.load(f)
+            if tr.get("kernel") == e["kernel"] and \
+                    tr.get("plan") == {k: plan[k] for k in tr.get("plan", {})}:
+                e["traffic"] = tr["bytes_per_dispatch"]
+    main, other = (degrid, grid) if degrid["launch_ms"] >= grid["launch_ms"] else (grid, degrid)
+    out = dict(main)
+    out["other_gridder"] = other
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(uvw_h, freq, dx, src, npix, plan, stride=200, w_max_m=100.0):
     """FP64 CPU w-gridder (oracle/wgridder_cpu.cpp, 'CPU restatement, not
-    ducc0') on a bounded sample of the same workload: the full image and the
-    same kernel (W, beta, nu, nv), one channel (the middle one) and the
-    central part of the track, sized for ~max_seconds on the host cores.
-    Returns invert + predict visibilities per second of the sample."""
+    ducc0') on a like-for-like sample of the same workload, all channels,
+    the full image and the same kernel and grid (W, beta, sigma, nu, nv).
+    The CPU times its visibility work (set-up, gridding, degridding) and its
+    plane work (FFTs, screens) separately.  The sample is every
+    `stride`-th of the track's rows with |w| < w_max_m: those rows span
+    nearly the full uv box (so the plane work per plane is the full job's)
+    but only the minimum W planes (so the sweep stays short).  The full
+    job's CPU time is the visibility work scaled by the visibility ratio
+    plus the plane work per plane times the job's plane count.  Returns
+    invert + predict visibilities per second of that full-job time."""
     from oracle import oracle
     from wsclean_amd import synth
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "64") or 64))
-    n_bl = 8128
-    n_t = uvw_h.shape[0] // n_bl
-    # central 1/4 of the track of one channel
-    t0, t1 = n_t * 3 // 8, n_t * 5 // 8
-    sub = uvw_h[t0 * n_bl:t1 * n_bl]
-    f = np.array([freq[len(freq) // 2]])
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    sel = np.flatnonzero(np.abs(uvw_h[:, 2]) < w_max_m)[::stride]
+    sub = np.ascontiguousarray(uvw_h[sel])
     l, m, S = src[:64, 0], src[:64, 1], src[:64, 3]
-    vis = synth.predict_sources(sub[:200000], f, l, m, S).astype(np.complex64)
-    vis = np.resize(vis, (sub.shape[0], 1)).astype(np.complex64)
+    vis = synth.predict_sources(sub[:20000], freq, l, m, S).astype(np.complex64)
+    vis = np.resize(vis, (sub.shape[0], len(freq))).astype(np.complex64)
     kw = dict(W=plan["W"], beta=plan["beta"], sigma=plan["sigma"], nu=plan["nu"],
               nv=plan["nv"], nthreads=cores)
     t = time.perf_counter()
-    img = oracle.cpu_wgrid_invert(npix, npix, dx, dx, 0.0, 0.0, sub, f, vis, **kw)
-    oracle.cpu_wgrid_predict(npix, npix, dx, dx, 0.0, 0.0, img, sub, f, **kw)
-    dt = time.perf_counter() - t
-    n = sub.shape[0]
-    return {"value": n / dt, "unit": "vis/s", "cores": cores, "kind": "port",
-            "sample": "CPU restatement (FP64, not ducc0): invert+predict of %d visibilities "
-                      "(1 channel at %.1f MHz, central quarter of the track), %d^2 image, "
-                      "same kernel W=%d sigma=%.2f; %.1f s" % (n, f[0] / 1e6, npix, plan["W"],
-                                                              plan["sigma"], dt)}
+    img = oracle.cpu_wgrid_invert(npix, npix, dx, dx, 0.0, 0.0, sub, freq, vis, **kw)
+    tv_i, tp_i, planes_i = oracle.cpu_wgrid_last_timing()
+    oracle.cpu_wgrid_predict(npix, npix, dx, dx, 0.0, 0.0, img, sub, freq, **kw)
+    tv_p, tp_p, planes_p = oracle.cpu_wgrid_last_timing()
+    wall = time.perf_counter() - t
+    n_sample = sub.shape[0] * len(freq)
+    n_full = uvw_h.shape[0] * len(freq)
+    full_s = (tv_i + tv_p) * (n_full / n_sample) + (tp_i + tp_p) * plan["P"] / max(1, planes_i)
+    return {"value": n_full / full_s, "unit": "vis/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": "CPU restatement (FP64, not ducc0): invert+predict of every %dth row with "
+                      "|w| < %g m (%d rows) x all %d channels = %d visibilities, %d^2 image, same "
+                      "kernel and grid W=%d sigma=%.2f nu=%d, %d w planes; visibility work "
+                      "scaled by %.0fx, plane work scaled to the job's %d planes; %.1f s wall" %
+                      (stride, w_max_m, sub.shape[0], len(freq), n_sample, npix, plan["W"],
+                       plan["sigma"], plan["nu"], planes_i, n_full / n_sample, plan["P"], wall),
+            "split_s": {"vis_invert": tv_i, "vis_predict": tv_p, "planes_invert": tp_i,
+                        "planes_predict": tp_p},
+            "full_job_s": full_s,
+            "sample_wall_s": wall}
 
 
 def main():
@@ -277,11 +307,15 @@ def main():
         "cpu_baseline": None,
         "gen_s": t_gen,
     }
+    if not args.no_accuracy:
+        line["accuracy"] = accuracy_leg(g, uvw_h, vis, out, freq, dx, n, n_vis)
     if not args.no_graph:
         line["graph"] = {"8192": graph_cycle(g, vis, out, n_vis, max(2, args.steps))}
     # the other legs make handles of their own: free this one's HBM first
     g.close()
     torch.cuda.empty_cache()
+    line["correction"] = fresh_handle_step(uvw, vis, out, freq, n, dx, args.eps, n_rows, n_chan,
+                                           ms)
     if not args.no_autotune:
         line["autotune"] = autotune_leg(uvw, freq, n, dx, args.eps, n_rows, n_chan, plan)
     if not args.no_per_channel:
@@ -297,8 +331,62 @@ def main():
     if args.facet_steps > 0:
         line["facets"] = facets_leg(args)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(uvw_h, freq, dx, src, n, plan, args.cpu_stride)
     print(json.dumps(line))
+
+
+def accuracy_leg(g, uvw_h, vis, out, freq, dx, n, n_vis, n_pix=16, n_sel=10000, seed=8192):
+    """Error of the timed pipeline's own results against the FP64 direct FT
+    (oracle, the DirectMSGridder restatement), relative L2 as epsilon is
+    defined (doc/source/wgridding.rst:19-20): the last step's dirty image
+    (x n_vis) on n_pix random pixels over all visibilities, and a predict of
+    a 500-source pixel model on n_sel random visibilities."""
+    import torch
+    from oracle import oracle
+    from wsclean_amd import synth
+    t = time.time()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    img = g.RealImage().reshape(n, n).astype(np.float64) * n_vis
+    xs, ys = rng.integers(0, n, n_pix), rng.integers(0, n, n_pix)
+    vis_h = vis.cpu().numpy().view(np.complex64).reshape(uvw_h.shape[0], len(freq))
+    ref = oracle.dft_invert_pixels_fast(n, n, dx, dx, 0.0, 0.0, uvw_h, freq, vis_h, xs, ys)
+    del vis_h
+    e_inv = float(np.linalg.norm(img[ys, xs] - ref) / np.linalg.norm(ref))
+    model = np.zeros((n, n))
+    _, _, flux, mx, my = synth.point_sky(rng, 500, n, dx, on_pixel=True)
+    model[my, mx] = flux
+    g.InitializePrediction(model.astype(np.float32).ravel())
+    g.PredictPrepared(out)
+    sel = rng.choice(out.shape[0] * out.shape[1], n_sel, replace=False)
+    got = out.view(-1, 2)[torch.from_numpy(sel).to(out.device)].cpu().numpy()
+    refv = oracle.dft_predict(n, n, dx, dx, 0.0, 0.0, model, uvw_h, freq, sel=sel)
+    e_pred = float(np.linalg.norm(got[:, 0] + 1j * got[:, 1] - refv) / np.linalg.norm(refv))
+    return {"invert_rel_l2": e_inv, "invert_pixels": n_pix, "predict_rel_l2": e_pred,
+            "predict_visibilities": n_sel, "reference": "FP64 direct FT", "s": time.time() - t}
+
+
+def fresh_handle_step(uvw, vis, out, freq, n, dx, eps, n_rows, n_chan, ms_cached):
+    """One step on a new handle, as WSClean creates a new gridder for every
+    inversion and prediction (wgridder/wgriddingmsgridder.cpp:267, :316):
+    it pays the grid-correction table (cached per handle in the timed
+    steps), the FFT twiddle tables and the buffer allocations."""
+    import torch
+    from wsclean_amd import WGriddingGridder_Simple
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, eps)
+    g.PrepareVisibilities(uvw, freq, n_rows, n_chan)
+    g.InitializeInversion()
+    g.AddInversionPrepared(vis)
+    g.FinalizeImage(1.0 / (n_rows * n_chan))
+    g.PredictPrepared(out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    g.close()
+    torch.cuda.empty_cache()
+    return {"cached_per_handle": True, "timed_step_ms": ms_cached, "fresh_handle_step_ms": ms,
+            "note": "the timed steps reuse one handle, which keeps the grid-correction table; "
+                    "a new handle per step also builds it and allocates its buffers"}
 
 
 def host_buffer_leg(uvw, vis, freq, n, dx, eps, n_rows, n_chan):

@@ -317,6 +317,18 @@ def cpu_wgrid_predict(nx, ny, dx, dy, l_shift, m_shift, image, uvw, freq, W, bet
     return (out[0::2] + 1j * out[1::2]).reshape(n_rows, n_chan)
 
 
+def cpu_wgrid_last_timing():
+    """(visibility-work seconds, plane-work seconds, planes) of the last
+    cpu_wgrid_invert / cpu_wgrid_predict call."""
+    lib = load()
+    out = np.zeros(3)
+    f = lib.oracle_wgrid_last_timing
+    f.argtypes = [ctypes.c_void_p]
+    f.restype = None
+    f(_p(out))
+    return float(out[0]), float(out[1]), int(out[2])
+
+
 def es_phi_hat(W, beta, t):
     """Phi(t) = int_{-W/2}^{W/2} exp(beta (sqrt(1 - (2s/W)^2) - 1)) cos(2 pi s t) ds."""
     from scipy.integrate import quad

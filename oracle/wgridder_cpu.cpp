@@ -17,6 +17,7 @@
 // Precision is FP64 throughout (WGriddingMSGridder::MakeGridder selects
 // double for epsilon <= 1.01e-5, wgridder/wgriddingmsgridder.cpp:43-48).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -26,6 +27,17 @@
 #include <vector>
 
 namespace {
+
+// Wall time of the last invert / predict split into the work that scales
+// with the visibilities (set-up, shifts, gridding / degridding) and the
+// work per w plane (FFTs, screens, crops): the CPU baseline extrapolates a
+// row sample to the full visibility count with the plane sweep unchanged.
+double g_t_vis = 0, g_t_plane = 0;
+int g_planes = 0;
+using Clock = std::chrono::steady_clock;
+double Since(Clock::time_point t) {
+  return std::chrono::duration<double>(Clock::now() - t).count();
+}
 
 using cd = std::complex<double>;
 constexpr double kC = 299792458.0;
@@ -287,6 +299,8 @@ struct Problem {
 };
 
 void Invert(Problem& pb, const float* vis_in, double* img_out /* [y][x] */) {
+  g_t_vis = g_t_plane = 0;
+  auto t0 = Clock::now();
   pb.Setup(vis_in);
   const int nx = pb.nx, ny = pb.ny, nu = pb.nu, nv = pb.nv, W = pb.W;
   const int T = pb.nthreads;
@@ -300,9 +314,11 @@ void Invert(Problem& pb, const float* vis_in, double* img_out /* [y][x] */) {
     if (v.flip) V = std::conj(V);
     shifted[i] = V * pb.Shift(v, -1.0);
   }
+  g_t_vis += Since(t0);
   Fft fu(nu), fv(nv);
   for (int p = 0; p < pb.P; ++p) {
     const size_t b0 = pb.p0_off[std::max(0, p - W + 1)], b1 = pb.p0_off[std::min(p, pb.P - W) + 1];
+    t0 = Clock::now();
     ParallelFor(T, b1 - b0, [&](size_t a, size_t b, int t) {
       std::vector<cd>& box = boxes[t];
       std::fill(box.begin(), box.end(), cd(0));
@@ -320,6 +336,8 @@ void Invert(Problem& pb, const float* vis_in, double* img_out /* [y][x] */) {
         }
       }
     });
+    g_t_vis += Since(t0);
+    t0 = Clock::now();
     // sum the thread boxes, FFT box rows and keep the cropped columns
     ParallelFor(T, pb.BV, [&](size_t a, size_t b, int) {
       std::vector<cd> line(nu), scr(nu);
@@ -350,14 +368,19 @@ void Invert(Problem& pb, const float* vis_in, double* img_out /* [y][x] */) {
         }
       }
     });
+    g_t_plane += Since(t0);
   }
+  g_planes = pb.P;
   for (int x = 0; x < nx; ++x)
     for (int y = 0; y < ny; ++y)
       img_out[size_t(y) * nx + x] = dirty[size_t(x) * ny + y] * pb.corr[size_t(x) * ny + y];
 }
 
 void Predict(Problem& pb, const double* img /* [y][x] */, double* vis_out) {
+  g_t_vis = g_t_plane = 0;
+  auto t0 = Clock::now();
   pb.Setup(nullptr);
+  g_t_vis += Since(t0);
   const int nx = pb.nx, ny = pb.ny, nu = pb.nu, nv = pb.nv, W = pb.W;
   const int T = pb.nthreads;
   std::vector<double> model(size_t(nx) * ny);
@@ -369,6 +392,7 @@ void Predict(Problem& pb, const double* img /* [y][x] */, double* vis_out) {
   Fft fu(nu), fv(nv);
   for (int p = 0; p < pb.P; ++p) {
     const double wp = pb.PlaneW(p);
+    t0 = Clock::now();
     ParallelFor(T, nx, [&](size_t a, size_t b, int) {
       std::vector<cd> line(nv), scr(nv);
       for (size_t x = a; x < b; ++x) {
@@ -392,6 +416,8 @@ void Predict(Problem& pb, const double* img /* [y][x] */, double* vis_out) {
         for (int i = 0; i < pb.BU; ++i) box[j * pb.BU + i] = line[pb.Wrap(pb.u_lo + i, nu)];
       }
     });
+    g_t_plane += Since(t0);
+    t0 = Clock::now();
     const size_t b0 = pb.p0_off[std::max(0, p - W + 1)], b1 = pb.p0_off[std::min(p, pb.P - W) + 1];
     ParallelFor(T, b1 - b0, [&](size_t a, size_t b, int) {
       double ku[16], kv[16];
@@ -410,7 +436,9 @@ void Predict(Problem& pb, const double* img /* [y][x] */, double* vis_out) {
         acc[i] += s * kw;
       }
     });
+    g_t_vis += Since(t0);
   }
+  g_planes = pb.P;
   for (size_t i = 0; i < pb.vis.size(); ++i) {
     const auto& v = pb.vis[i];
     cd V = acc[i] * pb.Shift(v, +1.0);
@@ -457,6 +485,14 @@ int oracle_wgrid_invert(int nx, int ny, double dx, double dy, double ls, double 
                            freq, nthreads);
   Invert(pb, vis, out);
   return pb.P;
+}
+
+// Wall seconds of the last invert or predict: out3 = {visibility work,
+// plane work, planes}.
+void oracle_wgrid_last_timing(double* out3) {
+  out3[0] = g_t_vis;
+  out3[1] = g_t_plane;
+  out3[2] = double(g_planes);
 }
 
 // Visibilities out[2 * (row * n_chan + chan) + {0, 1}] from img[y * nx + x].
