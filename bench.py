@@ -118,10 +118,14 @@ def kernel_roofline(st, plan):
     """Rooflines of the two gridding kernels from their own HIP events on
     the engine stream; the slower one is the dominant kernel.
 
-    For W <= 8 both run on the FP32 matrix cores (GridKernelMfma<W, fused>,
-    DegridKernelMfma<W, fused>): bound "mfma", algorithmic FLOPs = 4 W^3 per
-    visibility pass (each of the W^3 footprint cells takes a complex x real
-    multiply-add), against the 157.3 TFLOP/s dense FP32 MFMA peak.  With one
+    For W <= 8 both run on the matrix cores: GridKernelSplit<W, fused> /
+    DegridKernelSplit<W, fused> (FP32 operands as f16 pairs on the f16
+    instructions) where the plan says "split", else GridKernelMfma /
+    DegridKernelMfma (FP32 instructions).  Bound "mfma", algorithmic FLOPs =
+    4 W^3 per visibility pass (each of the W^3 footprint cells takes a
+    complex x real multiply-add, counted once, in FP32 terms), against the
+    157.3 TFLOP/s dense FP32 MFMA peak: the FP32-equivalent work, whatever
+    instructions carry it.  With one
     plane sub-slab (P <= Sg) the kernels gather / scatter the visibilities
     themselves (fused).  The HBM view is reported beside it: algorithmic
     bytes per launch = per visibility pass 16 B coordinates + 8 B visibility
@@ -137,8 +141,12 @@ def kernel_roofline(st, plan):
     mfma = W <= 8
     fused = mfma and plan["P"] <= plan["Sg"]
     fl = "true" if fused else "false"
-    names = (("GridKernelMfma<%d, %s>" % (W, fl), "DegridKernelMfma<%d, %s>" % (W, fl)) if mfma
-             else ("GridKernel<%d>" % W, "DegridKernel<%d>" % W))
+    if mfma and plan.get("split"):
+        names = ("GridKernelSplit<%d, %s>" % (W, fl), "DegridKernelSplit<%d, %s>" % (W, fl))
+    elif mfma:
+        names = ("GridKernelMfma<%d, %s>" % (W, fl), "DegridKernelMfma<%d, %s>" % (W, fl))
+    else:
+        names = ("GridKernel<%d>" % W, "DegridKernel<%d>" % W)
     grid = _kernel_entry(names[0], st["grid_kernel_ms"], st["grid_launches"],
                          4.0 * W ** 3 * st["grid_passes"], st["grid_bytes"], W)
     degrid = _kernel_entry(names[1], st["degrid_kernel_ms"], st["degrid_launches"],

@@ -204,6 +204,12 @@ int wg_stage_times(wg_handle* h, double* out14, int reset);
  * edge, planes per gridding sub-slab, FFT-slab byte budget. */
 int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab,
                   size_t slab_bytes);
+/* Arithmetic of the matrix-core gridding (W <= 8): -1 (default) split
+ * precision (each FP32 operand as a pair of f16 halves on the f16 matrix
+ * instructions) where its error estimate meets epsilon, FP32 otherwise;
+ * 0 always FP32; 1 split whenever W <= 8.  No counterpart in the
+ * reference (a tuning override like wg_set_tuning). */
+int wg_set_precision(wg_handle* h, int split);
 
 /* ------------------------------------------------------------------------
  * Facet batches.  WSClean images a faceted field with one gridder per facet

@@ -66,6 +66,7 @@ SIGNATURES = [
     ("wg_stage_times", ctypes.c_int, [ctypes.c_void_p, c_dbl_p, ctypes.c_int]),
     ("wg_set_tuning", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, c_size]),
+    ("wg_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("wgx_sort_keys", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),

@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwgrid_hip.so")
 
 SOURCES = ["plan.cpp", "partfile.cpp", "sort.hip", "vis.hip", "fft.hip", "grid.hip",
-           "engine.hip", "facets.hip", "weights.hip", "resample.hip", "capi.hip"]
+           "grid_split.hip", "engine.hip", "facets.hip", "weights.hip", "resample.hip",
+           "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

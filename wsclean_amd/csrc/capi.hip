@@ -755,6 +755,14 @@ int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab, siz
   });
 }
 
+int wg_set_precision(wg_handle* h, int split) {
+  return Guard([&] {
+    WG_CHECK(h, "null handle");
+    WG_CHECK(split >= -1 && split <= 1, "precision mode must be -1, 0 or 1");
+    h->eng->Options().split = split;
+  });
+}
+
 int wgx_sort_keys(wg_handle* h, size_t n_rows, size_t n_chan, const double* uvw,
                   const double* freq, const float* vis, uint32_t* keys_out, uint32_t* perm_out,
                   size_t* n_valid, char* plan_json, size_t plan_len) {

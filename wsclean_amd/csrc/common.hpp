@@ -75,6 +75,16 @@ inline double ShapeErrorEstimate(const KernelShape& k) {
   return 3.0 * k.err1d + kFp32AmpTerm * k.amp_rms * k.amp_rms * k.amp_rms + kFp32Floor;
 }
 
+// Expected relative RMS error of one shape (W <= 8) gridded and degridded
+// with split-precision f16 matrix operations: the FP32 estimate plus the
+// products' rounding (about 2^-21 of each product, amplified by the grid
+// correction like the FP32 rounding).
+constexpr double kSplitAmpTerm = 2.5e-8;
+constexpr double kSplitFloor = 1e-7;
+inline double SplitErrorEstimate(const KernelShape& k) {
+  return ShapeErrorEstimate(k) + kSplitAmpTerm * k.amp_rms * k.amp_rms * k.amp_rms + kSplitFloor;
+}
+
 // Smallest epsilon any shape of the table reaches with FP32 grids.
 double Fp32EpsilonFloor();
 
@@ -125,6 +135,9 @@ struct Plan {
   // plane batching
   int Sg = 4;                  // planes per gridding sub-slab (LDS bound)
   int Sf = 0;                  // planes per FFT slab (HBM bound)
+  // W <= 8: grid and degrid with split-precision f16 matrix operations
+  // (each FP32 operand as a pair of halves) instead of FP32 ones
+  bool split = false;
 
   std::string describe() const;
 };
@@ -138,6 +151,8 @@ struct PlanOptions {
   size_t slab_bytes = size_t(8) << 30;  // HBM budget for one FFT slab
   double sigma_min = 1.1, sigma_max = 2.0;
   long long nvis_hint = 0;     // for the cost model
+  int split = -1;              // -1: split precision where epsilon allows;
+                               // 0: FP32 matrix operations; 1: split for W <= 8
 };
 
 // Largest number of w planes of one gridding sub-slab (its LDS subgrid).

@@ -32,6 +32,7 @@ struct DevPlan {
   uint32_t nbuckets;  // (tile, p0) buckets
   // key = (bucket * key_T + (dv >> key_shift)) * key_T + (du >> key_shift)
   int key_shift, key_T;
+  int split;          // 1: split-precision matrix-core gridding (W <= 8)
 };
 
 // Sort key of a visibility in bucket `bucket` with first cell (du, dv).
@@ -110,13 +111,18 @@ __device__ __forceinline__ VisLoc LocateVis(double um, double vm, double wm,
 // against ~3e-7 for exp2 / sqrt of the closed form, whose sqrt(1 - x^2) - 1
 // cancels.
 template <int D>
-__device__ __forceinline__ float EsPoly(const float (&c)[D + 1], float f) {
-  const float g = __builtin_fmaf(f, 2.0f, -1.0f);
+__device__ __forceinline__ float EsPolyG(const float (&c)[D + 1], float g) {
   float p = c[D];
 #pragma unroll
   for (int d = D - 1; d >= 0; --d) p = __builtin_fmaf(p, g, c[d]);
   return p;
 }
+template <int D>
+__device__ __forceinline__ float EsPoly(const float (&c)[D + 1], float f) {
+  return EsPolyG<D>(c, __builtin_fmaf(f, 2.0f, -1.0f));
+}
+// phi is even, so tap W - 1 - k at f equals tap k at 1 - f, i.e. the
+// polynomial of tap k at -g: the taps k < W / 2 give all W.
 
 // The coefficients of tap k (any lane-varying k < kEsPolyTaps).
 template <int D>

@@ -58,6 +58,7 @@ DevPlan ToDevPlan(const Plan& p) {
   d.nkeys = p.nkeys;
   d.nbuckets = p.nbuckets;
   d.key_shift = p.key_shift;
+  d.split = p.split ? 1 : 0;
   d.key_T = p.key_T;
   return d;
 }

@@ -308,6 +308,20 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
 void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
                      const float2* scratch, int sub, int sub_lo, int sub_n,
                      int slab_lo, hipStream_t s);
+// Split-precision (f16 pairs) matrix-core gridding for W <= 8, same
+// arguments and results as LaunchGrid's matrix-core path (grid_split.hip);
+// LaunchGrid calls it when dp.split is set.
+void LaunchGridSplit(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
+                     float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
+                     int sub_n, hipStream_t s, const float2* vis_raw,
+                     const uint32_t* gather_perm);
+size_t SplitGridLdsBytes(const DevPlan& dp, int sub_n);
+// Split-precision matrix-core degridding for W <= 8 (grid_split.hip);
+// LaunchDegrid calls it when dp.split is set.
+void LaunchDegridSplit(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc,
+                       uint32_t* queue, int sub, int sub_lo, int sub_n, int slab_lo,
+                       hipStream_t s, const uint32_t* scatter_perm);
+size_t SplitDegridLdsBytes(const DevPlan& dp, int sub_n);
 size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub);
 // `queue`: one device word, the work-item counter of the persistent
 // matrix-core degridder (cleared by the launch).

@@ -16,6 +16,7 @@
 
 #include "device.hpp"
 #include "engine.hpp"
+#include "mfma_common.hpp"
 
 namespace wg {
 
@@ -182,67 +183,6 @@ __global__ void __launch_bounds__(kGridBlock) GridKernel(
 // with LDS atomics when the run changes.  Coordinates are staged per wave
 // through a 2 x 64-visibility LDS ring, loaded one batch ahead, and each
 // staged batch carries a 64-bit mask of the visibilities that start a run.
-constexpr int kMfmaWaves = 8;
-constexpr int kStage = 64;
-
-// Source of the visibilities the gridding kernel stages, one batch of 64 in
-// flight.  VisSource<false> reads the gathered (sorted, flipped, shifted)
-// visibilities.  VisSource<true> fuses the gather: it reads the caller's
-// visibility through the sorted permutation and applies the flip and the
-// sorted phasor itself.  The permutation word of a batch is loaded one
-// batch before its visibility, so the dependent random load never waits.
-template <bool FUSED>
-struct VisSource;
-
-template <>
-struct VisSource<false> {
-  const float2* vis;
-  float2 v;
-  __device__ __forceinline__ void Prime(uint32_t, uint32_t, int) {}
-  __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
-    v = first + uint32_t(lane) < end ? vis[first + lane] : make_float2(0.f, 0.f);
-  }
-  __device__ __forceinline__ float2 Value() const { return v; }
-};
-
-template <>
-struct VisSource<true> {
-  const float2* vis;
-  const uint32_t* perm;
-  const float2* phasor;
-  uint32_t pw_next, pw;
-  float2 raw, ph;
-  // permutation word of the batch the next Issue() loads
-  __device__ __forceinline__ void Prime(uint32_t first, uint32_t end, int lane) {
-    pw_next = first + uint32_t(lane) < end ? perm[first + lane] : 0u;
-  }
-  __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
-    pw = pw_next;
-    const bool in = first + uint32_t(lane) < end;
-    raw = in ? vis[pw & ~kPermFlip] : make_float2(0.f, 0.f);
-    ph = in ? phasor[first + lane] : make_float2(0.f, 0.f);
-    const uint32_t nxt = first + uint32_t(kStage) + uint32_t(lane);
-    pw_next = nxt < end ? perm[nxt] : 0u;
-  }
-  __device__ __forceinline__ float2 Value() const {
-    const float2 V = make_float2(raw.x, (pw & kPermFlip) ? -raw.y : raw.y);
-    return make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
-  }
-};
-
-// Run key (packed cell and first plane) of the lane's staged visibility;
-// -2 past the end of the range, so the end starts a "run".
-__device__ __forceinline__ int StageKey(uint32_t first, uint32_t end, int lane, const float4& c) {
-  return first + uint32_t(lane) < end ? (__float_as_int(c.w) & 0x7fffffff) : -2;
-}
-
-// Bit i set: staged visibility i starts a run (its key differs from that of
-// the visibility before it; prev_last is the key before lane 0).
-__device__ __forceinline__ uint64_t RunStarts(int key, int prev_last, int lane) {
-  int prev = __shfl_up(key, 1);
-  if (lane == 0) prev = prev_last;
-  return __ballot(key != prev);
-}
 
 // Stores the lane's staged visibility at ring position `slot` in both views.
 __device__ __forceinline__ void StoreStage(float4* ring, int slot, const float4& c,
@@ -258,15 +198,6 @@ struct alignas(16) Stage3 {
 __device__ __forceinline__ Stage3 ReadStage(const float4* view, int slot) {
   return *reinterpret_cast<const Stage3*>(view + slot);
 }
-
-// Orders one wave's LDS stores before its lanes read each other's values.
-__device__ __forceinline__ void WaveLdsSync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 // Lane exchanges of the gridding step.  The step runs with all 64 lanes
 // active, and every exchange must be executed by all of them: a DPP move
@@ -923,6 +854,11 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* scratch,
                 uint32_t* queue, const float* vmax, int sub, int sub_lo, int sub_n,
                 hipStream_t s, const float2* vis_raw, const uint32_t* gather_perm) {
+  if (dp.split && dp.W <= 8) {
+    LaunchGridSplit(dp, vs, vis_sorted, scratch, queue, vmax, sub, sub_lo, sub_n, s, vis_raw,
+                    gather_perm);
+    return;
+  }
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
@@ -980,6 +916,10 @@ size_t GridScratchBytes(const DevPlan& dp, int Sg, size_t max_items_per_sub) {
 void LaunchDegrid(const DevPlan& dp, const VisSet& vs, const float2* box, float2* acc,
                   uint32_t* queue, int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s,
                   const uint32_t* scatter_perm) {
+  if (dp.split && dp.W <= 8) {
+    LaunchDegridSplit(dp, vs, box, acc, queue, sub, sub_lo, sub_n, slab_lo, s, scatter_perm);
+    return;
+  }
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const size_t lds = GridLdsBytes(dp, sub_n);
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");

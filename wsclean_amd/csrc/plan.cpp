@@ -246,6 +246,11 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   p.beta = chosen.beta;
   p.err1d = chosen.err1d;
   p.err_est = ShapeErrorEstimate(chosen);
+  // split-precision matrix gridding for W <= 8 where its error still meets
+  // epsilon (SplitErrorEstimate), unless the options force either form
+  p.split = p.W <= 8 && (opt.split == 1 ||
+                         (opt.split < 0 && SplitErrorEstimate(chosen) <= g.epsilon));
+  if (p.split) p.err_est = std::max(p.err_est, SplitErrorEstimate(chosen));
   p.nu = GoodFFTSize(int(std::ceil(p.sigma * g.nx)));
   p.nv = GoodFFTSize(int(std::ceil(p.sigma * g.ny)));
   WG_CHECK(p.nu <= 16384 && p.nv <= 16384,
@@ -361,7 +366,7 @@ std::string Plan::describe() const {
     << ",\"Sg\":" << Sg << ",\"Sf\":" << Sf << ",\"dx\":" << img.dx
     << ",\"dy\":" << img.dy << ",\"zc\":" << zc << ",\"nkeys\":" << nkeys
     << ",\"nbuckets\":" << nbuckets << ",\"key_shift\":" << key_shift
-    << ",\"key_T\":" << key_T << "}";
+    << ",\"key_T\":" << key_T << ",\"split\":" << (split ? 1 : 0) << "}";
   return s.str();
 }
 

@@ -280,6 +280,11 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_set_tuning(self._h, int(W), float(sigma), int(tile),
                                            int(sub_slab), int(slab_bytes)))
 
+    def set_precision(self, split=-1):
+        """Matrix-core gridding arithmetic for W <= 8: -1 split f16 pairs
+        where epsilon allows, 0 FP32, 1 split whenever W <= 8."""
+        _lib.check(self._lib.wg_set_precision(self._h, int(split)))
+
     def sort_keys(self, uvw, freq, vis=None):
         """Test hook: (unsorted bucket keys, stable sorted permutation, plan)."""
         uvw = _f64(uvw)
