@@ -160,3 +160,41 @@ def test_more_than_2_31_visibilities(oracle):
     torch.cuda.empty_cache()
     assert e_inv < 1e-5
     assert e_pred < 1e-5
+
+
+def test_grid_sides_above_the_lds_fft(oracle):
+    # a 12288^2 image at eps 1e-5 pads to grid sides above one LDS FFT line
+    # (16384 points): the plane transforms run in two passes over R
+    # interleaved subsequences; invert and predict against the direct FT
+    p = synth.small_problem(seed=73, n_ant=24, n_times=30, n_chan=2, npix=12288,
+                            pixel_arcsec=1.0, radius_m=5000.0, n_src=12, noise=0.01)
+    n, eps = 12288, 1e-5
+    n_rows, n_chan = p["uvw"].shape[0], len(p["freq"])
+    g = WGriddingGridder_Simple(n, n, n, n, p["dx"], p["dx"], 0.0, 0.0, 1, eps)
+    g.InitializeInversion()
+    g.AddInversionData(n_rows, n_chan, p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(1.0)
+    plan = g.plan()
+    print("long-line plan", {k: plan[k] for k in ("W", "sigma", "nu", "nv", "P", "Sf")},
+          flush=True)
+    assert plan["nu"] > 16384 and plan["nv"] > 16384
+    img = g.RealImage().reshape(n, n)
+    rng = np.random.default_rng(73)
+    xs, ys = rng.integers(0, n, 300), rng.integers(0, n, 300)
+    ref = oracle.dft_invert_pixels(n, n, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
+                                   p["vis"], xs, ys)
+    e_inv = rel_l2(img[ys, xs], ref)
+    del img
+    model = np.zeros((n, n), np.float32)
+    k = 24
+    model[rng.integers(8, n - 8, k), rng.integers(8, n - 8, k)] = rng.uniform(-1.0, 1.0, k)
+    g.InitializePrediction(model.ravel())
+    out = np.zeros((n_rows, n_chan), np.complex64)
+    g.PredictVisibilities(n_rows, n_chan, p["uvw"], p["freq"], out)
+    g.close()
+    sel = rng.choice(out.size, 2000, replace=False)
+    refv = oracle.dft_predict(n, n, p["dx"], p["dx"], 0.0, 0.0, model.astype(np.float64),
+                              p["uvw"], p["freq"], sel=sel)
+    e_pred = rel_l2(out.ravel()[sel], refv)
+    print("long-line invert %.3e predict %.3e" % (e_inv, e_pred), flush=True)
+    assert e_inv < eps and e_pred < eps

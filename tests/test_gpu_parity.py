@@ -33,7 +33,9 @@ def invert(p, eps, ls=0.0, ms=0.0, n=None):
 # ------------------------------------------------------------------- FFT
 
 @pytest.mark.parametrize("n", [16, 18, 60, 81, 96, 162, 486, 1000, 1458, 2048, 2560, 3000, 4096,
-                               4374, 10240, 13122, 13500, 13824, 14580, 16384])
+                               4374, 10240, 13122, 13500, 13824, 14580, 16384,
+                               # two-pass lines: R = 3, 2, 2, 2, 3, 5, 4, 5 subsequences
+                               19683, 20000, 24576, 32768, 39366, 50625, 65536, 81920])
 def test_lds_fft_matches_numpy(n):
     rng = np.random.default_rng(n)
     x = (rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))).astype(np.complex64)

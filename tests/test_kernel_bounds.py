@@ -17,7 +17,19 @@ from oracle import oracle
 from wsclean_amd import _lib, synth
 
 CHUNK = 65536      # wsclean_amd/csrc/grid.hip kChunk
-MAX_FFT = 16384    # largest LDS FFT line
+MAX_LDS_FFT = 16384   # largest LDS FFT line
+MAX_FFT = 5 * MAX_LDS_FFT  # longest line: R <= 5 subsequences of an LDS line
+
+
+def split_radix(n):
+    """Subsequences of a line of n points (fft.hip LongFFTRadix): 1 up to
+    the LDS FFT, else the smallest R in 2..5 with n / R an LDS line."""
+    if n <= MAX_LDS_FFT:
+        return 1
+    for r in range(2, 6):
+        if n % r == 0 and n // r <= MAX_LDS_FFT:
+            return r
+    return 0
 C = synth.C_LIGHT
 
 
@@ -59,6 +71,9 @@ def check_plan_bounds(plan, uvw, freq, vis=None):
     assert kt == ((T - 1) >> plan["key_shift"]) + 1
     assert plan["nbuckets"] == nbuckets and plan["nkeys"] == nbuckets * kt * kt < 2 ** 32
     assert plan["nu"] <= MAX_FFT and plan["nv"] <= MAX_FFT
+    for side in (plan["nu"], plan["nv"]):
+        r = split_radix(side)
+        assert r >= 1 and side % r == 0 and side // r <= MAX_LDS_FFT
     assert Sf % Sg == 0 and 1 <= Sg <= 32
     assert Sg * E * E * 8 <= 160 * 1024                      # LDS subgrid
     if W <= 8:
@@ -287,3 +302,16 @@ def test_kernel_index_ranges_coarse_sort_keys():
     assert plan["key_shift"] >= 1 and plan["nkeys"] < 2 ** 32
     assert plan["nbuckets"] * plan["key_T"] ** 2 == plan["nkeys"]
     check_plan_bounds(plan, sub[::4], freq[[0, 63]], None)
+
+
+@pytest.mark.parametrize("npix", [12288, 20480, 30720])
+def test_plans_with_grid_sides_above_the_lds_fft(npix):
+    # images whose padded grid at eps 1e-5 exceeds one LDS FFT line (16384)
+    # plan with lines split into 2..5 subsequences; bench track, 2 arcsec,
+    # every 64th time step of one channel at both band edges
+    uvw, freq, _, _ = synth.large_wgridding_inputs(64, 8192)
+    sub = uvw.reshape(-1, 8128, 3)[::64].reshape(-1, 3)
+    plan = make_plan(npix, 2.0 * synth.ARCSEC, 1e-5, sub, freq[[0, 63]])
+    assert plan["nu"] > MAX_LDS_FFT and plan["nv"] > MAX_LDS_FFT
+    assert plan["nu"] >= plan["sigma"] * npix and plan["nv"] >= plan["sigma"] * npix
+    check_plan_bounds(plan, sub[::8], freq[[0, 63]], None)

@@ -888,14 +888,21 @@ int wgx_facet_atlas(const wg_params* facets, size_t n_facets, const double* exte
 
 int wgx_fft(int n, int nlines, int inverse, float* data) {
   return Guard([&] {
-    WG_CHECK(data && n > 0 && nlines > 0 && n <= 16384, "bad arguments");
-    const std::vector<float2> tw = wg::FFTTwiddleTable(n);
+    WG_CHECK(data && n > 0 && nlines > 0 && n <= wg::kMaxFFT, "bad arguments");
+    const int R = wg::LongFFTRadix(n);
+    WG_CHECK(R > 0, "FFT length not split by 2..5 into LDS FFT lines");
+    const int m = n / R;
+    const std::vector<float2> tw = wg::FFTTwiddleTable(m);
     const size_t bytes = size_t(n) * nlines * sizeof(float2);
-    DevBuf dtw(tw.size() * sizeof(float2)), dd(bytes);
+    DevBuf dtw(tw.size() * sizeof(float2)), dd(bytes), tmp(R > 1 ? bytes : 16);
     WG_HIP(hipMemcpy(dtw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
     WG_HIP(hipMemcpy(dd.p, data, bytes, hipMemcpyHostToDevice));
-    wg::LaunchLineFFT(dd.as<float2>(), nlines, wg::MakeFFTDesc(n, dtw.as<float2>()),
-                      inverse != 0, nullptr);
+    if (R > 1)
+      wg::LaunchLineFFTLong(dd.as<float2>(), nlines, n, R, wg::MakeFFTDesc(m, dtw.as<float2>()),
+                            tmp.as<float2>(), inverse != 0, nullptr);
+    else
+      wg::LaunchLineFFT(dd.as<float2>(), nlines, wg::MakeFFTDesc(n, dtw.as<float2>()),
+                        inverse != 0, nullptr);
     WG_HIP(hipDeviceSynchronize());
     WG_HIP(hipMemcpy(data, dd.p, bytes, hipMemcpyDeviceToHost));
   });

@@ -167,6 +167,10 @@ constexpr int kGridLdsTarget = 80 * 1024;
 // grid (ntu, ntv, T) and P0 are set.
 void PlanSortKeys(Plan& p);
 
+// Longest line the LDS FFT transforms in one pass; longer lines (up to
+// 5 x this) are split into R interleaved subsequences of at most this.
+constexpr int kMaxLdsFFT = 16384;
+constexpr int kMaxFFT = 5 * kMaxLdsFFT;
 int GoodFFTSize(int n);        // even 2^a 3^b 5^c >= n of least LDS-FFT work
 std::vector<int> FFTRadices(int n);  // Stockham stage radices for n
 

@@ -386,6 +386,22 @@ void Engine::BuildCorrection(VisSet& vs) {
   vs.corr_key = key;
 }
 
+Engine::LineFFTs Engine::PlaneFFTs(const DevPlan& dp, int nplanes) {
+  LineFFTs f;
+  f.ru = LongFFTRadix(dp.nu);
+  f.rv = LongFFTRadix(dp.nv);
+  WG_CHECK(f.ru > 0 && f.rv > 0, "padded grid side not split by 2..5 into LDS FFT lines");
+  f.fu = MakeFFTDesc(dp.nu / f.ru, Twiddles(dp.nu / f.ru));
+  f.fv = MakeFFTDesc(dp.nv / f.rv, Twiddles(dp.nv / f.rv));
+  if (f.ru > 1 || f.rv > 1) {
+    // subsequence spectra of one slab: the row pass's BV lines of nu or the
+    // column pass's nx lines of nv, per plane
+    const size_t per_plane = std::max(size_t(dp.BV) * size_t(dp.nu), size_t(dp.nx) * size_t(dp.nv));
+    f.tmp = static_cast<float2*>(arena_.Get("fft.long", size_t(nplanes) * per_plane * sizeof(float2)));
+  }
+  return f;
+}
+
 void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   const DevPlan& dp = vs.dp;
   const Plan& p = vs.plan;
@@ -409,8 +425,7 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
   LaunchFillZero(dirty, npix * 4, stream_);
-  const FFTDesc fu = MakeFFTDesc(dp.nu, Twiddles(dp.nu));
-  const FFTDesc fv = MakeFFTDesc(dp.nv, Twiddles(dp.nv));
+  const LineFFTs ft = PlaneFFTs(dp, p.Sf);
   const size_t plane_box = size_t(dp.BU) * dp.BV;
   const size_t plane_lines = size_t(dp.BV) * dp.nx;
   float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
@@ -446,9 +461,15 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
       }
     }
     EventTimer tm(*this, &times_.fft);
-    LaunchRowFFTInvert(dp, fu, box, rbuf, np, stream_);
+    if (ft.ru > 1)
+      LaunchRowFFTInvertLong(dp, ft.fu, dp.nu, ft.ru, box, ft.tmp, rbuf, np, stream_);
+    else
+      LaunchRowFFTInvert(dp, ft.fu, box, rbuf, np, stream_);
     LaunchTranspose(rbuf, cbuf, dp.BV, dp.nx, np, stream_);
-    LaunchColFFTInvert(dp, fv, cbuf, dirty, f0, np, stream_);
+    if (ft.rv > 1)
+      LaunchColFFTInvertLong(dp, ft.fv, dp.nv, ft.rv, cbuf, ft.tmp, dirty, f0, np, stream_);
+    else
+      LaunchColFFTInvert(dp, ft.fv, cbuf, dirty, f0, np, stream_);
   }
   LaunchAccumulateImage(dirty, vs.corr, img_dev, npix, stream_);
 }
@@ -465,8 +486,7 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
     acc = static_cast<float2*>(arena_.Get("vis_acc", vs.n_valid * sizeof(float2)));
     LaunchFillZero(acc, vs.n_valid * sizeof(float2), stream_);
   }
-  const FFTDesc fu = MakeFFTDesc(dp.nu, Twiddles(dp.nu));
-  const FFTDesc fv = MakeFFTDesc(dp.nv, Twiddles(dp.nv));
+  const LineFFTs ft = PlaneFFTs(dp, p.Sf);
   const size_t plane_box = size_t(dp.BU) * dp.BV;
   const size_t plane_lines = size_t(dp.BV) * dp.nx;
   float2* box = static_cast<float2*>(arena_.Get("box", p.Sf * plane_box * sizeof(float2)));
@@ -477,9 +497,16 @@ void Engine::Predict(VisSet& vs, const double* img_dev, float2* vis_out_dev) {
     const int np = std::min(p.Sf, p.P - f0);
     {
       EventTimer tm(*this, &times_.fft);
-      LaunchColFFTPredict(dp, fv, img_dev, vs.corr, cbuf, f0, np, stream_);
+      if (ft.rv > 1)
+        LaunchColFFTPredictLong(dp, ft.fv, dp.nv, ft.rv, img_dev, vs.corr, ft.tmp, cbuf, f0, np,
+                                stream_);
+      else
+        LaunchColFFTPredict(dp, ft.fv, img_dev, vs.corr, cbuf, f0, np, stream_);
       LaunchTranspose(cbuf, rbuf, dp.nx, dp.BV, np, stream_);
-      LaunchRowFFTPredict(dp, fu, rbuf, box, np, stream_);
+      if (ft.ru > 1)
+        LaunchRowFFTPredictLong(dp, ft.fu, dp.nu, ft.ru, rbuf, ft.tmp, box, np, stream_);
+      else
+        LaunchRowFFTPredict(dp, ft.fu, rbuf, box, np, stream_);
     }
     EventTimer tm(*this, &times_.degrid);
     const size_t E = size_t(dp.T + dp.W - 1);

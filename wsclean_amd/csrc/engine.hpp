@@ -32,6 +32,16 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
 void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
                          uint32_t stride, uint32_t* off, hipStream_t s);
 
+// One LDS FFT line length: Stockham stage radices and the stage index
+// divisors, with the device twiddle table.
+struct FFTDesc {
+  int n;
+  int nst;
+  int rad[16];
+  uint32_t magic[16];  // floor(2^32 / Ns) + 1 per stage (Ns: product of earlier radices)
+  const float2* tw;    // device copy of FFTTwiddleTable(n)
+};
+
 // A work item: the visibilities [begin, end) (sorted order) of one uv tile
 // that touch the planes of one gridding sub-slab.
 struct WorkItem {
@@ -170,6 +180,15 @@ class Engine {
  private:
   friend class FacetBatch;
   const float2* Twiddles(int n);
+  // Line transforms of a plane slab: descriptors of the LDS FFT lines and,
+  // for sides longer than the LDS FFT (ru, rv > 1 subsequences), the
+  // temporary of the two-pass transform.
+  struct LineFFTs {
+    FFTDesc fu{}, fv{};
+    int ru = 1, rv = 1;
+    float2* tmp = nullptr;
+  };
+  LineFFTs PlaneFFTs(const DevPlan& dp, int nplanes);
   void BuildCorrection(VisSet& vs);
   void BuildWorkItems(VisSet& vs);
 
@@ -266,13 +285,6 @@ class FacetBatch {
 };
 
 // ---- FFT / plane kernels (fft.hip) ----
-struct FFTDesc {
-  int n;
-  int nst;
-  int rad[16];
-  uint32_t magic[16];  // floor(2^32 / Ns) + 1 per stage (Ns: product of earlier radices)
-  const float2* tw;    // device copy of FFTTwiddleTable(n)
-};
 FFTDesc MakeFFTDesc(int n, const float2* tw);
 // Two-level twiddle table of the LDS FFT (host values, FP64-rounded to FP32).
 std::vector<float2> FFTTwiddleTable(int n);
@@ -286,6 +298,26 @@ void LaunchColFFTPredict(const DevPlan& dp, const FFTDesc& fv, const double* img
                          int nplanes, hipStream_t s);
 void LaunchRowFFTPredict(const DevPlan& dp, const FFTDesc& fu, const float2* rbuf,
                          float2* box, int nplanes, hipStream_t s);
+// Lines longer than the LDS FFT (kMaxLdsFFT) as N = R M, M <= kMaxLdsFFT:
+// LongFFTRadix(N) is R (1 for N <= kMaxLdsFFT, 0 when no R in 2..5
+// splits N).  The long launchers take fm = MakeFFTDesc(M) and a temporary
+// of R M complex values per line ([line][r][M]); results as their one-pass
+// counterparts.
+int LongFFTRadix(int n);
+void LaunchRowFFTInvertLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                            const float2* box, float2* tmp, float2* rbuf, int nplanes,
+                            hipStream_t s);
+void LaunchColFFTInvertLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                            const float2* cbuf, float2* tmp, float* dirty, int plane_lo,
+                            int nplanes, hipStream_t s);
+void LaunchColFFTPredictLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                             const double* img, const float* corr, float2* tmp, float2* cbuf,
+                             int plane_lo, int nplanes, hipStream_t s);
+void LaunchRowFFTPredictLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                             const float2* rbuf, float2* tmp, float2* box, int nplanes,
+                             hipStream_t s);
+void LaunchLineFFTLong(float2* data, int nlines, int N, int R, const FFTDesc& fm, float2* tmp,
+                       bool inverse, hipStream_t s);
 void LaunchTranspose(const float2* in, float2* out, int rows, int cols,
                      int nplanes, hipStream_t s);
 void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,

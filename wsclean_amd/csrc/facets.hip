@@ -54,6 +54,9 @@ FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
     const Plan& p = lay.plans[f];
     WG_CHECK(p.W == lead.W && p.T == lead.T && p.Sg == lead.Sg && p.beta == lead.beta,
              "facet plans disagree on the kernel shape");
+    WG_CHECK(p.nu <= kMaxLdsFFT && p.nv <= kMaxLdsFFT,
+             "a facet's padded grid is larger than " + std::to_string(kMaxLdsFFT) +
+                 ": image it with its own gridder");
   }
 
   // shelf packing of the facets' tile regions: facet f takes ntu + gap by

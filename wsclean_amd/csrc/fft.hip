@@ -663,6 +663,257 @@ __global__ void __launch_bounds__(BLOCK) LineFFTKernel(float2* __restrict__ data
   for (int i = threadIdx.x; i < d.n; i += BLOCK) line[i] = buf[i];
 }
 
+// ------------------------------------------------------------ long lines
+// A line of length N = R M (M <= 16384, R in 2..5) in two passes through HBM,
+// by one decimation-in-time step: pass 1 transforms the R interleaved
+// subsequences x[R m + r] with the LDS FFT of length M (one workgroup each,
+// loading from and folding into the period as the one-pass kernels do);
+// pass 2 forms only the outputs it needs, X[k] = sum_r w^(r k) F_r[k mod M]
+// with w = exp(-+2 pi i / N), fused with the line's consumer (crop, w-screen
+// and accumulate, or box store).  tmp holds F as [line][r][M].
+
+// exp(sign 2 pi i q / N)
+__device__ __forceinline__ float2 TwiddleN(long long q, int N, float sign) {
+  return Phasor(double(q % N) / double(N), sign);
+}
+
+// buf[(s - r) / R] += src[i] for the elements i < len whose slot s =
+// (origin + i) mod N (folding, one period at a time) has s mod R = r.
+template <int BLOCK>
+__device__ __forceinline__ void SubsequenceFoldLoad(float2* buf, const float2* __restrict__ src,
+                                                    int len, int origin, int N, int R, int r) {
+  const int i0 = Wrap(r - origin, R);
+  for (int k = 0; k * N < origin + len; ++k) {
+    const int lo = max(0, k * N - origin), hi = min(len, (k + 1) * N - origin);
+    const int first = lo + Wrap(i0 - lo, R);
+    for (int i = first + R * int(threadIdx.x); i < hi; i += R * BLOCK) {
+      float2& d = buf[(origin + i - k * N) / R];
+      const float2 a = src[i];
+      d.x += a.x;
+      d.y += a.y;
+    }
+    __syncthreads();
+  }
+}
+
+// Pass 1 of the invert row transform: subsequence r of box row L.
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) RowFFTInvertLongKernel(
+    DevPlan dp, FFTDesc fm, int N, int R, const float2* __restrict__ box,
+    float2* __restrict__ tmp) {
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
+  const int M = fm.n;
+  const int L = int(blockIdx.x) / R, r = int(blockIdx.x) % R;
+  const int j = L % dp.BV, pl = L / dp.BV;
+  LoadTwiddles<BLOCK>(twl, fm);
+  ZeroLds<BLOCK>(buf, M);
+  __syncthreads();
+  SubsequenceFoldLoad<BLOCK>(buf, box + size_t(pl) * dp.box_plane + size_t(j) * dp.box_pitch,
+                             dp.BU, dp.box_u0, N, R, r);
+  LdsFFT<false, BLOCK, MAXN>(buf, fm, twl);
+  float2* out = tmp + (size_t(L) * R + r) * M;
+  for (int m = threadIdx.x; m < M; m += BLOCK) out[m] = buf[m];
+}
+
+// Pass 2 of the invert row transform: image column x of line L.
+__global__ void __launch_bounds__(256) RowCombineInvertKernel(
+    DevPlan dp, int N, int R, int M, const float2* __restrict__ tmp, float2* __restrict__ rbuf,
+    int nlines) {
+  const size_t n = size_t(nlines) * dp.nx;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * 256) {
+    const int L = int(e / size_t(dp.nx)), x = int(e - size_t(L) * dp.nx);
+    const int k = Wrap(dp.nx / 2 - x, N);
+    const float2* f = tmp + size_t(L) * R * M + k % M;
+    float2 acc = f[0];
+    for (int r = 1; r < R; ++r)
+      acc = cadd(acc, cmul(f[size_t(r) * M], TwiddleN((long long)r * k, N, -1.f)));
+    rbuf[e] = acc;
+  }
+}
+
+// Pass 1 of the invert column transform: subsequence r of column line
+// (plane, x) of the column spectrum.
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) ColFFTInvertLongKernel(
+    DevPlan dp, FFTDesc fm, int N, int R, const float2* __restrict__ cbuf,
+    float2* __restrict__ tmp) {
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
+  const int M = fm.n;
+  const int line = int(blockIdx.x) / R, r = int(blockIdx.x) % R;
+  LoadTwiddles<BLOCK>(twl, fm);
+  ZeroLds<BLOCK>(buf, M);
+  __syncthreads();
+  SubsequenceFoldLoad<BLOCK>(buf, cbuf + size_t(line) * dp.BV, dp.BV, dp.box_v0, N, R, r);
+  LdsFFT<false, BLOCK, MAXN>(buf, fm, twl);
+  float2* out = tmp + (size_t(line) * R + r) * M;
+  for (int m = threadIdx.x; m < M; m += BLOCK) out[m] = buf[m];
+}
+
+// Pass 2 of the invert column transform: pixel (x, y) through the slab's
+// planes, w-screen, real part accumulated into the dirty image.
+__global__ void __launch_bounds__(256) ColCombineInvertKernel(
+    DevPlan dp, int N, int R, int M, const float2* __restrict__ tmp, float* __restrict__ dirty,
+    int plane_lo, int nplanes) {
+  const size_t n = size_t(dp.nx) * dp.ny;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * 256) {
+    const int x = int(e / size_t(dp.ny)), y = int(e - size_t(x) * dp.ny);
+    const int k = Wrap(y - dp.ny / 2, N);
+    // twiddles of the subsequences r < R (R <= 5; static indexes)
+    float2 tw[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+      tw[r] = r < R ? TwiddleN((long long)r * k, N, -1.f) : make_float2(0.f, 0.f);
+    const double zz = PixelZ(dp, x, y);
+    float acc = 0.f;
+    for (int pl = 0; pl < nplanes; ++pl) {
+      const float2* f = tmp + (size_t(pl) * dp.nx + x) * R * M + k % M;
+      float2 val = f[0];
+#pragma unroll
+      for (int r = 1; r < 5; ++r)
+        if (r < R) val = cadd(val, cmul(f[size_t(r) * M], tw[r]));
+      const float2 ph = Phasor(PlaneW(dp, plane_lo + pl) * zz, -1.f);
+      acc += val.x * ph.x - val.y * ph.y;
+    }
+    dirty[e] += acc;
+  }
+}
+
+// Pass 1 of the predict column transform: subsequence r of the screened
+// model column x of plane pl (image row y sits at slot (y - ny / 2) mod N).
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) ColFFTPredictLongKernel(
+    DevPlan dp, FFTDesc fm, int N, int R, const double* __restrict__ img,
+    const float* __restrict__ corr, float2* __restrict__ tmp, int plane_lo) {
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
+  const int M = fm.n;
+  const int line = int(blockIdx.x) / R, r = int(blockIdx.x) % R;
+  const int pl = line / dp.nx, x = line % dp.nx;
+  const double wp = PlaneW(dp, plane_lo + pl);
+  LoadTwiddles<BLOCK>(twl, fm);
+  for (int m = threadIdx.x; m < M; m += BLOCK) {
+    const int y = Wrap(R * m + r + dp.ny / 2, N);
+    float2 v = make_float2(0.f, 0.f);
+    if (y < dp.ny) {
+      const size_t idx = size_t(x) * dp.ny + y;
+      v = cscale(Phasor(wp * PixelZ(dp, x, y), 1.f), float(img[idx]) * corr[idx]);
+    }
+    buf[m] = v;
+  }
+  __syncthreads();
+  LdsFFT<true, BLOCK, MAXN>(buf, fm, twl);
+  float2* out = tmp + (size_t(line) * R + r) * M;
+  for (int m = threadIdx.x; m < M; m += BLOCK) out[m] = buf[m];
+}
+
+// Pass 2 of the predict column transform: box row j of column line
+// (plane, x).
+__global__ void __launch_bounds__(256) ColCombinePredictKernel(
+    DevPlan dp, int N, int R, int M, const float2* __restrict__ tmp, float2* __restrict__ cbuf,
+    int nplanes) {
+  const size_t n = size_t(nplanes) * dp.nx * dp.BV;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * 256) {
+    const size_t line = e / size_t(dp.BV);
+    const int j = int(e - line * dp.BV);
+    const int k = Wrap(dp.box_v0 + j, N);
+    const float2* f = tmp + line * R * M + k % M;
+    float2 acc = f[0];
+    for (int r = 1; r < R; ++r)
+      acc = cadd(acc, cmul(f[size_t(r) * M], TwiddleN((long long)r * k, N, 1.f)));
+    cbuf[e] = acc;
+  }
+}
+
+// Pass 1 of the predict row transform: subsequence r of line L of the
+// row spectrum (image column x sits at slot (nx / 2 - x) mod N).
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) RowFFTPredictLongKernel(
+    DevPlan dp, FFTDesc fm, int N, int R, const float2* __restrict__ rbuf,
+    float2* __restrict__ tmp) {
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
+  const int M = fm.n;
+  const int L = int(blockIdx.x) / R, r = int(blockIdx.x) % R;
+  const float2* src = rbuf + size_t(L) * dp.nx;
+  LoadTwiddles<BLOCK>(twl, fm);
+  ZeroLds<BLOCK>(buf, M);
+  __syncthreads();
+  const int x0 = Wrap(dp.nx / 2 - r, R);
+  for (int x = x0 + R * int(threadIdx.x); x < dp.nx; x += R * BLOCK)
+    buf[Wrap(dp.nx / 2 - x, N) / R] = src[x];
+  __syncthreads();
+  LdsFFT<true, BLOCK, MAXN>(buf, fm, twl);
+  float2* out = tmp + (size_t(L) * R + r) * M;
+  for (int m = threadIdx.x; m < M; m += BLOCK) out[m] = buf[m];
+}
+
+// Pass 2 of the predict row transform: box column i of line L.
+__global__ void __launch_bounds__(256) RowCombinePredictKernel(
+    DevPlan dp, int N, int R, int M, const float2* __restrict__ tmp, float2* __restrict__ box,
+    int nlines) {
+  const size_t n = size_t(nlines) * dp.BU;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * 256) {
+    const int L = int(e / size_t(dp.BU)), i = int(e - size_t(L) * dp.BU);
+    const int k = Wrap(dp.box_u0 + i, N);
+    const float2* f = tmp + size_t(L) * R * M + k % M;
+    float2 acc = f[0];
+    for (int r = 1; r < R; ++r)
+      acc = cadd(acc, cmul(f[size_t(r) * M], TwiddleN((long long)r * k, N, 1.f)));
+    const int row = L % dp.BV, pl = L / dp.BV;
+    box[size_t(pl) * dp.box_plane + size_t(row) * dp.box_pitch + i] = acc;
+  }
+}
+
+// Plain long-line transform of contiguous lines (test hook): pass 1.
+template <int BLOCK, int MAXN>
+__global__ void __launch_bounds__(BLOCK) LineFFTLongKernel(const float2* __restrict__ data,
+                                                           FFTDesc fm, int N, int R,
+                                                           float2* __restrict__ tmp,
+                                                           int inverse) {
+  extern __shared__ float2 lds[];
+  float2* twl = lds;
+  float2* buf = lds + kTwSlots;
+  const int M = fm.n;
+  const int line = int(blockIdx.x) / R, r = int(blockIdx.x) % R;
+  LoadTwiddles<BLOCK>(twl, fm);
+  for (int m = threadIdx.x; m < M; m += BLOCK) buf[m] = data[size_t(line) * N + R * m + r];
+  __syncthreads();
+  if (inverse)
+    LdsFFT<true, BLOCK, MAXN>(buf, fm, twl);
+  else
+    LdsFFT<false, BLOCK, MAXN>(buf, fm, twl);
+  float2* out = tmp + (size_t(line) * R + r) * M;
+  for (int m = threadIdx.x; m < M; m += BLOCK) out[m] = buf[m];
+}
+
+// Pass 2 of the plain long-line transform: every output.
+__global__ void __launch_bounds__(256) LineCombineKernel(const float2* __restrict__ tmp, int N,
+                                                         int R, int M, float2* __restrict__ data,
+                                                         int nlines, int inverse) {
+  const size_t n = size_t(nlines) * N;
+  const float sign = inverse ? 1.f : -1.f;
+  for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < n;
+       e += size_t(gridDim.x) * 256) {
+    const size_t line = e / size_t(N);
+    const int k = int(e - line * N);
+    const float2* f = tmp + line * R * M + k % M;
+    float2 acc = f[0];
+    for (int r = 1; r < R; ++r)
+      acc = cadd(acc, cmul(f[size_t(r) * M], TwiddleN((long long)r * k, N, sign)));
+    data[e] = acc;
+  }
+}
+
 // out[p][c][r] = in[p][r][c]; 32x32 tiles through LDS.
 __global__ void __launch_bounds__(256) TransposeKernel(const float2* __restrict__ in,
                                                        float2* __restrict__ out, int rows,
@@ -828,6 +1079,105 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse, hip
     LineFFTKernel<BLOCK, MAXN><<<dim3(nlines), BLOCK, LineLds(d.n), s>>>(
         data, d, inverse ? 1 : 0);
   });
+  WG_HIP(hipGetLastError());
+}
+
+// ---- long lines (N = R M > 16384) ----
+
+int LongFFTRadix(int n) {
+  if (n <= kMaxLdsFFT) return 1;
+  for (int R = 2; R <= 5; ++R)
+    if (n % R == 0 && n / R <= kMaxLdsFFT) return R;
+  return 0;
+}
+
+namespace {
+unsigned CombineBlocks(size_t n) {
+  return unsigned(std::min<size_t>((n + 255) / 256, size_t(1) << 20));
+}
+}  // namespace
+
+void LaunchRowFFTInvertLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                            const float2* box, float2* tmp, float2* rbuf, int nplanes,
+                            hipStream_t s) {
+  const int nlines = dp.BV * nplanes;
+  if (nlines <= 0) return;
+  WG_CHECK(size_t(fm.n) * R == size_t(N), "long FFT split does not match the line");
+  Dispatch(fm.n, [&](auto B, auto Mx) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(Mx)::value;
+    AllowLds(RowFFTInvertLongKernel<BLOCK, MAXN>, LineLds(MAXN));
+    RowFFTInvertLongKernel<BLOCK, MAXN><<<dim3(unsigned(nlines) * R), BLOCK, LineLds(fm.n), s>>>(
+        dp, fm, N, R, box, tmp);
+  });
+  const size_t n = size_t(nlines) * dp.nx;
+  RowCombineInvertKernel<<<CombineBlocks(n), 256, 0, s>>>(dp, N, R, fm.n, tmp, rbuf, nlines);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchColFFTInvertLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                            const float2* cbuf, float2* tmp, float* dirty, int plane_lo,
+                            int nplanes, hipStream_t s) {
+  if (nplanes <= 0) return;
+  WG_CHECK(size_t(fm.n) * R == size_t(N), "long FFT split does not match the line");
+  const unsigned lines = unsigned(nplanes) * unsigned(dp.nx);
+  Dispatch(fm.n, [&](auto B, auto Mx) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(Mx)::value;
+    AllowLds(ColFFTInvertLongKernel<BLOCK, MAXN>, LineLds(MAXN));
+    ColFFTInvertLongKernel<BLOCK, MAXN><<<dim3(lines * R), BLOCK, LineLds(fm.n), s>>>(
+        dp, fm, N, R, cbuf, tmp);
+  });
+  const size_t n = size_t(dp.nx) * dp.ny;
+  ColCombineInvertKernel<<<CombineBlocks(n), 256, 0, s>>>(dp, N, R, fm.n, tmp, dirty, plane_lo,
+                                                          nplanes);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchColFFTPredictLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                             const double* img, const float* corr, float2* tmp, float2* cbuf,
+                             int plane_lo, int nplanes, hipStream_t s) {
+  if (nplanes <= 0) return;
+  WG_CHECK(size_t(fm.n) * R == size_t(N), "long FFT split does not match the line");
+  const unsigned lines = unsigned(nplanes) * unsigned(dp.nx);
+  Dispatch(fm.n, [&](auto B, auto Mx) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(Mx)::value;
+    AllowLds(ColFFTPredictLongKernel<BLOCK, MAXN>, LineLds(MAXN));
+    ColFFTPredictLongKernel<BLOCK, MAXN><<<dim3(lines * R), BLOCK, LineLds(fm.n), s>>>(
+        dp, fm, N, R, img, corr, tmp, plane_lo);
+  });
+  const size_t n = size_t(nplanes) * dp.nx * dp.BV;
+  ColCombinePredictKernel<<<CombineBlocks(n), 256, 0, s>>>(dp, N, R, fm.n, tmp, cbuf, nplanes);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchRowFFTPredictLong(const DevPlan& dp, const FFTDesc& fm, int N, int R,
+                             const float2* rbuf, float2* tmp, float2* box, int nplanes,
+                             hipStream_t s) {
+  const int nlines = dp.BV * nplanes;
+  if (nlines <= 0) return;
+  WG_CHECK(size_t(fm.n) * R == size_t(N), "long FFT split does not match the line");
+  Dispatch(fm.n, [&](auto B, auto Mx) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(Mx)::value;
+    AllowLds(RowFFTPredictLongKernel<BLOCK, MAXN>, LineLds(MAXN));
+    RowFFTPredictLongKernel<BLOCK, MAXN><<<dim3(unsigned(nlines) * R), BLOCK, LineLds(fm.n),
+                                           s>>>(dp, fm, N, R, rbuf, tmp);
+  });
+  const size_t n = size_t(nlines) * dp.BU;
+  RowCombinePredictKernel<<<CombineBlocks(n), 256, 0, s>>>(dp, N, R, fm.n, tmp, box, nlines);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchLineFFTLong(float2* data, int nlines, int N, int R, const FFTDesc& fm, float2* tmp,
+                       bool inverse, hipStream_t s) {
+  WG_CHECK(size_t(fm.n) * R == size_t(N), "long FFT split does not match the line");
+  Dispatch(fm.n, [&](auto B, auto Mx) {
+    constexpr int BLOCK = decltype(B)::value, MAXN = decltype(Mx)::value;
+    AllowLds(LineFFTLongKernel<BLOCK, MAXN>, LineLds(MAXN));
+    LineFFTLongKernel<BLOCK, MAXN><<<dim3(unsigned(nlines) * R), BLOCK, LineLds(fm.n), s>>>(
+        data, fm, N, R, tmp, inverse ? 1 : 0);
+  });
+  const size_t n = size_t(nlines) * N;
+  LineCombineKernel<<<CombineBlocks(n), 256, 0, s>>>(tmp, N, R, fm.n, data, nlines,
+                                                     inverse ? 1 : 0);
   WG_HIP(hipGetLastError());
 }
 

@@ -114,20 +114,34 @@ bool Smooth235(int m) {
 }
 }  // namespace
 
+namespace {
+// Subsequences of a long line (as LongFFTRadix of the FFT code): 1 up to
+// kMaxLdsFFT, else the smallest R in 2..5 with N / R <= kMaxLdsFFT, 0 if none.
+int SplitRadix(int n) {
+  if (n <= kMaxLdsFFT) return 1;
+  for (int R = 2; R <= 5; ++R)
+    if (n % R == 0 && n / R <= kMaxLdsFFT) return R;
+  return 0;
+}
+}  // namespace
+
 int GoodFFTSize(int n) {
   if (n < 2) n = 2;
   int first = n + (n & 1);
-  while (!Smooth235(first)) first += 2;
-  // Among the even 2^a 3^b 5^c sizes up to 25 % above n (at most 16384),
-  // the one of least LDS-FFT work: the plane FFTs touch about m lines of m
-  // points, each in one LDS pass per Stockham stage plus the load, screen
-  // and store, so the cost goes as m^2 (stages + 2).
+  while (!Smooth235(first) || SplitRadix(first) == 0) first += 2;
+  // Among the even 2^a 3^b 5^c sizes up to 25 % above n, the one of least
+  // LDS-FFT work: the plane FFTs touch about m lines of m points, each in
+  // one LDS pass per Stockham stage plus the load, screen and store, so the
+  // cost goes as m^2 (stages + 2); a line longer than kMaxLdsFFT adds a
+  // pass through HBM for its R subsequences (+ 2 + R).
   int best = first;
   double best_cost = 1e300;
-  const int last = std::max(first, std::min(16384, int(1.25 * n)));
+  const int last = std::max(first, std::min(kMaxFFT, int(1.25 * n)));
   for (int m = first; m <= last; m += 2) {
-    if (!Smooth235(m)) continue;
-    const double cost = double(m) * double(m) * double(FFTRadices(m).size() + 2);
+    const int R = Smooth235(m) ? SplitRadix(m) : 0;
+    if (R == 0) continue;
+    const double stages = double(FFTRadices(m / R).size() + 2 + (R > 1 ? 2 + R : 0));
+    const double cost = double(m) * double(m) * stages;
     if (cost < best_cost) {
       best_cost = cost;
       best = m;
@@ -253,8 +267,9 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
   if (p.split) p.err_est = std::max(p.err_est, SplitErrorEstimate(chosen));
   p.nu = GoodFFTSize(int(std::ceil(p.sigma * g.nx)));
   p.nv = GoodFFTSize(int(std::ceil(p.sigma * g.ny)));
-  WG_CHECK(p.nu <= 16384 && p.nv <= 16384,
-           "padded grid larger than 16384 is not supported by the LDS FFT");
+  WG_CHECK(p.nu <= kMaxFFT && p.nv <= kMaxFFT && SplitRadix(p.nu) && SplitRadix(p.nv),
+           "padded grid larger than " + std::to_string(kMaxFFT) +
+               " is not supported by the FFT");
   p.ufac = double(p.nu) * g.dx;
   p.vfac = double(p.nv) * g.dy;
 
@@ -320,7 +335,11 @@ Plan MakePlan(const ImageGeometry& g, const VisExtent& e, const PlanOptions& opt
     // the FFT passes, which saves more than the second workgroup gains
     if (opt.Sg <= 0 && p.P > p.Sg && p.P <= max_sg && p.P <= kMaxSubSlab) p.Sg = p.P;
   }
-  const double per_plane = 8.0 * p.BV * (double(p.BU) + 2.0 * g.nx);
+  // box + row and column line buffers, + the subsequence spectra of a line
+  // longer than the LDS FFT
+  double per_plane = 8.0 * p.BV * (double(p.BU) + 2.0 * g.nx);
+  if (SplitRadix(p.nu) > 1 || SplitRadix(p.nv) > 1)
+    per_plane += 8.0 * std::max(double(p.BV) * p.nu, double(g.nx) * p.nv);
   int sf = int(std::max(1.0, std::floor(double(opt.slab_bytes) / per_plane)));
   sf = std::max(p.Sg, (sf / p.Sg) * p.Sg);
   p.Sf = std::min(sf, ((p.P + p.Sg - 1) / p.Sg) * p.Sg);
