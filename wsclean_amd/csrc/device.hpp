@@ -196,6 +196,21 @@ __host__ __device__ __forceinline__ float2 FromLinear(const float2* v, int pol) 
 // (the flat index itself is below 2^31).
 constexpr uint32_t kPermFlip = 0x80000000u;
 
+// Caller row of a renumbered row: the bits of the fourth word of its padded
+// uvw4 entry (w, tag).
+__device__ __forceinline__ uint32_t RowTag(double2 wt) {
+  return uint32_t(__double_as_longlong(wt.y));
+}
+
+// exp(sign 2 pi i (u l_shift + v m_shift + w zc)) for flipped coordinates.
+__device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L, float sign) {
+  const double cyc = L.u * dp.l_shift + L.v * dp.m_shift + L.w * dp.zc;
+  const double f = cyc - rint(cyc);
+  float s, c;
+  sincospif(2.0f * float(f), &s, &c);
+  return make_float2(c, sign * s);
+}
+
 // Packed per-visibility word: du (8 bits) | dv (8 bits) | p0 (15 bits) | flip.
 __device__ __forceinline__ uint32_t PackLoc(const VisLoc& l) {
   return uint32_t(l.du) | (uint32_t(l.dv) << 8) | (uint32_t(l.p0) << 16) |

@@ -265,16 +265,21 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
     LaunchPadUvw(uvw_dev, rperm, rperm, n_rows, uvw4, stream_);
   }
 
-  // ---- keys + stable sort ----
+  // ---- keys + stable sort; the sort's last pass writes the sorted
+  // coordinates and phasors and turns renumbered elements into the caller's
+  // flat indices with the flip bit ----
   uint32_t* keys = static_cast<uint32_t*>(arena_.Get(tag + ".keys", n_vis * 4));
   uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
+  vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", n_vis * sizeof(float4)));
+  vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", n_vis * sizeof(float2)));
   if (n_vis) {
     // the sort's first pass generates the index values itself
     LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_);
     uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
+    const SortCoords co{dp, uvw4, vs.scale, n_chan, 1.0 / double(n_chan), vs.coord, vs.phasor};
     if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_,
-                        true)) {
+                        true, &co)) {
       keys = keys2;
       vals = vals2;
     }
@@ -287,11 +292,6 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   WG_HIP(hipMemcpyAsync(&nvalid, vs.off + dp.nbuckets, 4, hipMemcpyDeviceToHost, stream_));
   WG_HIP(hipStreamSynchronize(stream_));
   vs.n_valid = nvalid;
-
-  vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", vs.n_valid * sizeof(float4)));
-  vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", vs.n_valid * sizeof(float2)));
-  LaunchCoords(dp, uvw4, vs.scale, n_chan, vs.perm, vs.n_valid, vs.coord, vs.phasor,
-               stream_);
   BuildWorkItems(vs);
   BuildCorrection(vs);
 }

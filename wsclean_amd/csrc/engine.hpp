@@ -25,9 +25,24 @@ size_t RadixSortScratchWords(size_t n);
 // the ping-pong buffers (all four 16-byte aligned).  index_vals: the input
 // values are the element indices and vals is not read.  Returns true when
 // the result is in the alt pair.
+// Last-pass epilogue of the visibility sort (DeviceRadixSort's coords):
+// for sorted entry p of renumbered element e = r * n_chan + ch (row r of
+// the padded renumbered uvw4), coord[p] = (fu, fv, fw, packed cell / plane
+// / flip), phasor[p] = the shift phasor exp(-2 pi i (u l + v m + w zc)), and
+// the value becomes the caller's flat index | kPermFlip.  coord and phasor
+// hold n entries (those of invalid keys, sorted last, are not used).
+struct SortCoords {
+  DevPlan dp;
+  const double* uvw4;
+  const double* scale;
+  size_t n_chan;
+  double inv_chan;  // 1 / n_chan
+  float4* coord;
+  float2* phasor;
+};
 bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                      size_t n, int key_bits, uint32_t* scratch, hipStream_t s,
-                     bool index_vals = false);
+                     bool index_vals = false, const SortCoords* coords = nullptr);
 // off[b] = first sorted position with key >= b * stride, b in [0, nbuckets].
 void DeviceBucketOffsets(const uint32_t* sorted_keys, size_t n, uint32_t nbuckets,
                          uint32_t stride, uint32_t* off, hipStream_t s);
@@ -414,12 +429,6 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
                           size_t n_rows, size_t n_chan, const float2* vis,
                           uint32_t* keys, hipStream_t s);
-// Sorted coordinates and phasors from the renumbered padded rows; perm
-// enters with sorted renumbered elements and leaves with the caller's flat
-// indices and the flip bit (kPermFlip).
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan,
-                  uint32_t* perm, size_t n_valid,
-                  float4* coord, float2* phasor, hipStream_t s);
 // out[i] = in[i] & mask
 void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,
                     hipStream_t s);

@@ -237,12 +237,29 @@ struct ScatterLds {
 // One tile of the scatter (see RadixScatterKernel).  FULL: the tile holds
 // kRsTile elements, so no element needs a bounds test.  BITS: the digit
 // width when known at compile time (0: the runtime `bits`).
-template <bool IDX, bool FULL, int BITS>
+// Coordinates epilogue of the last pass (SortCoords): the sorted entry p of
+// renumbered element e gets its gridding coordinates and shift phasor, and
+// its permutation word becomes the caller's flat index with the flip bit.
+// The elements of one tile come from 8192 consecutive renumbered elements,
+// so their rows are read from a few cache lines.
+__device__ __forceinline__ uint32_t SortedCoords(const SortCoords& co, uint32_t e, uint32_t p) {
+  const uint32_t nc = uint32_t(co.n_chan);
+  const uint32_t r = DivSmall(e, nc, co.inv_chan), ch = e - r * nc;
+  const double2 uv = reinterpret_cast<const double2*>(co.uvw4)[2 * size_t(r)];
+  const double2 wt = reinterpret_cast<const double2*>(co.uvw4)[2 * size_t(r) + 1];
+  const VisLoc L = LocateVis(uv.x, uv.y, wt.x, co.scale[ch], co.dp);
+  co.coord[p] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
+  co.phasor[p] = ShiftPhasor(co.dp, L, -1.f);
+  return (RowTag(wt) * nc + ch) | (L.flip ? kPermFlip : 0u);
+}
+
+template <bool IDX, bool FULL, int BITS, bool COORDS>
 __device__ __forceinline__ void ScatterTile(ScatterLds& L, const uint32_t* __restrict__ kin,
                                             const uint32_t* __restrict__ vin,
                                             uint32_t* __restrict__ kout,
                                             uint32_t* __restrict__ vout, uint32_t n,
-                                            uint32_t tile, int shift, int bits_rt) {
+                                            uint32_t tile, int shift, int bits_rt,
+                                            const SortCoords& co) {
   const int bits = BITS > 0 ? BITS : bits_rt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t mask = (1u << bits) - 1u;
@@ -361,19 +378,26 @@ __device__ __forceinline__ void ScatterTile(ScatterLds& L, const uint32_t* __res
   for (int it = 0; it < kRsPerThread; ++it)
     if (in(it)) L.buf[pos[it]] = val[it];
   __syncthreads();
-  for (int i = threadIdx.x; i < count; i += kRsThreads) vout[L.gbase[L.dig[i]] + i] = L.buf[i];
+  if constexpr (COORDS) {
+    for (int i = threadIdx.x; i < count; i += kRsThreads) {
+      const uint32_t p = L.gbase[L.dig[i]] + i;
+      vout[p] = SortedCoords(co, L.buf[i], p);
+    }
+  } else {
+    for (int i = threadIdx.x; i < count; i += kRsThreads) vout[L.gbase[L.dig[i]] + i] = L.buf[i];
+  }
   // the next tile's digits follow this tile's
   for (int d = threadIdx.x; d < R; d += kRsThreads) L.run_base[d] += L.start[d + 1] - L.start[d];
   __syncthreads();
 }
 
 // Two blocks per CU (LDS) need four waves per SIMD, i.e. at most 128 VGPRs.
-template <bool IDX, int BITS>
+template <bool IDX, int BITS, bool COORDS>
 __global__ void __launch_bounds__(kRsThreads)
     __attribute__((amdgpu_waves_per_eu(4, 4))) RadixScatterKernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
-    int bits, const uint32_t* __restrict__ offs, uint32_t nblocks) {
+    int bits, const uint32_t* __restrict__ offs, uint32_t nblocks, SortCoords co) {
   __shared__ ScatterLds L;
   const int R = 1 << (BITS > 0 ? BITS : bits);
   for (int d = threadIdx.x; d < R; d += kRsThreads)
@@ -382,9 +406,9 @@ __global__ void __launch_bounds__(kRsThreads)
     const uint32_t tile = (blockIdx.x * kRsSub + sub) * kRsTile;
     if (tile >= n) break;  // uniform over the block
     if (n - tile >= uint32_t(kRsTile))
-      ScatterTile<IDX, true, BITS>(L, kin, vin, kout, vout, n, tile, shift, bits);
+      ScatterTile<IDX, true, BITS, COORDS>(L, kin, vin, kout, vout, n, tile, shift, bits, co);
     else
-      ScatterTile<IDX, false, BITS>(L, kin, vin, kout, vout, n, tile, shift, bits);
+      ScatterTile<IDX, false, BITS, COORDS>(L, kin, vin, kout, vout, n, tile, shift, bits, co);
   }
 }
 
@@ -433,8 +457,10 @@ size_t RadixSortScratchWords(size_t n) {
 // says which pair holds the result (true: the alt pair).
 bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                      size_t n, int key_bits, uint32_t* scratch, hipStream_t s,
-                     bool index_vals) {
-  if (n <= 1) {
+                     bool index_vals, const SortCoords* coords) {
+  WG_CHECK(!coords || (index_vals && n < 0x80000000ull),
+           "the coordinates epilogue sorts renumbered element indices");
+  if (n <= 1 && !coords) {
     if (n == 1 && index_vals) LaunchFillZero(vals, sizeof(uint32_t), s);
     return false;
   }
@@ -444,6 +470,7 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
   const size_t nblocks = (n + kRsRun - 1) / kRsRun;
   // 32-bit element indices inside the scatter kernel
   WG_CHECK(n <= 0xFFFFFFFFull - kRsRun, "radix sort too large");
+  if (n == 0) return false;
   const int npass = (key_bits + kRsMaxBits - 1) / kRsMaxBits;
   const int digit = (key_bits + npass - 1) / npass;
   uint32_t* counts = scratch;
@@ -461,14 +488,25 @@ bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
         kin, n, shift, (1u << bits) - 1u, counts, uint32_t(nblocks));
     DeviceExclusiveScan(counts, offs, ncounts, scan_scratch, s, nullptr);
     const bool idx = index_vals && shift == 0;
+    const bool last = shift + digit >= key_bits;
+    const SortCoords co = coords ? *coords : SortCoords{};
     auto launch = [&](auto kernel) {
       kernel<<<dim3(unsigned(nblocks)), dim3(kRsThreads), 0, s>>>(
-          kin, idx ? nullptr : vin, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks));
+          kin, idx ? nullptr : vin, ko, vo, uint32_t(n), shift, bits, offs, uint32_t(nblocks),
+          co);
+    };
+    auto pick = [&](auto full_bits) {
+      constexpr int B = decltype(full_bits)::value;
+      if (coords && last)
+        idx ? launch(RadixScatterKernel<true, B, true>) : launch(RadixScatterKernel<false, B, true>);
+      else
+        idx ? launch(RadixScatterKernel<true, B, false>)
+            : launch(RadixScatterKernel<false, B, false>);
     };
     if (bits == kRsMaxBits)
-      idx ? launch(RadixScatterKernel<true, kRsMaxBits>) : launch(RadixScatterKernel<false, kRsMaxBits>);
+      pick(std::integral_constant<int, kRsMaxBits>());
     else
-      idx ? launch(RadixScatterKernel<true, 0>) : launch(RadixScatterKernel<false, 0>);
+      pick(std::integral_constant<int, 0>());
     std::swap(kin, ko);
     std::swap(vin, vo);
     in_alt = !in_alt;

@@ -75,11 +75,6 @@ __global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nbloc
 
 // Sort key of one visibility: ((tile * P0 + p0) * key_T + cell row) * key_T +
 // cell column (SortKey).
-// Caller row of a renumbered row: the bits of the fourth word of its padded
-// uvw4 entry (w, tag).
-__device__ __forceinline__ uint32_t RowTag(double2 wt) {
-  return uint32_t(__double_as_longlong(wt.y));
-}
 
 __device__ __forceinline__ uint32_t VisKey(const VisLoc& L, const DevPlan& dp) {
   const uint32_t bucket = uint32_t(L.tile) * uint32_t(dp.P0) + uint32_t(L.p0);
@@ -136,44 +131,6 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
   }
 }
 
-// exp(sign 2 pi i (u l_shift + v m_shift + w zc)) for flipped coordinates.
-__device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L, float sign) {
-  const double cyc = L.u * dp.l_shift + L.v * dp.m_shift + L.w * dp.zc;
-  const double f = cyc - rint(cyc);
-  float s, c;
-  sincospif(2.0f * float(f), &s, &c);
-  return make_float2(c, sign * s);
-}
-
-// Per sorted visibility: the gridding coordinates, the phase-centre /
-// w-centre phasor exp(-2 pi i (u l_shift + v m_shift + w zc)) and, in the top
-// bit of perm[i], whether (u, v, w, V) was flipped to w >= 0.  The gather
-// and scatter of visibilities then need no uvw reads and no FP64 work.
-//
-// perm enters holding renumbered elements e = r * n_chan + ch (renumbered
-// row r = caller row RowTag(uvw4, r), rows in uvw4 padded to 32 bytes) and
-// leaves holding the caller's flat index RowTag * n_chan + ch.  Rows are
-// renumbered in uv order, so the rows read by neighbouring sorted
-// visibilities lie close together in uvw4.
-__global__ void __launch_bounds__(kBlock) CoordsKernel(DevPlan dp, const double* __restrict__ uvw4,
-                                                       const double* __restrict__ scale,
-                                                       size_t n_chan, uint32_t* __restrict__ perm,
-                                                       size_t n_valid, float4* __restrict__ coord,
-                                                       float2* __restrict__ phasor) {
-  const double inv_chan = 1.0 / double(n_chan);
-  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
-       i += size_t(gridDim.x) * kBlock) {
-    const uint32_t e = perm[i];
-    const size_t r = DivSmall(e, uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
-    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
-    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
-    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
-    phasor[i] = ShiftPhasor(dp, L, -1.f);
-    const uint32_t idx = RowTag(wt) * uint32_t(n_chan) + uint32_t(ch);
-    perm[i] = idx | (L.flip ? kPermFlip : 0u);
-  }
-}
 
 // uvw4[r] = (uvw[rfetch[r]], tag): the renumbered rows padded to 32 bytes,
 // the fourth word holding rtag[r] (read back by RowTag), so that kernels
@@ -587,7 +544,8 @@ __global__ void __launch_bounds__(kBlock) FacetRowMapKernel(
   }
 }
 
-// Sorted coordinates and phasors of a facet batch (as CoordsKernel, with
+// Sorted coordinates and phasors of a facet batch (as the visibility sort's
+// coordinates epilogue, SortedCoords in sort.hip, with
 // the facet's own plan, phase centre shift and w centre).  perm enters with
 // sorted renumbered elements; it leaves with the virtual index (the predict
 // output layout) and perm_phys with the physical index into the shared
@@ -699,16 +657,6 @@ void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* s
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, scale, n_rows, n_chan,
                                                                   vis, keys);
-  WG_HIP(hipGetLastError());
-}
-
-void LaunchCoords(const DevPlan& dp, const double* uvw4, const double* scale, size_t n_chan,
-                  uint32_t* perm, size_t n_valid,
-                  float4* coord, float2* phasor, hipStream_t s) {
-  if (!n_valid) return;
-  WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
-  CoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(dp, uvw4, scale, n_chan, perm, n_valid,
-                                                  coord, phasor);
   WG_HIP(hipGetLastError());
 }
 
