@@ -204,6 +204,14 @@ int wg_stage_times(wg_handle* h, double* out14, int reset);
  * edge, planes per gridding sub-slab, FFT-slab byte budget. */
 int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab,
                   size_t slab_bytes);
+/* Smallest epsilon this library accepts: its grids and FFT are FP32 (with
+ * FP64 coordinates and phases), and below this no kernel shape meets the
+ * error estimate, so wg_add_inversion_data / wg_predict_visibilities with a
+ * smaller epsilon fail with a message.  No counterpart in the reference,
+ * which switches to FP64 grids at epsilon <= 1.01e-5
+ * (wgridder/wgriddingmsgridder.cpp:43-48): a caller keeps the reference's
+ * FP64 gridder below this value. */
+int wg_min_epsilon(double* epsilon);
 /* Arithmetic of the matrix-core gridding (W <= 8): -1 (default) split
  * precision (each FP32 operand as a pair of f16 halves on the f16 matrix
  * instructions) where its error estimate meets epsilon, FP32 otherwise;

@@ -12,6 +12,7 @@
 
 #include <complex>
 #include <cstddef>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,13 @@ namespace wgrid_hip {
 
 inline void Check(int status) {
   if (status != 0) throw std::runtime_error(std::string("wgrid_hip: ") + wg_last_error());
+}
+
+// Smallest epsilon the library accepts (wg_min_epsilon).
+inline double MinEpsilon() {
+  double e = 0.0;
+  Check(wg_min_epsilon(&e));
+  return e;
 }
 
 class Gridder {
@@ -85,6 +93,21 @@ class Gridder {
   void PredictVisibilities(size_t n_rows, size_t n_chan, const double* uvw, const double* freq,
                            std::complex<float>* vis) const {
     Check(wg_predict_visibilities(h_, n_rows, n_chan, uvw, freq, reinterpret_cast<float*>(vis)));
+  }
+
+  // AddInversionDataWithCorrectionCallback with the solutions passed
+  // explicitly (wg_add_inversion_data_corrected): gains [n_times][n_chan]
+  // [n_antennas][nparms], antennas [n_rows][2], vis [n_rows][n_chan][n_pol].
+  void AddInversionDataCorrected(int gain_mode, size_t n_pol, size_t n_rows, const double* uvw,
+                                 const double* freq, size_t n_chan, const uint32_t* antennas,
+                                 const std::complex<float>* vis, const uint64_t* time_offsets,
+                                 const std::complex<float>* gains, size_t n_times,
+                                 size_t n_antennas, size_t nparms, bool apply_forward,
+                                 int polarization) {
+    Check(wg_add_inversion_data_corrected(
+        h_, gain_mode, n_pol, n_rows, uvw, freq, n_chan, antennas,
+        reinterpret_cast<const float*>(vis), time_offsets, reinterpret_cast<const float*>(gains),
+        n_times, n_antennas, nparms, apply_forward ? 1 : 0, polarization));
   }
 
   wg_handle* Handle() const { return h_; }

@@ -67,6 +67,7 @@ SIGNATURES = [
     ("wg_set_tuning", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, c_size]),
     ("wg_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("wg_min_epsilon", ctypes.c_int, [c_dbl_p]),
     ("wgx_sort_keys", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(c_size), ctypes.c_char_p, c_size]),

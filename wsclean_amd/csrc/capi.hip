@@ -755,6 +755,13 @@ int wg_set_tuning(wg_handle* h, int W, double sigma, int tile, int sub_slab, siz
   });
 }
 
+int wg_min_epsilon(double* epsilon) {
+  return Guard([&] {
+    WG_CHECK(epsilon, "null argument");
+    *epsilon = wg::Fp32EpsilonFloor();
+  });
+}
+
 int wg_set_precision(wg_handle* h, int split) {
   return Guard([&] {
     WG_CHECK(h, "null handle");

@@ -280,6 +280,14 @@ class WGriddingGridder_Simple:
         _lib.check(self._lib.wg_set_tuning(self._h, int(W), float(sigma), int(tile),
                                            int(sub_slab), int(slab_bytes)))
 
+    @staticmethod
+    def min_epsilon():
+        """Smallest epsilon the library accepts (FP32 grids and FFT)."""
+        lib = _lib.load()
+        v = ctypes.c_double()
+        _lib.check(lib.wg_min_epsilon(ctypes.byref(v)))
+        return v.value
+
     def set_precision(self, split=-1):
         """Matrix-core gridding arithmetic for W <= 8: -1 split f16 pairs
         where epsilon allows, 0 FP32, 1 split whenever W <= 8."""
