@@ -134,6 +134,10 @@ int wg_add_inversion_data_corrected(wg_handle* h, int gain_mode, size_t n_pol, s
  * vis and vis_out are HIP device pointers (freq stays on the host).  They
  * correspond to the same reference members (AddInversionData,
  * PredictVisibilities); the caller keeps the data in HBM across calls.
+ * Device buffers must hold their data when a call is made: the library
+ * works on a HIP stream of its own, which does not wait for work the
+ * caller queued on other streams.  Every call has finished with its
+ * buffers when it returns.
  */
 int wg_add_inversion_data_device(wg_handle* h, size_t n_rows, size_t n_chan,
                                  const double* uvw_dev, const double* freq,

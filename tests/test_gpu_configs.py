@@ -179,7 +179,7 @@ SPECS = [(512, 1e5), (1024, 4e5), (8192, 2e6), (2048, 5e7), (4096, 8e6), (512, 2
 
 def test_served_stream_sizes(oracle):
     import torch
-    from wsclean_amd.serve import Server, latency_stats, make_requests, run_request
+    from wsclean_amd.serve import HandlePool, Server, latency_stats, make_requests, run_request
     reqs = make_requests(len(SPECS), rate_hz=20.0, seed=1005, specs=SPECS, cycles=5, eps=EPS)
     print("served stream:", [(r.rid, r.npix, r.n_rows, r.n_chan, r.n_vis) for r in reqs],
           flush=True)
@@ -191,8 +191,10 @@ def test_served_stream_sizes(oracle):
     torch.cuda.synchronize()
     for r in reqs:
         r.keep_residual = True
-    srv = Server(slots=4)
+    pool = HandlePool()
+    srv = Server(slots=4, pool=pool)
     out = srv.serve(reqs)
+    pool.close()
     s = latency_stats(out)
     print("served stats: p50 %.1f ms p99 %.1f ms, peak running %d"
           % (s["p50_ms"], s["p99_ms"], srv.peak_running), flush=True)

@@ -128,6 +128,7 @@ def test_facet_batch_predict_matches_single_gridders_and_direct_ft(oracle):
             fb.InitializePrediction(f, m)
     blocks = fb.block_offsets()
     out = torch.full((int(blocks[-1]), 2), float("nan"), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()  # the fill runs on torch's stream, the library on its own
     fb.Predict(out)
     outh = out.cpu().numpy().view(np.complex64).ravel()
     assert np.isfinite(outh).all()

@@ -28,6 +28,7 @@ def test_major_cycle_graph_replays_match_eager():
     out_e = torch.empty_like(vis)
     g.PredictPrepared(out_e)
     out_g = torch.full_like(vis, float("nan"))
+    torch.cuda.synchronize()
     g.CaptureMajorCycle(vis, out_g, fac)
     other = WGriddingGridder_Simple(64, 64, 64, 64, p["dx"], p["dx"], 0.0, 0.0, 1, 1e-4)
     for rep in range(4):
@@ -79,6 +80,7 @@ def test_facet_batch_graph_replays_match_eager():
     out_e = torch.empty((n_virtual, 2), dtype=torch.float32, device="cuda:0")
     fb.Predict(out_e)
     out_g = torch.full_like(out_e, float("nan"))
+    torch.cuda.synchronize()
     fb.CaptureMajorCycle(p["vis_d"], out_g, factors)
     for rep in range(3):
         out_g.fill_(float("nan"))

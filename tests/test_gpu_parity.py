@@ -317,6 +317,7 @@ def test_major_cycle_prepared_matches_stateless():
     img = g.RealImage().reshape(64, 64)
     assert rel_l2(img, ref_img) < 1e-5
     out = torch.zeros_like(vis)
+    torch.cuda.synchronize()  # the fill runs on torch's stream, the library on its own
     g.InitializePrediction(img.ravel())
     g.PredictPrepared(out)
     ref_out, _ = predict(64, p["dx"], 1e-4, img, p["uvw"], p["freq"])

@@ -1,6 +1,7 @@
 """Served request stream on the GPU: each request's result is the same
-alone and in the mix (bitwise: every kernel of the path is deterministic),
-and the requests of the mix do overlap on the device."""
+alone (on a fresh handle) and in the mix (on pooled handles reused across
+requests), bitwise: every kernel of the path is deterministic; and the
+requests of the mix do overlap on the device."""
 import itertools
 
 import numpy as np
@@ -11,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 def test_served_results_equal_alone_results():
     import torch
-    from wsclean_amd.serve import Server, latency_stats, make_requests, run_request
+    from wsclean_amd.serve import HandlePool, Server, latency_stats, make_requests, run_request
     reqs = make_requests(6, rate_hz=200.0, seed=17, npix_choices=[256, 512],
                          vis_range=(3e4, 4e5), cycles=2)
     print("served mix:", [(r.rid, r.npix, r.n_rows, r.n_chan) for r in reqs], flush=True)
@@ -21,8 +22,13 @@ def test_served_results_equal_alone_results():
         alone.append((img.copy(), r.out.cpu().numpy().copy()))
         r.out.zero_()
     torch.cuda.synchronize()
-    srv = Server(slots=3)
+    # the mix runs on pooled handles (reused across requests of one
+    # geometry) and must still equal the fresh-handle results bit for bit
+    pool = HandlePool()
+    srv = Server(slots=3, pool=pool)
     out = srv.serve(reqs)
+    assert pool.hits + pool.misses == len(reqs)
+    pool.close()
     for r, (img, vis) in zip(out, alone):
         assert np.isfinite(img).all()
         assert np.abs(img).max() > 0
@@ -33,6 +39,33 @@ def test_served_results_equal_alone_results():
     s = latency_stats(out)
     print("served stats:", s, flush=True)
     assert s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_pooled_handle_gives_the_fresh_handle_result():
+    """A handle reused by a later request of its geometry (its correction
+    table, twiddles and arena kept, its visibility set replaced) gives
+    bitwise what a fresh handle gives, also when the later request has more
+    visibilities (the arena grows) or fewer."""
+    import torch
+    from wsclean_amd.serve import HandlePool, make_requests, run_request
+    reqs = make_requests(3, rate_hz=100.0, seed=23, npix_choices=[384],
+                         vis_range=(5e4, 4e5), cycles=2)
+    fresh = []
+    for r in reqs:
+        img = run_request(r)
+        fresh.append((img.copy(), r.out.cpu().numpy().copy()))
+        r.out.zero_()
+    torch.cuda.synchronize()
+    pool = HandlePool()
+    for r, (img, vis) in zip(reqs, fresh):
+        h = pool.take(r.pool_key())
+        got = run_request(r, handle=h)
+        pool.put(r.pool_key(), h)
+        assert np.array_equal(got, img), "request %d image differs on a pooled handle" % r.rid
+        assert np.array_equal(r.out.cpu().numpy(), vis), "request %d prediction differs" % r.rid
+        assert r.allocated > 0
+    assert (pool.hits, pool.misses) == (2, 1)
+    pool.close()
 
 
 def test_device_image_calls_match_host_calls():
@@ -59,6 +92,8 @@ def test_device_image_calls_match_host_calls():
     g.FinalizeImage(1.0 / (2 * n_rows))
     host = g.RealImage()
     img = torch.full((w * h,), 7.0, dtype=torch.float32, device=dev)
+    # the fill runs on torch's stream, the library on its own
+    torch.cuda.synchronize()
     g.RealImageDevice(img)
     assert np.array_equal(img.cpu().numpy(), host, equal_nan=True)
     model = rng.standard_normal(w * h).astype(np.float32)

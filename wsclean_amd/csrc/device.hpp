@@ -124,6 +124,22 @@ __device__ __forceinline__ float EsPoly(const float (&c)[D + 1], float f) {
 // phi is even, so tap W - 1 - k at f equals tap k at 1 - f, i.e. the
 // polynomial of tap k at -g: the taps k < W / 2 give all W.
 
+// Even and odd parts of the tap polynomial c[0..D] (D odd) at y = g^2, so
+// that p(g) = E + g O and p(-g) = E - g O: two taps from D + 1 FMAs
+// (Horner in y for each part) and one FMA each.
+template <int D>
+__device__ __forceinline__ void EsPolyEvenOdd(const float (&c)[D + 1], float y, float& E,
+                                              float& O) {
+  static_assert(D % 2 == 1, "even / odd split of an odd-degree polynomial");
+  E = c[D - 1];
+  O = c[D];
+#pragma unroll
+  for (int d = D - 3; d >= 0; d -= 2) {
+    E = __builtin_fmaf(E, y, c[d]);
+    O = __builtin_fmaf(O, y, c[d + 1]);
+  }
+}
+
 // The coefficients of tap k (any lane-varying k < kEsPolyTaps).
 template <int D>
 __device__ __forceinline__ void EsPolyLoad(const DevPlan& dp, int k, float (&c)[D + 1]) {

@@ -224,7 +224,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         src.phasor = phasor;
       }
       auto load = [&](uint32_t first) {
-        pc = first + uint32_t(lane) < w1 ? coord[first + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // lanes past the range: their key marks the end and their V is 0
+        pc = coord[StageIndex(first, w1, lane)];
         src.Issue(first, w1, lane);
       };
       src.Prime(w0, w1, lane);
@@ -256,20 +257,29 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           const uint32_t gb = bb + 32u * uint32_t(half);
           if (gb >= w1) break;
           const uint32_t glen = min(32u, w1 - gb);
-          // kernel values of slot es into the table
-          const float fu = bu[half], fv = bv[half], fw = bw[half];
-          const float gu = __builtin_fmaf(fu, 2.f * gs, -gs);
-          const float gv = __builtin_fmaf(fv, 2.f * gs, -gs);
-          const float gw = __builtin_fmaf(fw, 2.f * gs, -gs);
+          // kernel values of slot es into the table.  The even and odd
+          // parts of tap q's polynomial at y = g^2 give tap q (+g) and tap
+          // W - 1 - q (-g) together: lanes eh = 0 / 1 take every tap of u /
+          // v, then both evaluate w and each writes its half of the taps.
+          {
+            const float g1 = __builtin_fmaf(eh ? bv[half] : bu[half], 2.f, -1.f);
+            const float gw = __builtin_fmaf(bw[half], 2.f, -1.f);
+            const float y1 = g1 * g1, yw = gw * gw;
+            const float sgw = gs * gw;
+            float* r1 = tbl + (eh ? kRowKv : kRowKu) * kSplitPitch + es;
+            float* rw = tbl + kRowKw * kSplitPitch + es;
 #pragma unroll
-          for (int q = 0; q < H; ++q) {
-            float c[D + 1];
+            for (int q = 0; q < H; ++q) {
+              float c[D + 1];
 #pragma unroll
-            for (int d = 0; d <= D; ++d) c[d] = dp.es_poly[q][d];
-            const int k = eh ? W - 1 - q : q;
-            tbl[(kRowKu + k) * kSplitPitch + es] = EsPolyG<D>(c, gu);
-            tbl[(kRowKv + k) * kSplitPitch + es] = EsPolyG<D>(c, gv);
-            tbl[(kRowKw + k) * kSplitPitch + es] = EsPolyG<D>(c, gw) * kSplitB;
+              for (int d = 0; d <= D; ++d) c[d] = dp.es_poly[q][d];
+              float E, O;
+              EsPolyEvenOdd<D>(c, y1, E, O);
+              r1[q * kSplitPitch] = __builtin_fmaf(g1, O, E);
+              r1[(W - 1 - q) * kSplitPitch] = __builtin_fmaf(-g1, O, E);
+              EsPolyEvenOdd<D>(c, yw, E, O);
+              rw[(eh ? W - 1 - q : q) * kSplitPitch] = __builtin_fmaf(sgw, O, E) * kSplitB;
+            }
           }
           tbl[(kRowV + eh) * kSplitPitch + es] = bx[half];
           WaveLdsSync();
@@ -421,16 +431,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
                          s4, s4, ah[h], al[h]);
         }
       };
-      float4 pc = w0 + uint32_t(lane) < w1 ? coord[w0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+      // lanes past the range: their key marks the end and their outputs
+      // are not stored
+      float4 pc = coord[StageIndex(w0, w1, lane)];
       int prev_last = -1;
       for (uint32_t bb = w0; bb < w1; bb += kStage) {
         const float4 cc = pc;
         const int key = StageKey(bb, w1, lane, cc);
         const uint64_t sm = RunStarts(key, prev_last, lane);
         prev_last = __builtin_amdgcn_readlane(key, 63);
-        if (bb + kStage < w1)
-          pc = bb + kStage + uint32_t(lane) < w1 ? coord[bb + kStage + lane]
-                                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (bb + kStage < w1) pc = coord[StageIndex(bb + kStage, w1, lane)];
         float bu[2], bv[2], bw[2];
         HalfBroadcast(cc.x, bu[0], bu[1]);
         HalfBroadcast(cc.y, bv[0], bv[1]);
@@ -458,22 +468,28 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
               }
             }
           }
-          // kernel values of slot es into the table
+          // kernel values of slot es into the table, as in the gridder:
+          // lanes eh = 0 / 1 take every tap of u / v from the even and odd
+          // parts of the tap polynomials, then both evaluate w and each
+          // writes its half of the taps
           {
-            const float fu = bu[half], fv = bv[half], fw = bw[half];
-            const float gu = __builtin_fmaf(fu, 2.f * gs, -gs);
-            const float gv = __builtin_fmaf(fv, 2.f * gs, -gs);
-            const float gw = __builtin_fmaf(fw, 2.f * gs, -gs);
+            const float g1 = __builtin_fmaf(eh ? bv[half] : bu[half], 2.f, -1.f);
+            const float gw = __builtin_fmaf(bw[half], 2.f, -1.f);
+            const float y1 = g1 * g1, yw = gw * gw;
+            const float sgw = gs * gw;
             float* row = tbl + es * kDgPitch;
+            float* r1 = row + (eh ? 8 : 0);
 #pragma unroll
             for (int q = 0; q < H; ++q) {
               float c[D + 1];
 #pragma unroll
               for (int d = 0; d <= D; ++d) c[d] = dp.es_poly[q][d];
-              const int k = eh ? W - 1 - q : q;
-              row[k] = EsPolyG<D>(c, gu);
-              row[8 + k] = EsPolyG<D>(c, gv);
-              row[16 + k] = EsPolyG<D>(c, gw);
+              float E, O;
+              EsPolyEvenOdd<D>(c, y1, E, O);
+              r1[q] = __builtin_fmaf(g1, O, E);
+              r1[W - 1 - q] = __builtin_fmaf(-g1, O, E);
+              EsPolyEvenOdd<D>(c, yw, E, O);
+              row[16 + (eh ? W - 1 - q : q)] = __builtin_fmaf(sgw, O, E);
             }
           }
           WaveLdsSync();

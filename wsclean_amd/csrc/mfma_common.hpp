@@ -24,15 +24,27 @@ constexpr int kStage = 64;
 template <bool FUSED>
 struct VisSource;
 
+// Index of lane `lane` of the batch at `first`, clamped to the range's last
+// element (first < end): every lane loads in bounds without a branch, and
+// lanes past the end are masked where their values are used.
+__device__ __forceinline__ uint32_t StageIndex(uint32_t first, uint32_t end, int lane) {
+  return min(first + uint32_t(lane), end - 1u);
+}
+
+// Visibilities of lanes past the end of the range are 0.
 template <>
 struct VisSource<false> {
   const float2* vis;
   float2 v;
+  bool in;
   __device__ __forceinline__ void Prime(uint32_t, uint32_t, int) {}
   __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
-    v = first + uint32_t(lane) < end ? vis[first + lane] : make_float2(0.f, 0.f);
+    in = first + uint32_t(lane) < end;
+    v = vis[StageIndex(first, end, lane)];
   }
-  __device__ __forceinline__ float2 Value() const { return v; }
+  __device__ __forceinline__ float2 Value() const {
+    return in ? v : make_float2(0.f, 0.f);
+  }
 };
 
 template <>
@@ -42,21 +54,22 @@ struct VisSource<true> {
   const float2* phasor;
   uint32_t pw_next, pw;
   float2 raw, ph;
+  bool in;
   // permutation word of the batch the next Issue() loads
   __device__ __forceinline__ void Prime(uint32_t first, uint32_t end, int lane) {
-    pw_next = first + uint32_t(lane) < end ? perm[first + lane] : 0u;
+    pw_next = perm[StageIndex(first, end, lane)];
   }
   __device__ __forceinline__ void Issue(uint32_t first, uint32_t end, int lane) {
     pw = pw_next;
-    const bool in = first + uint32_t(lane) < end;
-    raw = in ? vis[pw & ~kPermFlip] : make_float2(0.f, 0.f);
-    ph = in ? phasor[first + lane] : make_float2(0.f, 0.f);
-    const uint32_t nxt = first + uint32_t(kStage) + uint32_t(lane);
-    pw_next = nxt < end ? perm[nxt] : 0u;
+    in = first + uint32_t(lane) < end;
+    raw = vis[pw & ~kPermFlip];
+    ph = phasor[StageIndex(first, end, lane)];
+    pw_next = perm[min(first + uint32_t(kStage) + uint32_t(lane), end - 1u)];
   }
   __device__ __forceinline__ float2 Value() const {
     const float2 V = make_float2(raw.x, (pw & kPermFlip) ? -raw.y : raw.y);
-    return make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
+    return in ? make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x)
+              : make_float2(0.f, 0.f);
   }
 };
 

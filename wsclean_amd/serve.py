@@ -13,13 +13,16 @@ image goes to the host.
 
 Requests arrive at their own times (a seeded Poisson stream in
 ``make_requests``).  A request is admitted, in arrival order, as soon as a
-worker slot is free and its device memory fits under the HBM budget beside
-the requests already running.  Each admitted request drives a gridder handle
-of its own, and each handle owns a HIP stream and a stream-ordered arena,
-so the work of admitted requests overlaps on the device and no handle's
-allocation or release waits on another's work.  The library calls release
-the interpreter lock while the GPU works, so one Python thread per running
-request keeps the slots busy.
+worker slot is free and its device memory fits: the free memory is measured
+at each admission, less what the running requests have still to allocate.
+Each admitted request drives a gridder handle of its own, taken from a pool
+of idle handles of its geometry when there is one (so the grid-correction
+table, FFT twiddles and device arena of earlier requests are reused), and
+each handle owns a HIP stream and a stream-ordered arena, so the work of
+admitted requests overlaps on the device and no handle's allocation or
+release waits on another's work.  The library calls release the interpreter
+lock while the GPU works, so one Python thread per running request keeps
+the slots busy.
 
 A request's result does not depend on what runs beside it: handles share
 nothing but the device, and every kernel of the path is deterministic
@@ -55,6 +58,8 @@ class Request:
     # `residual`, so that the last dirty image can be checked
     keep_residual: bool = False
     residual: object = field(default=None, repr=False)
+    # device bytes the running request holds (reported by run_request)
+    allocated: int = 0
 
     @property
     def n_rows(self):
@@ -67,6 +72,10 @@ class Request:
     @property
     def n_vis(self):
         return self.n_rows * self.n_chan
+
+    def pool_key(self):
+        """Geometry of the request's gridder handle (``HandlePool``)."""
+        return (self.npix, self.pixel_rad, self.eps)
 
     def hbm_bytes(self, slab_bytes=8 << 30):
         """Upper estimate of the device memory the request allocates beyond
@@ -95,19 +104,44 @@ def major_cycle_model(img, out=None):
     return out.masked_fill_(~(img >= 0.5 * peak), 0.0)
 
 
-def run_request(req):
-    """Runs one request to completion on a gridder handle of its own and
-    returns the last cycle's dirty image (float32 [npix, npix], host);
-    ``req.out`` holds the last cycle's predicted visibilities."""
+def torch_bytes(req):
+    """Device bytes ``run_request`` allocates through torch beside the
+    handle: the residual visibilities and two float32 images."""
+    return int(8 * req.n_vis + 8 * req.npix * req.npix)
+
+
+def run_request(req, handle=None, on_alloc=None):
+    """Runs one request to completion and returns the last cycle's dirty
+    image (float32 [npix, npix], host); ``req.out`` holds the last cycle's
+    predicted visibilities.
+
+    ``handle``: a gridder handle of the request's geometry to run on (one
+    taken from a ``HandlePool``); None creates one and closes it at the
+    end.  After the visibility set is prepared and after the first cycle,
+    ``req.allocated`` is set to the device bytes the request holds (the
+    handle's arena plus ``torch_bytes``) and ``on_alloc()`` is called, so
+    that an admission loop can see how much of its estimate is still to
+    come."""
     import torch
     n = req.npix
     dev = req.vis.device
-    g = WGriddingGridder_Simple(n, n, n, n, req.pixel_rad, req.pixel_rad, 0.0, 0.0, 1, req.eps)
+    g = handle
+    own = g is None
+    if own:
+        g = WGriddingGridder_Simple(n, n, n, n, req.pixel_rad, req.pixel_rad, 0.0, 0.0, 1,
+                                    req.eps)
+
+    def report():
+        req.allocated = g.device_bytes() + torch_bytes(req)
+        if on_alloc is not None:
+            on_alloc()
+
     try:
         g.PrepareVisibilities(req.uvw, req.freq, req.n_rows, req.n_chan)
         residual = req.vis.clone()
         img = torch.empty(n * n, dtype=torch.float32, device=dev)
         model = torch.empty_like(img)
+        report()
         # torch work of this thread runs on its current stream; the library
         # calls synchronise their own stream before returning, so a host
         # synchronisation of this stream orders the two
@@ -122,6 +156,8 @@ def run_request(req):
             stream.synchronize()
             g.InitializePredictionDevice(model)
             g.PredictPrepared(req.out)
+            if c == 0:
+                report()
             if c + 1 < req.cycles:
                 torch.sub(req.vis, req.out, out=residual)
                 stream.synchronize()
@@ -129,49 +165,170 @@ def run_request(req):
             req.residual = residual
         return img.view(n, n).cpu().numpy()
     finally:
-        g.close()
+        if own:
+            g.close()
+
+
+class HandlePool:
+    """Idle gridder handles kept between requests, keyed by geometry
+    (``Request.pool_key``: image size, pixel size, epsilon).
+
+    A request of a pooled geometry runs on a handle that already holds the
+    geometry's grid-correction table, FFT twiddles and a grown device arena,
+    instead of building them again.  A handle serves one request at a time:
+    ``take`` removes it from the pool and ``put`` returns it.  At most
+    ``max_idle`` handles are kept; beyond that, and whenever admission is
+    short of device memory (``evict``), the least recently returned handle
+    is closed.  Handles are any objects with ``device_bytes()`` and
+    ``close()``; ``make(key)`` creates one for a request that finds none."""
+
+    def __init__(self, make=None, max_idle=8):
+        self.make = make or _make_handle
+        self.max_idle = int(max_idle)
+        self._idle = []          # (key, handle), least recently returned first
+        self._lock = threading.Lock()
+        self.hits = 0
+        self.misses = 0
+        self.evictions = 0
+
+    def take(self, key):
+        """A pooled handle of ``key``, or a new one."""
+        with self._lock:
+            for i in range(len(self._idle) - 1, -1, -1):
+                if self._idle[i][0] == key:
+                    self.hits += 1
+                    return self._idle.pop(i)[1]
+            self.misses += 1
+        return self.make(key)
+
+    def put(self, key, handle):
+        with self._lock:
+            self._idle.append((key, handle))
+            drop = self._idle[:-self.max_idle] if len(self._idle) > self.max_idle else []
+            self._idle = self._idle[len(drop):]
+            self.evictions += len(drop)
+        for _, h in drop:
+            h.close()
+
+    def evict(self):
+        """Closes the least recently returned idle handle; False when the
+        pool is empty."""
+        with self._lock:
+            if not self._idle:
+                return False
+            _, h = self._idle.pop(0)
+            self.evictions += 1
+        h.close()
+        return True
+
+    def idle(self):
+        with self._lock:
+            return len(self._idle)
+
+    def close(self):
+        while self.evict():
+            pass
+
+
+def _make_handle(key):
+    npix, pixel_rad, eps = key
+    return WGriddingGridder_Simple(npix, npix, npix, npix, pixel_rad, pixel_rad, 0.0, 0.0, 1,
+                                   eps)
+
+
+def device_free_bytes():
+    """Free device memory now: what the device reports free plus what
+    torch's caching allocator holds without using (it hands that out
+    first)."""
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    return int(free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
 
 
 class Server:
     """Continuous batching over ``slots`` concurrent requests.
 
     Admission is in arrival order: the head of the queue starts as soon as
-    a slot is free and ``hbm_bytes()`` of it fits under ``hbm_budget`` next
-    to the running requests (a request larger than the whole budget runs
-    alone).  Later arrivals wait behind the head, so large requests are not
-    starved.  ``hbm_budget=None`` takes 80% of the device memory free when
-    ``serve`` starts.  ``run`` is the per-request function (``run_request``).
-    """
+    a slot is free and its device memory fits; later arrivals wait behind
+    the head, so large requests are not starved.  A request that would not
+    fit even alone runs when nothing else does.
 
-    def __init__(self, slots=4, hbm_budget=None, run=run_request):
+    Memory (``hbm_budget=None``, the default): measured at each admission.
+    The head fits when its estimate (``hbm_bytes()``, less what its pooled
+    handle already holds) plus what the running requests have still to
+    allocate (their estimates less their reported ``allocated`` bytes) plus
+    ``reserve`` is at most ``free_bytes()`` (default: the device's free
+    memory plus torch's unused cache).  When it does not fit, idle pooled
+    handles are closed first, least recently used first.  A fixed
+    ``hbm_budget`` instead admits while the running requests' estimates sum
+    to at most the budget (no measurement).
+
+    ``pool``: a ``HandlePool``; each admitted request takes a handle of its
+    geometry from it and returns it when done (None: a new handle per
+    request).  ``run(req, handle, on_alloc)`` runs one request
+    (``run_request``)."""
+
+    def __init__(self, slots=4, hbm_budget=None, run=run_request, pool=None, free_bytes=None,
+                 reserve=1 << 30):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         self.slots = int(slots)
         self.hbm_budget = hbm_budget
         self.run = run
+        self.pool = pool
+        self.free_bytes = free_bytes or device_free_bytes
+        self.reserve = int(reserve)
         self._cv = threading.Condition()
         self._in_use = 0
-        self._running = 0
+        self._running = []
         self.peak_running = 0
         self.peak_bytes = 0
+        self.admissions = []     # (rid, measured free bytes, bytes still to come) per admission
 
-    def _admit(self, req, budget):
+    def _notify(self):
+        with self._cv:
+            self._cv.notify_all()
+
+    def _to_come(self):
+        return sum(max(0, need - getattr(r, "allocated", 0)) for r, need in self._running)
+
+    def _admit(self, req):
         need = req.hbm_bytes()
+        key = req.pool_key() if self.pool is not None else None
+        handle = None
         with self._cv:
             while True:
-                fits = budget is None or self._in_use + need <= budget or self._running == 0
-                if self._running < self.slots and fits:
-                    self._in_use += need
-                    self._running += 1
-                    self.peak_running = max(self.peak_running, self._running)
-                    self.peak_bytes = max(self.peak_bytes, self._in_use)
-                    return need
-                self._cv.wait()
+                if len(self._running) < self.slots:
+                    if handle is None and self.pool is not None:
+                        handle = self.pool.take(key)
+                    have = handle.device_bytes() if handle is not None else 0
+                    if self.hbm_budget is not None:
+                        fits = self._in_use + need <= self.hbm_budget
+                        free = to_come = None
+                    else:
+                        free, to_come = self.free_bytes(), self._to_come()
+                        fits = max(0, need - have) + to_come + self.reserve <= free
+                        if not fits and self.pool is not None and self.pool.evict():
+                            continue   # measure again with an idle handle closed
+                    if fits or not self._running:
+                        req.allocated = have
+                        self._in_use += need
+                        self._running.append((req, need))
+                        self.peak_running = max(self.peak_running, len(self._running))
+                        self.peak_bytes = max(self.peak_bytes, self._in_use)
+                        self.admissions.append((getattr(req, "rid", None), free, to_come))
+                        return need, handle
+                self._cv.wait(timeout=0.05)
 
-    def _release(self, need):
+    def _release(self, req, need, handle, ok):
+        if handle is not None:
+            if ok:
+                self.pool.put(req.pool_key(), handle)
+            else:
+                handle.close()
         with self._cv:
             self._in_use -= need
-            self._running -= 1
+            self._running = [(r, n) for r, n in self._running if r is not req]
             self._cv.notify_all()
 
     def serve(self, requests):
@@ -179,24 +336,22 @@ class Server:
         arrival order with start / done times (seconds from the stream
         start) and results.  The first error of any request is raised once
         every admitted request has ended."""
-        budget = self.hbm_budget
-        if budget is None:
-            import torch
-            budget = int(0.8 * torch.cuda.mem_get_info()[0])
         order = sorted(requests, key=lambda r: r.arrival)
         errors = []
         threads = []
         t0 = time.perf_counter()
 
-        def work(req, need):
+        def work(req, need, handle):
+            ok = False
             try:
                 req.start = time.perf_counter() - t0
-                req.image = self.run(req)
+                req.image = self.run(req, handle, self._notify)
                 req.done = time.perf_counter() - t0
+                ok = True
             except BaseException as e:
                 errors.append(e)
             finally:
-                self._release(need)
+                self._release(req, need, handle, ok)
 
         for req in order:
             delay = req.arrival - (time.perf_counter() - t0)
@@ -204,8 +359,8 @@ class Server:
                 time.sleep(delay)
             if errors:
                 break
-            need = self._admit(req, budget)
-            th = threading.Thread(target=work, args=(req, need), daemon=True)
+            need, handle = self._admit(req)
+            th = threading.Thread(target=work, args=(req, need, handle), daemon=True)
             th.start()
             threads.append(th)
         for th in threads:
