@@ -22,6 +22,7 @@ struct DevPlan {
   // fractional coordinate f (EsPolyCoefficients); rows of taps >= W are 0
   float es_poly[kEsPolyTaps][kEsPolyCoefs];
   int W, T, ntu, ntv, P0, P, nu, nv;
+  int tshift;         // log2(T) when T is a power of two, else -1
   int box_u0, box_v0, wrap_u, wrap_v;
   int BU, BV;
   // box storage: row pitch and plane stride in cells (BU and BU * BV for a
@@ -98,7 +99,9 @@ __device__ __forceinline__ VisLoc LocateVis(double um, double vm, double wm,
   } else {
     lv = a0v - dp.box_v0;
   }
-  const int tu_i = lu / dp.T, tv_i = lv / dp.T;
+  // lu, lv >= 0 (box or wrapped cells): a shift where T is a power of two
+  const int tu_i = dp.tshift >= 0 ? lu >> dp.tshift : lu / dp.T;
+  const int tv_i = dp.tshift >= 0 ? lv >> dp.tshift : lv / dp.T;
   r.du = lu - tu_i * dp.T;
   r.dv = lv - tv_i * dp.T;
   r.tile = tv_i * dp.ntu + tu_i;

@@ -39,6 +39,7 @@ DevPlan ToDevPlan(const Plan& p) {
   EsPolyCoefficients(p.W, p.beta, d.es_poly);
   d.W = p.W;
   d.T = p.T;
+  d.tshift = (p.T & (p.T - 1)) == 0 ? __builtin_ctz(unsigned(p.T)) : -1;
   d.ntu = p.ntu;
   d.ntv = p.ntv;
   d.P0 = p.P0;

@@ -353,7 +353,7 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
                 int sub_n, hipStream_t s, const float2* vis_raw = nullptr,
                 const uint32_t* gather_perm = nullptr);
 void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
-                     const float2* scratch, int sub, int sub_lo, int sub_n,
+                     float2* scratch, int sub, int sub_lo, int sub_n,
                      int slab_lo, hipStream_t s);
 // Split-precision (f16 pairs) matrix-core gridding for W <= 8, same
 // arguments and results as LaunchGrid's matrix-core path (grid_split.hip);

@@ -442,34 +442,66 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
   }
 }
 
+// The slots of every tile with more than one work item summed into the
+// tile's first slot (compensated, items in order), so that the box gather
+// reads one slot per covering tile.  Block (x, y) takes cells
+// [256 y, 256 y + 256) of the slot of work item x of the sub-slab, and does
+// the work only if item x is the first of a tile with several items; its
+// threads read each item's cell in turn (consecutive threads, consecutive
+// cells), so a dense tile's items are summed by many blocks at once.
+__global__ void __launch_bounds__(256) TileSlotsKernel(const WorkItem* __restrict__ items,
+                                                       const uint32_t* __restrict__ item_pos,
+                                                       int pos_lo, float2* __restrict__ scratch,
+                                                       uint32_t slot) {
+  const uint32_t base = item_pos[pos_lo];  // first work item of the sub-slab
+  const uint32_t it0 = base + blockIdx.x;
+  const uint32_t t = items[it0].tile;
+  const uint32_t a = item_pos[pos_lo + t], b = item_pos[pos_lo + t + 1];
+  if (it0 != a || b - a < 2) return;  // uniform over the block
+  const uint32_t i = blockIdx.y * 256u + threadIdx.x;
+  if (i >= slot) return;
+  float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;
+  for (uint32_t it = a; it < b; ++it) {
+    const float2 v = scratch[size_t(it - base) * slot + i];
+    const float yx = v.x - cx, yy = v.y - cy;
+    const float tx = sx + yx, ty = sy + yy;
+    cx = (tx - sx) - yx;
+    cy = (ty - sy) - yy;
+    sx = tx;
+    sy = ty;
+  }
+  scratch[size_t(a - base) * slot + i] = make_float2(sx, sy);
+}
+
 // box[p][r][c] for the planes of one sub-slab: the compensated (Kahan) sum
-// of the scratch slots of every work item whose tile footprint covers
-// (r, c), in a fixed order (tiles by index, items by index), so the box is
-// deterministic.  Every box cell of the sub-slab's planes is written.
+// of the tile slots (TileSlotsKernel) of every tile whose footprint covers
+// (r, c), tiles by index, so the box is deterministic.  Every box cell of
+// the sub-slab's planes is written.  Block (x, y, z): columns x * 256 +
+// threadIdx.x (grid-strided), box rows y, y + gridDim.y, ..., plane z of
+// the sub-slab, so a cell's plane, row and column need no division and a
+// row's covering tiles are the same for the whole block.
 __global__ void __launch_bounds__(256) GatherBoxKernel(
     DevPlan dp, int W, const uint32_t* __restrict__ item_pos, int pos_lo,
     const float2* __restrict__ scratch, float2* __restrict__ box, int sub_lo, int sub_n,
     int slab_lo) {
   const int E = dp.T + W - 1;
-  const size_t plane_cells = size_t(dp.BU) * dp.BV;
-  const size_t n = plane_cells * sub_n;
+  const int q = blockIdx.z;
   const uint32_t base = item_pos[pos_lo];  // first work item of the sub-slab
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const int q = int(i / plane_cells);
-    const size_t rc = i - size_t(q) * plane_cells;
-    const int r = int(rc / dp.BU), c = int(rc - size_t(r) * dp.BU);
-    // tiles whose footprint [t T, t T + E) contains the cell
+  const size_t slot = size_t(sub_n) * E * E;
+  for (int r = blockIdx.y; r < dp.BV; r += gridDim.y) {
+    // tiles whose footprint [t T, t T + E) contains the row / column
     const int tv0 = max(0, (r - E + dp.T) / dp.T), tv1 = min(dp.ntv - 1, r / dp.T);
-    const int tu0 = max(0, (c - E + dp.T) / dp.T), tu1 = min(dp.ntu - 1, c / dp.T);
-    float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;
-    for (int tv = tv0; tv <= tv1; ++tv) {
-      for (int tu = tu0; tu <= tu1; ++tu) {
-        const int t = tv * dp.ntu + tu;
-        const uint32_t a = item_pos[pos_lo + t], b = item_pos[pos_lo + t + 1];
-        const size_t off = (size_t(q) * E + (r - tv * dp.T)) * E + (c - tu * dp.T);
-        for (uint32_t it = a; it < b; ++it) {
-          const float2 v = scratch[size_t(it - base) * size_t(sub_n) * E * E + off];
+    float2* out = box + (size_t(sub_lo + q - slab_lo) * dp.BV + r) * dp.BU;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < dp.BU; c += gridDim.x * blockDim.x) {
+      const int tu0 = max(0, (c - E + dp.T) / dp.T), tu1 = min(dp.ntu - 1, c / dp.T);
+      float sx = 0.f, sy = 0.f, cx = 0.f, cy = 0.f;
+      for (int tv = tv0; tv <= tv1; ++tv) {
+        for (int tu = tu0; tu <= tu1; ++tu) {
+          const int t = tv * dp.ntu + tu;
+          const uint32_t a = item_pos[pos_lo + t], b = item_pos[pos_lo + t + 1];
+          if (a == b) continue;
+          const size_t off = (size_t(q) * E + (r - tv * dp.T)) * E + (c - tu * dp.T);
+          const float2 v = scratch[size_t(a - base) * slot + off];
           const float yx = v.x - cx, yy = v.y - cy;
           const float tx = sx + yx, ty = sy + yy;
           cx = (tx - sx) - yx;
@@ -478,8 +510,8 @@ __global__ void __launch_bounds__(256) GatherBoxKernel(
           sy = ty;
         }
       }
+      out[c] = make_float2(sx, sy);
     }
-    box[(size_t(sub_lo + q - slab_lo) * dp.BV + r) * dp.BU + c] = make_float2(sx, sy);
   }
 }
 
@@ -897,14 +929,22 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
   WG_HIP(hipGetLastError());
 }
 
-void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box, const float2* scratch,
+void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box, float2* scratch,
                      int sub, int sub_lo, int sub_n, int slab_lo, hipStream_t s) {
   const size_t ntiles = size_t(dp.ntu) * dp.ntv;
   const int pos_lo = int(sub * ntiles);
-  const size_t n = size_t(dp.BU) * dp.BV * sub_n;
-  const unsigned nb = unsigned(std::min<size_t>((n + 255) / 256, 65536));
-  GatherBoxKernel<<<nb, 256, 0, s>>>(dp, dp.W, vs.item_pos, pos_lo, scratch, box, sub_lo, sub_n,
-                                     slab_lo);
+  const size_t E = size_t(dp.T + dp.W - 1);
+  WG_CHECK(ntiles < 0x80000000ull && size_t(sub_n) * E * E < 0xFFFFFFFFull,
+           "tile grid too large for the box gather");
+  const uint32_t slot = uint32_t(size_t(sub_n) * E * E);
+  WG_CHECK(size_t(sub) < vs.sub_items.size(), "sub-slab out of range");
+  if (vs.sub_items[sub] > 0)
+    TileSlotsKernel<<<dim3(vs.sub_items[sub], (slot + 255) / 256), 256, 0, s>>>(
+        vs.items, vs.item_pos, pos_lo, scratch, slot);
+  const dim3 grid(unsigned((dp.BU + 255) / 256), unsigned(std::min(dp.BV, 65535)),
+                  unsigned(sub_n));
+  GatherBoxKernel<<<grid, 256, 0, s>>>(dp, dp.W, vs.item_pos, pos_lo, scratch, box, sub_lo, sub_n,
+                                       slab_lo);
   WG_HIP(hipGetLastError());
 }
 
