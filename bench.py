@@ -532,7 +532,8 @@ def serving_leg(args):
     cycle over the stream's span."""
     import torch
 
-    from wsclean_amd.serve import HandlePool, Server, latency_stats, make_requests, run_request
+    from wsclean_amd.serve import (HandlePool, NativeServer, Server, latency_stats,
+                                   make_requests, run_request)
     # one untimed request loads the kernels' code objects
     warm = make_requests(1, 1.0, seed=1, npix_choices=[512], vis_range=(1e5, 1.1e5), cycles=1,
                          eps=args.eps)
@@ -541,28 +542,42 @@ def serving_leg(args):
     t = time.time()
     reqs = make_requests(args.serve_requests, args.serve_rate, seed=1005, cycles=5, eps=args.eps)
     gen_s = time.time() - t
-    # handles of earlier requests of the same geometry are reused (the pool
-    # starts empty: each geometry's first request builds its handle)
+    # the library's request loop (C++ workers, per-geometry handle pool that
+    # starts empty, admission by measured free HBM)
+    nsrv = NativeServer(slots=args.serve_slots)
+    out = nsrv.serve(reqs)
+    s = latency_stats(out)
+    per_request = [{"npix": r.npix, "n_vis": r.n_vis, "n_chan": r.n_chan,
+                    "arrival_s": round(r.arrival, 4),
+                    "wait_ms": round((r.start - r.arrival) * 1e3, 2),
+                    "latency_ms": round((r.done - r.arrival) * 1e3, 2)} for r in out]
+    native_stats = nsrv.stats
+    for r in reqs:
+        r.start = r.done = 0.0
+    # the same stream through the Python loop (threads over the C-ABI calls)
     pool = HandlePool()
     srv = Server(slots=args.serve_slots, pool=pool)
     out = srv.serve(reqs)
     pool.close()
-    s = latency_stats(out)
+    sp = latency_stats(out)
+    sp.update({"loop": "Python threads (wsclean_amd.serve.Server)", "slots": srv.slots,
+               "peak_running": srv.peak_running,
+               "min_free_gb_at_admission": min(f for _, f, _ in srv.admissions) / 1e9,
+               "handle_pool": {"hits": pool.hits, "misses": pool.misses,
+                               "evictions": pool.evictions}})
     s.update({"workload": "served stream: %d requests, Poisson %.1f/s, 512^2-8192^2, "
                           "1e5-5e7 vis, 5 major cycles, eps %.0e" % (len(reqs), args.serve_rate,
                                                                      args.eps),
-              "slots": srv.slots, "peak_running": srv.peak_running,
-              "peak_admitted_gb": srv.peak_bytes / 1e9, "gen_s": gen_s,
+              "loop": "library (wg_server_*: C++ worker threads)", "slots": nsrv.slots,
+              "peak_running": native_stats["peak_running"], "gen_s": gen_s,
               "admission": "measured free HBM at each admission, less the running "
                            "requests' outstanding bytes, 1 GiB reserve",
-              "min_free_gb_at_admission": min(f for _, f, _ in srv.admissions) / 1e9,
-              "handle_pool": {"hits": pool.hits, "misses": pool.misses,
-                              "evictions": pool.evictions},
-              "per_request": [{"npix": r.npix, "n_vis": r.n_vis, "n_chan": r.n_chan,
-                               "arrival_s": round(r.arrival, 4),
-                               "wait_ms": round((r.start - r.arrival) * 1e3, 2),
-                               "latency_ms": round((r.done - r.arrival) * 1e3, 2)}
-                              for r in out]})
+              "min_free_gb_at_admission": native_stats["min_free_bytes_at_admission"] / 1e9,
+              "handle_pool": {"hits": native_stats["pool_hits"],
+                              "misses": native_stats["pool_misses"],
+                              "closed": native_stats["handles_closed"]},
+              "python_loop": sp,
+              "per_request": per_request})
     del reqs, out
     torch.cuda.empty_cache()
     return s

@@ -402,6 +402,59 @@ int wg_initialize_prediction_resampled(wg_handle* h, const float* image, size_t 
                                        size_t in_h);
 
 /* ------------------------------------------------------------------------
+ * Served request stream (no counterpart in the reference, which runs one
+ * imaging job per process; it is the request loop a service built on this
+ * library runs).  A request is one imaging job of `cycles` major cycles
+ * (main/wsclean.cpp:1519-1632) on device-resident data: per cycle the
+ * residual visibilities are inverted (wg_add_inversion_prepared on a
+ * prepared set), a model is taken from the dirty image (0.1 x the pixels
+ * at or above half its peak, standing in for the deconvolution), predicted
+ * (wg_predict_prepared) and subtracted from the data.
+ *
+ * wg_server_create starts `slots` worker threads on the current device.
+ * wg_server_submit queues a request and returns at once; requests start in
+ * submission order as soon as a worker is free and the request's device
+ * memory (an estimate, less what the pooled handle it will run on holds),
+ * plus what the running requests have still to allocate, plus
+ * reserve_bytes, fits in the device memory free at that moment
+ * (hipMemGetInfo); short of memory, idle pooled handles are closed first,
+ * and a request that does not fit even alone runs when nothing else does.
+ * Requests of one geometry (npix, pixel_size, epsilon) reuse idle gridder
+ * handles, at most max_idle_handles of which are kept.  wg_server_wait
+ * blocks until request `id` has ended and gives its submission, start and
+ * end times in seconds since wg_server_create (times3, may be NULL); a
+ * failed request returns non-zero with its message.  The request's buffers
+ * must stay valid, and its input buffers unchanged, until then.
+ * wg_server_stats: completed requests, peak concurrent requests, pool hits,
+ * pool misses, handles closed, the least free device memory seen at an
+ * admission (bytes), admission attempts that had to wait, idle handles.
+ * wg_server_destroy waits for every submitted request, then closes the
+ * idle handles. */
+typedef struct wg_server wg_server;
+typedef struct wg_request {
+  size_t npix;           /* image width = height (even, >= 8) */
+  double pixel_size;     /* radians */
+  double epsilon;        /* requested accuracy */
+  int cycles;            /* major cycles, >= 1 */
+  size_t n_rows;
+  size_t n_chan;
+  const double* uvw_dev; /* [n_rows][3] metres, device */
+  const double* freq;    /* [n_chan] Hz, host (copied at submission) */
+  const float* vis_dev;  /* [n_rows][n_chan] complex64 data, device */
+  float* vis_out_dev;    /* [n_rows][n_chan] complex64, device: the last
+                            cycle's predicted visibilities */
+  float* image_host;     /* npix * npix float, host: the last dirty image
+                            (NULL: not copied) */
+  float* residual_dev;   /* [n_rows][n_chan] complex64, device: the residual
+                            the last image was made from (NULL: not copied) */
+} wg_request;
+int wg_server_create(int slots, size_t reserve_bytes, int max_idle_handles, wg_server** out);
+int wg_server_submit(wg_server* s, const wg_request* request, uint64_t* id);
+int wg_server_wait(wg_server* s, uint64_t id, double* times3);
+int wg_server_stats(wg_server* s, uint64_t* out8);
+int wg_server_destroy(wg_server* s);
+
+/* ------------------------------------------------------------------------
  * Test entry points (no reference counterpart): expose internals so that
  * the bucket sort and the LDS FFT can be checked bit-for-bit / against a
  * library FFT. */

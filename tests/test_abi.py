@@ -58,3 +58,22 @@ def test_cpp_wrapper_compiles_and_links(tmp_path):
                         "-Wl,--allow-shlib-undefined"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_request_struct_layout(tmp_path):
+    # wg_request as the C compiler lays it out, against the ctypes mirror
+    fields = [f for f, _ in _lib.WgRequest._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "wgrid_hip.h"\n'
+                   "int main(void) {\n"
+                   '  printf("%zu", sizeof(wg_request));\n' +
+                   "".join('  printf(" %%zu", offsetof(wg_request, %s));\n' % f for f in fields) +
+                   "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = [ctypes.sizeof(_lib.WgRequest)] + [getattr(_lib.WgRequest, f).offset for f in fields]
+    assert got == want

@@ -68,6 +68,61 @@ def test_pooled_handle_gives_the_fresh_handle_result():
     pool.close()
 
 
+def test_native_server_mix_equals_alone_and_the_python_loop():
+    """The library's request loop (wg_server_*): each request's last dirty
+    image, prediction and residual are bitwise the same when it runs alone
+    on the native server, in the native server's mix, and in the Python
+    loop (run_request: model and residual by torch); the mix overlaps."""
+    import torch
+    from wsclean_amd.serve import NativeServer, latency_stats, make_requests, run_request
+    reqs = make_requests(6, rate_hz=200.0, seed=19, npix_choices=[256, 512],
+                         vis_range=(3e4, 4e5), cycles=2)
+    py = []
+    for r in reqs:
+        r.keep_residual = True
+        img = run_request(r)
+        py.append((img.copy(), r.out.cpu().numpy().copy(), r.residual.cpu().numpy().copy()))
+        r.out.zero_()
+    torch.cuda.synchronize()
+    alone = []
+    for r in reqs:
+        srv = NativeServer(slots=1)
+        r.arrival = 0.0
+        srv.serve([r])
+        alone.append((r.image.copy(), r.out.cpu().numpy().copy(), r.residual.cpu().numpy().copy()))
+        r.out.zero_()
+    torch.cuda.synchronize()
+    for i, r in enumerate(reqs):
+        r.arrival = 0.002 * i
+    srv = NativeServer(slots=3)
+    out = srv.serve(reqs)
+    for r, a, p in zip(out, alone, py):
+        mix = (r.image, r.out.cpu().numpy(), r.residual.cpu().numpy())
+        for k, what in enumerate(("image", "prediction", "residual")):
+            assert np.array_equal(mix[k], a[k], equal_nan=True), \
+                "request %d %s differs in the native mix" % (r.rid, what)
+            assert np.array_equal(a[k], p[k], equal_nan=True), \
+                "request %d %s differs from the Python loop" % (r.rid, what)
+    st = srv.stats
+    print("native served stats:", st, latency_stats(out), flush=True)
+    assert st["completed"] == len(reqs) and st["peak_running"] >= 2
+    assert st["pool_hits"] + st["pool_misses"] == len(reqs)
+    assert any(a.start < b.done and b.start < a.done for a, b in itertools.combinations(out, 2))
+
+
+def test_native_server_reports_request_errors():
+    import torch
+    from wsclean_amd import WgError
+    from wsclean_amd.serve import NativeServer, make_requests
+    reqs = make_requests(2, rate_hz=100.0, seed=29, npix_choices=[256], vis_range=(3e4, 5e4),
+                         cycles=1)
+    reqs[1].eps = 1e-9   # below what the library serves: that request fails
+    with pytest.raises(WgError, match="FP32"):
+        NativeServer(slots=2).serve(reqs)
+    torch.cuda.synchronize()
+    assert reqs[0].done > 0   # the other request still ran to completion
+
+
 def test_device_image_calls_match_host_calls():
     """RealImageDevice / InitializePredictionDevice give what RealImage /
     InitializePrediction give (trimmed image centred, NaN border)."""

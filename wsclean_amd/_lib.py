@@ -26,6 +26,16 @@ class WgParams(ctypes.Structure):
     ]
 
 
+class WgRequest(ctypes.Structure):
+    _fields_ = [
+        ("npix", c_size), ("pixel_size", ctypes.c_double), ("epsilon", ctypes.c_double),
+        ("cycles", ctypes.c_int), ("n_rows", c_size), ("n_chan", c_size),
+        ("uvw_dev", ctypes.c_void_p), ("freq", ctypes.c_void_p), ("vis_dev", ctypes.c_void_p),
+        ("vis_out_dev", ctypes.c_void_p), ("image_host", ctypes.c_void_p),
+        ("residual_dev", ctypes.c_void_p),
+    ]
+
+
 # (name, restype, argtypes) for every symbol declared in include/wgrid_hip.h
 SIGNATURES = [
     ("wg_create", ctypes.c_int, [ctypes.POINTER(WgParams), ctypes.POINTER(ctypes.c_void_p)]),
@@ -160,6 +170,13 @@ SIGNATURES = [
     ("wgx_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_size)]),
     ("wgx_es_poly", ctypes.c_int,
      [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    ("wg_server_create", ctypes.c_int,
+     [ctypes.c_int, c_size, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wg_server_submit", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(WgRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    ("wg_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_dbl_p]),
+    ("wg_server_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    ("wg_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("wgx_synth_visibilities", ctypes.c_int,
      [ctypes.c_void_p, c_size, c_size, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]),

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libwgrid_hip.so")
 
 SOURCES = ["plan.cpp", "partfile.cpp", "sort.hip", "vis.hip", "fft.hip", "grid.hip",
            "grid_split.hip", "engine.hip", "facets.hip", "weights.hip", "resample.hip",
-           "capi.hip"]
+           "capi.hip", "server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -277,6 +277,11 @@ struct PinnedBuf {
 
 }  // namespace
 
+namespace wg {
+// The thread-local message of wg_last_error, for entry points defined in
+// other files of the library (server.hip).
+void SetLastError(const std::string& msg) { g_last_error = msg; }
+}  // namespace wg
 
 extern "C" {
 

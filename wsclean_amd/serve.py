@@ -370,6 +370,80 @@ class Server:
         return order
 
 
+class NativeServer:
+    """The library's own request loop (wg_server_*, csrc/server.hip): the
+    same continuous batching as ``Server`` -- worker slots, admission in
+    arrival order by measured free device memory, a per-geometry pool of
+    idle gridder handles -- with the major cycles of ``run_request`` (model
+    and residual by device kernels) run by C++ worker threads, each with a
+    HIP stream of its own.
+
+    ``serve(requests)`` submits each request at its arrival time and returns
+    them in arrival order with start / done times (seconds from the stream
+    start), ``image`` (the last dirty image), ``out`` filled and, with
+    ``keep_residual``, ``residual``.  ``stats`` holds the server's counters
+    after the stream."""
+
+    def __init__(self, slots=4, reserve=1 << 30, max_idle=8):
+        self.slots = int(slots)
+        self.reserve = int(reserve)
+        self.max_idle = int(max_idle)
+        self.stats = None
+
+    def serve(self, requests):
+        import torch
+        lib = _lib.load()
+        c_void_p = _lib.ctypes.c_void_p
+        srv = c_void_p()
+        _lib.check(lib.wg_server_create(self.slots, self.reserve, self.max_idle,
+                                        _lib.ctypes.byref(srv)))
+        order = sorted(requests, key=lambda r: r.arrival)
+        pending = []
+        try:
+            for r in order:
+                n = r.npix
+                r.image = np.empty((n, n), np.float32)
+                if r.keep_residual:
+                    r.residual = torch.empty_like(r.vis)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for r in order:
+                delay = r.arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                freq = np.ascontiguousarray(r.freq, dtype=np.float64)
+                q = _lib.WgRequest(
+                    r.npix, r.pixel_rad, r.eps, r.cycles, r.n_rows, r.n_chan,
+                    c_void_p(r.uvw.data_ptr()), freq.ctypes.data_as(c_void_p),
+                    c_void_p(r.vis.data_ptr()), c_void_p(r.out.data_ptr()),
+                    r.image.ctypes.data_as(c_void_p),
+                    c_void_p(r.residual.data_ptr()) if r.keep_residual else None)
+                rid = _lib.ctypes.c_uint64()
+                _lib.check(lib.wg_server_submit(srv, _lib.ctypes.byref(q), _lib.ctypes.byref(rid)))
+                pending.append((r, rid.value, time.perf_counter() - t0, freq))
+            errors = []
+            times = (_lib.ctypes.c_double * 3)()
+            for r, rid, t_sub, _ in pending:
+                st = lib.wg_server_wait(srv, rid, times)
+                if st != 0:
+                    errors.append(_lib.WgError(lib.wg_last_error().decode()))
+                    continue
+                # library times are seconds since the server started: shift
+                # them by the submission time seen here
+                off = t_sub - times[0]
+                r.start, r.done = times[1] + off, times[2] + off
+            stats = (_lib.ctypes.c_uint64 * 8)()
+            _lib.check(lib.wg_server_stats(srv, stats))
+            self.stats = dict(zip(("completed", "peak_running", "pool_hits", "pool_misses",
+                                   "handles_closed", "min_free_bytes_at_admission",
+                                   "admission_waits", "idle_handles"), list(stats)))
+            if errors:
+                raise errors[0]
+        finally:
+            lib.wg_server_destroy(srv)
+        return order
+
+
 def latency_stats(requests):
     """p50 / p99 / mean latency (arrival to completion) and the stream's
     throughput in visibilities per second (every cycle's invert and
