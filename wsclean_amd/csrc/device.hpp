@@ -220,6 +220,10 @@ constexpr uint32_t kPermFlip = 0x80000000u;
 __device__ __forceinline__ uint32_t RowTag(double2 wt) {
   return uint32_t(__double_as_longlong(wt.y));
 }
+// The high half of that word: a facet batch's facet of the row (else 0).
+__device__ __forceinline__ uint32_t RowHi(double2 wt) {
+  return uint32_t(uint64_t(__double_as_longlong(wt.y)) >> 32);
+}
 
 // exp(sign 2 pi i (u l_shift + v m_shift + w zc)) for flipped coordinates.
 __device__ __forceinline__ float2 ShiftPhasor(const DevPlan& dp, const VisLoc& L, float sign) {

@@ -402,9 +402,10 @@ void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vr
                                const double* uvw4, const double* scale,
                                size_t n_vrows, size_t n_chan, int atlas_ntu, int atlas_P0,
                                int key_shift, int key_T, uint32_t* keys, hipStream_t s);
-// rphys[r] = physical row of renumbered virtual row r.
+// rphys[r] = physical row and rfac[r] = facet of renumbered virtual row r.
 void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                       const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, hipStream_t s);
+                       const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, uint32_t* rfac,
+                       hipStream_t s);
 // Sorted coordinates / phasors; perm enters with sorted renumbered elements
 // and leaves with the virtual index, perm_phys gets the physical index
 // (both with the flip bit).
@@ -423,7 +424,7 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
 // fourth word holding the bits of rtag[r] (the row number the coordinate and
 // key kernels report for renumbered row r).
 void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rtag, size_t n_rows,
-                  double* uvw4, hipStream_t s);
+                  double* uvw4, hipStream_t s, const uint32_t* rhi = nullptr);
 // Keys of the visibilities in renumbered row order (element r * n_chan + ch
 // = channel ch of caller row tag(r)); zero visibilities get nkeys.
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,

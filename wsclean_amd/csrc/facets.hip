@@ -240,12 +240,14 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
     uint32_t* rk2 = static_cast<uint32_t*>(A.Get("rows.keys2", n_vrows * 4));
     uint32_t* rp2 = static_cast<uint32_t*>(A.Get("rows.perm2", n_vrows * 4));
     uint32_t* rphys = static_cast<uint32_t*>(A.Get("rows.phys", n_vrows * 4));
+    uint32_t* rfac = static_cast<uint32_t*>(A.Get("rows.facet", n_vrows * 4));
     LaunchFacetKeys(dfac, dvrb, F, uvw_dev, scale_ + n_chan / 2, 1, n_vrows, pa.ntu, pa.P0,
                     pa.key_shift, pa.key_T, rk, nullptr, s);
     if (DeviceRadixSort(rk, rperm, rk2, rp2, n_vrows, pa.key_bits, scratch, s, true))
       rperm = rp2;
-    LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, s);
-    LaunchPadUvw(uvw_dev, rphys, rperm, n_vrows, uvw4, s);
+    LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, rfac, s);
+    // the padded rows carry their virtual row and their facet
+    LaunchPadUvw(uvw_dev, rphys, rperm, n_vrows, uvw4, s, rfac);
   }
 
   // ---- keys, one stable sort, bucket offsets ----

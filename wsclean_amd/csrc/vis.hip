@@ -133,19 +133,22 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
 
 
 // uvw4[r] = (uvw[rfetch[r]], tag): the renumbered rows padded to 32 bytes,
-// the fourth word holding rtag[r] (read back by RowTag), so that kernels
-// reading a renumbered row find its caller row in the same cache line.
+// the fourth word holding rtag[r] in its low half (read back by RowTag) and
+// rhi[r] (or 0) in its high half (RowHi), so that kernels reading a
+// renumbered row find its caller row -- and a facet batch its facet -- in
+// the same cache line.
 __global__ void __launch_bounds__(kBlock) PadUvwKernel(const double* __restrict__ uvw,
                                                        const uint32_t* __restrict__ rfetch,
                                                        const uint32_t* __restrict__ rtag,
+                                                       const uint32_t* __restrict__ rhi,
                                                        size_t n_rows,
                                                        double* __restrict__ uvw4) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < 4 * n_rows;
        i += size_t(gridDim.x) * kBlock) {
     const size_t r = i >> 2;
     const int k = int(i & 3);
-    uvw4[i] = k < 3 ? uvw[3 * size_t(rfetch[r]) + k]
-                    : __longlong_as_double((long long)(rtag[r]));
+    const uint64_t tag = uint64_t(rtag[r]) | (rhi ? uint64_t(rhi[r]) << 32 : 0ull);
+    uvw4[i] = k < 3 ? uvw[3 * size_t(rfetch[r]) + k] : __longlong_as_double((long long)tag);
   }
 }
 
@@ -521,26 +524,34 @@ __global__ void __launch_bounds__(kBlock) FacetKeysRenumberedKernel(
     const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
     const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
     const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
-    const int f = FacetOf(vrbase, F, RowTag(wt));
-    const DevPlan& dp = fac[f].dp;
-    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
-    const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
-    const uint32_t tile = uint32_t(fac[f].tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fac[f].tu0 + tu);
-    const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
-    keys[e] = SortKey(bucket, L.du, L.dv, key_shift, key_T);
+    const int f = int(RowHi(wt));  // the row's facet, from its padded entry
+    auto key = [&](const FacetDev& fd) {
+      const DevPlan& dp = fd.dp;
+      const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
+      const int tu = L.tile % dp.ntu, tv = L.tile / dp.ntu;
+      const uint32_t tile = uint32_t(fd.tv0 + tv) * uint32_t(atlas_ntu) + uint32_t(fd.tu0 + tu);
+      const uint32_t bucket = tile * uint32_t(atlas_P0) + uint32_t(L.p0);
+      return SortKey(bucket, L.du, L.dv, key_shift, key_T);
+    };
+    // a wave's elements mostly share one facet: its plan is then read with
+    // scalar loads
+    const int f0 = __builtin_amdgcn_readfirstlane(f);
+    keys[e] = __all(f == f0) ? key(fac[f0]) : key(fac[f]);
   }
 }
 
-// Physical row of every renumbered virtual row: rphys[r] = row_off_f +
-// (rperm[r] - vrbase[f]).
+// Physical row and facet of every renumbered virtual row: rphys[r] =
+// row_off_f + (rperm[r] - vrbase[f]), rfac[r] = f.
 __global__ void __launch_bounds__(kBlock) FacetRowMapKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
-    const uint32_t* __restrict__ rperm, size_t n_vrows, uint32_t* __restrict__ rphys) {
+    const uint32_t* __restrict__ rperm, size_t n_vrows, uint32_t* __restrict__ rphys,
+    uint32_t* __restrict__ rfac) {
   for (size_t r = size_t(blockIdx.x) * kBlock + threadIdx.x; r < n_vrows;
        r += size_t(gridDim.x) * kBlock) {
     const uint32_t vr = rperm[r];
     const int f = FacetOf(vrbase, F, vr);
     rphys[r] = uint32_t(fac[f].row_off + (vr - vrbase[f]));
+    rfac[r] = uint32_t(f);
   }
 }
 
@@ -564,15 +575,24 @@ __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
     const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
     const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
     const uint32_t vr = RowTag(wt);
-    const int f = FacetOf(vrbase, F, vr);
-    const DevPlan& dp = fac[f].dp;
-    const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
-    coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
-    phasor[i] = ShiftPhasor(dp, L, -1.f);
-    const uint32_t flip = L.flip ? kPermFlip : 0u;
-    const size_t row = fac[f].row_off + (vr - vrbase[f]);
-    perm[i] = uint32_t(size_t(vr) * n_chan + ch) | flip;
-    perm_phys[i] = uint32_t(row * n_chan + ch) | flip;
+    const int f = int(RowHi(wt));  // the row's facet, from its padded entry
+    auto write = [&](const FacetDev& fd, unsigned long long vrb) {
+      const DevPlan& dp = fd.dp;
+      const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
+      coord[i] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
+      phasor[i] = ShiftPhasor(dp, L, -1.f);
+      const uint32_t flip = L.flip ? kPermFlip : 0u;
+      const size_t row = fd.row_off + (vr - vrb);
+      perm[i] = uint32_t(size_t(vr) * n_chan + ch) | flip;
+      perm_phys[i] = uint32_t(row * n_chan + ch) | flip;
+    };
+    // neighbouring sorted visibilities mostly share one facet: its plan is
+    // then read with scalar loads
+    const int f0 = __builtin_amdgcn_readfirstlane(f);
+    if (__all(f == f0))
+      write(fac[f0], vrbase[f0]);
+    else
+      write(fac[f], vrbase[f]);
   }
 }
 
@@ -626,9 +646,11 @@ void LaunchFacetKeysRenumbered(const FacetDev* fac, const unsigned long long* vr
 }
 
 void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, int F,
-                       const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, hipStream_t s) {
+                       const uint32_t* rperm, size_t n_vrows, uint32_t* rphys, uint32_t* rfac,
+                       hipStream_t s) {
   if (!n_vrows) return;
-  FacetRowMapKernel<<<Blocks(n_vrows), kBlock, 0, s>>>(fac, vrbase, F, rperm, n_vrows, rphys);
+  FacetRowMapKernel<<<Blocks(n_vrows), kBlock, 0, s>>>(fac, vrbase, F, rperm, n_vrows, rphys,
+                                                        rfac);
   WG_HIP(hipGetLastError());
 }
 
@@ -644,9 +666,9 @@ void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, in
 }
 
 void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rtag, size_t n_rows,
-                  double* uvw4, hipStream_t s) {
+                  double* uvw4, hipStream_t s, const uint32_t* rhi) {
   if (!n_rows) return;
-  PadUvwKernel<<<Blocks(4 * n_rows), kBlock, 0, s>>>(uvw, rfetch, rtag, n_rows, uvw4);
+  PadUvwKernel<<<Blocks(4 * n_rows), kBlock, 0, s>>>(uvw, rfetch, rtag, rhi, n_rows, uvw4);
   WG_HIP(hipGetLastError());
 }
 
