@@ -117,6 +117,46 @@ class Gridder {
   size_t width_, height_;
 };
 
+// The library's served request stream (wg_server_*): Submit queues a
+// request and returns its id, Wait blocks until it has ended (rethrowing its
+// failure) and gives its submission, start and end times in seconds since
+// the server started; the destructor waits for every submitted request.
+class Server {
+ public:
+  explicit Server(int slots = 4, size_t reserve_bytes = size_t(1) << 30, int max_idle = 8) {
+    Check(wg_server_create(slots, reserve_bytes, max_idle, &s_));
+  }
+  ~Server() {
+    if (s_) wg_server_destroy(s_);
+  }
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  uint64_t Submit(const wg_request& r) {
+    uint64_t id = 0;
+    Check(wg_server_submit(s_, &r, &id));
+    return id;
+  }
+  struct Times {
+    double submitted, started, ended;
+  };
+  Times Wait(uint64_t id) {
+    double t[3] = {0, 0, 0};
+    Check(wg_server_wait(s_, id, t));
+    return Times{t[0], t[1], t[2]};
+  }
+  // completed, peak running, pool hits, pool misses, handles closed, least
+  // free device bytes at an admission, waited admissions, idle handles
+  std::vector<uint64_t> Stats() {
+    std::vector<uint64_t> v(8);
+    Check(wg_server_stats(s_, v.data()));
+    return v;
+  }
+
+ private:
+  wg_server* s_ = nullptr;
+};
+
 }  // namespace wgrid_hip
 
 #endif  // WGRID_HIP_HPP_

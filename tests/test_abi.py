@@ -44,11 +44,16 @@ def test_cpp_wrapper_compiles_and_links(tmp_path):
         '#include "wgrid_hip.hpp"\n'
         "#include <type_traits>\n"
         "static_assert(!std::is_copy_constructible<wgrid_hip::Gridder>::value, \"\");\n"
+        "static_assert(!std::is_copy_constructible<wgrid_hip::Server>::value, \"\");\n"
         "int run(size_t n) {\n"
         "  wgrid_hip::Gridder g(n, n, n, n, 1e-5, 1e-5, 0.0, 0.0, 1, 1e-5);\n"
         "  g.InitializeInversion();\n"
         "  std::vector<float> img = g.RealImage();\n"
-        "  return int(img.size() + g.ConstantMemoryUsage());\n"
+        "  wgrid_hip::Server s(2);\n"
+        "  wg_request r{};\n"
+        "  const uint64_t id = s.Submit(r);\n"
+        "  const double t = s.Wait(id).ended;\n"
+        "  return int(img.size() + g.ConstantMemoryUsage() + s.Stats()[0] + size_t(t));\n"
         "}\n"
         "int main(int argc, char**) { return argc > 5 ? run(64) : 0; }\n")
     exe = tmp_path / "use"
