@@ -163,6 +163,12 @@ __device__ __forceinline__ void EsTaps(const DevPlan& dp, float f, float* out) {
   }
 }
 
+// |V|^2 with one fixed rounding sequence (so every kernel that takes the
+// max-modulus of the same visibilities gets the same bits).
+__device__ __forceinline__ float SquaredModulus(float2 v) {
+  return __builtin_fmaf(v.x, v.x, v.y * v.y);
+}
+
 // i / d for i, d < 2^31 without an integer division: the FP64 product with
 // the rounded reciprocal is within one of the quotient, and one remainder
 // test corrects it.

@@ -273,9 +273,17 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
   vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", n_vis * sizeof(float4)));
   vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", n_vis * sizeof(float2)));
+  vs.vmax_of = nullptr;
   if (n_vis) {
-    // the sort's first pass generates the index values itself
-    LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_);
+    // the sort's first pass generates the index values itself.  When the
+    // keys kernel reads the visibilities (zero skipping), it also leaves
+    // their max |V| in the gridder's word for the Invert that follows
+    uint32_t* vmax_bits = nullptr;
+    if (vis_dev) {
+      vmax_bits = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t))) + 1;
+      vs.vmax_of = vis_dev;
+    }
+    LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_, vmax_bits);
     uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
     const SortCoords co{dp, uvw4, vs.scale, n_chan, 1.0 / double(n_chan), vs.coord, vs.phasor};
@@ -417,12 +425,15 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   {
     EventTimer tm(*this, &times_.sort);
     if (fused) {
-      LaunchVisMax(vis_dev, vs.n_vis, words + 1, stream_);
+      // the max of these visibilities is already in words[1] when the
+      // Prepare just before read them
+      if (vs.vmax_of != vis_dev) LaunchVisMax(vis_dev, vs.n_vis, words + 1, stream_);
     } else {
       vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
       LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, words + 1, stream_);
     }
   }
+  vs.vmax_of = nullptr;
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
   LaunchFillZero(dirty, npix * 4, stream_);

@@ -121,6 +121,10 @@ struct VisSet {
   DevPlan dp;
   size_t n_rows = 0, n_chan = 0, n_vis = 0, n_valid = 0;
   bool skip_zero = false;  // zero visibilities were left out (invert only)
+  // Prepare read these visibilities and left their max |V| in the gridder's
+  // word (grid.words[1]): the Invert that follows in the same library call
+  // skips its max-modulus pass (cleared by that Invert)
+  const float2* vmax_of = nullptr;
   // device arrays (owned by the arena under names prefixed with `tag`)
   std::string tag;
   double* uvw = nullptr;      // [n_rows][3]
@@ -426,10 +430,12 @@ void LaunchKeys(const DevPlan& dp, const double* uvw, const double* scale,
 void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rtag, size_t n_rows,
                   double* uvw4, hipStream_t s, const uint32_t* rhi = nullptr);
 // Keys of the visibilities in renumbered row order (element r * n_chan + ch
-// = channel ch of caller row tag(r)); zero visibilities get nkeys.
+// = channel ch of caller row tag(r)); zero visibilities get nkeys.  With
+// vis and vmax_bits, also the bits of max |V| over all visibilities (the
+// value LaunchVisMax gives), from the same reads.
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
                           size_t n_rows, size_t n_chan, const float2* vis,
-                          uint32_t* keys, hipStream_t s);
+                          uint32_t* keys, hipStream_t s, uint32_t* vmax_bits = nullptr);
 // out[i] = in[i] & mask
 void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,
                     hipStream_t s);

@@ -112,9 +112,11 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
 __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     DevPlan dp, const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_rows, size_t n_chan,
-    const float2* __restrict__ vis, uint32_t* __restrict__ keys) {
+    const float2* __restrict__ vis, uint32_t* __restrict__ keys,
+    uint32_t* __restrict__ vmax_bits) {
   const size_t n = n_rows * n_chan;
   const double inv_chan = 1.0 / double(n_chan);
+  float m2 = 0.f;
   for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
        e += size_t(gridDim.x) * kBlock) {
     const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
@@ -125,9 +127,14 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     if (vis) {
       const float2 V = vis[size_t(RowTag(wt)) * n_chan + ch];
       valid = (V.x != 0.f) || (V.y != 0.f);
+      m2 = fmaxf(m2, SquaredModulus(V));
     }
     if (valid) key = VisKey(LocateVis(uv.x, uv.y, wt.x, scale[ch], dp), dp);
     keys[e] = key;
+  }
+  if (vmax_bits) {
+    for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
+    if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(sqrtf(m2)));
   }
 }
 
@@ -198,22 +205,19 @@ __global__ void __launch_bounds__(kBlock) VisMaxKernel(const float2* __restrict_
       for (int k = 0; k < 4; ++k) q[k] = v4[i + k * stride];
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        m2 = fmaxf(m2, fmaxf(q[k].x * q[k].x + q[k].y * q[k].y, q[k].z * q[k].z + q[k].w * q[k].w));
+        m2 = fmaxf(m2, fmaxf(SquaredModulus(make_float2(q[k].x, q[k].y)),
+                             SquaredModulus(make_float2(q[k].z, q[k].w))));
     }
     for (; i < n4; i += stride) {
       const float4 q = v4[i];
-      m2 = fmaxf(m2, fmaxf(q.x * q.x + q.y * q.y, q.z * q.z + q.w * q.w));
+      m2 = fmaxf(m2, fmaxf(SquaredModulus(make_float2(q.x, q.y)),
+                           SquaredModulus(make_float2(q.z, q.w))));
     }
-    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-      const float2 v = vis[n - 1];
-      m2 = fmaxf(m2, v.x * v.x + v.y * v.y);
-    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) m2 = fmaxf(m2, SquaredModulus(vis[n - 1]));
   } else {
     for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += size_t(gridDim.x) * kBlock) {
-      const float2 v = vis[i];
-      m2 = fmaxf(m2, v.x * v.x + v.y * v.y);
-    }
+         i += size_t(gridDim.x) * kBlock)
+      m2 = fmaxf(m2, SquaredModulus(vis[i]));
   }
   for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
   if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(sqrtf(m2)));
@@ -674,11 +678,13 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
 
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
                           size_t n_rows, size_t n_chan, const float2* vis,
-                          uint32_t* keys, hipStream_t s) {
+                          uint32_t* keys, hipStream_t s, uint32_t* vmax_bits) {
+  WG_CHECK(!vmax_bits || vis, "the max-modulus needs the visibilities");
+  if (vmax_bits) LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
   if (!n_rows) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, scale, n_rows, n_chan,
-                                                                  vis, keys);
+                                                                  vis, keys, vmax_bits);
   WG_HIP(hipGetLastError());
 }
 
