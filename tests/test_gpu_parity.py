@@ -3,6 +3,8 @@ direct-FT oracle (DirectMSGridder restatement) and against exact
 properties.  Tolerances are the requested epsilon in relative L2 (the
 meaning of epsilon in doc/source/wgridding.rst:19-20) unless stated.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -174,6 +176,63 @@ def test_invert_dense_core(oracle):
     xs = rng.integers(0, 256, 400)
     ys = rng.integers(0, 256, 400)
     ref = oracle.dft_invert_pixels(256, 256, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
+                                   p["vis"], xs, ys)
+    assert rel_l2(img[ys, xs], ref) < 1e-5
+
+
+@functools.lru_cache(maxsize=1)
+def uniform_weighted_problem():
+    # 48 antennas in a log-radial 3 km layout (dense core, sparse outskirts),
+    # 240 x 60 s, 8 channels: 2.2e6 visibilities times uniform density
+    # weights (1 / visibilities in the image's uv cell), which span about
+    # four decades; plus 20 long-baseline visibilities x 1e3 (unflagged
+    # interference).  The gridders' fixed-point scale must follow each work
+    # item's own visibilities for the faint, densely sampled core to keep
+    # epsilon.
+    rng = np.random.Generator(np.random.PCG64(31))
+    enu = synth.antennas_logradial(rng, 48, 3000.0)
+    uvw = synth.earth_rotation_uvw(enu, 240, 60.0, np.deg2rad(-30.0))
+    freq = 140e6 + 4e6 * np.arange(8)
+    npix, dx = 256, 20.0 * synth.ARCSEC
+    l, m, flux, _, _ = synth.point_sky(rng, 12, npix, dx)
+    vis = synth.predict_sources(uvw, freq, l, m, flux).astype(np.complex64)
+    s = freq[None, :] / 299792458.0 * npix * dx
+    u, v = uvw[:, 0:1] * s, uvw[:, 1:2] * s
+    flip = v < 0
+    u, v = np.where(flip, -u, u), np.where(flip, -v, v)
+    cell = (np.floor(u).astype(np.int64) % npix) * npix + np.floor(v).astype(np.int64) % npix
+    count = np.bincount(cell.ravel(), minlength=npix * npix)
+    wt = 1.0 / count[cell]
+    vis = (vis * wt).astype(np.complex64)
+    r = np.hypot(uvw[:, 0], uvw[:, 1])
+    far = np.argsort(r)[-20:]
+    vis[far, 0] *= 1e3
+    return dict(uvw=uvw, freq=freq, vis=vis, npix=npix, dx=dx, wt_range=wt.max() / wt.min())
+
+
+@pytest.mark.parametrize("precision,sub_slab", [(-1, 0), (0, 0), (-1, 2)])
+def test_invert_uniform_weighted_dynamic_range(oracle, precision, sub_slab):
+    # default split kernels with the fused gather; FP32 matrix kernels; and
+    # several plane sub-slabs (pre-gathered visibilities)
+    p = uniform_weighted_problem()
+    assert p["wt_range"] > 1e3
+    n = p["npix"]
+    g = make(n, p["dx"], 1e-5)
+    g.set_precision(precision)
+    if sub_slab:
+        g.set_tuning(sub_slab=sub_slab)
+    g.InitializeInversion()
+    g.AddInversionData(p["uvw"].shape[0], len(p["freq"]), p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(1.0)
+    img = g.RealImage().reshape(n, n)
+    plan = g.plan()
+    assert plan["W"] <= 8
+    if sub_slab:
+        assert plan["P"] > sub_slab
+    rng = np.random.default_rng(2)
+    xs = rng.integers(0, n, 300)
+    ys = rng.integers(0, n, 300)
+    ref = oracle.dft_invert_pixels(n, n, p["dx"], p["dx"], 0.0, 0.0, p["uvw"], p["freq"],
                                    p["vis"], xs, ys)
     assert rel_l2(img[ys, xs], ref) < 1e-5
 

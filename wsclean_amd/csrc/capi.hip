@@ -326,12 +326,13 @@ int wg_destroy(wg_handle* h) {
 // FFT slab (w planes of the occupied uv box and two line buffers, budgeted
 // by the plan at slab_bytes) and tables.  Per visibility of one engine call:
 // sort keys and permutation with their ping-pong copies, sorted coordinates
-// and phasors, sorted or accumulated visibilities and the staging of host
-// calls, grown by the arena's 1/8 rounding; per row the staged and
-// renumbered uvw and the row sort.
+// and phasors, each sorted visibility's renumbered row, sorted or
+// accumulated visibilities and the staging of host calls, grown by the
+// arena's 1/8 rounding; per row the staged and renumbered uvw, the row sort
+// and the row maxima.
 constexpr size_t kDevBytesPerPixel = 28;
 constexpr size_t kDevBytesTables = size_t(64) << 20;
-constexpr size_t kDevBytesPerVis = 64;
+constexpr size_t kDevBytesPerVis = 72;
 constexpr size_t kDevBytesPerRow = 96;
 
 int wg_constant_memory_usage(const wg_handle* h, size_t* bytes) {

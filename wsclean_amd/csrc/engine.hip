@@ -252,19 +252,22 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   // caller row in the pad word: the coordinates kernel then reads the rows
   // of neighbouring sorted visibilities from neighbouring addresses instead
   // of at random, and each row's caller index from the same cache line.
-  uint32_t* rperm = static_cast<uint32_t*>(arena_.Get("rows.perm", n_rows * 4));
+  // the renumbered rows' caller rows stay with the set (the row maxima of
+  // its inversions read the rows through them)
+  uint32_t* rperm = static_cast<uint32_t*>(arena_.Get(tag + ".rows", n_rows * 4));
   double* uvw4 = static_cast<double*>(arena_.Get("uvw4", n_rows * 4 * sizeof(double)));
   uint32_t* scratch = static_cast<uint32_t*>(
       arena_.Get("sort.scratch", RadixSortScratchWords(std::max(n_vis, n_rows)) * 4));
   if (n_rows) {
     uint32_t* rk = static_cast<uint32_t*>(arena_.Get("rows.keys", n_rows * 4));
     uint32_t* rk2 = static_cast<uint32_t*>(arena_.Get("rows.keys2", n_rows * 4));
-    uint32_t* rp2 = static_cast<uint32_t*>(arena_.Get("rows.perm2", n_rows * 4));
+    uint32_t* rp2 = static_cast<uint32_t*>(arena_.Get(tag + ".rows2", n_rows * 4));
     LaunchKeys(dp, uvw_dev, vs.scale + n_chan / 2, n_rows, 1, nullptr, rk, nullptr, stream_);
     if (DeviceRadixSort(rk, rperm, rk2, rp2, n_rows, vs.plan.key_bits, scratch, stream_, true))
       rperm = rp2;
     LaunchPadUvw(uvw_dev, rperm, rperm, n_rows, uvw4, stream_);
   }
+  vs.rows = rperm;
 
   // ---- keys + stable sort; the sort's last pass writes the sorted
   // coordinates and phasors and turns renumbered elements into the caller's
@@ -273,20 +276,26 @@ void Engine::Prepare(VisSet& vs, const std::string& tag, size_t n_rows, size_t n
   uint32_t* vals = static_cast<uint32_t*>(arena_.Get(tag + ".perm", n_vis * 4));
   vs.coord = static_cast<float4*>(arena_.Get(tag + ".coord", n_vis * sizeof(float4)));
   vs.phasor = static_cast<float2*>(arena_.Get(tag + ".phasor", n_vis * sizeof(float2)));
-  vs.vmax_of = nullptr;
+  // sets that the matrix-core kernel grids in one sub-slab keep each sorted
+  // visibility's renumbered row for the per-item bounds
+  const bool fused = dp.W <= 8 && dp.P <= vs.plan.Sg;
+  vs.vrow = fused ? static_cast<uint32_t*>(arena_.Get(tag + ".vrow", n_vis * 4)) : nullptr;
+  vs.rowmax_of = nullptr;
   if (n_vis) {
     // the sort's first pass generates the index values itself.  When the
     // keys kernel reads the visibilities (zero skipping), it also leaves
-    // their max |V| in the gridder's word for the Invert that follows
-    uint32_t* vmax_bits = nullptr;
+    // the max |V| of every row for the Invert that follows
+    uint32_t* rowmax_bits = nullptr;
     if (vis_dev) {
-      vmax_bits = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t))) + 1;
-      vs.vmax_of = vis_dev;
+      rowmax_bits = static_cast<uint32_t*>(arena_.Get("grid.rowmax", n_rows * sizeof(float)));
+      vs.rowmax_of = vis_dev;
     }
-    LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_, vmax_bits);
+    LaunchKeysRenumbered(dp, uvw4, vs.scale, n_rows, n_chan, vis_dev, keys, stream_,
+                         rowmax_bits);
     uint32_t* keys2 = static_cast<uint32_t*>(arena_.Get(tag + ".keys2", n_vis * 4));
     uint32_t* vals2 = static_cast<uint32_t*>(arena_.Get(tag + ".perm2", n_vis * 4));
-    const SortCoords co{dp, uvw4, vs.scale, n_chan, 1.0 / double(n_chan), vs.coord, vs.phasor};
+    const SortCoords co{dp,        uvw4,      vs.scale, n_chan, 1.0 / double(n_chan),
+                        vs.coord, vs.phasor, vs.vrow};
     if (DeviceRadixSort(keys, vals, keys2, vals2, n_vis, vs.plan.key_bits, scratch, stream_,
                         true, &co)) {
       keys = keys2;
@@ -416,24 +425,33 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
   const Plan& p = vs.plan;
   if (vs.n_valid == 0) return;
   float2* vsorted = nullptr;
-  // [0] work-item queue of the gridding kernel, [1] max |component| of the
-  // sorted visibilities (sets the gridding's fixed-point scale)
+  // work-item queue of the gridding kernel
   uint32_t* words = static_cast<uint32_t*>(arena_.Get("grid.words", 4 * sizeof(uint32_t)));
   // one gridding pass per visibility (a single sub-slab): the matrix-core
   // kernel gathers through the permutation itself
   const bool fused = dp.W <= 8 && vs.nsub == 1;
+  // the matrix-core gridders' per-item scales
+  float2* bounds = nullptr;
   {
     EventTimer tm(*this, &times_.sort);
+    if (dp.W <= 8)
+      bounds = static_cast<float2*>(arena_.Get("grid.bounds", vs.n_items * sizeof(float2)));
     if (fused) {
-      // the max of these visibilities is already in words[1] when the
+      // the row maxima of these visibilities are already there when the
       // Prepare just before read them
-      if (vs.vmax_of != vis_dev) LaunchVisMax(vis_dev, vs.n_vis, words + 1, stream_);
+      float* rowmax = static_cast<float*>(arena_.Get("grid.rowmax", vs.n_rows * sizeof(float)));
+      if (vs.rowmax_of != vis_dev)
+        LaunchRowMax(vis_dev, vs.rows, vs.n_rows, vs.n_chan, reinterpret_cast<uint32_t*>(rowmax),
+                     stream_);
+      LaunchItemBounds(vs.items, vs.n_items, nullptr, vs.vrow, rowmax, bounds, stream_);
     } else {
       vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));
-      LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, words + 1, stream_);
+      LaunchGatherVis(vs.perm, vs.phasor, vs.n_valid, vis_dev, vsorted, stream_);
+      if (bounds)
+        LaunchItemBounds(vs.items, vs.n_items, vsorted, nullptr, nullptr, bounds, stream_);
     }
   }
-  vs.vmax_of = nullptr;
+  vs.rowmax_of = nullptr;
   const size_t npix = size_t(dp.nx) * dp.ny;
   float* dirty = static_cast<float*>(arena_.Get("dirty", npix * 4));
   LaunchFillZero(dirty, npix * 4, stream_);
@@ -456,9 +474,8 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
         const int sub_n = std::min(p.Sg, p.P - sub_lo);
         {
           EventTimer tk(*this, &times_.grid_kernel_ms);
-          LaunchGrid(dp, vs, vsorted, scratch, words, reinterpret_cast<const float*>(words + 1),
-                     g, sub_lo, sub_n, stream_, fused ? vis_dev : nullptr,
-                     fused ? vs.perm : nullptr);
+          LaunchGrid(dp, vs, vsorted, scratch, words, bounds, g, sub_lo, sub_n, stream_,
+                     fused ? vis_dev : nullptr, fused ? vs.perm : nullptr);
         }
         if (timing) {
           // coordinates (16 B) and visibility (8 B) per pass -- fused: also

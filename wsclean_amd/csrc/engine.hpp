@@ -39,6 +39,7 @@ struct SortCoords {
   double inv_chan;  // 1 / n_chan
   float4* coord;
   float2* phasor;
+  uint32_t* vrow;  // renumbered row of each sorted element (or null)
 };
 bool DeviceRadixSort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                      size_t n, int key_bits, uint32_t* scratch, hipStream_t s,
@@ -121,15 +122,19 @@ struct VisSet {
   DevPlan dp;
   size_t n_rows = 0, n_chan = 0, n_vis = 0, n_valid = 0;
   bool skip_zero = false;  // zero visibilities were left out (invert only)
-  // Prepare read these visibilities and left their max |V| in the gridder's
-  // word (grid.words[1]): the Invert that follows in the same library call
-  // skips its max-modulus pass (cleared by that Invert)
-  const float2* vmax_of = nullptr;
+  // Prepare read these visibilities and left the max |V| of every row in
+  // the arena's "grid.rowmax": the Invert that follows in the same library
+  // call skips its row-maxima pass (cleared by that Invert)
+  const float2* rowmax_of = nullptr;
   // device arrays (owned by the arena under names prefixed with `tag`)
   std::string tag;
   double* uvw = nullptr;      // [n_rows][3]
   double* scale = nullptr;    // [n_chan] freq / c
   uint32_t* perm = nullptr;   // [n_valid] sorted -> flat index | kPermFlip
+  uint32_t* rows = nullptr;   // [n_rows] caller row of renumbered row r
+  // [n_valid] renumbered row of each sorted visibility (sets gridded with
+  // the fused gather only: the per-item bounds read the row maxima by it)
+  uint32_t* vrow = nullptr;
   float2* phasor = nullptr;   // [n_valid] exp(-2 pi i (u l_s + v m_s + w zc))
   uint32_t* off = nullptr;    // [nkeys + 1] bucket offsets
   float4* coord = nullptr;    // [n_valid] fu, fv, fw, packed cell/plane/flip
@@ -300,6 +305,8 @@ class FacetBatch {
   size_t n_chan_ = 0;
   double* scale_ = nullptr;
   uint32_t* perm_phys_ = nullptr;
+  uint32_t* rphys_ = nullptr;  // physical row of each renumbered virtual row
+  size_t n_vrows_ = 0;
   bool prepared_ = false;
 };
 
@@ -347,13 +354,15 @@ void LaunchLineFFT(float2* data, int nlines, const FFTDesc& d, bool inverse,
 // [sub_lo, sub_lo + sub_n) into its scratch slot (GridScratchBytes); then
 // LaunchGatherBox writes every box cell of those planes.
 // `queue` is one device word, the work-item counter of the persistent
-// matrix-core kernel (cleared by the launch); `vmax` points to the largest
-// |component| of vis_sorted (from LaunchGatherVis).
+// matrix-core kernel (cleared by the launch).  `bounds` (matrix-core
+// gridders, W <= 8) holds per work item the (max, sum) of a bound on the
+// components of its visibilities (LaunchItemBounds), from which each item
+// takes its operand and fixed-point scales.
 // With vis_raw (matrix-core gridder only), the gather is fused: the kernel
 // reads vis_raw[gather_perm[i]] (flip bit in the word) times phasor[i]
 // instead of vis_sorted[i].
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
+                float2* scratch, uint32_t* queue, const float2* bounds, int sub, int sub_lo,
                 int sub_n, hipStream_t s, const float2* vis_raw = nullptr,
                 const uint32_t* gather_perm = nullptr);
 void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
@@ -363,8 +372,8 @@ void LaunchGatherBox(const DevPlan& dp, const VisSet& vs, float2* box,
 // arguments and results as LaunchGrid's matrix-core path (grid_split.hip);
 // LaunchGrid calls it when dp.split is set.
 void LaunchGridSplit(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                     float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
-                     int sub_n, hipStream_t s, const float2* vis_raw,
+                     float2* scratch, uint32_t* queue, const float2* bounds, int sub,
+                     int sub_lo, int sub_n, hipStream_t s, const float2* vis_raw,
                      const uint32_t* gather_perm);
 size_t SplitGridLdsBytes(const DevPlan& dp, int sub_n);
 // Split-precision matrix-core degridding for W <= 8 (grid_split.hip);
@@ -412,11 +421,11 @@ void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, in
                        hipStream_t s);
 // Sorted coordinates / phasors; perm enters with sorted renumbered elements
 // and leaves with the virtual index, perm_phys gets the physical index
-// (both with the flip bit).
+// (both with the flip bit); vrow (if not null) the renumbered virtual row.
 void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, int F,
                        const double* uvw4, const double* scale,
                        size_t n_chan, uint32_t* perm, uint32_t* perm_phys, size_t n_valid,
-                       float4* coord, float2* phasor, hipStream_t s);
+                       float4* coord, float2* phasor, hipStream_t s, uint32_t* vrow = nullptr);
 void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
                    double* partial, double* out, hipStream_t s);
 // Unsorted sort keys of every visibility (nkeys for zero ones); vals[i] = i
@@ -431,24 +440,30 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
                   double* uvw4, hipStream_t s, const uint32_t* rhi = nullptr);
 // Keys of the visibilities in renumbered row order (element r * n_chan + ch
 // = channel ch of caller row tag(r)); zero visibilities get nkeys.  With
-// vis and vmax_bits, also the bits of max |V| over all visibilities (the
-// value LaunchVisMax gives), from the same reads.
+// vis and rowmax_bits ([n_rows] words), also the bits of max |V| of every
+// renumbered row (the values LaunchRowMax gives with the rows' caller
+// map), from the same reads.
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
                           size_t n_rows, size_t n_chan, const float2* vis,
-                          uint32_t* keys, hipStream_t s, uint32_t* vmax_bits = nullptr);
+                          uint32_t* keys, hipStream_t s, uint32_t* rowmax_bits = nullptr);
 // out[i] = in[i] & mask
 void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out,
                     hipStream_t s);
-// vis_sorted[i] = flip(vis[perm[i]]) * phasor[i].  Also writes the bits of
-// max |Re|, |Im| of the result to *vmax_bits (a non-negative float; the
-// launch clears the word first).
+// vis_sorted[i] = flip(vis[perm[i]]) * phasor[i].
 void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
-                     const float2* vis, float2* vis_sorted, uint32_t* vmax_bits,
-                     hipStream_t s);
-// vmax_bits = bits of max |V| over vis[0, n) (a non-negative float; the
-// launch clears the word first): bounds every component of the rotated
-// visibilities, the fused gather's fixed-point scale.
-void LaunchVisMax(const float2* vis, size_t n, uint32_t* vmax_bits, hipStream_t s);
+                     const float2* vis, float2* vis_sorted, hipStream_t s);
+// rowmax_bits[r] = bits of max |V| over the n_chan visibilities of row
+// row_map[r] of vis ([rows][n_chan]), for r < n_rows: a non-negative float
+// (the launch clears the words first).
+void LaunchRowMax(const float2* vis, const uint32_t* row_map, size_t n_rows, size_t n_chan,
+                  uint32_t* rowmax_bits, hipStream_t s);
+// bounds[it] = (max, sum) over the visibilities of work item it (of all
+// n_items) of a bound on |Re| and |Im| of each rotated visibility: with
+// vis_sorted, its own larger component; else max |V| of its row,
+// rowmax[vrow[i]].  The matrix-core gridders take each item's scales from
+// these.
+void LaunchItemBounds(const WorkItem* items, size_t n_items, const float2* vis_sorted,
+                      const uint32_t* vrow, const float* rowmax, float2* bounds, hipStream_t s);
 // vis_out[perm[i]] = flip(acc[i] * conj(phasor[i])).
 void LaunchScatterVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
                       const float2* acc, float2* vis_out, hipStream_t s);

@@ -239,7 +239,7 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
     uint32_t* rk = static_cast<uint32_t*>(A.Get("rows.keys", n_vrows * 4));
     uint32_t* rk2 = static_cast<uint32_t*>(A.Get("rows.keys2", n_vrows * 4));
     uint32_t* rp2 = static_cast<uint32_t*>(A.Get("rows.perm2", n_vrows * 4));
-    uint32_t* rphys = static_cast<uint32_t*>(A.Get("rows.phys", n_vrows * 4));
+    uint32_t* rphys = static_cast<uint32_t*>(A.Get("fb.rows.phys", n_vrows * 4));
     uint32_t* rfac = static_cast<uint32_t*>(A.Get("rows.facet", n_vrows * 4));
     LaunchFacetKeys(dfac, dvrb, F, uvw_dev, scale_ + n_chan / 2, 1, n_vrows, pa.ntu, pa.P0,
                     pa.key_shift, pa.key_T, rk, nullptr, s);
@@ -248,7 +248,9 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
     LaunchFacetRowMap(dfac, dvrb, F, rperm, n_vrows, rphys, rfac, s);
     // the padded rows carry their virtual row and their facet
     LaunchPadUvw(uvw_dev, rphys, rperm, n_vrows, uvw4, s, rfac);
+    rphys_ = rphys;
   }
+  n_vrows_ = n_vrows;
 
   // ---- keys, one stable sort, bucket offsets ----
   atlas_.n_vis = n_virtual;
@@ -277,8 +279,12 @@ void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* u
   atlas_.coord = static_cast<float4*>(A.Get("atlas.coord", nvalid * sizeof(float4)));
   atlas_.phasor = static_cast<float2*>(A.Get("atlas.phasor", nvalid * sizeof(float2)));
   perm_phys_ = static_cast<uint32_t*>(A.Get("atlas.perm_phys", nvalid * 4));
+  // a batch that the matrix-core kernel grids in one sub-slab keeps each
+  // sorted visibility's renumbered virtual row for the per-item bounds
+  const bool fused = pa.W <= 8 && pa.P <= pa.Sg;
+  atlas_.vrow = fused ? static_cast<uint32_t*>(A.Get("atlas.vrow", nvalid * 4)) : nullptr;
   LaunchFacetCoords(dfac, dvrb, F, uvw4, scale_, n_chan, atlas_.perm, perm_phys_, nvalid,
-                    atlas_.coord, atlas_.phasor, s);
+                    atlas_.coord, atlas_.phasor, s, atlas_.vrow);
   eng_.BuildWorkItems(atlas_);
   for (int f = 0; f < F; ++f) eng_.BuildCorrection(fvs_[f]);
   prepared_ = true;
@@ -301,15 +307,23 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
   // a single sub-slab: the matrix-core gridder gathers through perm_phys
   const bool fused = da.W <= 8 && atlas_.nsub == 1;
   float2* vsorted = nullptr;
+  // the matrix-core gridders' per-item scales
+  float2* bounds = nullptr;
   {
     EventTimer tm(eng_, &eng_.Times().sort);
+    if (da.W <= 8)
+      bounds = static_cast<float2*>(A.Get("grid.bounds", atlas_.n_items * sizeof(float2)));
     if (fused) {
-      size_t rows_end = 0;
-      for (int f = 0; f < F; ++f) rows_end = std::max(rows_end, row_off_[f] + n_rows_[f]);
-      LaunchVisMax(vis_dev, rows_end * n_chan_, words + 1, s);
+      // row maxima of the renumbered virtual rows, read through their
+      // physical rows
+      float* rowmax = static_cast<float*>(A.Get("grid.rowmax", n_vrows_ * sizeof(float)));
+      LaunchRowMax(vis_dev, rphys_, n_vrows_, n_chan_, reinterpret_cast<uint32_t*>(rowmax), s);
+      LaunchItemBounds(atlas_.items, atlas_.n_items, nullptr, atlas_.vrow, rowmax, bounds, s);
     } else {
       vsorted = static_cast<float2*>(A.Get("vis_sorted", atlas_.n_valid * sizeof(float2)));
-      LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, words + 1, s);
+      LaunchGatherVis(perm_phys_, atlas_.phasor, atlas_.n_valid, vis_dev, vsorted, s);
+      if (bounds)
+        LaunchItemBounds(atlas_.items, atlas_.n_items, vsorted, nullptr, nullptr, bounds, s);
     }
   }
   StageTimes& st = eng_.Times();
@@ -338,8 +352,7 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
         {
           // one persistent launch over the work items of every facet
           EventTimer tk(eng_, &st.grid_kernel_ms);
-          LaunchGrid(da, atlas_, vsorted, scratch, words,
-                     reinterpret_cast<const float*>(words + 1), g, sub_lo, sub_n, s,
+          LaunchGrid(da, atlas_, vsorted, scratch, words, bounds, g, sub_lo, sub_n, s,
                      fused ? vis_dev : nullptr, fused ? perm_phys_ : nullptr);
         }
         if (eng_.timing) {

@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
     const float2* __restrict__ vis, const uint32_t* __restrict__ perm,
     const float2* __restrict__ phasor, float2* __restrict__ scratch, int sub_lo, int sub_n,
-    uint32_t* __restrict__ queue, const float* __restrict__ vmax_ptr) {
+    uint32_t* __restrict__ queue, const float2* __restrict__ bounds) {
   static_assert(W <= 8 && W % 2 == 0, "matrix-core gridding covers W <= 8");
   __shared__ uint32_t next_item;
   // [sub_n][E][E] fixed-point re, im; then the staging rings.  The dynamic
@@ -250,7 +250,6 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
   const int E = dp.T + W - 1;
   const int ncell = sub_n * E * E;
   int* sub_im = sub_re + ncell;
-  const float vmax = *vmax_ptr;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // staging rings after the subgrid, 16-byte aligned
@@ -287,16 +286,22 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     }
     __syncthreads();
     const uint32_t n = wi.end - wi.begin;
-    // Fixed-point subgrid: every cell component is a sum of |V component| x
-    // kernel weights (<= 1) over at most n visibilities, so |cell| <=
-    // n * vmax <= 2^e and values scaled by 2^(30 - e) fit an int32 with
-    // headroom for the rounding of each flushed term.  Integer LDS atomics
-    // are fast and exact, so the subgrid is also independent of the order
-    // of the flushes.
+    // Fixed-point subgrid: every cell component is a sum of V components x
+    // kernel weights (<= 1) over the item's n visibilities, so |cell| <=
+    // B = min(n max b_i, sum b_i) <= 2^e for bounds b_i on the components
+    // (the item's bounds), and values scaled by 2^(30 - e) fit an int32
+    // with a factor 2 of headroom for the rounding of the bound's sum and
+    // of each flushed term.  The scale follows the item's own visibilities,
+    // so an item of faint ones (a densely sampled region under uniform
+    // weighting) is not quantised on the scale of the brightest.  Integer
+    // LDS atomics are fast and exact, so the subgrid is also independent of
+    // the order of the flushes.
+    const float2 bd = bounds[it];
+    const float bnd = fminf(float(n) * bd.x, bd.y);
     int e = 0;
-    (void)frexpf(fmaxf(float(n) * vmax, 1e-30f), &e);
+    (void)frexpf(fmaxf(bnd, 1e-30f), &e);
     e = max(e, -90);  // all-zero items: keep the scale finite
-    const float fx_scale = (float(n) * vmax < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
+    const float fx_scale = (bnd < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
     const float fx_inv = ldexpf(1.0f, e - 30);
     const uint32_t per = (n + kMfmaWaves - 1) / kMfmaWaves;
     const uint32_t w0 = wi.begin + min(n, wave * per);
@@ -884,10 +889,11 @@ void BuildItems(const DevPlan& dp, const uint32_t* off, int Sg, int nsub, uint32
 }
 
 void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, float2* scratch,
-                uint32_t* queue, const float* vmax, int sub, int sub_lo, int sub_n,
+                uint32_t* queue, const float2* bounds, int sub, int sub_lo, int sub_n,
                 hipStream_t s, const float2* vis_raw, const uint32_t* gather_perm) {
+  WG_CHECK(dp.W > 8 || bounds, "the matrix-core gridders need the items' bounds");
   if (dp.split && dp.W <= 8) {
-    LaunchGridSplit(dp, vs, vis_sorted, scratch, queue, vmax, sub, sub_lo, sub_n, s, vis_raw,
+    LaunchGridSplit(dp, vs, vis_sorted, scratch, queue, bounds, sub, sub_lo, sub_n, s, vis_raw,
                     gather_perm);
     return;
   }
@@ -912,7 +918,7 @@ void LaunchGrid(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted, f
         LaunchFillZero(queue, sizeof(uint32_t), s);
         kernel<<<blocks, 64 * kMfmaWaves, total, s>>>(dp, vs.items, vs.item_pos, pos_lo, pos_hi,
                                                       vs.coord, v, pm, vs.phasor, scratch, sub_lo,
-                                                      sub_n, queue, vmax);
+                                                      sub_n, queue, bounds);
       };
       if (vis_raw)
         launch(GridKernelMfma<W, true>, vis_raw, gather_perm);

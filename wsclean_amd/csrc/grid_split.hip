@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     int pos_lo, int pos_hi, const float4* __restrict__ coord,
     const float2* __restrict__ vis, const uint32_t* __restrict__ perm,
     const float2* __restrict__ phasor, float2* __restrict__ scratch, int sub_lo, int sub_n,
-    uint32_t* __restrict__ queue, const float* __restrict__ vmax_ptr) {
+    uint32_t* __restrict__ queue, const float2* __restrict__ bounds) {
   static_assert(W <= 8 && W % 2 == 0, "split-precision gridding covers W <= 8");
   constexpr int H = W / 2;  // taps per coordinate and lane; plane pairs
   constexpr int D = EsPolyDegree(W);
@@ -149,11 +149,6 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
       reinterpret_cast<float*>(sub_re + ((2 * ncell + 3) & ~3)) + wave * kSplitTable, 16));
   // rows of taps >= W are never written and stay zero
   for (int i = lane; i < kSplitTable; i += 64) tbl[i] = 0.f;
-  // V sA <= 2^14 in magnitude
-  const float vmax = *vmax_ptr;
-  int ev = 0;
-  (void)frexpf(fmaxf(vmax, 1e-30f), &ev);
-  const float sA = ldexpf(1.0f, 14 - ev);
   // evaluation lane: slot es of the group, +g (eh = 0) or -g (eh = 1)
   const int es = lane & 31, eh = lane >> 5;
   const float gs = eh ? -1.f : 1.f;
@@ -179,11 +174,18 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     }
     __syncthreads();
     const uint32_t n = wi.end - wi.begin;
-    // fixed-point subgrid as in the FP32 kernel: |cell| <= n vmax <= 2^e
+    // operands: V sA <= 2^14 in magnitude for the item's largest component
+    // bound; fixed-point subgrid as in the FP32 kernel, |cell| <=
+    // min(n max b_i, sum b_i) <= 2^e
+    const float2 bd = bounds[it];
+    int ev = 0;
+    (void)frexpf(fmaxf(bd.x, 1e-30f), &ev);
+    const float sA = ldexpf(1.0f, 14 - ev);
+    const float bnd = fminf(float(n) * bd.x, bd.y);
     int e = 0;
-    (void)frexpf(fmaxf(float(n) * vmax, 1e-30f), &e);
+    (void)frexpf(fmaxf(bnd, 1e-30f), &e);
     e = max(e, -90);
-    const float fx_scale = (float(n) * vmax < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
+    const float fx_scale = (bnd < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
     const float fx_inv = ldexpf(1.0f, e - 30);
     // D holds sA kSplitB times the sums
     const float fxs = fx_scale / (sA * kSplitB);
@@ -573,8 +575,8 @@ size_t SplitGridLdsBytes(const DevPlan& dp, int sub_n) {
 }
 
 void LaunchGridSplit(const DevPlan& dp, const VisSet& vs, const float2* vis_sorted,
-                     float2* scratch, uint32_t* queue, const float* vmax, int sub, int sub_lo,
-                     int sub_n, hipStream_t s, const float2* vis_raw,
+                     float2* scratch, uint32_t* queue, const float2* bounds, int sub,
+                     int sub_lo, int sub_n, hipStream_t s, const float2* vis_raw,
                      const uint32_t* gather_perm) {
   WG_CHECK(dp.W <= 8 && dp.W % 2 == 0, "split-precision gridding covers even W <= 8");
   WG_CHECK(sub_n >= 1 && sub_n <= kMaxSubSlab, "sub-slab depth out of range");
@@ -590,7 +592,7 @@ void LaunchGridSplit(const DevPlan& dp, const VisSet& vs, const float2* vis_sort
     LaunchFillZero(queue, sizeof(uint32_t), s);
     kernel<<<blocks, 64 * kMfmaWaves, total, s>>>(dp, vs.items, vs.item_pos, pos_lo, pos_hi,
                                                   vs.coord, v, pm, vs.phasor, scratch, sub_lo,
-                                                  sub_n, queue, vmax);
+                                                  sub_n, queue, bounds);
   };
   switch (dp.W) {
     case 4:

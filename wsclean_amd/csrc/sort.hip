@@ -239,7 +239,8 @@ struct ScatterLds {
 // width when known at compile time (0: the runtime `bits`).
 // Coordinates epilogue of the last pass (SortCoords): the sorted entry p of
 // renumbered element e gets its gridding coordinates and shift phasor, and
-// its permutation word becomes the caller's flat index with the flip bit.
+// its permutation word becomes the caller's flat index with the flip bit
+// (and, with co.vrow, its renumbered row is kept).
 // The elements of one tile come from 8192 consecutive renumbered elements,
 // so their rows are read from a few cache lines.
 __device__ __forceinline__ uint32_t SortedCoords(const SortCoords& co, uint32_t e, uint32_t p) {
@@ -250,6 +251,7 @@ __device__ __forceinline__ uint32_t SortedCoords(const SortCoords& co, uint32_t 
   const VisLoc L = LocateVis(uv.x, uv.y, wt.x, co.scale[ch], co.dp);
   co.coord[p] = make_float4(L.fu, L.fv, L.fw, __uint_as_float(PackLoc(L)));
   co.phasor[p] = ShiftPhasor(co.dp, L, -1.f);
+  if (co.vrow) co.vrow[p] = r;
   return (RowTag(wt) * nc + ch) | (L.flip ? kPermFlip : 0u);
 }
 

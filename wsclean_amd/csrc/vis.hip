@@ -105,39 +105,165 @@ __global__ void __launch_bounds__(kBlock) KeysKernel(DevPlan dp, const double* _
   }
 }
 
+// Row maxima of |V| from one wave's 64 consecutive elements: lane L holds
+// the squared modulus m2 of an element of row `row` (~0u past the end), and
+// the lanes of one row are consecutive.  Each row segment's max of
+// sqrt(m2) goes to rowmax_bits[row] with atomicMax on the bits of a
+// non-negative float (the words start at zero), which also joins a row
+// split between waves.  sqrt is monotonic and correctly rounded, so this is
+// max |V| of the row exactly.  Every lane of the wave takes part.
+__device__ __forceinline__ void RowSegmentMax(float m2, uint32_t row,
+                                              uint32_t* __restrict__ rowmax_bits) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t r0 = __builtin_amdgcn_readfirstlane(row);
+  if (__all(row == r0)) {
+    for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
+    if (lane == 0 && r0 != ~0u) atomicMax(rowmax_bits + r0, __float_as_uint(sqrtf(m2)));
+    return;
+  }
+  // lane L ends with the max over [L, end of its segment): windows that
+  // double in length, taken only from lanes of the same row
+  for (int off = 1; off < 64; off <<= 1) {
+    const float o = __shfl_down(m2, off);
+    const uint32_t ro = __shfl_down(row, off);
+    if (lane + off < 64 && ro == row) m2 = fmaxf(m2, o);
+  }
+  const uint32_t rp = __shfl_up(row, 1);
+  if ((lane == 0 || rp != row) && row != ~0u)
+    atomicMax(rowmax_bits + row, __float_as_uint(sqrtf(m2)));
+}
+
 // Keys in renumbered row order: element e = r * n_chan + ch is channel ch of
 // renumbered row r, i.e. of caller row RowTag(uvw4, r); uvw4 holds the
 // renumbered rows padded to 32 bytes.  Keys are the same as KeysKernel's for
-// the same visibility, only their order differs.
+// the same visibility, only their order differs.  With rowmax_bits, also
+// max |V| of every renumbered row (the values RowMaxKernel gives with the
+// rows' caller map) from the same reads.  The loop steps whole waves, so
+// every lane takes part in the row maxima.
 __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     DevPlan dp, const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_rows, size_t n_chan,
     const float2* __restrict__ vis, uint32_t* __restrict__ keys,
-    uint32_t* __restrict__ vmax_bits) {
+    uint32_t* __restrict__ rowmax_bits) {
   const size_t n = n_rows * n_chan;
   const double inv_chan = 1.0 / double(n_chan);
-  float m2 = 0.f;
-  for (size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x; e < n;
-       e += size_t(gridDim.x) * kBlock) {
-    const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
-    uint32_t key = dp.nkeys;
-    bool valid = true;
-    const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
-    const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
-    if (vis) {
-      const float2 V = vis[size_t(RowTag(wt)) * n_chan + ch];
-      valid = (V.x != 0.f) || (V.y != 0.f);
-      m2 = fmaxf(m2, SquaredModulus(V));
+  const size_t lane = threadIdx.x & 63;
+  for (size_t e0 = size_t(blockIdx.x) * kBlock + (threadIdx.x & ~63u); e0 < n;
+       e0 += size_t(gridDim.x) * kBlock) {
+    const size_t e = e0 + lane;
+    const bool in = e < n;
+    float m2 = 0.f;
+    uint32_t rrow = ~0u;
+    if (in) {
+      const size_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan), ch = e - r * n_chan;
+      uint32_t key = dp.nkeys;
+      bool valid = true;
+      const double2 uv = reinterpret_cast<const double2*>(uvw4)[2 * r];
+      const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
+      if (vis) {
+        const float2 V = vis[size_t(RowTag(wt)) * n_chan + ch];
+        valid = (V.x != 0.f) || (V.y != 0.f);
+        m2 = SquaredModulus(V);
+        rrow = uint32_t(r);
+      }
+      if (valid) key = VisKey(LocateVis(uv.x, uv.y, wt.x, scale[ch], dp), dp);
+      keys[e] = key;
     }
-    if (valid) key = VisKey(LocateVis(uv.x, uv.y, wt.x, scale[ch], dp), dp);
-    keys[e] = key;
-  }
-  if (vmax_bits) {
-    for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
-    if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(sqrtf(m2)));
+    if (rowmax_bits) RowSegmentMax(m2, rrow, rowmax_bits);
   }
 }
 
+// rowmax_bits[r] = bits of max |V| over the n_chan visibilities of row
+// row_map[r] (row r without a map) of vis; the words start at zero.  Each
+// wave step reads four 64-element chunks before reducing them.
+__global__ void __launch_bounds__(kBlock) RowMaxKernel(const float2* __restrict__ vis,
+                                                      const uint32_t* __restrict__ row_map,
+                                                      size_t n_rows, size_t n_chan,
+                                                      uint32_t* __restrict__ rowmax_bits) {
+  const size_t n = n_rows * n_chan;
+  const double inv_chan = 1.0 / double(n_chan);
+  const size_t lane = threadIdx.x & 63;
+  const size_t stride = size_t(gridDim.x) * kBlock;
+  for (size_t e0 = size_t(blockIdx.x) * kBlock + (threadIdx.x & ~63u); e0 < n;
+       e0 += 4 * stride) {
+    float2 v[4];
+    uint32_t row[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t e = e0 + k * stride + lane;
+      row[k] = ~0u;
+      v[k] = make_float2(0.f, 0.f);
+      if (e < n) {
+        const uint32_t r = DivSmall(uint32_t(e), uint32_t(n_chan), inv_chan);
+        const size_t src = row_map ? row_map[r] : r;
+        v[k] = vis[src * n_chan + (e - size_t(r) * n_chan)];
+        row[k] = r;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (e0 + k * stride >= n) break;  // the same for every lane
+      RowSegmentMax(SquaredModulus(v[k]), row[k], rowmax_bits);
+    }
+  }
+}
+
+// bounds[it] = (max, sum) over work item it's visibilities of a bound b_i
+// on |Re| and |Im| of each rotated visibility: SORTED, the larger component
+// of vis_sorted[i] itself; else max |V| of the visibility's renumbered row
+// (rowmax[vrow[i]]), which bounds both components of V times a unit
+// phasor.  Rows are renumbered in uv order, so an item reads the maxima of
+// a few short row ranges.  One block per item; the sums are formed in a
+// fixed order, so the bounds (and the gridding scales taken from them)
+// repeat bitwise.
+template <bool SORTED>
+__global__ void __launch_bounds__(256) ItemBoundsKernel(const WorkItem* __restrict__ items,
+                                                       const float2* __restrict__ vis_sorted,
+                                                       const uint32_t* __restrict__ vrow,
+                                                       const float* __restrict__ rowmax,
+                                                       float2* __restrict__ bounds) {
+  __shared__ float part[2][4];
+  const WorkItem wi = items[blockIdx.x];
+  auto bound = [&](uint32_t i) {
+    if constexpr (SORTED) {
+      const float2 v = vis_sorted[i];
+      return fmaxf(fabsf(v.x), fabsf(v.y));
+    } else {
+      return rowmax[vrow[i]];
+    }
+  };
+  float mx = 0.f, sm = 0.f;
+  uint32_t i = wi.begin + threadIdx.x;
+  for (; i + 3u * 256u < wi.end; i += 4u * 256u) {
+    float b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = bound(i + 256u * k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      mx = fmaxf(mx, b[k]);
+      sm += b[k];
+    }
+  }
+  for (; i < wi.end; i += 256u) {
+    const float b = bound(i);
+    mx = fmaxf(mx, b);
+    sm += b;
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, m));
+    sm += __shfl_xor(sm, m);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][wave] = mx;
+    part[1][wave] = sm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    bounds[blockIdx.x] =
+        make_float2(fmaxf(fmaxf(part[0][0], part[0][1]), fmaxf(part[0][2], part[0][3])),
+                    (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]));
+}
 
 // uvw4[r] = (uvw[rfetch[r]], tag): the renumbered rows padded to 32 bytes,
 // the fourth word holding rtag[r] in its low half (read back by RowTag) and
@@ -168,59 +294,15 @@ __global__ void __launch_bounds__(kBlock) MaskKernel(const uint32_t* __restrict_
 
 __global__ void __launch_bounds__(kBlock) GatherVisKernel(
     const uint32_t* __restrict__ perm, const float2* __restrict__ phasor, size_t n_valid,
-    const float2* __restrict__ vis, float2* __restrict__ out, uint32_t* __restrict__ vmax_bits) {
-  float vmax = 0.f;  // largest |component| of the gathered visibilities
+    const float2* __restrict__ vis, float2* __restrict__ out) {
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
     const uint32_t pw = perm[i];
     float2 V = vis[pw & ~kPermFlip];
     if (pw & kPermFlip) V.y = -V.y;
     const float2 ph = phasor[i];
-    const float2 o = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
-    out[i] = o;
-    vmax = fmaxf(vmax, fmaxf(fabsf(o.x), fabsf(o.y)));
+    out[i] = make_float2(V.x * ph.x - V.y * ph.y, V.x * ph.y + V.y * ph.x);
   }
-  // non-negative floats order like their bit patterns
-  for (int m = 32; m >= 1; m >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, m));
-  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(vmax));
-}
-
-// max |V| over n visibilities, as the bits of a non-negative float
-// (atomicMax on the bits orders non-negative floats).  The squared moduli
-// are compared and one square root is taken at the end (sqrt is monotonic
-// and correctly rounded, so this is max |V| exactly).  VEC: 16-byte loads
-// of two visibilities, four in flight per thread.
-template <bool VEC>
-__global__ void __launch_bounds__(kBlock) VisMaxKernel(const float2* __restrict__ vis, size_t n,
-                                                      uint32_t* __restrict__ vmax_bits) {
-  float m2 = 0.f;
-  if (VEC) {
-    const float4* v4 = reinterpret_cast<const float4*>(vis);
-    const size_t n4 = n / 2;
-    const size_t stride = size_t(gridDim.x) * kBlock;
-    size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-      float4 q[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) q[k] = v4[i + k * stride];
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        m2 = fmaxf(m2, fmaxf(SquaredModulus(make_float2(q[k].x, q[k].y)),
-                             SquaredModulus(make_float2(q[k].z, q[k].w))));
-    }
-    for (; i < n4; i += stride) {
-      const float4 q = v4[i];
-      m2 = fmaxf(m2, fmaxf(SquaredModulus(make_float2(q.x, q.y)),
-                           SquaredModulus(make_float2(q.z, q.w))));
-    }
-    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) m2 = fmaxf(m2, SquaredModulus(vis[n - 1]));
-  } else {
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += size_t(gridDim.x) * kBlock)
-      m2 = fmaxf(m2, SquaredModulus(vis[i]));
-  }
-  for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
-  if ((threadIdx.x & 63) == 0) atomicMax(vmax_bits, __float_as_uint(sqrtf(m2)));
 }
 
 __global__ void __launch_bounds__(kBlock) ScatterVisKernel(
@@ -564,13 +646,14 @@ __global__ void __launch_bounds__(kBlock) FacetRowMapKernel(
 // the facet's own plan, phase centre shift and w centre).  perm enters with
 // sorted renumbered elements; it leaves with the virtual index (the predict
 // output layout) and perm_phys with the physical index into the shared
-// visibility buffer; both carry the flip bit.
+// visibility buffer; both carry the flip bit.  With vrow, each sorted
+// element's renumbered virtual row is kept.
 __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
     const FacetDev* __restrict__ fac, const unsigned long long* __restrict__ vrbase, int F,
     const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_chan, uint32_t* __restrict__ perm,
     uint32_t* __restrict__ perm_phys, size_t n_valid, float4* __restrict__ coord,
-    float2* __restrict__ phasor) {
+    float2* __restrict__ phasor, uint32_t* __restrict__ vrow) {
   const double inv_chan = 1.0 / double(n_chan);
   for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n_valid;
        i += size_t(gridDim.x) * kBlock) {
@@ -580,6 +663,7 @@ __global__ void __launch_bounds__(kBlock) FacetCoordsKernel(
     const double2 wt = reinterpret_cast<const double2*>(uvw4)[2 * r + 1];
     const uint32_t vr = RowTag(wt);
     const int f = int(RowHi(wt));  // the row's facet, from its padded entry
+    if (vrow) vrow[i] = uint32_t(r);
     auto write = [&](const FacetDev& fd, unsigned long long vrb) {
       const DevPlan& dp = fd.dp;
       const VisLoc L = LocateVis(uv.x, uv.y, wt.x, scale[ch], dp);
@@ -661,11 +745,12 @@ void LaunchFacetRowMap(const FacetDev* fac, const unsigned long long* vrbase, in
 void LaunchFacetCoords(const FacetDev* fac, const unsigned long long* vrbase, int F,
                        const double* uvw4, const double* scale,
                        size_t n_chan, uint32_t* perm, uint32_t* perm_phys, size_t n_valid,
-                       float4* coord, float2* phasor, hipStream_t s) {
+                       float4* coord, float2* phasor, hipStream_t s, uint32_t* vrow) {
   if (!n_valid) return;
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   FacetCoordsKernel<<<Blocks(n_valid), kBlock, 0, s>>>(fac, vrbase, F, uvw4, scale, n_chan,
-                                                        perm, perm_phys, n_valid, coord, phasor);
+                                                        perm, perm_phys, n_valid, coord, phasor,
+                                                        vrow);
   WG_HIP(hipGetLastError());
 }
 
@@ -678,13 +763,13 @@ void LaunchPadUvw(const double* uvw, const uint32_t* rfetch, const uint32_t* rta
 
 void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* scale,
                           size_t n_rows, size_t n_chan, const float2* vis,
-                          uint32_t* keys, hipStream_t s, uint32_t* vmax_bits) {
-  WG_CHECK(!vmax_bits || vis, "the max-modulus needs the visibilities");
-  if (vmax_bits) LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
+                          uint32_t* keys, hipStream_t s, uint32_t* rowmax_bits) {
+  WG_CHECK(!rowmax_bits || vis, "the row maxima need the visibilities");
   if (!n_rows) return;
+  if (rowmax_bits) LaunchFillZero(rowmax_bits, n_rows * sizeof(uint32_t), s);
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
   KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, scale, n_rows, n_chan,
-                                                                  vis, keys, vmax_bits);
+                                                                  vis, keys, rowmax_bits);
   WG_HIP(hipGetLastError());
 }
 
@@ -695,23 +780,34 @@ void LaunchMaskBits(const uint32_t* in, size_t n, uint32_t mask, uint32_t* out, 
 }
 
 void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
-                     const float2* vis, float2* vis_sorted, uint32_t* vmax_bits, hipStream_t s) {
-  LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
+                     const float2* vis, float2* vis_sorted, hipStream_t s) {
   if (!n_valid) return;
-  GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(perm, phasor, n_valid, vis, vis_sorted,
-                                                     vmax_bits);
+  GatherVisKernel<<<Blocks(n_valid), kBlock, 0, s>>>(perm, phasor, n_valid, vis, vis_sorted);
   WG_HIP(hipGetLastError());
 }
 
-void LaunchVisMax(const float2* vis, size_t n, uint32_t* vmax_bits, hipStream_t s) {
-  LaunchFillZero(vmax_bits, sizeof(uint32_t), s);
-  if (!n) return;
-  if ((reinterpret_cast<uintptr_t>(vis) & 15) == 0) {
-    // about 8 visibilities per thread per pass over the grid
-    const unsigned nb = unsigned(std::min<size_t>((n / 2 + 4 * kBlock - 1) / (4 * kBlock), 8192));
-    VisMaxKernel<true><<<std::max(nb, 1u), kBlock, 0, s>>>(vis, n, vmax_bits);
+void LaunchRowMax(const float2* vis, const uint32_t* row_map, size_t n_rows, size_t n_chan,
+                  uint32_t* rowmax_bits, hipStream_t s) {
+  if (!n_rows || !n_chan) return;
+  LaunchFillZero(rowmax_bits, n_rows * sizeof(uint32_t), s);
+  const size_t n = n_rows * n_chan;
+  // four 64-element chunks per wave step, about four steps per wave
+  const unsigned nb = unsigned(std::min<size_t>((n + 16 * kBlock - 1) / (16 * kBlock), 16384));
+  RowMaxKernel<<<std::max(nb, 1u), kBlock, 0, s>>>(vis, row_map, n_rows, n_chan, rowmax_bits);
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchItemBounds(const WorkItem* items, size_t n_items, const float2* vis_sorted,
+                      const uint32_t* vrow, const float* rowmax, float2* bounds, hipStream_t s) {
+  if (!n_items) return;
+  WG_CHECK(n_items < (size_t(1) << 31), "too many work items");
+  if (vis_sorted) {
+    ItemBoundsKernel<true><<<unsigned(n_items), 256, 0, s>>>(items, vis_sorted, nullptr, nullptr,
+                                                              bounds);
   } else {
-    VisMaxKernel<false><<<Blocks(n), kBlock, 0, s>>>(vis, n, vmax_bits);
+    WG_CHECK(vrow && rowmax, "item bounds need the sorted rows and the row maxima");
+    ItemBoundsKernel<false><<<unsigned(n_items), 256, 0, s>>>(items, nullptr, vrow, rowmax,
+                                                               bounds);
   }
   WG_HIP(hipGetLastError());
 }
