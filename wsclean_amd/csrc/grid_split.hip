@@ -376,9 +376,19 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
       reinterpret_cast<float*>(dsplit_lds + ((ncell + 1) & ~1)) + wave * kDgTable, 16));
   for (int i = lane; i < kDgTable; i += 64) tbl[i] = 0.f;
   const int es = lane & 31, eh = lane >> 5;
-  const float gs = eh ? -1.f : 1.f;
   const int jc = lane & 15, og = lane >> 4;
   const int a_t = jc >> 1, a_c = jc & 1;  // A row (plane, component)
+  // w taps: lane half eh evaluates the even / odd pairs j + eh HW (taps q
+  // and W - 1 - q of its slot), so each pair is evaluated once per slot;
+  // their coefficients are held per lane
+  constexpr int HW = (H + 1) / 2;
+  float cw[HW][D + 1];
+#pragma unroll
+  for (int j = 0; j < HW; ++j)
+#pragma unroll
+    for (int d = 0; d <= D; ++d)
+      cw[j][d] = eh ? (j + HW < H ? dp.es_poly[j + HW < H ? j + HW : j][d] : 0.f)
+                    : dp.es_poly[j][d];
   const uint32_t it0 = item_pos[pos_lo], it1 = item_pos[pos_hi];
   for (;;) {
     if (threadIdx.x == 0) {
@@ -411,7 +421,10 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     const uint32_t w1 = wi.begin + min(n, (wave + 1) * per);
     if (w0 < w1) {
       half8_t ah[2], al[2];
-      // the run's box block as A operands
+      // the run's box block as A operands.  Every lane reads its W cells
+      // from a valid address (row 0 of the block where its row is outside
+      // the block or the kernel), so the reads issue together; the values
+      // of such rows are then zero.
       auto load_a = [&](int key) {
         const int du = key & 255, dv = (key >> 8) & 255;
         const int q = ((key >> 16) & 0x7fff) - sub_lo + a_t;
@@ -419,15 +432,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int l = 4 * h + og;
+          const bool ok = row_ok && l < W;
+          const float2* src = sub + ((ok ? q * E + dv + l : 0) * E + (ok ? du : 0));
+          float2 g[W];
+#pragma unroll
+          for (int e = 0; e < W; ++e) g[e] = src[e];
           float x[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            x[e] = 0.f;
-            if (row_ok && l < W && e < W) {
-              const float2 g = sub[(q * E + dv + l) * E + du + e];
-              x[e] = a_c ? g.y : g.x;
-            }
-          }
+          for (int e = 0; e < 8; ++e) x[e] = (e < W && ok) ? (a_c ? g[e < W ? e : 0].y
+                                                                   : g[e < W ? e : 0].x)
+                                                          : 0.f;
           const float4 s4 = make_float4(sG, sG, sG, sG);
           SplitProducts8(make_float4(x[0], x[1], x[2], x[3]), make_float4(x[4], x[5], x[6], x[7]),
                          s4, s4, ah[h], al[h]);
@@ -470,15 +484,14 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
               }
             }
           }
-          // kernel values of slot es into the table, as in the gridder:
-          // lanes eh = 0 / 1 take every tap of u / v from the even and odd
-          // parts of the tap polynomials, then both evaluate w and each
-          // writes its half of the taps
+          // kernel values of slot es into the table: lanes eh = 0 / 1 take
+          // every tap of u / v from the even and odd parts of the tap
+          // polynomials (as in the gridder), and each half evaluates its
+          // share of the w tap pairs
           {
             const float g1 = __builtin_fmaf(eh ? bv[half] : bu[half], 2.f, -1.f);
             const float gw = __builtin_fmaf(bw[half], 2.f, -1.f);
             const float y1 = g1 * g1, yw = gw * gw;
-            const float sgw = gs * gw;
             float* row = tbl + es * kDgPitch;
             float* r1 = row + (eh ? 8 : 0);
 #pragma unroll
@@ -490,8 +503,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
               EsPolyEvenOdd<D>(c, y1, E, O);
               r1[q] = __builtin_fmaf(g1, O, E);
               r1[W - 1 - q] = __builtin_fmaf(-g1, O, E);
-              EsPolyEvenOdd<D>(c, yw, E, O);
-              row[16 + (eh ? W - 1 - q : q)] = __builtin_fmaf(sgw, O, E);
+            }
+#pragma unroll
+            for (int j = 0; j < HW; ++j) {
+              const int q = j + (eh ? HW : 0);
+              float E, O;
+              EsPolyEvenOdd<D>(cw[j], yw, E, O);
+              if (q < H) {
+                row[16 + q] = __builtin_fmaf(gw, O, E);
+                row[16 + W - 1 - q] = __builtin_fmaf(-gw, O, E);
+              }
             }
           }
           WaveLdsSync();
