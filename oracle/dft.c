@@ -196,7 +196,8 @@ typedef struct {
   double s0, ds;
   const double* s;     /* per-channel scale (fallback) */
   int uniform;
-  double* acc;         /* npix partial sums */
+  int per_chan;        /* acc holds npix x n_chan sums (else npix) */
+  double* acc;         /* partial sums */
 } fast_job_t;
 
 static void* invert_fast_worker(void* arg) {
@@ -208,11 +209,16 @@ static void* invert_fast_worker(void* arg) {
       const double theta =
           -2.0 * M_PI * (u * j->lmn[3 * p] + v * j->lmn[3 * p + 1] + w * j->lmn[3 * p + 2]);
       double a = 0.0;
+      double* ac = j->per_chan ? j->acc + p * j->n_chan : NULL;
       if (j->uniform) {
         double zr = cos(theta * j->s0), zi = sin(theta * j->s0);
         const double dr = cos(theta * j->ds), di = sin(theta * j->ds);
         for (size_t c = 0; c < j->n_chan; ++c) {
-          a += (double)vr[2 * c] * zr - (double)vr[2 * c + 1] * zi;
+          const double term = (double)vr[2 * c] * zr - (double)vr[2 * c + 1] * zi;
+          if (ac)
+            ac[c] += term;
+          else
+            a += term;
           const double t = zr * dr - zi * di;
           zi = zr * di + zi * dr;
           zr = t;
@@ -220,25 +226,31 @@ static void* invert_fast_worker(void* arg) {
       } else {
         for (size_t c = 0; c < j->n_chan; ++c) {
           const double ang = theta * j->s[c];
-          a += (double)vr[2 * c] * cos(ang) - (double)vr[2 * c + 1] * sin(ang);
+          const double term = (double)vr[2 * c] * cos(ang) - (double)vr[2 * c + 1] * sin(ang);
+          if (ac)
+            ac[c] += term;
+          else
+            a += term;
         }
       }
-      j->acc[p] += a;
+      if (!ac) j->acc[p] += a;
     }
   }
   return NULL;
 }
 
-void oracle_dft_invert_pixels_fast(int nx, int ny, double dx, double dy, double l_shift,
-                                   double m_shift, size_t n_rows, size_t n_chan,
-                                   const double* uvw, const double* freq, const float* vis,
-                                   size_t npix, const int* xs, const int* ys, double* out,
-                                   int nthreads, int* used_recurrence) {
+static void dft_invert_pixels_fast_impl(int nx, int ny, double dx, double dy, double l_shift,
+                                        double m_shift, size_t n_rows, size_t n_chan,
+                                        const double* uvw, const double* freq, const float* vis,
+                                        size_t npix, const int* xs, const int* ys, double* out,
+                                        int nthreads, int* used_recurrence, int per_chan) {
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
+  /* sums per thread: npix, or npix x n_chan per channel */
+  const size_t nsum = (npix ? npix : 1) * (per_chan && n_chan ? n_chan : 1);
   double* lmn = (double*)malloc(sizeof(double) * 3 * (npix ? npix : 1));
   double* s = (double*)malloc(sizeof(double) * (n_chan ? n_chan : 1));
-  double* acc = (double*)calloc((size_t)nthreads * (npix ? npix : 1), sizeof(double));
+  double* acc = (double*)calloc((size_t)nthreads * nsum, sizeof(double));
   for (size_t p = 0; p < npix; ++p) {
     const double l = pixel_l(xs[p], nx, dx, l_shift), m = pixel_m(ys[p], ny, dy, m_shift);
     lmn[3 * p] = l;
@@ -267,17 +279,40 @@ void oracle_dft_invert_pixels_fast(int nx, int ny, double dx, double dy, double 
     j->ds = ds;
     j->s = s;
     j->uniform = uniform;
-    j->acc = acc + (size_t)t * npix;
+    j->per_chan = per_chan;
+    j->acc = acc + (size_t)t * nsum;
     pthread_create(&th[t], NULL, invert_fast_worker, j);
   }
   for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
-  for (size_t p = 0; p < npix; ++p) {
+  const size_t nout = per_chan ? npix * n_chan : npix;
+  for (size_t p = 0; p < nout; ++p) {
     double a = 0.0;
-    for (int t = 0; t < nthreads; ++t) a += acc[(size_t)t * npix + p];
+    for (int t = 0; t < nthreads; ++t) a += acc[(size_t)t * nsum + p];
     out[p] = a;
   }
   if (used_recurrence) *used_recurrence = uniform;
   free(acc);
   free(s);
   free(lmn);
+}
+
+void oracle_dft_invert_pixels_fast(int nx, int ny, double dx, double dy, double l_shift,
+                                   double m_shift, size_t n_rows, size_t n_chan,
+                                   const double* uvw, const double* freq, const float* vis,
+                                   size_t npix, const int* xs, const int* ys, double* out,
+                                   int nthreads, int* used_recurrence) {
+  dft_invert_pixels_fast_impl(nx, ny, dx, dy, l_shift, m_shift, n_rows, n_chan, uvw, freq, vis,
+                              npix, xs, ys, out, nthreads, used_recurrence, 0);
+}
+
+/* As oracle_dft_invert_pixels_fast, with one sum per channel:
+ * out[p * n_chan + c] = the dirty image of channel c alone at pixel p. */
+void oracle_dft_invert_pixels_per_channel(int nx, int ny, double dx, double dy, double l_shift,
+                                          double m_shift, size_t n_rows, size_t n_chan,
+                                          const double* uvw, const double* freq,
+                                          const float* vis, size_t npix, const int* xs,
+                                          const int* ys, double* out, int nthreads,
+                                          int* used_recurrence) {
+  dft_invert_pixels_fast_impl(nx, ny, dx, dy, l_shift, m_shift, n_rows, n_chan, uvw, freq, vis,
+                              npix, xs, ys, out, nthreads, used_recurrence, 1);
 }

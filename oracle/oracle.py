@@ -115,6 +115,29 @@ def dft_invert_pixels_fast(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, xs,
     return (out, bool(used.value)) if return_mode else out
 
 
+def dft_invert_pixels_per_channel(nx, ny, dx, dy, l_shift, m_shift, uvw, freq, vis, xs, ys,
+                                  nthreads=0, return_mode=False):
+    """dft_invert_pixels_fast with one sum per channel: [len(xs), n_chan],
+    column c = dft_invert_pixels of channel c alone.  Pinned against
+    dft_invert_pixels per channel by tests/test_oracle.py."""
+    lib = load()
+    uvw = np.ascontiguousarray(uvw, np.float64)
+    freq = np.ascontiguousarray(freq, np.float64)
+    vis = np.ascontiguousarray(vis, np.complex64)
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    out = np.zeros(xs.size * freq.size, np.float64)
+    used = ctypes.c_int(0)
+    f = lib.oracle_dft_invert_pixels_per_channel
+    f.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
+        [ctypes.c_size_t, ctypes.c_size_t] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t] + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p]
+    f(nx, ny, dx, dy, l_shift, m_shift, uvw.size // 3, freq.size, _p(uvw), _p(freq), _p(vis),
+      xs.size, _p(xs), _p(ys), _p(out), _nthreads(nthreads), ctypes.byref(used))
+    out = out.reshape(xs.size, freq.size)
+    return (out, bool(used.value)) if return_mode else out
+
+
 def dft_predict(nx, ny, dx, dy, l_shift, m_shift, image, uvw, freq, sel=None, nthreads=0):
     """Visibilities from image [ny, nx]; complex128 [n_rows, n_chan] (or [len(sel)])."""
     lib = load()

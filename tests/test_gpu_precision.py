@@ -3,10 +3,12 @@ array in a 3 km disc, 300 x 10 s at dec -30 deg, 8 channels over
 130-170 MHz, 1 arcsec pixels, 20 power-law point sources on pixel
 positions plus noise.  Per channel and per epsilon in {1e-3, 1e-4, 1e-5,
 1e-6}, the HIP gridder's dirty image is compared with the FP64 direct FT
-(the DirectMSGridder restatement) on 1000 random pixels, and its predicted
+(the DirectMSGridder restatement) on 1e4 random pixels, and its predicted
 visibilities of the source model with the direct FT on 1e5 random rows.
 The tolerance is epsilon in relative L2 (doc/source/wgridding.rst:19-20),
-for every channel.
+for every channel.  At epsilon 1e-5 and 1e-6 the first and last channels'
+whole dirty images, and the first channel's predictions, are also compared
+with the FP64 CPU w-gridder of the oracle on the same kernel and grid.
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 N = 4096
 DX = synth.ARCSEC
-N_PIX = 1000
+N_PIX = 10000
 N_VIS = 100000
 EPS = [1e-3, 1e-4, 1e-5, 1e-6]
 
@@ -40,11 +42,13 @@ def study(oracle):
     model = np.zeros((N, N))
     model[ys, xs] = flux
     sel = rng.choice(uvw.shape[0], N_VIS, replace=False)
+    # every channel's sums in one pass over the visibilities
+    per_chan = oracle.dft_invert_pixels_per_channel(N, N, DX, DX, 0.0, 0.0, uvw, freq, vis, px,
+                                                    py)
     dirty, pred = [], []
     for c in range(len(freq)):
         f1 = freq[c:c + 1]
-        v1 = np.ascontiguousarray(vis[:, c:c + 1])
-        dirty.append(oracle.dft_invert_pixels(N, N, DX, DX, 0.0, 0.0, uvw, f1, v1, px, py))
+        dirty.append(per_chan[:, c])
         pred.append(oracle.dft_predict(N, N, DX, DX, 0.0, 0.0, model, uvw, f1, sel=sel))
     print("precision study: %d rows, %d channels" % (uvw.shape[0], len(freq)), flush=True)
     return dict(uvw=uvw, freq=freq, vis=vis, px=px, py=py, model=model, sel=sel, dirty=dirty,
@@ -75,6 +79,38 @@ def test_precision_study_per_channel(study, eps):
              np.mean(err_p)), flush=True)
     assert max(err_i) < eps
     assert max(err_p) < eps
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-6])
+def test_precision_study_against_fp64_wgridder(study, oracle, eps):
+    # the oracle's FP64 CPU w-gridder on the same kernel and grid (W, beta,
+    # sigma, nu, nv): whole dirty images of the first and last channel, and
+    # the first channel's predictions on the 1e5 selected rows
+    uvw, freq, vis = study["uvw"], study["freq"], study["vis"]
+    n_rows = uvw.shape[0]
+    g = WGriddingGridder_Simple(N, N, N, N, DX, DX, 0.0, 0.0, 1, eps)
+    errs = []
+    for c in (0, len(freq) - 1):
+        f1, v1 = freq[c:c + 1], np.ascontiguousarray(vis[:, c:c + 1])
+        g.InitializeInversion()
+        g.AddInversionData(n_rows, 1, uvw, f1, v1)
+        g.FinalizeImage(1.0)
+        img = g.RealImage().reshape(N, N)
+        pl = g.plan()
+        kw = dict(W=pl["W"], beta=pl["beta"], sigma=pl["sigma"], nu=pl["nu"], nv=pl["nv"])
+        ref = oracle.cpu_wgrid_invert(N, N, DX, DX, 0.0, 0.0, uvw, f1, v1, **kw)
+        errs.append(rel_l2(img, ref))
+    f1 = freq[:1]
+    g.InitializePrediction(study["model"].astype(np.float32).ravel())
+    out = np.zeros((n_rows, 1), np.complex64)
+    g.PredictVisibilities(n_rows, 1, uvw, f1, out)
+    refv = oracle.cpu_wgrid_predict(N, N, DX, DX, 0.0, 0.0, study["model"], uvw, f1, **kw)
+    err_p = rel_l2(out[study["sel"], 0], refv[study["sel"], 0])
+    print("eps %.0e W %d sigma %.2f: invert vs FP64 w-gridder (channels 0, %d) %.2e %.2e, "
+          "predict %.2e" % (eps, kw["W"], kw["sigma"], len(freq) - 1, errs[0], errs[1], err_p),
+          flush=True)
+    assert max(errs) < eps
+    assert err_p < eps
 
 
 def test_plain_invert_predict_2048(oracle):

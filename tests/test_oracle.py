@@ -147,6 +147,30 @@ def test_fast_pixel_dft_matches_direct(oracle, uniform):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("uniform", [True, False])
+def test_per_channel_pixel_dft_matches_direct(oracle, uniform):
+    # one sum per channel against the per-term form on that channel alone:
+    # a channel's sum landing in another channel's column, or a phasor step
+    # applied out of order, fails at far more than 1e-11
+    p = synth.small_problem(seed=43, n_ant=12, n_times=12, n_chan=16, npix=128,
+                            pixel_arcsec=30.0, radius_m=4000.0, df=2e6, noise=0.1)
+    freq = p["freq"].copy()
+    if not uniform:
+        freq[5] += 0.4e6
+    rng = np.random.default_rng(3)
+    xs, ys = rng.integers(0, 128, 30), rng.integers(0, 128, 30)
+    got, used = oracle.dft_invert_pixels_per_channel(128, 128, p["dx"], p["dx"], 0.01, -0.02,
+                                                     p["uvw"], freq, p["vis"], xs, ys,
+                                                     nthreads=3, return_mode=True)
+    assert used == uniform
+    assert got.shape == (30, 16)
+    for c in range(16):
+        ref = oracle.dft_invert_pixels(128, 128, p["dx"], p["dx"], 0.01, -0.02, p["uvw"],
+                                       freq[c:c + 1], np.ascontiguousarray(p["vis"][:, c:c + 1]),
+                                       xs, ys)
+        np.testing.assert_allclose(got[:, c], ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
 def test_golden_fixture(oracle):
     g = np.load(os.path.join(HERE, "golden", "dft_small.npz"))
     img = oracle.dft_invert(int(g["nx"]), int(g["ny"]), float(g["dx"]), float(g["dy"]),
