@@ -136,10 +136,11 @@ __device__ __forceinline__ void RowSegmentMax(float m2, uint32_t row,
 // Keys in renumbered row order: element e = r * n_chan + ch is channel ch of
 // renumbered row r, i.e. of caller row RowTag(uvw4, r); uvw4 holds the
 // renumbered rows padded to 32 bytes.  Keys are the same as KeysKernel's for
-// the same visibility, only their order differs.  With rowmax_bits, also
+// the same visibility, only their order differs.  ROWMAX (with vis): also
 // max |V| of every renumbered row (the values RowMaxKernel gives with the
-// rows' caller map) from the same reads.  The loop steps whole waves, so
+// rows' caller map) from the same reads; its loop steps whole waves, so
 // every lane takes part in the row maxima.
+template <bool ROWMAX>
 __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     DevPlan dp, const double* __restrict__ uvw4,
     const double* __restrict__ scale, size_t n_rows, size_t n_chan,
@@ -147,11 +148,11 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
     uint32_t* __restrict__ rowmax_bits) {
   const size_t n = n_rows * n_chan;
   const double inv_chan = 1.0 / double(n_chan);
-  const size_t lane = threadIdx.x & 63;
-  for (size_t e0 = size_t(blockIdx.x) * kBlock + (threadIdx.x & ~63u); e0 < n;
-       e0 += size_t(gridDim.x) * kBlock) {
+  const size_t lane = ROWMAX ? threadIdx.x & 63 : threadIdx.x;
+  const size_t first = size_t(blockIdx.x) * kBlock + (ROWMAX ? (threadIdx.x & ~63u) : 0u);
+  for (size_t e0 = first; ROWMAX ? e0 < n : e0 + lane < n; e0 += size_t(gridDim.x) * kBlock) {
     const size_t e = e0 + lane;
-    const bool in = e < n;
+    const bool in = !ROWMAX || e < n;
     float m2 = 0.f;
     uint32_t rrow = ~0u;
     if (in) {
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) KeysRenumberedKernel(
       if (valid) key = VisKey(LocateVis(uv.x, uv.y, wt.x, scale[ch], dp), dp);
       keys[e] = key;
     }
-    if (rowmax_bits) RowSegmentMax(m2, rrow, rowmax_bits);
+    if constexpr (ROWMAX) RowSegmentMax(m2, rrow, rowmax_bits);
   }
 }
 
@@ -768,8 +769,12 @@ void LaunchKeysRenumbered(const DevPlan& dp, const double* uvw4, const double* s
   if (!n_rows) return;
   if (rowmax_bits) LaunchFillZero(rowmax_bits, n_rows * sizeof(uint32_t), s);
   WG_CHECK((reinterpret_cast<uintptr_t>(uvw4) & 31) == 0, "padded uvw must be 32-byte aligned");
-  KeysRenumberedKernel<<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(dp, uvw4, scale, n_rows, n_chan,
-                                                                  vis, keys, rowmax_bits);
+  if (rowmax_bits)
+    KeysRenumberedKernel<true><<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(
+        dp, uvw4, scale, n_rows, n_chan, vis, keys, rowmax_bits);
+  else
+    KeysRenumberedKernel<false><<<Blocks(n_rows * n_chan), kBlock, 0, s>>>(
+        dp, uvw4, scale, n_rows, n_chan, vis, keys, nullptr);
   WG_HIP(hipGetLastError());
 }
 
