@@ -151,7 +151,6 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
   for (int i = lane; i < kSplitTable; i += 64) tbl[i] = 0.f;
   // evaluation lane: slot es of the group, +g (eh = 0) or -g (eh = 1)
   const int es = lane & 31, eh = lane >> 5;
-  const float gs = eh ? -1.f : 1.f;
   // operand lane: row / column o16, slots 8 og .. 8 og + 7
   const int o16 = lane & 15, og = lane >> 4;
   const int a_l = o16 & 7, a_c = o16 >> 3;  // A row: v tap, component
@@ -243,10 +242,26 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
         if (bb + kStage < w1) load(bb + kStage);
         // both 32-visibility groups of the batch, seen by every lane
         // (evaluation lanes eh = 0 / 1 store the real / imaginary part)
-        float bu[2], bv[2], bw[2], bx[2];
+        float bu[2], bv[2], bx[2];
         HalfBroadcast(cc.x, bu[0], bu[1]);
         HalfBroadcast(cc.y, bv[0], bv[1]);
-        HalfBroadcast(cc.z, bw[0], bw[1]);
+        // the w taps of the lane's own visibility, slot es of group eh:
+        // each visibility's w polynomial pairs are evaluated once
+        float tw[W];
+        {
+          const float gw = __builtin_fmaf(cc.z, 2.f, -1.f);
+          const float yw = gw * gw;
+#pragma unroll
+          for (int q = 0; q < H; ++q) {
+            float c[D + 1];
+#pragma unroll
+            for (int d = 0; d <= D; ++d) c[d] = dp.es_poly[q][d];
+            float E, O;
+            EsPolyEvenOdd<D>(c, yw, E, O);
+            tw[q] = __builtin_fmaf(gw, O, E) * kSplitB;
+            tw[W - 1 - q] = __builtin_fmaf(-gw, O, E) * kSplitB;
+          }
+        }
         {
           float r[2], i[2];
           HalfBroadcast(vre, r[0], r[1]);
@@ -262,12 +277,10 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           // kernel values of slot es into the table.  The even and odd
           // parts of tap q's polynomial at y = g^2 give tap q (+g) and tap
           // W - 1 - q (-g) together: lanes eh = 0 / 1 take every tap of u /
-          // v, then both evaluate w and each writes its half of the taps.
+          // v; the w taps come from the lanes of this group's visibilities.
           {
             const float g1 = __builtin_fmaf(eh ? bv[half] : bu[half], 2.f, -1.f);
-            const float gw = __builtin_fmaf(bw[half], 2.f, -1.f);
-            const float y1 = g1 * g1, yw = gw * gw;
-            const float sgw = gs * gw;
+            const float y1 = g1 * g1;
             float* r1 = tbl + (eh ? kRowKv : kRowKu) * kSplitPitch + es;
             float* rw = tbl + kRowKw * kSplitPitch + es;
 #pragma unroll
@@ -279,8 +292,10 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
               EsPolyEvenOdd<D>(c, y1, E, O);
               r1[q * kSplitPitch] = __builtin_fmaf(g1, O, E);
               r1[(W - 1 - q) * kSplitPitch] = __builtin_fmaf(-g1, O, E);
-              EsPolyEvenOdd<D>(c, yw, E, O);
-              rw[(eh ? W - 1 - q : q) * kSplitPitch] = __builtin_fmaf(sgw, O, E) * kSplitB;
+            }
+            if (eh == half) {
+#pragma unroll
+              for (int k = 0; k < W; ++k) rw[k * kSplitPitch] = tw[k];
             }
           }
           tbl[(kRowV + eh) * kSplitPitch + es] = bx[half];
