@@ -481,24 +481,21 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
           const uint32_t gb = bb + 32u * uint32_t(half);
           if (gb >= w1) break;
           const uint32_t glen = min(32u, w1 - gb);
-          // outputs of this group's two column groups: lanes og == 0
-          uint32_t pw[2] = {0u, 0u};
-          float2 ph[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-          float2 av[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-          if (og == 0) {
-#pragma unroll
-            for (int cg = 0; cg < 2; ++cg) {
-              const uint32_t v = gb + uint32_t(16 * cg + jc);
-              if (v < w1) {
-                if constexpr (FUSED) {
-                  pw[cg] = perm[v];
-                  ph[cg] = phasor[v];
-                } else {
-                  av[cg] = acc[v];
-                }
-              }
+          // outputs of this group: lanes og = 0 / 1 write column group
+          // cg = og (visibility 16 og + jc of the group)
+          const uint32_t vout = gb + uint32_t(16 * (og & 1) + jc);
+          const bool store = og < 2 && vout < w1;
+          uint32_t pw = 0u;
+          float2 ph = make_float2(0.f, 0.f), av = make_float2(0.f, 0.f);
+          if (store) {
+            if constexpr (FUSED) {
+              pw = perm[vout];
+              ph = phasor[vout];
+            } else {
+              av = acc[vout];
             }
           }
+          float sre[2], sim[2];
           // kernel values of slot es into the table: lanes eh = 0 / 1 take
           // every tap of u / v from the even and odd parts of the tap
           // polynomials (as in the gridder), and each half evaluates its
@@ -575,18 +572,18 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
             im += __shfl_xor(im, 16);
             re += __shfl_xor(re, 32);
             im += __shfl_xor(im, 32);
-            re *= inv_s;
-            im *= inv_s;
-            const uint32_t v = gb + uint32_t(16 * cg + jc);
-            if (og == 0 && v < w1) {
-              if constexpr (FUSED) {
-                float2 V = make_float2(re * ph[cg].x + im * ph[cg].y,
-                                       im * ph[cg].x - re * ph[cg].y);
-                if (pw[cg] & kPermFlip) V.y = -V.y;
-                acc[pw[cg] & ~kPermFlip] = V;
-              } else {
-                acc[v] = make_float2(av[cg].x + re, av[cg].y + im);
-              }
+            sre[cg] = re;
+            sim[cg] = im;
+          }
+          if (store) {
+            const float re = (og ? sre[1] : sre[0]) * inv_s;
+            const float im = (og ? sim[1] : sim[0]) * inv_s;
+            if constexpr (FUSED) {
+              float2 V = make_float2(re * ph.x + im * ph.y, im * ph.x - re * ph.y);
+              if (pw & kPermFlip) V.y = -V.y;
+              acc[pw & ~kPermFlip] = V;
+            } else {
+              acc[vout] = make_float2(av.x + re, av.y + im);
             }
           }
           WaveLdsSync();
