@@ -440,9 +440,15 @@ void Engine::Invert(VisSet& vs, const float2* vis_dev, double* img_dev) {
       // the row maxima of these visibilities are already there when the
       // Prepare just before read them
       float* rowmax = static_cast<float*>(arena_.Get("grid.rowmax", vs.n_rows * sizeof(float)));
-      if (vs.rowmax_of != vis_dev)
-        LaunchRowMax(vis_dev, vs.rows, vs.n_rows, vs.n_chan, reinterpret_cast<uint32_t*>(rowmax),
-                     stream_);
+      if (vs.rowmax_of != vis_dev) {
+        // in caller row order, streaming through the visibilities, then
+        // gathered into renumbered row order
+        float* rowmax_c =
+            static_cast<float*>(arena_.Get("grid.rowmax_c", vs.n_rows * sizeof(float)));
+        LaunchRowMax(vis_dev, nullptr, vs.n_rows, vs.n_chan,
+                     reinterpret_cast<uint32_t*>(rowmax_c), stream_);
+        LaunchGatherRows(rowmax_c, vs.rows, vs.n_rows, rowmax, stream_);
+      }
       LaunchItemBounds(vs.items, vs.n_items, nullptr, vs.vrow, rowmax, bounds, stream_);
     } else {
       vsorted = static_cast<float2*>(arena_.Get("vis_sorted", vs.n_valid * sizeof(float2)));

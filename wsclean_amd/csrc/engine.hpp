@@ -457,6 +457,9 @@ void LaunchGatherVis(const uint32_t* perm, const float2* phasor, size_t n_valid,
 // (the launch clears the words first).
 void LaunchRowMax(const float2* vis, const uint32_t* row_map, size_t n_rows, size_t n_chan,
                   uint32_t* rowmax_bits, hipStream_t s);
+// out[r] = in[rows[r]] for r < n.
+void LaunchGatherRows(const float* in, const uint32_t* rows, size_t n, float* out,
+                      hipStream_t s);
 // bounds[it] = (max, sum) over the visibilities of work item it (of all
 // n_items) of a bound on |Re| and |Im| of each rotated visibility: with
 // vis_sorted, its own larger component; else max |V| of its row,

@@ -209,6 +209,49 @@ __global__ void __launch_bounds__(kBlock) RowMaxKernel(const float2* __restrict_
   }
 }
 
+// As RowMaxKernel without a row map, for an even n_chan: each lane reads
+// two visibilities of one row with a 16-byte load (no pair spans two rows),
+// four 128-element chunks per wave step.
+__global__ void __launch_bounds__(kBlock) RowMaxPairsKernel(const float4* __restrict__ vis2,
+                                                           size_t n_rows, size_t n_chan,
+                                                           uint32_t* __restrict__ rowmax_bits) {
+  const size_t npair = n_rows * n_chan / 2;
+  const double inv_chan = 1.0 / double(n_chan);
+  const size_t lane = threadIdx.x & 63;
+  const size_t stride = size_t(gridDim.x) * kBlock;
+  for (size_t p0 = size_t(blockIdx.x) * kBlock + (threadIdx.x & ~63u); p0 < npair;
+       p0 += 4 * stride) {
+    float4 v[4];
+    uint32_t row[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t p = p0 + k * stride + lane;
+      row[k] = ~0u;
+      v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (p < npair) {
+        v[k] = vis2[p];
+        row[k] = DivSmall(uint32_t(2 * p), uint32_t(n_chan), inv_chan);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (p0 + k * stride >= npair) break;  // the same for every lane
+      RowSegmentMax(fmaxf(SquaredModulus(make_float2(v[k].x, v[k].y)),
+                          SquaredModulus(make_float2(v[k].z, v[k].w))),
+                    row[k], rowmax_bits);
+    }
+  }
+}
+
+// out[r] = in[rows[r]]: per-row values in another row order.
+__global__ void __launch_bounds__(kBlock) GatherRowsKernel(const float* __restrict__ in,
+                                                          const uint32_t* __restrict__ rows,
+                                                          size_t n, float* __restrict__ out) {
+  for (size_t r = size_t(blockIdx.x) * kBlock + threadIdx.x; r < n;
+       r += size_t(gridDim.x) * kBlock)
+    out[r] = in[rows[r]];
+}
+
 // bounds[it] = (max, sum) over work item it's visibilities of a bound b_i
 // on |Re| and |Im| of each rotated visibility: SORTED, the larger component
 // of vis_sorted[i] itself; else max |V| of the visibility's renumbered row
@@ -796,9 +839,24 @@ void LaunchRowMax(const float2* vis, const uint32_t* row_map, size_t n_rows, siz
   if (!n_rows || !n_chan) return;
   LaunchFillZero(rowmax_bits, n_rows * sizeof(uint32_t), s);
   const size_t n = n_rows * n_chan;
-  // four 64-element chunks per wave step, about four steps per wave
-  const unsigned nb = unsigned(std::min<size_t>((n + 16 * kBlock - 1) / (16 * kBlock), 16384));
-  RowMaxKernel<<<std::max(nb, 1u), kBlock, 0, s>>>(vis, row_map, n_rows, n_chan, rowmax_bits);
+  if (!row_map && n_chan % 2 == 0 && (reinterpret_cast<uintptr_t>(vis) & 15) == 0) {
+    // four 128-element chunks per wave step, about four steps per wave
+    const unsigned nb =
+        unsigned(std::min<size_t>((n / 2 + 16 * kBlock - 1) / (16 * kBlock), 16384));
+    RowMaxPairsKernel<<<std::max(nb, 1u), kBlock, 0, s>>>(
+        reinterpret_cast<const float4*>(vis), n_rows, n_chan, rowmax_bits);
+  } else {
+    // four 64-element chunks per wave step, about four steps per wave
+    const unsigned nb = unsigned(std::min<size_t>((n + 16 * kBlock - 1) / (16 * kBlock), 16384));
+    RowMaxKernel<<<std::max(nb, 1u), kBlock, 0, s>>>(vis, row_map, n_rows, n_chan, rowmax_bits);
+  }
+  WG_HIP(hipGetLastError());
+}
+
+void LaunchGatherRows(const float* in, const uint32_t* rows, size_t n, float* out,
+                      hipStream_t s) {
+  if (!n) return;
+  GatherRowsKernel<<<Blocks(n), kBlock, 0, s>>>(in, rows, n, out);
   WG_HIP(hipGetLastError());
 }
 
