@@ -93,6 +93,50 @@ def test_chunked_correction_callback_equals_one_chunk(oracle):
     assert rel_l2(imgs[1], imgs[0]) < 1e-6
 
 
+def test_host_calls_equal_device_calls_bitwise():
+    # host-buffer calls stage transfers above 4 MiB through page-locked
+    # 64 MiB chunks: 1.09e7 visibilities (83 MiB, two chunks, the
+    # second partial; uvw 2.5 MiB, direct) give the device calls' image and
+    # predictions bit for bit, twice in a row on one handle
+    import torch
+    rng = np.random.Generator(np.random.PCG64(77))
+    enu = synth.antennas_disc(rng, 40, 3000.0)
+    uvw = synth.earth_rotation_uvw(enu, 140, 30.0, np.deg2rad(-30.0))
+    freq = np.linspace(140e6, 160e6, 100)
+    n_rows, n_chan = uvw.shape[0], len(freq)
+    assert n_rows * n_chan * 8 > (64 << 20) and n_rows * 24 < (4 << 20)
+    dev = torch.device("cuda:0")
+    duvw = torch.from_numpy(uvw).to(dev)
+    src = np.array([[0.002, -0.001, 0.0, 1.0, 0.0], [-0.003, 0.002, 0.0, 0.5, 0.0]])
+    src[:, 2] = np.sqrt(1.0 - src[:, 0] ** 2 - src[:, 1] ** 2) - 1.0
+    dvis = torch.empty((n_rows, n_chan, 2), dtype=torch.float32, device=dev)
+    synth_visibilities(duvw, freq, src, 0.05, 77, dvis)
+    torch.cuda.synchronize()
+    vis = dvis.cpu().numpy().view(np.complex64).reshape(n_rows, n_chan).copy()
+    n, dx = 512, 10.0 * synth.ARCSEC
+    model = np.zeros((n, n), np.float32)
+    model[rng.integers(8, n - 8, 12), rng.integers(8, n - 8, 12)] = 1.0
+    g = WGriddingGridder_Simple(n, n, n, n, dx, dx, 0.0, 0.0, 1, 1e-5)
+    for _ in range(2):
+        g.InitializeInversion()
+        g.AddInversionData(n_rows, n_chan, uvw, freq, vis)
+        g.FinalizeImage(1.0)
+        img_h = g.RealImage()
+        g.InitializeInversion()
+        g.AddInversionDataDevice(duvw, freq, dvis)
+        g.FinalizeImage(1.0)
+        np.testing.assert_array_equal(img_h, g.RealImage())
+        g.InitializePrediction(model.ravel())
+        out_h = np.zeros((n_rows, n_chan), np.complex64)
+        g.PredictVisibilities(n_rows, n_chan, uvw, freq, out_h)
+        out_d = torch.empty_like(dvis)
+        g.PredictVisibilitiesDevice(duvw, freq, out_d)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out_h.view(np.float32).reshape(n_rows, n_chan, 2),
+                                      out_d.cpu().numpy())
+    g.close()
+
+
 def test_device_memory_within_accounting():
     # the device memory a handle holds after a call stays within what
     # ConstantMemoryUsage + PerVisibilityMemoryUsage x visibilities report
