@@ -384,12 +384,18 @@ def test_major_cycle_prepared_matches_stateless():
     assert rel_l2(got, ref_out) < 1e-5
 
 
-def test_one_call_inversion_equals_prepared_bitwise():
-    """The one-call inversion takes the max-modulus of the visibilities from
-    its keys kernel, the prepared path from its own pass over them: both
-    give the same bits, so the gridding scale and the images are identical."""
+@pytest.mark.parametrize("n_chan", [3, 16, 64])
+def test_one_call_inversion_equals_prepared_bitwise(n_chan):
+    """The one-call inversion takes the row maxima of |V| from its keys
+    kernel, the prepared path from its own pass over the visibilities (pairs
+    of channels per load for an even channel count, in caller row order,
+    then gathered into renumbered order): both give the same bits, so the
+    work items' gridding scales and the images are identical.  3, 16 and 64
+    channels put several rows, rows split between waves, and one row in a
+    wave's elements."""
     import torch
-    p = synth.small_problem(seed=23, n_ant=16, n_times=24, n_chan=3, npix=96)
+    p = synth.small_problem(seed=23, n_ant=16, n_times=24, n_chan=n_chan, npix=96,
+                            df=40e6 / n_chan)
     n_rows, n_chan = p["uvw"].shape[0], len(p["freq"])
     assert np.all(p["vis"] != 0)   # no zero skipping: the same sorted set
     dev = torch.device("cuda:0")
