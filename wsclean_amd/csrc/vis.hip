@@ -62,13 +62,32 @@ __global__ void __launch_bounds__(kBlock) ExtentsKernel(const double* __restrict
   }
 }
 
-__global__ void ExtentsFinalKernel(const double* __restrict__ partial, int nblocks,
-                                   double* __restrict__ out) {
+// The blocks' partial extents reduced by one block: each thread takes every
+// kBlock-th partial, then the waves and the block combine (min / max are
+// exact, so the order does not matter).
+__global__ void __launch_bounds__(kBlock) ExtentsFinalKernel(const double* __restrict__ partial,
+                                                             int nblocks,
+                                                             double* __restrict__ out) {
+  double r[6] = {DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX, DBL_MAX, -DBL_MAX};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double v = partial[size_t(b) * 6 + k];
+      r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
+    }
+  }
+  __shared__ double sh[kBlock / 64][6];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double v = (k & 1) ? WaveMax(r[k]) : WaveMin(r[k]);
+    if (lane == 0) sh[wave][k] = v;
+  }
+  __syncthreads();
   if (threadIdx.x < 6) {
     const int k = threadIdx.x;
-    double v = (k & 1) ? -DBL_MAX : DBL_MAX;
-    for (int b = 0; b < nblocks; ++b)
-      v = (k & 1) ? fmax(v, partial[size_t(b) * 6 + k]) : fmin(v, partial[size_t(b) * 6 + k]);
+    double v = sh[0][k];
+    for (int w = 1; w < kBlock / 64; ++w) v = (k & 1) ? fmax(v, sh[w][k]) : fmin(v, sh[w][k]);
     out[k] = v;
   }
 }
@@ -743,7 +762,7 @@ void LaunchExtents(const double* uvw, size_t n_rows, double smin, double smax,
                    double* partial, double* out, hipStream_t s) {
   const unsigned nb = std::min<unsigned>(Blocks(n_rows), 1024);
   ExtentsKernel<<<nb, kBlock, 0, s>>>(uvw, n_rows, smin, smax, partial);
-  ExtentsFinalKernel<<<1, 64, 0, s>>>(partial, int(nb), out);
+  ExtentsFinalKernel<<<1, kBlock, 0, s>>>(partial, int(nb), out);
   WG_HIP(hipGetLastError());
 }
 
