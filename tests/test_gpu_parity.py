@@ -237,6 +237,41 @@ def test_invert_uniform_weighted_dynamic_range(oracle, precision, sub_slab):
     assert rel_l2(img[ys, xs], ref) < 1e-5
 
 
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+@pytest.mark.parametrize("precision,sub_slab,one_call",
+                         [(-1, 0, True), (-1, 0, False), (0, 0, False), (-1, 2, False)])
+def test_non_finite_visibility_reaches_the_image(bad, precision, sub_slab, one_call):
+    # one NaN or infinite visibility makes the dirty image non-finite, as an
+    # FP gridder's FFT spreads it, instead of being dropped with its work
+    # item: its row maximum and item bounds carry it, and the item's
+    # subgrid is written as NaN.  Paths: the one-call row maxima from the
+    # keys pass, the prepared row maxima, the FP32 matrix kernel, and the
+    # pre-gathered components of several sub-slabs.
+    import torch
+    p = uniform_weighted_problem()
+    vis = p["vis"].copy()
+    vis[len(vis) // 3, 5] = complex(bad, 0.0)
+    n = p["npix"]
+    g = make(n, p["dx"], 1e-5)
+    g.set_precision(precision)
+    if sub_slab:
+        g.set_tuning(sub_slab=sub_slab)
+    n_rows, n_chan = vis.shape
+    g.InitializeInversion()
+    if one_call:
+        g.AddInversionData(n_rows, n_chan, p["uvw"], p["freq"], vis)
+    else:
+        dev = torch.device("cuda:0")
+        g.PrepareVisibilities(torch.from_numpy(p["uvw"]).to(dev), p["freq"], n_rows, n_chan)
+        g.AddInversionPrepared(
+            torch.from_numpy(vis.view(np.float32).reshape(n_rows, n_chan, 2).copy()).to(dev))
+    g.FinalizeImage(1.0)
+    img = g.RealImage()
+    if sub_slab:
+        assert g.plan()["P"] > sub_slab
+    assert not np.isfinite(img).any()
+
+
 def test_invert_is_deterministic():
     # integer subgrid accumulation + fixed-order gather: bitwise repeatable
     p = dense_core_problem()

@@ -163,6 +163,14 @@ __device__ __forceinline__ void EsTaps(const DevPlan& dp, float f, float* out) {
   }
 }
 
+// max(a, b) that keeps a NaN operand (fmaxf returns the other one): the
+// bounds of a work item holding a NaN or infinite visibility become NaN,
+// and its subgrid is written as NaN, so the image shows the bad input
+// instead of silently losing the item's visibilities.
+__device__ __forceinline__ float MaxKeepNaN(float a, float b) {
+  return (a > b || a != a) ? a : b;
+}
+
 // |V|^2 with one fixed rounding sequence (so every kernel that takes the
 // max-modulus of the same visibilities gets the same bits).
 __device__ __forceinline__ float SquaredModulus(float2 v) {

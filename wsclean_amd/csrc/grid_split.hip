@@ -185,7 +185,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves)
     (void)frexpf(fmaxf(bnd, 1e-30f), &e);
     e = max(e, -90);
     const float fx_scale = (bnd < 3.0e38f) ? ldexpf(1.0f, 30 - e) : 0.f;
-    const float fx_inv = ldexpf(1.0f, e - 30);
+    // a non-finite bound (NaN or infinite visibilities) writes the subgrid as NaN
+    const float fx_inv = (bnd < 3.0e38f) ? ldexpf(1.0f, e - 30) : __builtin_nanf("");
     // D holds sA kSplitB times the sums
     const float fxs = fx_scale / (sA * kSplitB);
     const uint32_t per = (n + kMfmaWaves - 1) / kMfmaWaves;

@@ -136,7 +136,7 @@ __device__ __forceinline__ void RowSegmentMax(float m2, uint32_t row,
   const int lane = threadIdx.x & 63;
   const uint32_t r0 = __builtin_amdgcn_readfirstlane(row);
   if (__all(row == r0)) {
-    for (int m = 32; m >= 1; m >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, m));
+    for (int m = 32; m >= 1; m >>= 1) m2 = MaxKeepNaN(m2, __shfl_xor(m2, m));
     if (lane == 0 && r0 != ~0u) atomicMax(rowmax_bits + r0, __float_as_uint(sqrtf(m2)));
     return;
   }
@@ -145,7 +145,7 @@ __device__ __forceinline__ void RowSegmentMax(float m2, uint32_t row,
   for (int off = 1; off < 64; off <<= 1) {
     const float o = __shfl_down(m2, off);
     const uint32_t ro = __shfl_down(row, off);
-    if (lane + off < 64 && ro == row) m2 = fmaxf(m2, o);
+    if (lane + off < 64 && ro == row) m2 = MaxKeepNaN(m2, o);
   }
   const uint32_t rp = __shfl_up(row, 1);
   if ((lane == 0 || rp != row) && row != ~0u)
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) RowMaxPairsKernel(const float4* __rest
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (p0 + k * stride >= npair) break;  // the same for every lane
-      RowSegmentMax(fmaxf(SquaredModulus(make_float2(v[k].x, v[k].y)),
+      RowSegmentMax(MaxKeepNaN(SquaredModulus(make_float2(v[k].x, v[k].y)),
                           SquaredModulus(make_float2(v[k].z, v[k].w))),
                     row[k], rowmax_bits);
     }
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) ItemBoundsKernel(const WorkItem* __restri
   auto bound = [&](uint32_t i) {
     if constexpr (SORTED) {
       const float2 v = vis_sorted[i];
-      return fmaxf(fabsf(v.x), fabsf(v.y));
+      return MaxKeepNaN(fabsf(v.x), fabsf(v.y));
     } else {
       return rowmax[vrow[i]];
     }
@@ -303,17 +303,17 @@ __global__ void __launch_bounds__(256) ItemBoundsKernel(const WorkItem* __restri
     for (int k = 0; k < 4; ++k) b[k] = bound(i + 256u * k);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      mx = fmaxf(mx, b[k]);
+      mx = MaxKeepNaN(mx, b[k]);
       sm += b[k];
     }
   }
   for (; i < wi.end; i += 256u) {
     const float b = bound(i);
-    mx = fmaxf(mx, b);
+    mx = MaxKeepNaN(mx, b);
     sm += b;
   }
   for (int m = 32; m >= 1; m >>= 1) {
-    mx = fmaxf(mx, __shfl_xor(mx, m));
+    mx = MaxKeepNaN(mx, __shfl_xor(mx, m));
     sm += __shfl_xor(sm, m);
   }
   const int wave = threadIdx.x >> 6;
@@ -324,7 +324,8 @@ __global__ void __launch_bounds__(256) ItemBoundsKernel(const WorkItem* __restri
   __syncthreads();
   if (threadIdx.x == 0)
     bounds[blockIdx.x] =
-        make_float2(fmaxf(fmaxf(part[0][0], part[0][1]), fmaxf(part[0][2], part[0][3])),
+        make_float2(MaxKeepNaN(MaxKeepNaN(part[0][0], part[0][1]),
+                               MaxKeepNaN(part[0][2], part[0][3])),
                     (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]));
 }
 
