@@ -467,6 +467,32 @@ def test_errors_leave_handle_usable():
     assert np.isfinite(g.RealImage()).all()
 
 
+@pytest.mark.parametrize("axis", [0, 2])
+def test_non_finite_uvw_is_refused(axis):
+    # a NaN coordinate reaches the extents (their min / max keep NaN) and
+    # the call fails with the non-finite-coordinates error instead of
+    # gridding visibilities located from NaN; the handle stays usable
+    import torch
+    p = synth.small_problem(seed=5, npix=32)
+    n_rows = p["uvw"].shape[0]
+    bad = p["uvw"].copy()
+    bad[n_rows // 2, axis] = np.nan
+    g = make(32, p["dx"], 1e-4)
+    g.InitializeInversion()
+    with pytest.raises(WgError, match="non-finite uvw"):
+        g.AddInversionData(n_rows, 2, bad, p["freq"], p["vis"])
+    with pytest.raises(WgError, match="non-finite uvw"):
+        g.PrepareVisibilities(torch.from_numpy(bad).to("cuda:0"), p["freq"], n_rows, 2)
+    out = np.zeros((n_rows, 2), np.complex64)
+    g.InitializePrediction(np.zeros((32, 32), np.float32))
+    with pytest.raises(WgError, match="non-finite uvw"):
+        g.PredictVisibilities(n_rows, 2, bad, p["freq"], out)
+    g.InitializeInversion()
+    g.AddInversionData(n_rows, 2, p["uvw"], p["freq"], p["vis"])
+    g.FinalizeImage(1.0)
+    assert np.isfinite(g.RealImage()).all()
+
+
 # ------------------------------------------------ correction callback (f3)
 
 @pytest.mark.parametrize("mode,nparms,forward,pol", [

@@ -14,12 +14,20 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// min / max that keep a NaN operand (fmin / fmax return the other one), so
+// a NaN coordinate reaches the extents and the host's finiteness check
+__device__ __forceinline__ double MinKeepNaNd(double a, double b) {
+  return (a < b || a != a) ? a : b;
+}
+__device__ __forceinline__ double MaxKeepNaNd(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
 __device__ __forceinline__ double WaveMin(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = MinKeepNaNd(v, __shfl_xor(v, o, 64));
   return v;
 }
 __device__ __forceinline__ double WaveMax(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = MaxKeepNaNd(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -40,12 +48,12 @@ __global__ void __launch_bounds__(kBlock) ExtentsKernel(const double* __restrict
     const double u1 = u * smin, u2 = u * smax;
     const double v1 = v * smin, v2 = v * smax;
     const double w1 = w * smin, w2 = w * smax;
-    r[0] = fmin(r[0], fmin(u1, u2));
-    r[1] = fmax(r[1], fmax(u1, u2));
-    r[2] = fmin(r[2], fmin(v1, v2));
-    r[3] = fmax(r[3], fmax(v1, v2));
-    r[4] = fmin(r[4], fmin(w1, w2));
-    r[5] = fmax(r[5], fmax(w1, w2));
+    r[0] = MinKeepNaNd(r[0], MinKeepNaNd(u1, u2));
+    r[1] = MaxKeepNaNd(r[1], MaxKeepNaNd(u1, u2));
+    r[2] = MinKeepNaNd(r[2], MinKeepNaNd(v1, v2));
+    r[3] = MaxKeepNaNd(r[3], MaxKeepNaNd(v1, v2));
+    r[4] = MinKeepNaNd(r[4], MinKeepNaNd(w1, w2));
+    r[5] = MaxKeepNaNd(r[5], MaxKeepNaNd(w1, w2));
   }
   __shared__ double sh[kBlock / 64][6];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -57,7 +65,8 @@ __global__ void __launch_bounds__(kBlock) ExtentsKernel(const double* __restrict
   if (threadIdx.x < 6) {
     const int k = threadIdx.x;
     double v = sh[0][k];
-    for (int w = 1; w < kBlock / 64; ++w) v = (k & 1) ? fmax(v, sh[w][k]) : fmin(v, sh[w][k]);
+    for (int w = 1; w < kBlock / 64; ++w)
+      v = (k & 1) ? MaxKeepNaNd(v, sh[w][k]) : MinKeepNaNd(v, sh[w][k]);
     partial[size_t(blockIdx.x) * 6 + k] = v;
   }
 }
@@ -73,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) ExtentsFinalKernel(const double* __res
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double v = partial[size_t(b) * 6 + k];
-      r[k] = (k & 1) ? fmax(r[k], v) : fmin(r[k], v);
+      r[k] = (k & 1) ? MaxKeepNaNd(r[k], v) : MinKeepNaNd(r[k], v);
     }
   }
   __shared__ double sh[kBlock / 64][6];
@@ -87,7 +96,8 @@ __global__ void __launch_bounds__(kBlock) ExtentsFinalKernel(const double* __res
   if (threadIdx.x < 6) {
     const int k = threadIdx.x;
     double v = sh[0][k];
-    for (int w = 1; w < kBlock / 64; ++w) v = (k & 1) ? fmax(v, sh[w][k]) : fmin(v, sh[w][k]);
+    for (int w = 1; w < kBlock / 64; ++w)
+      v = (k & 1) ? MaxKeepNaNd(v, sh[w][k]) : MinKeepNaNd(v, sh[w][k]);
     out[k] = v;
   }
 }
