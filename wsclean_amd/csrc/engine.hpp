@@ -280,6 +280,9 @@ FacetLayout PlanFacets(const std::vector<ImageGeometry>& geoms,
 class FacetBatch {
  public:
   FacetBatch(const std::vector<ImageGeometry>& facets, const PlanOptions& opt);
+  ~FacetBatch();
+  FacetBatch(const FacetBatch&) = delete;
+  FacetBatch& operator=(const FacetBatch&) = delete;
   int NumFacets() const { return int(geoms_.size()); }
   const ImageGeometry& Geometry(int f) const { return geoms_[f]; }
   Engine& Eng() { return eng_; }
@@ -308,6 +311,12 @@ class FacetBatch {
   uint32_t* rphys_ = nullptr;  // physical row of each renumbered virtual row
   size_t n_vrows_ = 0;
   bool prepared_ = false;
+  // Consecutive facets' line FFTs alternate between the engine stream and
+  // this one, each with its own row and column buffers, so one facet's
+  // column pass overlaps the next facet's transforms (fork / join events
+  // around the loop).
+  hipStream_t aux_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
 };
 
 // ---- FFT / plane kernels (fft.hip) ----

@@ -131,12 +131,26 @@ FacetBatch::FacetBatch(const std::vector<ImageGeometry>& facets, const PlanOptio
     WG_CHECK(g.nx > 0 && g.ny > 0, "empty facet image");
 }
 
+FacetBatch::~FacetBatch() {
+  if (aux_) {
+    (void)hipStreamSynchronize(aux_);
+    (void)hipStreamDestroy(aux_);
+  }
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+}
+
 void FacetBatch::Prepare(size_t n_chan, const double* freq_host, const double* uvw_dev,
                          const size_t* row_off, const size_t* n_rows) {
   const int F = NumFacets();
   hipStream_t s = eng_.Stream();
   DeviceArena& A = eng_.Arena();
   EventTimer tm(eng_, &eng_.Times().plan);
+  if (!aux_) {
+    WG_HIP(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+    WG_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    WG_HIP(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+  }
   prepared_ = false;
   WG_CHECK(n_chan > 0, "n_chan must be positive");
   row_off_.assign(row_off, row_off + F);
@@ -337,8 +351,13 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
   }
   float2* box =
       static_cast<float2*>(A.Get("box", size_t(pa.Sf) * da.box_plane * sizeof(float2)));
-  float2* rbuf = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
-  float2* cbuf = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  float2* rbuf[2], *cbuf[2];
+  rbuf[0] = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  cbuf[0] = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  rbuf[1] = F > 1 ? static_cast<float2*>(A.Get("rbuf2", size_t(pa.Sf) * lines * sizeof(float2)))
+                  : rbuf[0];
+  cbuf[1] = F > 1 ? static_cast<float2*>(A.Get("cbuf2", size_t(pa.Sf) * lines * sizeof(float2)))
+                  : cbuf[0];
   float2* scratch = static_cast<float2*>(
       A.Get("grid.scratch", GridScratchBytes(da, pa.Sg, atlas_.max_items_per_sub)));
   for (int f0 = 0; f0 < pa.P; f0 += pa.Sf) {
@@ -365,16 +384,23 @@ void FacetBatch::Invert(const float2* vis_dev, double* const* img_dev) {
       }
     }
     EventTimer tf(eng_, &st.fft);
+    WG_HIP(hipEventRecord(fork_, s));
+    WG_HIP(hipStreamWaitEvent(aux_, fork_, 0));
+    int k = 0;
     for (int f = 0; f < F; ++f) {
       const VisSet& v = fvs_[f];
       const int npf = std::min(np, v.plan.P - f0);
       if (npf <= 0) continue;
       const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
       const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
-      LaunchRowFFTInvert(v.dp, fu, Region(box, f), rbuf, npf, s);
-      LaunchTranspose(rbuf, cbuf, v.dp.BV, v.dp.nx, npf, s);
-      LaunchColFFTInvert(v.dp, fv, cbuf, dirty[f], f0, npf, s);
+      const int b = k++ & 1;
+      hipStream_t sf = b ? aux_ : s;
+      LaunchRowFFTInvert(v.dp, fu, Region(box, f), rbuf[b], npf, sf);
+      LaunchTranspose(rbuf[b], cbuf[b], v.dp.BV, v.dp.nx, npf, sf);
+      LaunchColFFTInvert(v.dp, fv, cbuf[b], dirty[f], f0, npf, sf);
     }
+    WG_HIP(hipEventRecord(join_, aux_));
+    WG_HIP(hipStreamWaitEvent(s, join_, 0));
   }
   for (int f = 0; f < F; ++f)
     LaunchAccumulateImage(dirty[f], fvs_[f].corr, img_dev[f],
@@ -402,8 +428,13 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
     lines = std::max(lines, size_t(fvs_[f].plan.BV) * fvs_[f].plan.img.nx);
   float2* box =
       static_cast<float2*>(A.Get("box", size_t(pa.Sf) * da.box_plane * sizeof(float2)));
-  float2* rbuf = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
-  float2* cbuf = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  float2* rbuf[2], *cbuf[2];
+  rbuf[0] = static_cast<float2*>(A.Get("rbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  cbuf[0] = static_cast<float2*>(A.Get("cbuf", size_t(pa.Sf) * lines * sizeof(float2)));
+  rbuf[1] = F > 1 ? static_cast<float2*>(A.Get("rbuf2", size_t(pa.Sf) * lines * sizeof(float2)))
+                  : rbuf[0];
+  cbuf[1] = F > 1 ? static_cast<float2*>(A.Get("cbuf2", size_t(pa.Sf) * lines * sizeof(float2)))
+                  : cbuf[0];
   uint32_t* queue = static_cast<uint32_t*>(A.Get("grid.words", 4 * sizeof(uint32_t)));
   for (int f0 = 0; f0 < pa.P; f0 += pa.Sf) {
     const int np = std::min(pa.Sf, pa.P - f0);
@@ -411,16 +442,23 @@ void FacetBatch::Predict(const double* const* img_dev, float2* vis_out_dev) {
     // only reach planes below P
     {
       EventTimer tf(eng_, &st.fft);
+      WG_HIP(hipEventRecord(fork_, s));
+      WG_HIP(hipStreamWaitEvent(aux_, fork_, 0));
+      int k = 0;
       for (int f = 0; f < F; ++f) {
         const VisSet& v = fvs_[f];
         const int npf = std::min(np, v.plan.P - f0);
         if (npf <= 0) continue;
         const FFTDesc fu = MakeFFTDesc(v.dp.nu, eng_.Twiddles(v.dp.nu));
         const FFTDesc fv = MakeFFTDesc(v.dp.nv, eng_.Twiddles(v.dp.nv));
-        LaunchColFFTPredict(v.dp, fv, img_dev[f], v.corr, cbuf, f0, npf, s);
-        LaunchTranspose(cbuf, rbuf, v.dp.nx, v.dp.BV, npf, s);
-        LaunchRowFFTPredict(v.dp, fu, rbuf, Region(box, f), npf, s);
+        const int b = k++ & 1;
+        hipStream_t sf = b ? aux_ : s;
+        LaunchColFFTPredict(v.dp, fv, img_dev[f], v.corr, cbuf[b], f0, npf, sf);
+        LaunchTranspose(cbuf[b], rbuf[b], v.dp.nx, v.dp.BV, npf, sf);
+        LaunchRowFFTPredict(v.dp, fu, rbuf[b], Region(box, f), npf, sf);
       }
+      WG_HIP(hipEventRecord(join_, aux_));
+      WG_HIP(hipStreamWaitEvent(s, join_, 0));
     }
     EventTimer td(eng_, &st.degrid);
     const size_t E = size_t(da.T + da.W - 1);
